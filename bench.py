@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark: forward + adjoint solve pairs per second on one MI355X.
+
+Workload: 201 x 201 lon/lat grid at 1 km, 100 virtual sources x 30 periods =
+3,000 (source, period) pairs with 99 receivers each; velocity 3.2 km/s with an
+independent seeded +-10 % Gaussian random field (15 km correlation) per
+period; observed times from the 3.2 km/s background (synthetic data).
+
+A step is one pass of the whole hot path over the batch -- forward solve,
+receiver interpolation, residual, adjoint solve, mask, per-period kernel
+accumulation and chi for all 3,000 pairs -- with inputs resident in HBM
+(src/preproc.cpp:71-148 for every event of an inversion iteration).
+
+`value` is the batched FP32 path.  The line also carries the FP64 batched and
+the plain per-solve FP64 GPU numbers, the FP32 kernel error against the FP64
+path (which equals the FP64 oracle to 1e-12), the roofline of the solve
+kernel and the CPU baseline (the FP64 oracle on the host's cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fwd+adjoint solves/s @201² grid: batched FP32 vs plain FP64, ≤1e-5 rel err"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(nx, ny, iters_f, iters_a, w):
+    """Per-pair algorithmic bytes of the solve kernel (sweeps + set-up):
+    forward 9 words per node per sweep (fun, a, b, c, T0, T0x, T0y, tau read;
+    tau written), adjoint 5 words per interior node per sweep (x-flux, y-flux,
+    delta, Ta read; Ta written), set-up 3 + 6 words per node.  Iteration counts
+    are the FP64 path's (identical to the FP64 oracle's)."""
+    N = nx * ny
+    Ni = (nx - 2) * (ny - 2)
+    return (w * N * (4 * 9 * iters_f.astype(np.float64)) + w * Ni * (4 * 5 * iters_a.astype(np.float64))
+            + w * N * (3 + 6))
+
+
+def _cpu_worker(args):
+    """One host core: its round-robin share of pairs (include/parallel.h:39-41),
+    stopping at the deadline."""
+    batch, ids, deadline = args
+    from oracle import oracle as O
+    from surfattpp_amd import synth
+    done = 0
+    for e in ids:
+        if time.time() > deadline:
+            break
+        O.forward_for_event(synth.subset_events(batch, [e]), lib=O.bench_lib())
+        done += 1
+    return done
+
+
+def cpu_baseline(batch, seconds: float, cores: int):
+    ne = len(batch["ev_iper"])
+    deadline = time.time() + seconds
+    shares = [(batch, list(range(r, ne, cores)), deadline) for r in range(cores)]
+    t0 = time.time()
+    ctx = mp.get_context("fork")
+    with ctx.Pool(cores) as pool:
+        done = pool.map(_cpu_worker, shares)
+    dt = time.time() - t0
+    n = int(sum(done))
+    return dict(value=n / dt, unit="fwd+adjoint solves/s", cores=cores, kind="port",
+                sample=f"{n} of {ne} pairs of the same workload (FP64 oracle, -O3 -mavx2 -mfma), "
+                       f"round-robin over {cores} processes, {dt:.1f} s wall")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-pairs", type=int, default=30, help="pairs timed in per-solve FP64 mode")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the (source, period) "
+                                   "pairs shard by period across GPUs, see DESIGN.md)"}))
+        sys.exit(2)
+
+    from surfattpp_amd import PREC_F32, PREC_F64, RUN_PER_SOLVE, _lib, synth
+    from surfattpp_amd import preproc as PP
+
+    dev = _lib.device_info()
+    b = synth.config_grf()
+    nx, ny = b["lon"].size, b["lat"].size
+    ne = len(b["ev_iper"])
+
+    # FP64 batched (reference arithmetic): iteration counts and the FP64 kernels.
+    with PP.EikonalBatch(b, precision=PREC_F64) as b64:
+        b64.run()
+        b64.run()
+        f64_ms = b64.last_ms()
+        r64 = b64.results()
+    iters_f, iters_a = r64["iters_fwd"], r64["iters_adj"]
+
+    # plain per-solve FP64: one launch per pair, on a subset
+    sub = synth.subset_events(b, np.linspace(0, ne - 1, args.plain_pairs).astype(int))
+    with PP.EikonalBatch(sub, precision=PREC_F64) as bp:
+        bp.run(mode=RUN_PER_SOLVE)
+        bp.run(mode=RUN_PER_SOLVE)
+        plain_ms = bp.last_ms()[0]
+    plain_rate = args.plain_pairs / (plain_ms * 1e-3)
+
+    # batched FP32: the measured path
+    bt = PP.EikonalBatch(b, precision=PREC_F32)
+    for _ in range(args.warmup):
+        bt.run()
+    bt.sync()
+    solve_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bt.run(sync=False)
+        bt.sync()  # per step, so the per-launch event timing is read for every step
+        solve_ms.append(bt.last_ms()[1])
+    bt.sync()
+    wall = time.perf_counter() - t0
+    r32 = bt.results()
+    bt.close()
+    ms_per_step = wall / args.steps * 1e3
+    value = ne * args.steps / wall
+
+    err_k = max(float(np.abs(r32["adj_s"][p] - r64["adj_s"][p]).max() / np.abs(r64["adj_s"][p]).max())
+                for p in range(r64["adj_s"].shape[0]))
+    err_syn = float(np.abs(r32["syn"] - r64["syn"]).max() / np.abs(r64["syn"]).max())
+    err_chi = abs(r32["chi"] - r64["chi"]) / abs(r64["chi"])
+
+    kern_ms = float(np.mean(solve_ms))
+    bytes_launch = float(algorithmic_bytes(nx, ny, iters_f, iters_a, 4).sum())
+    achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_pair_kernel_f32_grf201.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(b, args.cpu_seconds, cores)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "fwd+adjoint solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "grf-201x201-1km-100src-30per", "grid": [nx, ny], "pairs": ne,
+                   "receivers_per_pair": int(b["ev_nrec"][0]), "sweep_precision": "fp32 (sigma = tau - 1)",
+                   "fp64_parts": "T0 init, T, adjoint coefficients, convergence norm, misfit, kernel accumulation",
+                   "device": dev},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "pair_kernel<SAT_PREC_F32,4>", "kernel_ms": round(kern_ms, 3),
+                     "algorithmic_bytes_per_launch": bytes_launch},
+        "cpu_baseline": cpu,
+        "err_vs_fp64": {"kernel_max_per_period": err_k, "syn": err_syn, "chi": err_chi},
+        "batched_f64": {"value": round(ne / (f64_ms[0] * 1e-3), 2), "ms_per_step": round(f64_ms[0], 3)},
+        "plain_f64_per_solve": {"value": round(plain_rate, 3), "pairs_timed": args.plain_pairs},
+        "speedup": {"vs_plain_f64": round(value / plain_rate, 1),
+                    "vs_cpu": (round(value / cpu["value"], 1) if cpu else None)},
+        "iters": {"fwd_mean": float(iters_f.mean()), "adj_mean": float(iters_a.mean()),
+                  "fwd_hist": np.bincount(iters_f).tolist(), "adj_hist": np.bincount(iters_a).tolist()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,159 @@
+/*
+ * surfatt_eik.h -- C ABI of the MI355X (gfx950) eikonal forward + adjoint path.
+ *
+ * Plain C, POD arguments only: no HIP, torch or Eigen types.  Library:
+ * surfattpp_amd/libsurfatt_eik.so.
+ *
+ * Field layout everywhere: a 2-D field on an nx-by-ny lon/lat grid is a
+ * column-major (ix fastest) array of nx*ny doubles, element (ix, iy) at
+ * ix + iy*nx -- the storage of the reference's Eigen::MatrixX<real_t>
+ * (src/eikonal_solver.cpp, Eigen default) and of its Fortran twin.
+ * Per-period planes in the batched API are period-major: plane iper starts
+ * at iper*nx*ny (the reference keeps [ix][iy][iper], include/surf_grid.h:63-65;
+ * the C++ shim transposes once per period).
+ *
+ * Status codes: SAT_OK (0) or a positive errno-style value.  Warnings of the
+ * reference (non-convergence, src/eikonal_solver.cpp:304-313; source on the
+ * boundary, :123-125) are reported as SAT_FLAG_* bits in sat_info.flags and
+ * never abort.
+ */
+#ifndef SURFATT_EIK_H
+#define SURFATT_EIK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SAT_OK 0
+#define SAT_EINVAL 22  /* shape < 3x3, non-uniform axis, null pointer, bad descriptor */
+#define SAT_ENOMEM 12  /* device allocation failed */
+#define SAT_EDEVICE 5  /* HIP runtime / kernel launch error, or no gfx950 device */
+
+/* Arithmetic of the sweeps.  SAT_PREC_F64 reproduces the reference's FP64
+ * operation order (kernels within 1e-12 of the FP64 oracle).  SAT_PREC_F32
+ * sweeps the factored perturbation in FP32 with FP64 source initialisation,
+ * convergence norm, misfit and kernel accumulation. */
+#define SAT_PREC_F64 0
+#define SAT_PREC_F32 1
+
+#define SAT_FLAG_NOT_CONVERGED 1   /* forward hit MAXITER (eikonal_solver.cpp:304) */
+#define SAT_FLAG_SRC_ON_BOUNDARY 2 /* a fixed near-source node is on the edge (:123) */
+#define SAT_FLAG_ADJ_MAXITER 4     /* adjoint hit maxiter (silent in the reference, :471) */
+#define SAT_FLAG_REC_OUTSIDE 8     /* a receiver lies outside the grid: syn = NaN (utils.h:379) */
+
+typedef struct sat_info {
+  int iters;     /* sweeping iterations performed (4 sweeps each) */
+  int flags;     /* SAT_FLAG_* */
+  double l1;     /* last convergence L1 (mean |dtau|*T0 or |dTa|) */
+  double linf;   /* last convergence Linf */
+} sat_info;
+
+/* ------------------------------------------------------------------------
+ * Per-solve calls (host pointers, synchronous, one solve per launch).
+ * ------------------------------------------------------------------------ */
+
+/* Replaces eikonal::FSM_UW_PS_lonlat_2d (include/eikonal_solver.h:32-40,
+ * src/eikonal_solver.cpp:22-318).  lon_deg[nx], lat_deg[ny] uniformly spaced
+ * degrees; spha/sphb/sphc/fun nx*ny; T_out nx*ny travel time (s). */
+int sat_eik_forward(int precision, int nx, int ny, const double* lon_deg, const double* lat_deg,
+                    const double* spha, const double* sphb, const double* sphc, const double* fun,
+                    double src_lon_deg, double src_lat_deg, double* T_out, sat_info* info);
+
+/* Replaces eikonal::FSM_O1_JSE_lonlat_2d (include/eikonal_solver.h:74-83,
+ * src/eikonal_solver.cpp:323-512): the adjoint-state field for receiver
+ * adjoint sources adj[nr] at (rec_lon_deg, rec_lat_deg), given the forward
+ * travel time T (nx*ny). */
+int sat_eik_backward(int precision, int nx, int ny, const double* lon_deg, const double* lat_deg,
+                     const double* spha, const double* sphb, const double* sphc, const double* T,
+                     int nr, const double* rec_lon_deg, const double* rec_lat_deg, const double* adj,
+                     double* Ta_out, sat_info* info);
+
+/* Replaces eikonal::mask_uniform_grid (include/eikonal_solver.h:85-89,
+ * src/eikonal_solver.cpp:514-536), in place on a host field.  Coordinates in
+ * the caller's units (degrees at src/preproc.cpp:125). */
+int sat_eik_mask(int nx, int ny, const double* xx, const double* yy, double* Ta, double x0, double y0);
+
+/* ------------------------------------------------------------------------
+ * Batched path: every (source, period) pair of one inversion iteration in one
+ * persistent launch.  Replaces the per-rank serial loop of
+ * preproc::forward_for_event (include/preproc.h:14, src/preproc.cpp:71-148)
+ * and its MPI round-robin over events (include/parallel.h:39-41).
+ * ------------------------------------------------------------------------ */
+
+typedef struct sat_grid_desc {
+  int nx, ny;            /* nodes (>= 3 each) */
+  int nperiod;           /* planes below */
+  const double* lon_deg; /* [nx] uniform, ascending */
+  const double* lat_deg; /* [ny] uniform, ascending */
+  const double* svel;    /* [nperiod][ny][nx] phase velocity (km/s) */
+  const double* m11;     /* [nperiod][ny][nx] metric (surf_grid.cpp:184-190) */
+  const double* m12;     /* [nperiod][ny][nx]; the solver is called with sphc = -m12 */
+  const double* m22;     /* [nperiod][ny][nx] */
+} sat_grid_desc;
+
+typedef struct sat_event_desc {
+  int grid;           /* index into grids */
+  int iper;           /* period plane */
+  double src_lon_deg; /* evlo */
+  double src_lat_deg; /* evla */
+  int rec_off;        /* first receiver in the flattened receiver arrays */
+  int nrec;           /* receivers of this event (may be 0) */
+} sat_event_desc;
+
+typedef struct sat_batch_desc {
+  int ngrid;
+  const sat_grid_desc* grids;
+  int nevent;
+  const sat_event_desc* events; /* accumulation and chi order = this order */
+  int nrec_total;
+  const double* rec_lon_deg;    /* [nrec_total] */
+  const double* rec_lat_deg;
+  const double* tt_obs;         /* observed travel times */
+  const double* weight;         /* receiver weights */
+  int precision;                /* SAT_PREC_* */
+  int calc_adj;                 /* 0: forward + syn only (preproc.cpp:113 is_calc_adj) */
+} sat_batch_desc;
+
+typedef struct sat_batch sat_batch; /* opaque: device inputs, workspace, results */
+
+/* Upload inputs and allocate all device memory; no allocation afterwards. */
+int sat_batch_create(const sat_batch_desc* desc, sat_batch** out);
+/* Replace one grid's per-period planes (next model of an inversion). */
+int sat_batch_update_media(sat_batch* b, int grid, const double* svel, const double* m11,
+                           const double* m12, const double* m22);
+/* Replace observed times / weights (same receiver layout). */
+int sat_batch_update_obs(sat_batch* b, const double* tt_obs, const double* weight);
+/* Queue the whole pipeline (forward -> misfit -> adjoint -> mask -> kernel
+ * accumulate -> chi) on the batch's stream.  mode: SAT_RUN_BATCHED (one
+ * persistent launch) or SAT_RUN_PER_SOLVE (one launch per pair, same kernels). */
+#define SAT_RUN_BATCHED 0
+#define SAT_RUN_PER_SOLVE 1
+int sat_batch_run(sat_batch* b, int mode);
+int sat_batch_sync(sat_batch* b);
+/* Copy results to host.  syn[nrec_total]; adj_s: for each grid in order,
+ * [nperiod][ny][nx] FP64 kernels (preproc.cpp:31 adj_s_local); chi scalar
+ * (preproc.cpp:142); iters_fwd/iters_adj/flags [nevent].  Any pointer may be NULL. */
+int sat_batch_results(sat_batch* b, double* syn, double* adj_s, double* chi, int* iters_fwd,
+                      int* iters_adj, int* flags);
+/* FP32 batches: a pair whose FP32 forward stop/continue decision lies within
+ * a relative `band` of the reference tolerance (1e-4) is re-solved by the FP64
+ * path in the same run, so forward iteration counts equal the reference's.
+ * Default 0.02; 0 disables the re-solves. */
+int sat_batch_set_f32_band(sat_batch* b, double band);
+/* Per event: Linf of the last and the previous forward round (of the launch
+ * that produced the result), and the number of FP64 re-solves of the last run. */
+int sat_batch_conv(sat_batch* b, double* linf_last, double* linf_prev, int* n_resolved);
+/* Duration in ms of the last sat_batch_run, measured with HIP events on the
+ * batch's stream; also the ms of its main solve kernel launches alone. */
+int sat_batch_last_ms(sat_batch* b, double* total_ms, double* solve_ms);
+int sat_batch_destroy(sat_batch* b);
+
+/* Library identity: device name, build arch; returns SAT_EDEVICE without a GPU. */
+int sat_device_info(char* buf, int buflen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

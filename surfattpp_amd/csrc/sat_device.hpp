@@ -1,0 +1,247 @@
+// sat_device.hpp -- device-side building blocks of the gfx950 eikonal path.
+//
+// Storage on the device ("segmented rows"): a wave of 64 lanes sweeps one
+// grid; lane l owns the C consecutive columns [l*C, l*C+C) of every row.
+// A plane of an nx-by-ny grid is stored row after row with pitch P = La*C
+// (La = ceil(nx/C) active lanes), so lane l's part of row iy starts at
+// iy*P + l*C.  Columns ix >= nx in the last lane are padding.
+//
+// Gauss-Seidel order.  The reference visits, per sweep, x outer and y inner
+// (src/eikonal_solver.cpp:149-154, :474-478) and every node update reads only
+// its 4 axis neighbours, so node (ix,iy) depends on exactly the neighbours
+// that precede it in that order.  Lane l processes row r of the sweep at step
+// t = r + lag(l), lag = l for +x sweeps and La-1-l for -x sweeps: the x-side
+// neighbour of its first column was finished by the adjacent lane one step
+// earlier (passed by a cross-lane shuffle), the y-side neighbour by itself.
+// Every node sees the same neighbour values as in the sequential loop, so the
+// result is identical iterate for iterate.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace sat {
+
+constexpr double kPI = 3.14159265358979323846264338327950288;  // include/config.h:54
+constexpr double kR = 6371.0;                                   // include/config.h:55
+constexpr double kDEG2RAD = kPI / 180.0;                        // include/config.h:68
+constexpr int kFwdMaxIter = 2000;                               // src/eikonal_solver.cpp:15
+constexpr double kFwdTol = 1e-4;                                // :16
+constexpr int kAdjMaxIter = 1000;                               // :339
+constexpr double kAdjTol = 1e-9;                                // :338
+constexpr double kAdjEps = 1e-14;                               // :337
+constexpr int kWave = 64;
+
+constexpr int FLAG_NOT_CONVERGED = 1;
+constexpr int FLAG_SRC_ON_BOUNDARY = 2;
+constexpr int FLAG_ADJ_MAXITER = 4;
+constexpr int FLAG_REC_OUTSIDE = 8;
+
+__host__ __device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
+__host__ __device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
+__host__ __device__ inline int imin(int a, int b) { return (b < a) ? b : a; }
+__host__ __device__ inline int imax(int a, int b) { return (a < b) ? b : a; }
+
+// Geometry and per-period media of one grid, as seen by the kernels.
+struct GridDev {
+  int nx, ny, nperiod;
+  int C, La, P;            // columns per lane, active lanes, segmented row pitch
+  size_t seg_plane;        // P*ny
+  size_t plane;            // nx*ny
+  double dx, dy;           // radians
+  const double* lon;       // degrees [nx]
+  const double* lat;       // degrees [ny]
+  const double* xx;        // radians [nx]   (lon * DEG2RAD, eikonal_solver.cpp:41)
+  const double* yy;        // radians [ny]
+  // per-period media, [nperiod] planes each
+  const double* f_plain;   // slowness 1/svel, column-major (bilinear at source)
+  const double* aA_plain;  // adjoint metric a  (spha/(R2*cos*cos), :385)
+  const double* b_plain;   // b = sphb/R2
+  const double* c_plain;   // c = sphc/(R2*cos)
+  const double* vinv3;     // svel^-3 (preproc.cpp:29)
+  // forward media, segmented, sweep precision: FP64 a, b, c, f; FP32 a, b, c, f^2,
+  // dis_x = sqrt(f^2 b/det), dis_y = sqrt(f^2 a/det) (-1 where det <= 0)
+  const void* seg[6];
+};
+
+__device__ inline int seg_index(const GridDev& g, int ix, int iy) {
+  return iy * g.P + ix;  // lane l, column k -> l*C + k == ix
+}
+
+// ----------------------------------------------------------------------------
+// Wave-level helpers (blockDim.x == 64, one wave per work-group).
+// ----------------------------------------------------------------------------
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = imin(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline int wave_max_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = imax(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline int wave_or_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+
+// include/utils.h:28-36 bilinear at fractional index (idx0+r1, idy0+r2), column-major.
+__device__ inline double bilinear_cm(const double* M, int nx, int idx0, int idy0, double r1, double r2) {
+  return (1 - r1) * (1 - r2) * M[idx0 + idy0 * nx]
+       + (1 - r1) * r2 * M[idx0 + (idy0 + 1) * nx]
+       + r1 * (1 - r2) * M[idx0 + 1 + idy0 * nx]
+       + r1 * r2 * M[idx0 + 1 + (idy0 + 1) * nx];
+}
+
+// std::lower_bound over an ascending array.
+__device__ inline int lower_bound_d(const double* v, int n, double q) {
+  int lo = 0, cnt = n;
+  while (cnt > 0) {
+    const int step = cnt / 2;
+    const int it = lo + step;
+    if (v[it] < q) { lo = it + 1; cnt -= step + 1; } else cnt = step;
+  }
+  return lo;
+}
+
+// include/utils.h:358-409 interp2d at one query point (NaN outside the grid).
+__device__ inline double interp2d_point(const double* xg, int nx, const double* yg, int ny,
+                                        const double* z, double qx, double qy) {
+  if (qx < xg[0] || qx > xg[nx - 1] || qy < yg[0] || qy > yg[ny - 1]) return __builtin_nan("");
+  int i = lower_bound_d(xg, nx, qx);
+  if (i == nx) --i;
+  if (i > 0 && xg[i] > qx) --i;
+  i = imax(0, imin(i, nx - 2));
+  int j = lower_bound_d(yg, ny, qy);
+  if (j == ny) --j;
+  if (j > 0 && yg[j] > qy) --j;
+  j = imax(0, imin(j, ny - 2));
+  const double x1 = xg[i], x2 = xg[i + 1];
+  const double y1 = yg[j], y2 = yg[j + 1];
+  const double tx = (qx - x1) / (x2 - x1);
+  const double ty = (qy - y1) / (y2 - y1);
+  return (1 - tx) * (1 - ty) * z[i + j * nx]
+       + (tx) * (1 - ty) * z[i + 1 + j * nx]
+       + (1 - tx) * (ty) * z[i + (j + 1) * nx]
+       + (tx) * (ty) * z[i + 1 + (j + 1) * nx];
+}
+
+// ----------------------------------------------------------------------------
+// FP64 node updates, in the reference's operation order (compiled with
+// -ffp-contract=off, IEEE division and square root).
+// ----------------------------------------------------------------------------
+
+// One forward node (src/eikonal_solver.cpp:155-283).  W/E/S/N = (ix-1,iy),
+// (ix+1,iy), (ix,iy-1), (ix,iy+1); h* says the neighbour exists; t* its tau,
+// T0* its factor T0.
+__device__ inline double fwd_update_f64(double tau_self, double t0, double t0x, double t0y,
+                                        double a, double b, double c, double f,
+                                        double dx, double dy,
+                                        bool hW, double tW, double T0W, bool hE, double tE, double T0E,
+                                        bool hS, double tS, double T0S, bool hN, double tN, double T0N) {
+  const double det = a * b - c * c;
+  double tn = tau_self;
+  // :173-219 two-point quadrants (sx, sy) = (-1,-1), (-1,+1), (+1,-1), (+1,+1)
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int sx = (k < 2) ? -1 : 1;
+    const int sy = (k & 1) ? 1 : -1;
+    const bool hx = (sx < 0) ? hW : hE;
+    const bool hy = (sy < 0) ? hS : hN;
+    if (!(hx && hy)) continue;
+    const double tX = (sx < 0) ? tW : tE;
+    const double tY = (sy < 0) ? tS : tN;
+    const double px = t0x - sx * (t0 / dx);
+    const double qx = sx * (t0 / dx) * tX;
+    const double py = t0y - sy * (t0 / dy);
+    const double qy = sy * (t0 / dy) * tY;
+    const double eq_a = a * px * px + b * py * py - 2.0 * c * px * py;
+    const double eq_b = 2.0 * (a * px * qx + b * py * qy - c * (px * qy + py * qx));
+    const double eq_c = a * qx * qx + b * qy * qy - 2.0 * c * qx * qy - f * f;
+    const double Delta = eq_b * eq_b - 4.0 * eq_a * eq_c;
+    if (Delta < 0.0 || fabs(eq_a) < 1e-30) continue;
+    const double sq = sqrt(Delta);
+    const double two_a = 2.0 * eq_a;
+#pragma unroll
+    for (int sgn = 0; sgn < 2; sgn++) {
+      const double tmp_tau = (-eq_b + (sgn == 0 ? sq : -sq)) / two_a;
+      const double Tx = px * tmp_tau + qx;
+      const double Ty = py * tmp_tau + qy;
+      const double char_x = a * Tx - c * Ty;
+      const double char_y = b * Ty - c * Tx;
+      if (-sx * char_x >= 0.0 && -sy * char_y >= 0.0) tn = dmin(tn, tmp_tau);
+    }
+  }
+  // :224-279 one-point stencils
+  if (hW) {
+    const double px = t0x + t0 / dx;
+    const double qx = -(t0 / dx) * tW;
+    if (det > 0.0) {
+      const double dis = sqrt(f * f * b / det);
+#pragma unroll
+      for (int sgn = 0; sgn < 2; sgn++) {
+        const double tmp_tau = ((sgn == 0 ? dis : -dis) - qx) / px;
+        if (tmp_tau * t0 >= tW * T0W && tmp_tau > tW * 0.5) tn = dmin(tn, tmp_tau);
+      }
+    }
+  }
+  if (hE) {
+    const double px = t0x - t0 / dx;
+    const double qx = (t0 / dx) * tE;
+    if (det > 0.0) {
+      const double dis = sqrt(f * f * b / det);
+#pragma unroll
+      for (int sgn = 0; sgn < 2; sgn++) {
+        const double tmp_tau = ((sgn == 0 ? dis : -dis) - qx) / px;
+        if (tmp_tau * t0 >= tE * T0E && tmp_tau > tE * 0.5) tn = dmin(tn, tmp_tau);
+      }
+    }
+  }
+  if (hS) {
+    const double py = t0y + t0 / dy;
+    const double qy = -(t0 / dy) * tS;
+    if (det > 0.0) {
+      const double dis = sqrt(f * f * a / det);
+#pragma unroll
+      for (int sgn = 0; sgn < 2; sgn++) {
+        const double tmp_tau = ((sgn == 0 ? dis : -dis) - qy) / py;
+        if (tmp_tau * t0 >= tS * T0S && tmp_tau > tS * 0.5) tn = dmin(tn, tmp_tau);
+      }
+    }
+  }
+  if (hN) {
+    const double py = t0y - t0 / dy;
+    const double qy = (t0 / dy) * tN;
+    if (det > 0.0) {
+      const double dis = sqrt(f * f * a / det);
+#pragma unroll
+      for (int sgn = 0; sgn < 2; sgn++) {
+        const double tmp_tau = ((sgn == 0 ? dis : -dis) - qy) / py;
+        if (tmp_tau * t0 >= tN * T0N && tmp_tau > tN * 0.5) tn = dmin(tn, tmp_tau);
+      }
+    }
+  }
+  return tn;
+}
+
+// One adjoint node (src/eikonal_solver.cpp:479-490).
+__device__ inline double adj_update_f64(double a1p, double a2m, double b1p, double b2m, double dlt, double d,
+                                        double tW, double tE, double tS, double tN, double dx, double dy) {
+  if (fabs(d) < kAdjEps) return 0.0;
+  const double e = (tW * a1p - tE * a2m) / dx + (tS * b1p - tN * b2m) / dy;
+  return (dlt + e) / d;
+}
+
+}  // namespace sat

@@ -1,0 +1,169 @@
+// sat_engine.hpp -- the lane-skewed Gauss-Seidel sweep shared by every solve.
+//
+// One 64-lane wave sweeps one grid.  Lane l owns columns [l*C, l*C+C) of
+// every row (segmented planes, pitch P = La*C).  In a sweep with directions
+// (XD, YD) lane l handles sweep-row r at step t = r + lag(l), lag = l for
+// XD > 0 and La-1-l for XD < 0, and visits its C columns in XD order.
+//
+// A node reads its 4 axis neighbours.  The neighbours that precede it in the
+// reference's x-outer / y-inner order (src/eikonal_solver.cpp:149-154,
+// :474-478) are exactly those already updated here: the previous column in
+// the lane (this step), the previous row (this lane, last step) and, for the
+// first column, the adjacent lane's last column (one step earlier; it arrives
+// by a cross-lane shuffle of that lane's previous row).  The following
+// neighbours still hold the previous sweep's values: the next column, the
+// next row (prefetched) and, for the last column, the other adjacent lane's
+// first column (shuffled from that lane's prefetched row).  Each sweep reads
+// `tin` and writes `tout`, so every update sees the same values as the
+// sequential loop and the result is identical iterate for iterate.
+//
+// Rows flow through a 3-deep register pipeline (cur, nxt, nn): the row two
+// steps ahead is loaded while the current one is computed.
+#pragma once
+#include "sat_device.hpp"
+
+namespace sat {
+
+// Policy P provides:
+//   using V;                                  swept value type (float / double)
+//   using A;                                  neighbour auxiliary value type (T0 of the forward)
+//   template <int C> struct Row               per-row register data; has V u[C], A aux[C], V st[C]
+//   template <int C> static void load(const Ctx&, Row<C>&, size_t base, const V* tin, const V* tst)
+//   static bool fixed(const Ctx&, int ix, int j)     node keeps its value
+//   template <int C> static V update(const Ctx&, const Row<C>&, int k, int ix, int j,
+//                                    V uW, A aW, bool hW, V uE, A aE, bool hE,
+//                                    V uS, A aS, bool hS, V uN, A aN, bool hN)
+//   template <int C> static double norm(const Row<C>&, int k, V newv)   convergence term
+template <class Pol, int C, int XD, int YD, bool NORM>
+__device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typename Pol::V* __restrict__ tin,
+                                      typename Pol::V* __restrict__ tout,
+                                      const typename Pol::V* __restrict__ tst, double& lmax, double& lsum) {
+  using V = typename Pol::V;
+  using A = typename Pol::A;
+  using Row = typename Pol::template Row<C>;
+  const int lane = threadIdx.x;
+  const int La = cx.g->La, nx = cx.g->nx, ny = cx.g->ny, P = cx.g->P;
+  const bool lane_ok = lane < La;
+  const int lag = (XD > 0) ? lane : (La - 1 - lane);
+  const int nsteps = ny + La - 1;
+
+  auto load = [&](Row& R, int r) {
+    if (lane_ok && r >= 0 && r < ny) {
+      const int j = (YD > 0) ? r : (ny - 1 - r);
+      Pol::template load<C>(cx, R, (size_t)j * P + (size_t)lane * C, tin, NORM ? tst : nullptr);
+    }
+  };
+
+  Row cur, nxt, nn;
+  V prev_u[C];
+  A prev_a[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) { prev_u[k] = V(0); prev_a[k] = A(0); }
+  load(cur, -lag);
+  load(nxt, 1 - lag);
+
+  for (int t = 0; t < nsteps; ++t) {
+    const int r = t - lag;
+    load(nn, r + 2);
+    const V nw_u = __shfl((XD > 0) ? prev_u[C - 1] : prev_u[0], lane - XD);
+    const A nw_a = Pol::kAux ? __shfl((XD > 0) ? prev_a[C - 1] : prev_a[0], lane - XD) : A(0);
+    const V od_u = __shfl((XD > 0) ? nxt.u[0] : nxt.u[C - 1], lane + XD);
+    const A od_a = Pol::kAux ? __shfl((XD > 0) ? nxt.aux[0] : nxt.aux[C - 1], lane + XD) : A(0);
+    if (lane_ok && r >= 0 && r < ny) {
+      const int j = (YD > 0) ? r : (ny - 1 - r);
+      const bool hS = j > 0, hN = j < ny - 1;
+      V newv[C];
+#pragma unroll
+      for (int kk = 0; kk < C; kk++) {
+        const int k = (XD > 0) ? kk : (C - 1 - kk);
+        const int ix = lane * C + k;
+        V uW, uE;
+        A aW, aE;
+        if (XD > 0) {
+          if (k > 0) { uW = newv[k - 1]; aW = cur.aux[k - 1]; } else { uW = nw_u; aW = nw_a; }
+          if (k < C - 1) { uE = cur.u[k + 1]; aE = cur.aux[k + 1]; } else { uE = od_u; aE = od_a; }
+        } else {
+          if (k < C - 1) { uE = newv[k + 1]; aE = cur.aux[k + 1]; } else { uE = nw_u; aE = nw_a; }
+          if (k > 0) { uW = cur.u[k - 1]; aW = cur.aux[k - 1]; } else { uW = od_u; aW = od_a; }
+        }
+        V uS, uN;
+        A aS, aN;
+        if (YD > 0) { uS = prev_u[k]; aS = prev_a[k]; uN = nxt.u[k]; aN = nxt.aux[k]; }
+        else { uN = prev_u[k]; aN = prev_a[k]; uS = nxt.u[k]; aS = nxt.aux[k]; }
+        const bool valid = ix < nx;
+        if (!valid || Pol::fixed(cx, ix, j)) {
+          newv[k] = cur.u[k];
+        } else {
+          newv[k] = Pol::template update<C>(cx, cur, k, ix, j, uW, aW, ix > 0, uE, aE, ix < nx - 1,
+                                            uS, aS, hS, uN, aN, hN);
+        }
+        if (NORM && valid) {
+          const double d = Pol::template norm<C>(cur, k, newv[k]);
+          lsum += d;
+          lmax = dmax(lmax, d);
+        }
+      }
+      const size_t base = (size_t)j * P + (size_t)lane * C;
+#pragma unroll
+      for (int k = 0; k < C; k++) tout[base + k] = newv[k];
+#pragma unroll
+      for (int k = 0; k < C; k++) { prev_u[k] = newv[k]; prev_a[k] = cur.aux[k]; }
+    }
+    cur = nxt;
+    nxt = nn;
+  }
+}
+
+struct IterOut {
+  int iters;
+  bool converged;
+  double l1, linf;
+  double linf_prev;  // Linf of the round before the last (-1 if none)
+};
+
+// Iterate 4-sweep rounds until L1 < tol and Linf < tol (src/eikonal_solver.cpp:146-314,
+// :471-509).  Three rotating planes: b[0] holds the iterate at the start of a round.
+// Returns the plane holding the result.
+template <class Pol, int C>
+__device__ typename Pol::V* iterate(const typename Pol::Ctx& cx, typename Pol::V* b0, typename Pol::V* b1,
+                                    typename Pol::V* b2, int maxiter, double tol, IterOut& io) {
+  using V = typename Pol::V;
+  V* B0 = b0;
+  V* B1 = b1;
+  V* B2 = b2;
+  V* res = B0;
+  io.iters = maxiter;
+  io.converged = false;
+  io.linf = -1.0;
+  io.linf_prev = -1.0;
+  const double npts = (double)(cx.g->nx * cx.g->ny);
+  for (int it = 0; it < maxiter; ++it) {
+    double lmax = 0.0, lsum = 0.0;
+    sweep<Pol, C, 1, 1, false>(cx, B0, B1, nullptr, lmax, lsum);
+    __syncthreads();
+    sweep<Pol, C, 1, -1, false>(cx, B1, B2, nullptr, lmax, lsum);
+    __syncthreads();
+    sweep<Pol, C, -1, 1, false>(cx, B2, B1, nullptr, lmax, lsum);
+    __syncthreads();
+    sweep<Pol, C, -1, -1, true>(cx, B1, B2, B0, lmax, lsum);
+    __syncthreads();
+    const double Linf = wave_max(lmax);
+    const double L1 = wave_sum(lsum) / npts;
+    io.l1 = L1;
+    io.linf_prev = io.linf;
+    io.linf = Linf;
+    res = B2;
+    if (L1 < tol && Linf < tol) {
+      io.iters = it + 1;
+      io.converged = true;
+      return res;
+    }
+    V* n0 = B2;
+    V* n1 = B0;
+    V* n2 = B1;
+    B0 = n0; B1 = n1; B2 = n2;
+  }
+  return res;
+}
+
+}  // namespace sat

@@ -1,0 +1,560 @@
+// sat_policies.hpp -- node updates and per-solve set-up for the two precisions.
+//
+//   FwdF64 / AdjF64   the reference's FP64 arithmetic in its operation order
+//                     (src/eikonal_solver.cpp); kernels within 1e-12 of the oracle.
+//   FwdF32 / AdjF32   FP32 sweeps.  The forward node update runs in FP32 on the
+//                     factored perturbation sigma = tau - 1: each quadratic is solved
+//                     for the increment w about the smaller upwind neighbour, which
+//                     removes the O((dist/dx)^2) cancellation of the reference's
+//                     coefficients (a*px^2*tau^2 ~ 1e3 against f^2 ~ 0.1 at 1 km
+//                     cells) that FP32 cannot carry.  sigma itself is kept in FP64
+//                     (sigma_new = sigma_ref + w), so the per-update rounding is that
+//                     of w (~1e-3) rather than of sigma (~0.1): FP32 storage let it
+//                     accumulate along rays to ~1e-5 in the kernels.  The analytic
+//                     factor T0, the travel time T = T0 + sigma*T0, the adjoint
+//                     coefficients, the convergence norm and the kernel
+//                     accumulation are FP64.
+#pragma once
+#include "sat_crmath.hpp"
+#include "sat_device.hpp"
+#include "sat_engine.hpp"
+
+namespace sat {
+
+struct FixedBox {  // near-source nodes held at tau = 1 (src/eikonal_solver.cpp:119-129)
+  int x0, x1, y0, y1;
+};
+
+// Per-solve workspace: segmented planes in sweep precision, plain FP64 planes.
+struct Slot {
+  void* seg[9];
+  double* pl[4];
+};
+
+// ============================================================================
+// FP64, reference operation order
+// ============================================================================
+struct FwdF64 {
+  using V = double;
+  using A = double;
+  static constexpr bool kAux = true;
+  struct Ctx {
+    const GridDev* g;
+    const double *t0, *t0x, *t0y, *a, *b, *c, *f;
+    FixedBox fb;
+    double dx, dy;
+  };
+  template <int C>
+  struct Row {
+    double u[C], aux[C], t0x[C], t0y[C], a[C], b[C], c[C], f[C], st[C];
+  };
+  template <int C>
+  __device__ static void load(const Ctx& cx, Row<C>& R, size_t base, const double* tin, const double* tst) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      R.u[k] = tin[base + k];
+      R.aux[k] = cx.t0[base + k];
+      R.t0x[k] = cx.t0x[base + k];
+      R.t0y[k] = cx.t0y[base + k];
+      R.a[k] = cx.a[base + k];
+      R.b[k] = cx.b[base + k];
+      R.c[k] = cx.c[base + k];
+      R.f[k] = cx.f[base + k];
+      R.st[k] = tst ? tst[base + k] : 0.0;
+    }
+  }
+  __device__ static bool fixed(const Ctx& cx, int ix, int j) {
+    return j >= cx.fb.y0 && j <= cx.fb.y1 && ix >= cx.fb.x0 && ix <= cx.fb.x1;
+  }
+  template <int C>
+  __device__ static double update(const Ctx& cx, const Row<C>& r, int k, int, int, double uW, double aW, bool hW,
+                                  double uE, double aE, bool hE, double uS, double aS, bool hS, double uN,
+                                  double aN, bool hN) {
+    return fwd_update_f64(r.u[k], r.aux[k], r.t0x[k], r.t0y[k], r.a[k], r.b[k], r.c[k], r.f[k], cx.dx, cx.dy, hW,
+                          uW, aW, hE, uE, aE, hS, uS, aS, hN, uN, aN);
+  }
+  template <int C>
+  __device__ static double norm(const Row<C>& r, int k, double v) {
+    return fabs(v - r.st[k]) * r.aux[k];  // :295
+  }
+};
+
+struct AdjF64 {
+  using V = double;
+  using A = double;
+  static constexpr bool kAux = false;
+  struct Ctx {
+    const GridDev* g;
+    const double *a1p, *a2m, *b1p, *b2m, *dl, *d;
+    double dx, dy;
+  };
+  template <int C>
+  struct Row {
+    double u[C], aux[C], a1p[C], a2m[C], b1p[C], b2m[C], dl[C], d[C], st[C];
+  };
+  template <int C>
+  __device__ static void load(const Ctx& cx, Row<C>& R, size_t base, const double* tin, const double* tst) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      R.u[k] = tin[base + k];
+      R.aux[k] = 0.0;
+      R.a1p[k] = cx.a1p[base + k];
+      R.a2m[k] = cx.a2m[base + k];
+      R.b1p[k] = cx.b1p[base + k];
+      R.b2m[k] = cx.b2m[base + k];
+      R.dl[k] = cx.dl[base + k];
+      R.d[k] = cx.d[base + k];
+      R.st[k] = tst ? tst[base + k] : 0.0;
+    }
+  }
+  __device__ static bool fixed(const Ctx& cx, int ix, int j) {
+    return ix == 0 || ix >= cx.g->nx - 1 || j == 0 || j == cx.g->ny - 1;
+  }
+  template <int C>
+  __device__ static double update(const Ctx& cx, const Row<C>& r, int k, int, int, double uW, double, bool,
+                                  double uE, double, bool, double uS, double, bool, double uN, double, bool) {
+    return adj_update_f64(r.a1p[k], r.a2m[k], r.b1p[k], r.b2m[k], r.dl[k], r.d[k], uW, uE, uS, uN, cx.dx, cx.dy);
+  }
+  template <int C>
+  __device__ static double norm(const Row<C>& r, int k, double v) {
+    return fabs(v - r.st[k]);  // :501
+  }
+};
+
+// ============================================================================
+// FP32 sweeps on sigma = tau - 1
+// ============================================================================
+
+// One forward node in increment form.  Same candidate set and acceptance tests
+// as src/eikonal_solver.cpp:173-283, written for sigma:
+//   two-point (sx, sy): with sr = min(sX, sY), tau = 1 + sr + w,
+//     Tx = Tx0 + px*w,  Tx0 = t0x*(1+sr) + sx*h*(sX - sr),  px = t0x - sx*h
+//     (y alike), a*Tx^2 + b*Ty^2 - 2c*Tx*Ty = f^2 solved for w with the stable
+//     quadratic formula; accepted if -sx*(a*Tx - c*Ty) >= 0 and -sy*(b*Ty - c*Tx) >= 0.
+//   one-point (W): tau = tau_W + w, w = (+-dis - t0x*tau_W)/(t0x + h), accepted if
+//     tau*T0 >= tau_W*T0_W and tau > tau_W/2 (E, S, N alike).
+// Neighbour sigmas and the result are FP64; w is computed in FP32.
+__device__ __forceinline__ double fwd_update_f32(double u_self, float t0, float t0x, float t0y, float a, float b,
+                                                 float c, float f2, float dsx, float dsy, float h, float kk,
+                                                 bool hW, double uW, float T0W, bool hE, double uE, float T0E,
+                                                 bool hS, double uS, float T0S, bool hN, double uN, float T0N) {
+#pragma clang fp contract(fast)
+  double un = u_self;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const float sx = (q < 2) ? -1.0f : 1.0f;
+    const float sy = (q & 1) ? 1.0f : -1.0f;
+    const bool hx = (q < 2) ? hW : hE;
+    const bool hy = (q & 1) ? hN : hS;
+    if (!(hx && hy)) continue;
+    const double uX = (q < 2) ? uW : uE;
+    const double uY = (q & 1) ? uN : uS;
+    const double ur = (uY < uX) ? uY : uX;
+    const float px = t0x - sx * h;
+    const float py = t0y - sy * kk;
+    const float tr = (float)(ur + 1.0);
+    const float Tx0 = t0x * tr + sx * h * (float)(uX - ur);
+    const float Ty0 = t0y * tr + sy * kk * (float)(uY - ur);
+    const float Aq = a * px * px + b * py * py - 2.0f * c * px * py;
+    const float Bh = a * px * Tx0 + b * py * Ty0 - c * (px * Ty0 + py * Tx0);
+    const float Cq = a * Tx0 * Tx0 + b * Ty0 * Ty0 - 2.0f * c * Tx0 * Ty0 - f2;
+    const float D = Bh * Bh - Aq * Cq;
+    if (D < 0.0f || fabsf(Aq) < 1e-30f) continue;
+    const float sq = __builtin_amdgcn_sqrtf(D);
+    const float sv = -(Bh + ((Bh >= 0.0f) ? sq : -sq));
+    const float w0 = sv * __builtin_amdgcn_rcpf(Aq);
+    const float w1 = (sv != 0.0f) ? Cq * __builtin_amdgcn_rcpf(sv) : w0;
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+      const float w = g ? w1 : w0;
+      const float Tx = Tx0 + px * w;
+      const float Ty = Ty0 + py * w;
+      const float cxv = a * Tx - c * Ty;
+      const float cyv = b * Ty - c * Tx;
+      if (-sx * cxv >= 0.0f && -sy * cyv >= 0.0f) {
+        const double cand = ur + (double)w;
+        un = (cand < un) ? cand : un;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const bool hq = (q == 0) ? hW : (q == 1) ? hE : (q == 2) ? hS : hN;
+    const float dis = (q < 2) ? dsx : dsy;
+    if (!hq || dis < 0.0f) continue;
+    const double un_ = (q == 0) ? uW : (q == 1) ? uE : (q == 2) ? uS : uN;
+    const float T0n = (q == 0) ? T0W : (q == 1) ? T0E : (q == 2) ? T0S : T0N;
+    const float td = (q < 2) ? t0x : t0y;
+    const float hh = (q < 2) ? h : kk;
+    const float p = (q == 0 || q == 2) ? td + hh : td - hh;
+    const double tnd = un_ + 1.0;
+    const float Td0 = td * (float)tnd;
+    const float rp = __builtin_amdgcn_rcpf(p);
+    const double Tn = tnd * (double)T0n;
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+      const float w = ((g == 0) ? dis - Td0 : -dis - Td0) * rp;
+      const double cu = un_ + (double)w;
+      const double tc = cu + 1.0;
+      if (tc * (double)t0 >= Tn && tc > tnd * 0.5) un = (cu < un) ? cu : un;
+    }
+  }
+  return un;
+}
+
+struct FwdF32 {
+  using V = double;  // sigma storage
+  using A = float;   // T0 of a neighbour
+  static constexpr bool kAux = true;
+  struct Ctx {
+    const GridDev* g;
+    const float *t0, *t0x, *t0y, *a, *b, *c, *f2, *dsx, *dsy;
+    FixedBox fb;
+    float idx, idy;  // 1/dx, 1/dy (radians)
+  };
+  template <int C>
+  struct Row {
+    double u[C], st[C];
+    float aux[C], t0x[C], t0y[C], a[C], b[C], c[C], f2[C], dsx[C], dsy[C];
+  };
+  template <int C>
+  __device__ static void load(const Ctx& cx, Row<C>& R, size_t base, const double* tin, const double* tst) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      R.u[k] = tin[base + k];
+      R.aux[k] = cx.t0[base + k];
+      R.t0x[k] = cx.t0x[base + k];
+      R.t0y[k] = cx.t0y[base + k];
+      R.a[k] = cx.a[base + k];
+      R.b[k] = cx.b[base + k];
+      R.c[k] = cx.c[base + k];
+      R.f2[k] = cx.f2[base + k];
+      R.dsx[k] = cx.dsx[base + k];
+      R.dsy[k] = cx.dsy[base + k];
+      R.st[k] = tst ? tst[base + k] : 0.0;
+    }
+  }
+  __device__ static bool fixed(const Ctx& cx, int ix, int j) {
+    return j >= cx.fb.y0 && j <= cx.fb.y1 && ix >= cx.fb.x0 && ix <= cx.fb.x1;
+  }
+  template <int C>
+  __device__ static double update(const Ctx& cx, const Row<C>& r, int k, int, int, double uW, float aW, bool hW,
+                                  double uE, float aE, bool hE, double uS, float aS, bool hS, double uN, float aN,
+                                  bool hN) {
+    const float t0 = r.aux[k];
+    return fwd_update_f32(r.u[k], t0, r.t0x[k], r.t0y[k], r.a[k], r.b[k], r.c[k], r.f2[k], r.dsx[k], r.dsy[k],
+                          t0 * cx.idx, t0 * cx.idy, hW, uW, aW, hE, uE, aE, hS, uS, aS, hN, uN, aN);
+  }
+  template <int C>
+  __device__ static double norm(const Row<C>& r, int k, double v) {
+    return fabs(v - r.st[k]) * (double)r.aux[k];
+  }
+};
+
+struct AdjF32 {
+  using V = float;
+  using A = float;
+  static constexpr bool kAux = false;
+  struct Ctx {
+    const GridDev* g;
+    const float *cW, *cE, *cS, *cN, *cD;
+  };
+  template <int C>
+  struct Row {
+    float u[C], aux[C], cW[C], cE[C], cS[C], cN[C], cD[C], st[C];
+  };
+  template <int C>
+  __device__ static void load(const Ctx& cx, Row<C>& R, size_t base, const float* tin, const float* tst) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      R.u[k] = tin[base + k];
+      R.aux[k] = 0.0f;
+      R.cW[k] = cx.cW[base + k];
+      R.cE[k] = cx.cE[base + k];
+      R.cS[k] = cx.cS[base + k];
+      R.cN[k] = cx.cN[base + k];
+      R.cD[k] = cx.cD[base + k];
+      R.st[k] = tst ? tst[base + k] : 0.0f;
+    }
+  }
+  __device__ static bool fixed(const Ctx& cx, int ix, int j) {
+    return ix == 0 || ix >= cx.g->nx - 1 || j == 0 || j == cx.g->ny - 1;
+  }
+  template <int C>
+  __device__ static float update(const Ctx&, const Row<C>& r, int k, int, int, float uW, float, bool, float uE,
+                                 float, bool, float uS, float, bool, float uN, float, bool) {
+#pragma clang fp contract(fast)
+    return r.cD[k] + r.cW[k] * uW + r.cE[k] * uE + r.cS[k] * uS + r.cN[k] * uN;
+  }
+  template <int C>
+  __device__ static double norm(const Row<C>& r, int k, float v) {
+    return fabs((double)v - (double)r.st[k]);
+  }
+};
+
+// ============================================================================
+// Per-solve set-up and finish
+// ============================================================================
+
+// src/eikonal_solver.cpp:67-131.  Analytic factor in FP64 for every node of the
+// grid; stores it in the sweep precision (t0, t0x, t0y segmented), the FP64 T0
+// (plain, FP32 path only) and the initial sweep value (tau or sigma).
+template <typename V, typename U>
+__device__ inline FixedBox fwd_init(const GridDev& g, int iper, double src_lon, double src_lat, V* t0s, V* t0xs,
+                                    V* t0ys, U* u0, double* T0d, int& flags) {
+  const int lane = threadIdx.x;
+  const int nx = g.nx, ny = g.ny, P = g.P;
+  const double x0 = src_lon * kDEG2RAD;
+  const double y0 = src_lat * kDEG2RAD;
+  const double dx = g.dx, dy = g.dy;
+  int idx0 = (int)floor((x0 - g.xx[0]) / dx);
+  int idy0 = (int)floor((y0 - g.yy[0]) / dy);
+  idx0 = imax(0, imin(idx0, nx - 2));
+  idy0 = imax(0, imin(idy0, ny - 2));
+  const double r1 = dmin(1.0, (x0 - g.xx[idx0]) / dx);
+  const double r2 = dmin(1.0, (y0 - g.yy[idy0]) / dy);
+  const double fun0 = bilinear_cm(g.f_plain + (size_t)iper * g.plane, nx, idx0, idy0, r1, r2);
+
+  // The fixed set {|(xx-x0)/dx| <= 1} x {|(yy-y0)/dy| <= 1} is an index box
+  // (the test is monotone in the coordinate).
+  int bx0 = 1 << 30, bx1 = -1, by0 = 1 << 30, by1 = -1;
+  for (int ix = lane; ix < nx; ix += kWave)
+    if (fabs((g.xx[ix] - x0) / dx) <= 1.0) { bx0 = imin(bx0, ix); bx1 = imax(bx1, ix); }
+  for (int iy = lane; iy < ny; iy += kWave)
+    if (fabs((g.yy[iy] - y0) / dy) <= 1.0) { by0 = imin(by0, iy); by1 = imax(by1, iy); }
+  FixedBox fb;
+  fb.x0 = wave_min_i(bx0);
+  fb.x1 = wave_max_i(bx1);
+  fb.y0 = wave_min_i(by0);
+  fb.y1 = wave_max_i(by1);
+  if (fb.x0 <= fb.x1 && fb.y0 <= fb.y1 && (fb.x0 == 0 || fb.x1 == nx - 1 || fb.y0 == 0 || fb.y1 == ny - 1))
+    flags |= FLAG_SRC_ON_BOUNDARY;
+
+  constexpr bool kSigma = sizeof(V) == 4;
+  // Correctly rounded sin/cos/acos (sat_crmath.hpp): the same bits as any
+  // correctly rounded CPU evaluation.  sin/cos of the longitude offsets are
+  // evaluated once per column (a lane owns at most kMaxCols columns).
+  constexpr int kMaxCols = 8;
+  double sdl[kMaxCols], cdl[kMaxCols];
+#pragma unroll
+  for (int m = 0; m < kMaxCols; m++) {
+    const int q = lane + kWave * m;
+    sdl[m] = 0.0;
+    cdl[m] = 1.0;
+    if (q < nx) cr::sincos_cr(g.xx[q] - x0, sdl[m], cdl[m]);
+  }
+  double sy0, cy0;
+  cr::sincos_cr(y0, sy0, cy0);
+  for (int iy = 0; iy < ny; iy++) {
+    double syy, cyy;
+    cr::sincos_cr(g.yy[iy], syy, cyy);
+#pragma unroll
+    for (int m = 0; m < kMaxCols; m++) {
+      const int q = lane + kWave * m;
+      if (q >= P) continue;
+      const size_t o = (size_t)iy * P + q;
+      if (q >= nx) {
+        t0s[o] = V(0); t0xs[o] = V(0); t0ys[o] = V(0); u0[o] = kSigma ? U(9) : U(10);
+        continue;
+      }
+      const double tmp = syy * sy0 + cyy * cy0 * cdl[m];
+      const double tmp_c = dmax(-1.0, dmin(1.0, tmp));
+      const double T0 = cr::acos_cr(tmp_c) * kR * fun0;
+      const double sin_d = sqrt(dmax(0.0, 1.0 - tmp_c * tmp_c));
+      double T0x, T0y;
+      if (sin_d < 1e-14) {
+        T0x = 0.0;
+        T0y = 0.0;
+      } else {
+        T0x = cyy * cy0 * sdl[m] / sin_d * kR * fun0;
+        T0y = -(cyy * sy0 - syy * cy0 * cdl[m]) / sin_d * kR * fun0;
+      }
+      t0s[o] = (V)T0;
+      t0xs[o] = (V)T0x;
+      t0ys[o] = (V)T0y;
+      if (kSigma) T0d[q + (size_t)iy * nx] = T0;
+      const bool fixed = (q >= fb.x0 && q <= fb.x1 && iy >= fb.y0 && iy <= fb.y1);
+      u0[o] = kSigma ? (fixed ? U(0) : U(9)) : (fixed ? U(1) : U(10));
+    }
+  }
+  return fb;
+}
+
+// :317 T = tau .* T0 (FP64) or T = T0 + sigma*T0 (FP32 sweeps), column-major.
+template <typename V, typename U>
+__device__ inline void fwd_finish(const GridDev& g, const U* u, const V* t0s, const double* T0d, double* T) {
+  const int nx = g.nx, ny = g.ny;
+  for (int iy = 0; iy < ny; iy++)
+    for (int ix = threadIdx.x; ix < nx; ix += kWave) {
+      const int o = seg_index(g, ix, iy);
+      const size_t p = ix + (size_t)iy * nx;
+      if (sizeof(V) == 8) T[p] = (double)u[o] * (double)t0s[o];
+      else T[p] = T0d[p] + (double)u[o] * T0d[p];
+    }
+}
+
+// src/eikonal_solver.cpp:407-441 face fluxes at interior node (ix, iy).
+struct Faces {
+  double a1m, a1p, a2m, a2p, b1m, b1p, b2m, b2p;
+};
+__device__ inline Faces faces_at(const GridDev& g, const double* __restrict__ T, const double* __restrict__ a,
+                                 const double* __restrict__ b, const double* __restrict__ c, int ix, int iy) {
+  const int nx = g.nx;
+  const double dx = g.dx, dy = g.dy;
+#define TT(i, j) T[(i) + (size_t)(j) * nx]
+#define AA(i, j) a[(i) + (size_t)(j) * nx]
+#define BB(i, j) b[(i) + (size_t)(j) * nx]
+#define CC(i, j) c[(i) + (size_t)(j) * nx]
+  Faces F;
+  const double a1 = -(TT(ix, iy) - TT(ix - 1, iy)) / dx * (AA(ix, iy) + AA(ix - 1, iy)) / 2.0
+      + (TT(ix, iy + 1) - TT(ix, iy - 1) + TT(ix - 1, iy + 1) - TT(ix - 1, iy - 1)) / (4.0 * dy)
+        * (CC(ix, iy) + CC(ix - 1, iy)) / 2.0;
+  F.a1m = (a1 - fabs(a1)) / 2.0;
+  F.a1p = (a1 + fabs(a1)) / 2.0;
+  const double a2 = -(TT(ix + 1, iy) - TT(ix, iy)) / dx * (AA(ix + 1, iy) + AA(ix, iy)) / 2.0
+      + (TT(ix + 1, iy + 1) - TT(ix + 1, iy - 1) + TT(ix, iy + 1) - TT(ix, iy - 1)) / (4.0 * dy)
+        * (CC(ix + 1, iy) + CC(ix, iy)) / 2.0;
+  F.a2m = (a2 - fabs(a2)) / 2.0;
+  F.a2p = (a2 + fabs(a2)) / 2.0;
+  const double b1 = -(TT(ix, iy) - TT(ix, iy - 1)) / dy * (BB(ix, iy) + BB(ix, iy - 1)) / 2.0
+      + (TT(ix + 1, iy) - TT(ix - 1, iy) + TT(ix + 1, iy - 1) - TT(ix - 1, iy - 1)) / (4.0 * dx)
+        * (CC(ix, iy) + CC(ix, iy - 1)) / 2.0;
+  F.b1m = (b1 - fabs(b1)) / 2.0;
+  F.b1p = (b1 + fabs(b1)) / 2.0;
+  const double b2 = -(TT(ix, iy + 1) - TT(ix, iy)) / dy * (BB(ix, iy + 1) + BB(ix, iy)) / 2.0
+      + (TT(ix + 1, iy + 1) - TT(ix - 1, iy + 1) + TT(ix + 1, iy) - TT(ix - 1, iy)) / (4.0 * dx)
+        * (CC(ix, iy + 1) + CC(ix, iy)) / 2.0;
+  F.b2m = (b2 - fabs(b2)) / 2.0;
+  F.b2p = (b2 + fabs(b2)) / 2.0;
+#undef TT
+#undef AA
+#undef BB
+#undef CC
+  return F;
+}
+
+// FP64 adjoint set-up: a1p, a2m, b1p, b2m, dl = 0, d, ta0 (segmented, seg[0..6]).
+__device__ inline void adj_setup_f64(const GridDev& g, int iper, const double* __restrict__ T, const Slot& ws) {
+  const int nx = g.nx, ny = g.ny, P = g.P;
+  const double* a = g.aA_plain + (size_t)iper * g.plane;
+  const double* b = g.b_plain + (size_t)iper * g.plane;
+  const double* c = g.c_plain + (size_t)iper * g.plane;
+  double* S[7];
+#pragma unroll
+  for (int i = 0; i < 7; i++) S[i] = static_cast<double*>(ws.seg[i]);
+  for (int iy = 0; iy < ny; iy++)
+    for (int q = threadIdx.x; q < P; q += kWave) {
+      const size_t o = (size_t)iy * P + q;
+      double v[7] = {0, 0, 0, 0, 0, 0, 0};
+      if (q >= 1 && q < nx - 1 && iy >= 1 && iy < ny - 1) {
+        const Faces F = faces_at(g, T, a, b, c, q, iy);
+        v[0] = F.a1p; v[1] = F.a2m; v[2] = F.b1p; v[3] = F.b2m;
+        v[5] = (F.a2p - F.a1m) / g.dx + (F.b2p - F.b1m) / g.dy;  // :479-481
+        v[6] = 100.0;                                            // :446-449
+      }
+#pragma unroll
+      for (int i = 0; i < 7; i++) S[i][o] = v[i];
+    }
+}
+
+// FP32 adjoint set-up: cW, cE, cS, cN (segmented seg[0..3], float), cD = 0 (seg[4]),
+// ta0 (seg[5]); the FP64 diagonal d into pl[2].  Ta = cD + cW*Ta_W + cE*Ta_E +
+// cS*Ta_S + cN*Ta_N is src/eikonal_solver.cpp:483-490 divided through by d.
+__device__ inline void adj_setup_f32(const GridDev& g, int iper, const double* __restrict__ T, const Slot& ws) {
+  const int nx = g.nx, ny = g.ny, P = g.P;
+  const double* a = g.aA_plain + (size_t)iper * g.plane;
+  const double* b = g.b_plain + (size_t)iper * g.plane;
+  const double* c = g.c_plain + (size_t)iper * g.plane;
+  float* S[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) S[i] = static_cast<float*>(ws.seg[i]);
+  double* dpl = ws.pl[2];
+  for (int iy = 0; iy < ny; iy++)
+    for (int q = threadIdx.x; q < P; q += kWave) {
+      const size_t o = (size_t)iy * P + q;
+      float v[6] = {0, 0, 0, 0, 0, 0};
+      if (q >= 1 && q < nx - 1 && iy >= 1 && iy < ny - 1) {
+        const Faces F = faces_at(g, T, a, b, c, q, iy);
+        const double d = (F.a2p - F.a1m) / g.dx + (F.b2p - F.b1m) / g.dy;
+        dpl[q + (size_t)iy * nx] = d;
+        if (!(fabs(d) < kAdjEps)) {
+          const double rdx = 1.0 / (g.dx * d), rdy = 1.0 / (g.dy * d);
+          v[0] = (float)(F.a1p * rdx);
+          v[1] = (float)(-F.a2m * rdx);
+          v[2] = (float)(F.b1p * rdy);
+          v[3] = (float)(-F.b2m * rdy);
+        }
+        v[5] = 100.0f;
+      }
+#pragma unroll
+      for (int i = 0; i < 6; i++) S[i][o] = v[i];
+    }
+}
+
+// src/eikonal_solver.cpp:356-375 receivers spread onto delta, in receiver order,
+// by lane 0.  FP64: into the dense dl plane (seg[4]).  FP32: into the persistent
+// all-zero FP64 plane pl[3], then cD = dl/d at the touched nodes, then pl[3] is
+// cleared again.
+template <typename V>
+__device__ inline void adj_delta(const GridDev& g, int nr, const double* rlon, const double* rlat, const double* adj,
+                                 const Slot& ws) {
+  if (threadIdx.x != 0) return;
+  const int nx = g.nx, ny = g.ny;
+  const double dx = g.dx, dy = g.dy;
+  constexpr bool k64 = sizeof(V) == 8;
+  double* dl = k64 ? static_cast<double*>(ws.seg[4]) : ws.pl[3];
+  auto at = [&](int i, int j) -> size_t { return k64 ? (size_t)seg_index(g, i, j) : (size_t)i + (size_t)j * nx; };
+  for (int pass = 0; pass < (k64 ? 1 : 3); pass++) {
+    for (int ir = 0; ir < nr; ir++) {
+      const double xrec = rlon[ir] * kDEG2RAD;
+      const double yrec = rlat[ir] * kDEG2RAD;
+      int idx0 = (int)floor((xrec - g.xx[0]) / dx);
+      int idy0 = (int)floor((yrec - g.yy[0]) / dy);
+      idx0 = imax(0, imin(idx0, nx - 2));
+      idy0 = imax(0, imin(idy0, ny - 2));
+      if (pass == 0) {
+        const double r1 = dmin(1.0, (xrec - g.xx[idx0]) / dx);
+        const double r2 = dmin(1.0, (yrec - g.yy[idy0]) / dy);
+        const double area = dx * kR * cr::cos_cr(yrec) * dy * kR;
+        const double w = adj[ir] / area;
+        dl[at(idx0, idy0)] += w * (1 - r1) * (1 - r2);
+        dl[at(idx0, idy0 + 1)] += w * (1 - r1) * r2;
+        dl[at(idx0 + 1, idy0)] += w * r1 * (1 - r2);
+        dl[at(idx0 + 1, idy0 + 1)] += w * r1 * r2;
+      } else {
+        for (int m = 0; m < 4; m++) {
+          const int i = idx0 + (m >> 1), j = idy0 + (m & 1);
+          const size_t p = (size_t)i + (size_t)j * nx;
+          if (pass == 1) {
+            const bool interior = i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1;
+            const double d = ws.pl[2][p];
+            static_cast<float*>(ws.seg[4])[seg_index(g, i, j)] =
+                (interior && !(fabs(d) < kAdjEps)) ? (float)(dl[p] / d) : 0.0f;
+          } else {
+            dl[p] = 0.0;
+          }
+        }
+      }
+    }
+  }
+}
+
+// src/eikonal_solver.cpp:514-536 in degrees; writes the masked Ta column-major.
+template <typename V, typename O>
+__device__ inline void mask_store(const GridDev& g, double src_lon, double src_lat, const V* __restrict__ ta,
+                                  O* __restrict__ out) {
+  const int nx = g.nx, ny = g.ny;
+  const double dxd = g.lon[1] - g.lon[0];
+  const double dyd = g.lat[1] - g.lat[0];
+  const int ix_lo = imax(0, (int)floor((src_lon - 1.5 * dxd - g.lon[0]) / dxd) + 1);
+  const int ix_hi = imin(nx - 1, (int)ceil((src_lon + 1.5 * dxd - g.lon[0]) / dxd) - 1);
+  const int iy_lo = imax(0, (int)floor((src_lat - 1.5 * dyd - g.lat[0]) / dyd) + 1);
+  const int iy_hi = imin(ny - 1, (int)ceil((src_lat + 1.5 * dyd - g.lat[0]) / dyd) - 1);
+  for (int iy = 0; iy < ny; iy++)
+    for (int ix = threadIdx.x; ix < nx; ix += kWave) {
+      const bool m = ix >= ix_lo && ix <= ix_hi && iy >= iy_lo && iy <= iy_hi;
+      out[ix + (size_t)iy * nx] = m ? O(0) : (O)ta[seg_index(g, ix, iy)];
+    }
+}
+
+}  // namespace sat

@@ -1,0 +1,170 @@
+"""Batched mirror of ``preproc::forward_for_event`` (include/preproc.h:14,
+src/preproc.cpp:71-148).
+
+The reference loops serially over a rank's events (src/preproc.cpp:79) after
+an MPI round-robin deal (include/parallel.h:39-41) and reduces chi and the
+per-period kernels with MPI_Allreduce (:146, :238).  Here one ``EikonalBatch``
+holds every (source, period) pair of an inversion iteration on the GPU and one
+``run()`` executes forward -> receiver interpolation -> residual -> adjoint ->
+mask -> kernel accumulation -> chi for all of them.
+
+A batch dict (see surfattpp_amd.synth) has: lon (nx,), lat (ny,), svel/m11/
+m12/m22 (nperiod, ny, nx), ev_iper, ev_lon, ev_lat, ev_rec_off, ev_nrec,
+rec_lon, rec_lat, tt_obs, weight.  A list of such dicts (one per grid) is an
+uneven multi-grid batch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+
+
+def _c64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+class EikonalBatch:
+    """Device-resident batch of forward+adjoint pairs over one or more grids."""
+
+    def __init__(self, grids: dict | Sequence[dict], precision: int = _lib.PREC_F64, calc_adj: bool = True):
+        if isinstance(grids, dict):
+            grids = [grids]
+        self._keep = []
+        gdesc = (_lib.sat_grid_desc * len(grids))()
+        events, rl, rb, tt, w = [], [], [], [], []
+        rec_base = 0
+        self.shapes = []
+        self.ev_slices = []
+        for gi, b in enumerate(grids):
+            lon, lat = _c64(b["lon"]), _c64(b["lat"])
+            planes = [_c64(b[k]) for k in ("svel", "m11", "m12", "m22")]
+            nper = planes[0].shape[0]
+            for p in planes:
+                if p.shape != (nper, lat.size, lon.size):
+                    raise ValueError(f"plane shape {p.shape} != ({nper}, {lat.size}, {lon.size})")
+            self._keep += [lon, lat, *planes]
+            gdesc[gi] = _lib.sat_grid_desc(lon.size, lat.size, nper, lon.ctypes.data, lat.ctypes.data,
+                                           planes[0].ctypes.data, planes[1].ctypes.data,
+                                           planes[2].ctypes.data, planes[3].ctypes.data)
+            self.shapes.append((nper, lat.size, lon.size))
+            n0 = len(events)
+            for e in range(len(b["ev_iper"])):
+                events.append((gi, int(b["ev_iper"][e]), float(b["ev_lon"][e]), float(b["ev_lat"][e]),
+                               int(b["ev_rec_off"][e]) + rec_base, int(b["ev_nrec"][e])))
+            self.ev_slices.append(slice(n0, len(events)))
+            nrec_g = len(b["rec_lon"])
+            rl.append(_c64(b["rec_lon"]))
+            rb.append(_c64(b["rec_lat"]))
+            tt.append(_c64(b["tt_obs"]))
+            w.append(_c64(b["weight"]))
+            rec_base += nrec_g
+        self.nevent = len(events)
+        self.nrec = rec_base
+        edesc = (_lib.sat_event_desc * max(self.nevent, 1))()
+        for i, ev in enumerate(events):
+            edesc[i] = _lib.sat_event_desc(*ev)
+        self._rec = [np.concatenate(x) if x else np.zeros(0) for x in (rl, rb, tt, w)]
+        self.rec_counts = [len(x) for x in rl]
+        d = _lib.sat_batch_desc()
+        d.ngrid = len(grids)
+        d.grids = gdesc
+        d.nevent = self.nevent
+        d.events = edesc
+        d.nrec_total = self.nrec
+        d.rec_lon_deg, d.rec_lat_deg, d.tt_obs, d.weight = (a.ctypes.data if a.size else None for a in self._rec)
+        d.precision = int(precision)
+        d.calc_adj = 1 if calc_adj else 0
+        self.precision = precision
+        self.calc_adj = calc_adj
+        h = C.c_void_p()
+        rc = _lib.load().sat_batch_create(C.byref(d), C.byref(h))
+        _lib.check("sat_batch_create", rc)
+        self._h = h
+        self._keep = None  # inputs are on the device now
+
+    def run(self, mode: int = _lib.RUN_BATCHED, sync: bool = True):
+        _lib.check("sat_batch_run", _lib.load().sat_batch_run(self._h, int(mode)))
+        if sync:
+            self.sync()
+
+    def sync(self):
+        _lib.check("sat_batch_sync", _lib.load().sat_batch_sync(self._h))
+
+    def last_ms(self):
+        t, s = C.c_double(), C.c_double()
+        _lib.check("sat_batch_last_ms", _lib.load().sat_batch_last_ms(self._h, C.byref(t), C.byref(s)))
+        return t.value, s.value
+
+    def update_media(self, grid: int, svel, m11, m12, m22):
+        planes = [_c64(x) for x in (svel, m11, m12, m22)]
+        for p in planes:
+            if p.shape != self.shapes[grid]:
+                raise ValueError("plane shape mismatch")
+        rc = _lib.load().sat_batch_update_media(self._h, grid, *(p.ctypes.data for p in planes))
+        _lib.check("sat_batch_update_media", rc)
+
+    def update_obs(self, tt_obs, weight):
+        tt, w = _c64(tt_obs), _c64(weight)
+        if tt.size != self.nrec or w.size != self.nrec:
+            raise ValueError("receiver count mismatch")
+        rc = _lib.load().sat_batch_update_obs(self._h, tt.ctypes.data, w.ctypes.data)
+        _lib.check("sat_batch_update_obs", rc)
+
+    def results(self):
+        syn = np.zeros(max(self.nrec, 1))
+        K = [np.zeros(s) for s in self.shapes]
+        Kflat = np.zeros(sum(int(np.prod(s)) for s in self.shapes))
+        chi = np.zeros(1)
+        ne = max(self.nevent, 1)
+        itf, ita, fl = (np.zeros(ne, dtype=np.int32) for _ in range(3))
+        rc = _lib.load().sat_batch_results(self._h, syn.ctypes.data, Kflat.ctypes.data, chi.ctypes.data,
+                                           itf.ctypes.data, ita.ctypes.data, fl.ctypes.data)
+        _lib.check("sat_batch_results", rc)
+        off = 0
+        for i, s in enumerate(self.shapes):
+            n = int(np.prod(s))
+            K[i] = Kflat[off:off + n].reshape(s)
+            off += n
+        return dict(syn=syn[: self.nrec], adj_s=K if len(K) > 1 else K[0], chi=float(chi[0]),
+                    iters_fwd=itf[: self.nevent], iters_adj=ita[: self.nevent], flags=fl[: self.nevent])
+
+    def set_f32_band(self, band: float):
+        _lib.check("sat_batch_set_f32_band", _lib.load().sat_batch_set_f32_band(self._h, float(band)))
+
+    def conv(self):
+        """Per event Linf of the last two forward rounds, and the FP64 re-solve count."""
+        ne = max(self.nevent, 1)
+        last, prev = np.zeros(ne), np.zeros(ne)
+        n = C.c_int()
+        rc = _lib.load().sat_batch_conv(self._h, last.ctypes.data, prev.ctypes.data, C.byref(n))
+        _lib.check("sat_batch_conv", rc)
+        return last[: self.nevent], prev[: self.nevent], n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().sat_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def forward_for_event(batch: dict | Sequence[dict], is_calc_adj: bool = True,
+                      precision: int = _lib.PREC_F64) -> dict:
+    """One-shot: create a batch, run it, return results (syn, adj_s, chi, iters, flags)."""
+    with EikonalBatch(batch, precision=precision, calc_adj=is_calc_adj) as b:
+        b.run()
+        return b.results()

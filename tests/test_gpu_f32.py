@@ -1,0 +1,96 @@
+"""GPU parity of the FP32 path (SAT_PREC_F32) against the FP64 oracle.
+
+Bar (the metric's): ||K - K_oracle||_inf / ||K_oracle||_inf <= 1e-5 for the
+per-period kernels; travel times and syn within 1e-6 relative; chi within 1e-5.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from surfattpp_amd import PREC_F32
+from surfattpp_amd import eikonal as E
+from surfattpp_amd import preproc as PP
+from surfattpp_amd import synth
+from tests import refcases as RC
+
+pytestmark = pytest.mark.gpu
+TOL_K = 1e-5
+TOL_T = 1e-6
+
+
+@pytest.mark.parametrize("case", ["case9", "case10"])
+def test_forward_f32_close_to_oracle(case):
+    xx, yy, spha, sphb, sphc, fun, x0, y0 = getattr(RC, case + "_inputs")()
+    T, it, _ = O.forward(xx, yy, spha, sphb, sphc, fun, x0, y0)
+    Tg = E.FSM_UW_PS_lonlat_2d(xx, yy, spha, sphb, sphc, fun, x0, y0, precision=PREC_F32)
+    err = RC.rel_inf(Tg, T)
+    print(case, "T rel err", err, "iters", E.last_info["iters"], "oracle", it, flush=True)
+    assert err <= TOL_T
+
+
+@pytest.mark.parametrize("case", ["case9", "case10"])
+def test_adjoint_f32_close_to_oracle(case):
+    xx, yy, spha, sphb, sphc, fun, x0, y0 = getattr(RC, case + "_inputs")()
+    T, _, _ = O.forward(xx, yy, spha, sphb, sphc, fun, x0, y0)
+    xr, yr, sa = RC.receivers_for(xx, yy, 30, 5)
+    Ta, _ = O.adjoint(xx, yy, spha, sphb, sphc, T, xr, yr, sa)
+    Tag = E.FSM_O1_JSE_lonlat_2d(xx, yy, spha, sphb, sphc, T, xr, yr, sa, precision=PREC_F32)
+    err = RC.rel_inf(Tag, Ta)
+    print(case, "Ta rel err", err, flush=True)
+    assert err <= TOL_K
+
+
+def test_ref_known_answers_f32():
+    f = lambda *a: E.FSM_UW_PS_lonlat_2d(*a, precision=PREC_F32)  # noqa: E731
+    g = lambda *a: E.FSM_O1_JSE_lonlat_2d(*a, precision=PREC_F32)  # noqa: E731
+    RC.test1_isotropic_homogeneous(f)
+    RC.test2_near_source_small(f)
+    RC.test3_monotone_along_row(f)
+    RC.test4_anisotropic_runs(f)
+    RC.test5_grid_refinement(f)
+    RC.test6_adjoint_zero_source(f, g)
+    RC.test7_adjoint_finite_boundary(f, g)
+    # linearity of the FP32 adjoint holds to FP32 rounding of the solution
+    xx, yy, a, b, c, T = RC.iso_grid_T(f, 21, 21)
+    Ta_a = g(xx, yy, a, b, c, T, np.array([5.0]), np.array([0.0]), np.array([2.0]))
+    Ta_b = g(xx, yy, a, b, c, T, np.array([-5.0]), np.array([3.0]), np.array([-1.5]))
+    Ta_ab = g(xx, yy, a, b, c, T, np.array([5.0, -5.0]), np.array([0.0, 3.0]), np.array([2.0, -1.5]))
+    assert RC.rel_inf(Ta_a + Ta_b, Ta_ab) < 1e-6
+
+
+def test_batch_grf_201_f32_vs_oracle():
+    b = synth.config_grf(nperiod=2, nsta=10)
+    ro = O.forward_for_event(b)
+    with PP.EikonalBatch(b, precision=PREC_F32) as bt:
+        bt.run()
+        rg = bt.results()
+        _, _, nres = bt.conv()
+    ek = RC.rel_inf(rg["adj_s"], ro["adj_s"])
+    es = RC.rel_inf(rg["syn"], ro["syn"])
+    ec = abs(rg["chi"] - ro["chi"]) / abs(ro["chi"])
+    print("K err", ek, "syn err", es, "chi err", ec, flush=True)
+    print("FP64 re-solves", nres, flush=True)
+    # forward rounds equal the reference's (decisions near the tolerance were re-solved in FP64)
+    np.testing.assert_array_equal(rg["iters_fwd"], ro["iters_fwd"])
+    for p in range(rg["adj_s"].shape[0]):
+        assert RC.rel_inf(rg["adj_s"][p], ro["adj_s"][p]) <= TOL_K
+    assert es <= TOL_T
+    assert ec <= TOL_K
+
+
+def test_batch_f32_matches_f64_gpu_full_workload():
+    # The metric's workload: 201x201, 100 sources x 30 periods.  Every period's
+    # kernel within the bar of the FP64 path (which equals the oracle to 1e-12,
+    # test_gpu_parity.py), every forward iteration count equal.
+    b = synth.config_grf()
+    with PP.EikonalBatch(b) as b64, PP.EikonalBatch(b, precision=PREC_F32) as b32:
+        b64.run()
+        b32.run()
+        r64, r32 = b64.results(), b32.results()
+        _, _, nres = b32.conv()
+    errs = [RC.rel_inf(r32["adj_s"][p], r64["adj_s"][p]) for p in range(r64["adj_s"].shape[0])]
+    print("per-period K err max", max(errs), "syn err", RC.rel_inf(r32["syn"], r64["syn"]), "FP64 re-solves", nres,
+          flush=True)
+    np.testing.assert_array_equal(r32["iters_fwd"], r64["iters_fwd"])
+    assert max(errs) <= TOL_K
+    assert RC.rel_inf(r32["syn"], r64["syn"]) <= TOL_T

@@ -1,0 +1,184 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+FP64 bar: ||x - x_oracle||_inf / ||x_oracle||_inf <= 1e-12 for T, Ta, syn and the
+per-period kernels, chi within 1e-12 relative, identical iteration counts.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from surfattpp_amd import eikonal as E
+from surfattpp_amd import preproc as PP
+from surfattpp_amd import synth
+from tests import refcases as RC
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+
+gfwd = lambda *a: E.FSM_UW_PS_lonlat_2d(*a)  # noqa: E731
+gadj = lambda *a: E.FSM_O1_JSE_lonlat_2d(*a)  # noqa: E731
+
+
+def test_device_present():
+    from surfattpp_amd import _lib
+    info = _lib.device_info()
+    print(info, flush=True)
+    assert "gfx950" in info
+
+
+@pytest.mark.parametrize("case", ["case9", "case10"])
+def test_forward_f64_matches_oracle(case):
+    xx, yy, spha, sphb, sphc, fun, x0, y0 = getattr(RC, case + "_inputs")()
+    T, it, _ = O.forward(xx, yy, spha, sphb, sphc, fun, x0, y0)
+    Tg = gfwd(xx, yy, spha, sphb, sphc, fun, x0, y0)
+    assert E.last_info["iters"] == it
+    assert RC.rel_inf(Tg, T) <= TOL64
+
+
+@pytest.mark.parametrize("case", ["case9", "case10"])
+def test_adjoint_f64_matches_oracle(case):
+    xx, yy, spha, sphb, sphc, fun, x0, y0 = getattr(RC, case + "_inputs")()
+    T, _, _ = O.forward(xx, yy, spha, sphb, sphc, fun, x0, y0)
+    xr, yr, sa = RC.receivers_for(xx, yy, 30, 5)
+    Ta, ita = O.adjoint(xx, yy, spha, sphb, sphc, T, xr, yr, sa)
+    Tag = gadj(xx, yy, spha, sphb, sphc, T, xr, yr, sa)
+    assert E.last_info["iters"] == ita
+    assert RC.rel_inf(Tag, Ta) <= TOL64
+
+
+def test_golden_twin_vectors_gpu():
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "twin_vectors.npz"))
+    for i in range(int(z["ncase"])):
+        g = lambda k: z[f"c{i}_{k}"]  # noqa: E731
+        Tg = gfwd(g("xx"), g("yy"), g("spha"), g("sphb"), g("sphc"), g("fun"), float(g("x0")), float(g("y0")))
+        assert RC.rel_inf(Tg, g("T")) < 1e-12
+        Tag = gadj(g("xx"), g("yy"), g("spha"), g("sphb"), g("sphc"), g("T"), g("xr"), g("yr"), g("adj"))
+        assert RC.rel_inf(Tag, g("Ta")) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_ref1_homogeneous_gpu(n):
+    RC.test1_isotropic_homogeneous(gfwd) if n == 1 else None
+    if n == 2:
+        RC.test2_near_source_small(gfwd)
+    if n == 3:
+        RC.test3_monotone_along_row(gfwd)
+    if n == 4:
+        RC.test4_anisotropic_runs(gfwd)
+    if n == 5:
+        RC.test5_grid_refinement(gfwd)
+    if n == 6:
+        RC.test6_adjoint_zero_source(gfwd, gadj)
+    if n == 7:
+        RC.test7_adjoint_finite_boundary(gfwd, gadj)
+    if n == 8:
+        RC.test8_adjoint_linearity(gfwd, gadj)
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 3), (5, 4), (64, 9), (65, 7), (129, 11), (200, 13), (512, 5)])
+def test_forward_ragged_shapes(nx, ny):
+    # every lane layout (columns per lane C = 1..8, partial last lane)
+    xx, yy = np.linspace(-1, 1, nx), np.linspace(-0.5, 0.5, ny)
+    rng = np.random.default_rng(nx * 1000 + ny)
+    fun = 1.0 / (3.0 + 0.3 * rng.uniform(size=(nx, ny)))
+    a = 1.0 + 0.05 * rng.uniform(size=(nx, ny))
+    b = 1.0 + 0.05 * rng.uniform(size=(nx, ny))
+    c = 0.02 * rng.uniform(-1, 1, size=(nx, ny))
+    x0, y0 = xx[nx // 2] + 0.1 * (xx[1] - xx[0]), yy[ny // 2] - 0.2 * (yy[1] - yy[0])
+    T, it, _ = O.forward(xx, yy, a, b, c, fun, x0, y0)
+    Tg = gfwd(xx, yy, a, b, c, fun, x0, y0)
+    assert E.last_info["iters"] == it
+    assert RC.rel_inf(Tg, T) <= TOL64
+    xr, yr, sa = RC.receivers_for(xx, yy, 7, nx) if nx > 5 and ny > 5 else (np.array([xx[1]]), np.array([yy[1]]), np.array([1.0]))
+    Ta, ita = O.adjoint(xx, yy, a, b, c, T, xr, yr, sa)
+    Tag = gadj(xx, yy, a, b, c, T, xr, yr, sa)
+    assert E.last_info["iters"] == ita
+    assert RC.rel_inf(Tag, Ta) <= TOL64 or np.abs(Ta).max() == 0.0
+
+
+def test_errors_follow_reference():
+    with pytest.raises(ValueError):
+        gfwd(np.array([0.0, 1.0]), np.linspace(0, 1, 5), np.ones((2, 5)), np.ones((2, 5)), np.zeros((2, 5)),
+             np.ones((2, 5)), 0.5, 0.5)
+    nx = ny = 21
+    xx, yy = np.linspace(0, 2, nx), np.linspace(0, 2, ny)
+    gfwd(xx, yy, *RC.iso(nx, ny, 0.3), 0.0, 1.0)
+    assert E.last_info["flags"] & 2  # source on the boundary: a warning flag, not an error
+
+
+def _batch_vs_oracle(b, tol=TOL64):
+    ro = O.forward_for_event(b)
+    with PP.EikonalBatch(b) as bt:
+        bt.run()
+        rg = bt.results()
+        bt.run(mode=1)  # one launch per pair: same kernels, same bits
+        rp = bt.results()
+    np.testing.assert_array_equal(rg["iters_fwd"], ro["iters_fwd"])
+    np.testing.assert_array_equal(rg["iters_adj"], ro["iters_adj"])
+    assert RC.rel_inf(rg["syn"], ro["syn"]) <= tol
+    assert RC.rel_inf(rg["adj_s"], ro["adj_s"]) <= tol
+    assert abs(rg["chi"] - ro["chi"]) <= tol * abs(ro["chi"])
+    np.testing.assert_array_equal(rp["adj_s"], rg["adj_s"])
+    np.testing.assert_array_equal(rp["syn"], rg["syn"])
+    return rg, ro
+
+
+def test_batch_checkerboard_61x71_f64():
+    cb = synth.checkerboard(61, 71, 3.0, 0.06, 2, 3)
+    b0 = synth.config_checkerboard()
+    # observed = forward times in the checkerboard (computed by the oracle)
+    bt = dict(b0, svel=cb[None])
+    obs = O.forward_for_event(bt, is_calc_adj=False)["syn"]
+    b = synth.config_checkerboard(tt_obs=obs)
+    rg, ro = _batch_vs_oracle(b)
+    print("chi", rg["chi"], "iters", rg["iters_fwd"].tolist(), rg["iters_adj"].tolist(), flush=True)
+
+
+def test_batch_grf_201_subset_f64():
+    b = synth.config_grf(nperiod=3, nsta=12)
+    _batch_vs_oracle(b)
+
+
+def test_batch_multi_grid_uneven_f64():
+    g1 = synth.config_grf(seed=11, nx=41, ny=33, nsta=6, nperiod=2, spacing_km=1.5)
+    g2 = synth.config_grf(seed=12, nx=131, ny=57, nsta=5, nperiod=2, spacing_km=1.5)
+    with PP.EikonalBatch([g1, g2]) as bt:
+        bt.run()
+        r = bt.results()
+    for gi, g in enumerate([g1, g2]):
+        ro = O.forward_for_event(g)
+        sl = slice(sum(bt.rec_counts[:gi]), sum(bt.rec_counts[:gi + 1]))
+        assert RC.rel_inf(r["syn"][sl], ro["syn"]) <= TOL64
+        assert RC.rel_inf(r["adj_s"][gi], ro["adj_s"]) <= TOL64
+        np.testing.assert_array_equal(r["iters_fwd"][bt.ev_slices[gi]], ro["iters_fwd"])
+
+
+def test_batch_zero_receivers_and_forward_only():
+    b = synth.config_grf(seed=5, nx=33, ny=29, nsta=4, nperiod=1)
+    b["ev_nrec"] = b["ev_nrec"].copy()
+    b["ev_nrec"][1] = 0  # an event without receivers: Ta == 0 (test_eikonal.cpp:348)
+    ro = O.forward_for_event(b)
+    with PP.EikonalBatch(b) as bt:
+        bt.run()
+        rg = bt.results()
+    assert RC.rel_inf(rg["adj_s"], ro["adj_s"]) <= TOL64
+    with PP.EikonalBatch(b, calc_adj=False) as bt:
+        bt.run()
+        rf = bt.results()
+    assert RC.rel_inf(rf["syn"], ro["syn"]) <= TOL64
+    assert rf["chi"] == 0.0 and not rf["adj_s"].any()
+
+
+def test_batch_update_media_and_obs():
+    b = synth.config_grf(seed=6, nx=45, ny=41, nsta=5, nperiod=2)
+    b2 = dict(b, svel=b["svel"] * 1.02, tt_obs=b["tt_obs"] + 0.1)
+    ro = O.forward_for_event(b2)
+    with PP.EikonalBatch(b) as bt:
+        bt.run()
+        bt.update_media(0, b2["svel"], b2["m11"], b2["m12"], b2["m22"])
+        bt.update_obs(b2["tt_obs"], b2["weight"])
+        bt.run()
+        rg = bt.results()
+    assert RC.rel_inf(rg["adj_s"], ro["adj_s"]) <= TOL64
+    assert abs(rg["chi"] - ro["chi"]) <= TOL64 * abs(ro["chi"])
