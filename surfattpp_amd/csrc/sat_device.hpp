@@ -62,6 +62,9 @@ struct GridDev {
   // forward media, segmented, sweep precision: FP64 a, b, c, f; FP32 a, b, c, f^2,
   // dis_x = sqrt(f^2 b/det), dis_y = sqrt(f^2 a/det) (-1 where det <= 0)
   const void* seg[6];
+  const double2* trig_yy;  // per row: sin(yy), cos(yy), correctly rounded
+  int iso;                 // spha = sphb = 1, sphc = 0 on every node of every period
+  const float4* rowtab;    // iso grids, per row: a, b, 1/sqrt(a), 1/sqrt(b) (FP32 sweeps)
 };
 
 __device__ inline int seg_index(const GridDev& g, int ix, int iy) {

@@ -19,11 +19,13 @@
 #include <cstring>
 #include <new>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/surfatt_eik.h"
 #include "sat_device.hpp"
 #include "sat_engine.hpp"
+#include "sat_phases.hpp"
 #include "sat_policies.hpp"
 
 namespace sat {
@@ -152,10 +154,26 @@ __global__ void media_kernel(GridDev g, int mode, int prec, const double* __rest
       static_cast<float*>(s0)[so] = (float)va;
       static_cast<float*>(s1)[so] = (float)vb;
       static_cast<float*>(s2)[so] = (float)vc;
-      static_cast<float*>(s3)[so] = (float)(vf * vf);
+      static_cast<float*>(s3)[so] = (float)vf;
       static_cast<float*>(s4)[so] = (det > 0.0) ? (float)sqrt(vf * vf * vb / det) : -1.0f;
       static_cast<float*>(s5)[so] = (det > 0.0) ? (float)sqrt(vf * vf * va / det) : -1.0f;
     }
+  }
+}
+
+// Per row: correctly rounded sin/cos of the latitude; isotropic grids also the
+// FP32 row metric {a, b, 1/sqrt(a), 1/sqrt(b)} (src/eikonal_solver.cpp:55-65 with
+// spha = sphb = 1, sphc = 0).
+__global__ void row_kernel(GridDev g, double2* trig_yy, float4* rowtab) {
+  const int iy = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iy >= g.ny) return;
+  double s, c;
+  cr::sincos_cr(g.yy[iy], s, c);
+  trig_yy[iy] = make_double2(s, c);
+  if (rowtab) {
+    const double R2 = kR * kR;
+    const double a = 1.0 / (R2 * (c * c)), b = 1.0 / R2;
+    rowtab[iy] = make_float4((float)a, (float)b, (float)(1.0 / sqrt(a)), (float)(1.0 / sqrt(b)));
   }
 }
 
@@ -177,7 +195,7 @@ __device__ inline typename Prec<PREC>::Fwd::Ctx fwd_ctx(const GridDev& g, int ip
     cx.dx = g.dx;
     cx.dy = g.dy;
   } else {
-    cx.f2 = static_cast<const V*>(g.seg[3]) + po;
+    cx.f = static_cast<const V*>(g.seg[3]) + po;
     cx.dsx = static_cast<const V*>(g.seg[4]) + po;
     cx.dsy = static_cast<const V*>(g.seg[5]) + po;
     cx.idx = (float)(1.0 / g.dx);
@@ -495,9 +513,11 @@ struct GridHost {
   char* d_seg = nullptr;      // 6 segmented media planes per period (sweep precision)
   char* d_seg64 = nullptr;    // FP32 batches: the FP64 media for the re-solves
   GridDev g64{};              // FP32 batches: g with the FP64 media
+  double2* d_trig_yy = nullptr;
+  float4* d_rowtab = nullptr;
 
   void release() {
-    void* ps[] = {d_lon, d_lat, d_xx, d_yy, d_model, d_fp, d_aA, d_b, d_c, d_v3, d_seg, d_seg64};
+    void* ps[] = {d_lon, d_lat, d_xx, d_yy, d_model, d_fp, d_aA, d_b, d_c, d_v3, d_seg, d_seg64, d_trig_yy, d_rowtab};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = GridHost();
@@ -543,6 +563,14 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
   g.lon = G.d_lon; g.lat = G.d_lat; g.xx = G.d_xx; g.yy = G.d_yy;
   g.f_plain = G.d_fp; g.aA_plain = G.d_aA; g.b_plain = G.d_b; g.c_plain = G.d_c; g.vinv3 = G.d_v3;
   for (int i = 0; i < 6; i++) g.seg[i] = G.d_seg + (size_t)i * seg_bytes;
+  SAT_CHECK(dalloc(&G.d_trig_yy, ny));
+  SAT_CHECK(dalloc(&G.d_rowtab, ny));
+  g.trig_yy = G.d_trig_yy;
+  g.rowtab = G.d_rowtab;
+  g.iso = 0;
+  hipLaunchKernelGGL(row_kernel, dim3((ny + 63) / 64), dim3(64), 0, nullptr, g, G.d_trig_yy, G.d_rowtab);
+  SAT_CHECK(hipGetLastError());
+  SAT_CHECK(hipDeviceSynchronize());
   G.g64 = g;
   if (G.d_seg64)
     for (int i = 0; i < 6; i++) G.g64.seg[i] = G.d_seg64 + (size_t)i * seg64_bytes;
@@ -554,8 +582,18 @@ static int grid_media(GridHost& G, int mode, const double* p0, const double* p1,
                       const double* p3, hipStream_t stream) {
   const size_t N = (size_t)G.nx * G.ny, n = (size_t)G.nper * N;
   const double* src[4] = {p0, p1, p2, p3};
-  for (int i = 0; i < 4; i++) {
+  for (int i = 0; i < 4; i++)
     if (!src[i]) return SAT_EINVAL;
+  {  // isotropic metric: spha = sphb = 1 and sphc = 0 everywhere
+    const double* ma = (mode == 0) ? p1 : p0;   // m11 | spha
+    const double* mb = (mode == 0) ? p3 : p1;   // m22 | sphb
+    const double* mc = p2;                      // m12 | sphc
+    bool iso = true;
+    for (size_t k = 0; k < n && iso; k++) iso = ma[k] == 1.0 && mb[k] == 1.0 && mc[k] == 0.0;
+    G.g.iso = iso ? 1 : 0;
+    G.g64.iso = G.g.iso;
+  }
+  for (int i = 0; i < 4; i++) {
     SAT_CHECK(hipMemcpyAsync(G.d_model + i * n, src[i], sizeof(double) * n, hipMemcpyHostToDevice, stream));
   }
   const size_t total = (size_t)G.nper * G.ny * G.g.P;
@@ -719,29 +757,53 @@ struct sat_batch {
   int calc_adj = 1;
   int nevent = 0, nrec = 0;
   std::vector<GridHost> grids;
+  std::vector<EventDev> he;        // host copy of the events
+  std::vector<EventX> hx;          // host copy of the phase events
+  std::vector<int> base_flags;     // per event, set at creation (source on the boundary)
   GridDev* d_grids = nullptr;
+  GridDev* d_grids64 = nullptr;    // FP32 batches: grids with FP64 media (re-solves)
   EventDev* d_events = nullptr;
+  EventX* d_evx = nullptr;
   int* d_order = nullptr;
-  int* d_counter = nullptr;        // one per C group
+  int* d_counter = nullptr;        // re-solve counts | fwd heads | adj heads | re-solve heads (one per group),
+                                   // then one head per event for per-solve launches
   double* d_rec = nullptr;         // rec_lon | rec_lat | tt | w | syn | adjsrc
+  int* d_rec_ev = nullptr;
+  int2* d_rcell = nullptr;
+  double* d_rwt = nullptr;
   double* d_chi_ev = nullptr;
   double* d_chi = nullptr;
   int* d_iters = nullptr;          // iters_f | iters_a | flags
-  char* d_ta = nullptr;
-  size_t ta_total = 0;             // elements
+  int* d_base_flags = nullptr;
+  double* d_conv = nullptr;        // 2 per event
+  unsigned char* d_skip = nullptr;
+  int* d_fb = nullptr;             // re-solve lists, group g at group_first[g]
+  double f32_band = 0.02;
+  // per-event planes of the phase pipeline
+  size_t seg_total = 0, pl_total = 0, col_total = 0;
+  double* d_evs = nullptr;
+  double2* d_coltab = nullptr;
+  char* d_t0 = nullptr;            // 3 planes (t0, t0x, t0y), sweep precision
+  double* d_T0d = nullptr;
+  double* d_T = nullptr;
+  char* d_co = nullptr;            // 6 (FP64) / 5 (FP32) coefficient planes
+  char* d_ta = nullptr;            // masked Ta, sweep precision
+  char* d_slots = nullptr;         // phase sweep slots
+  size_t slot_plane = 0, slot_bytes = 0;
+  int nslots_phase = 0;
+  // fused path: per-solve mode and FP64 re-solves
   char* d_ws = nullptr;
   SlotGeom sg{};
-  int nslots = 0;
-  GridDev* d_grids64 = nullptr;    // FP32 batches: grids with FP64 media (re-solves)
-  char* d_ws64 = nullptr;          // FP32 batches: FP64 slots for the re-solves
+  char* d_ws64 = nullptr;
   SlotGeom sg64{};
   int nslots64 = 0;
-  double* d_conv = nullptr;        // 2 per event
-  int* d_fb = nullptr;             // re-solve lists (ne) | counters (ngroups)
-  double f32_band = 0.02;
   double* d_K = nullptr;
   size_t K_total = 0;
-  std::vector<int> group_C, group_first, group_count;  // events grouped by C, largest grids first
+  // launch groups: events grouped by (columns per lane C, isotropic grid)
+  std::vector<int> group_C, group_iso, group_first, group_count;
+  bool groups_dirty = true;
+  int max_nx = 0;
+  size_t max_seg = 0;
   int nplanes = 0;
   size_t* d_red_sz = nullptr;      // plane_off | node_prefix
   int* d_red_i = nullptr;          // plane_n | ev_off | ev_ids
@@ -752,17 +814,96 @@ struct sat_batch {
 
   void release() {
     for (auto& G : grids) G.release();
-    void* ps[] = {d_grids, d_events, d_order, d_counter, d_rec, d_chi_ev, d_chi, d_iters, d_ta, d_ws, d_K, d_red_sz,
-                  d_red_i, (void*)d_red_v3, d_grids64, d_ws64, d_conv, d_fb};
+    void* ps[] = {d_grids, d_grids64, d_events, d_evx, d_order, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
+                  d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_evs, d_coltab, d_t0, d_T0d, d_T, d_co, d_ta,
+                  d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3};
     for (void* p : ps)
       if (p) (void)hipFree(p);
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-    if (ev2) (void)hipEventDestroy(ev2);
-    if (evA) (void)hipEventDestroy(evA);
+    hipEvent_t es[] = {ev0, ev1, ev2, evA};
+    for (hipEvent_t e : es)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+namespace {
+
+// Sort events into launch groups (same C and isotropy), largest grids first.
+int build_groups(sat_batch* b) {
+  const int ne = b->nevent;
+  std::vector<int> order(ne);
+  std::iota(order.begin(), order.end(), 0);
+  auto key = [&](int e) {
+    const GridDev& g = b->grids[b->he[e].grid].g;
+    return std::make_pair(g.C, g.iso);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    if (key(a) != key(c)) return key(a) < key(c);
+    return b->grids[b->he[a].grid].g.plane > b->grids[b->he[c].grid].g.plane;
+  });
+  b->group_C.clear();
+  b->group_iso.clear();
+  b->group_first.clear();
+  b->group_count.clear();
+  for (int i = 0; i < ne;) {
+    const auto k = key(order[i]);
+    int j = i;
+    while (j < ne && key(order[j]) == k) j++;
+    b->group_C.push_back(k.first);
+    b->group_iso.push_back(k.second);
+    b->group_first.push_back(i);
+    b->group_count.push_back(j - i);
+    i = j;
+  }
+  if (ne) SAT_CHECK(hipMemcpy(b->d_order, order.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+  std::vector<GridDev> hg(b->grids.size()), hg64(b->grids.size());
+  for (size_t gi = 0; gi < b->grids.size(); gi++) {
+    hg[gi] = b->grids[gi].g;
+    hg64[gi] = b->grids[gi].g64;
+  }
+  SAT_CHECK(hipMemcpy(b->d_grids, hg.data(), sizeof(GridDev) * hg.size(), hipMemcpyHostToDevice));
+  if (b->d_grids64) SAT_CHECK(hipMemcpy(b->d_grids64, hg64.data(), sizeof(GridDev) * hg64.size(), hipMemcpyHostToDevice));
+  b->groups_dirty = false;
+  return SAT_OK;
+}
+
+template <int PREC, int C, bool ISO> struct FwdSweepK {
+  static void go(dim3 g, hipStream_t s, PhaseDev a, int first, int count, int group) {
+    hipLaunchKernelGGL((fwd_sweep_kernel<PREC, C, ISO>), g, dim3(kWave), 0, s, a, first, count, group);
+  }
+};
+template <int PREC, int C> struct FwdSweepIsoK : FwdSweepK<PREC, C, true> {};
+template <int PREC, int C> struct FwdSweepGenK : FwdSweepK<PREC, C, false> {};
+template <int PREC, int C> struct AdjSweepK {
+  static void go(dim3 g, hipStream_t s, PhaseDev a, int first, int count) {
+    hipLaunchKernelGGL((adj_sweep_kernel<PREC, C>), g, dim3(kWave), 0, s, a, first, count);
+  }
+};
+
+// Host copy of the index boxes, computed with the same IEEE double operations
+// as the device (src/eikonal_solver.cpp:119-129 and :528-531).
+void event_boxes(const GridHost& G, const std::vector<double>& xx, const std::vector<double>& yy,
+                 const double* lon, const double* lat, double elon, double elat, EventX& x, int& flags) {
+  const int nx = G.nx, ny = G.ny;
+  const double x0 = elon * kDEG2RAD, y0 = elat * kDEG2RAD;
+  const double dx = xx[1] - xx[0], dy = yy[1] - yy[0];
+  int bx0 = 1 << 30, bx1 = -1, by0 = 1 << 30, by1 = -1;
+  for (int ix = 0; ix < nx; ix++)
+    if (std::fabs((xx[ix] - x0) / dx) <= 1.0) { bx0 = std::min(bx0, ix); bx1 = std::max(bx1, ix); }
+  for (int iy = 0; iy < ny; iy++)
+    if (std::fabs((yy[iy] - y0) / dy) <= 1.0) { by0 = std::min(by0, iy); by1 = std::max(by1, iy); }
+  x.fb[0] = bx0; x.fb[1] = bx1; x.fb[2] = by0; x.fb[3] = by1;
+  flags = 0;
+  if (bx0 <= bx1 && by0 <= by1 && (bx0 == 0 || bx1 == nx - 1 || by0 == 0 || by1 == ny - 1))
+    flags |= FLAG_SRC_ON_BOUNDARY;
+  const double dxd = lon[1] - lon[0], dyd = lat[1] - lat[0];
+  x.mb[0] = imax(0, (int)std::floor((elon - 1.5 * dxd - lon[0]) / dxd) + 1);
+  x.mb[1] = imin(nx - 1, (int)std::ceil((elon + 1.5 * dxd - lon[0]) / dxd) - 1);
+  x.mb[2] = imax(0, (int)std::floor((elat - 1.5 * dyd - lat[0]) / dyd) + 1);
+  x.mb[3] = imin(ny - 1, (int)std::ceil((elat + 1.5 * dyd - lat[0]) / dyd) - 1);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -804,12 +945,17 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   if (d->precision != SAT_PREC_F64 && d->precision != SAT_PREC_F32) return SAT_EINVAL;
   if (d->nrec_total < 0 || (d->nrec_total > 0 && (!d->rec_lon_deg || !d->rec_lat_deg || !d->tt_obs || !d->weight)))
     return SAT_EINVAL;
+  std::vector<int> rec_ev((size_t)std::max(d->nrec_total, 1), -1);
   for (int e = 0; e < d->nevent; e++) {
     const sat_event_desc& ev = d->events[e];
     if (ev.grid < 0 || ev.grid >= d->ngrid) return SAT_EINVAL;
     if (ev.iper < 0 || ev.iper >= d->grids[ev.grid].nperiod) return SAT_EINVAL;
     if (ev.nrec < 0 || ev.rec_off < 0 || (long)ev.rec_off + ev.nrec > d->nrec_total) return SAT_EINVAL;
     if (!std::isfinite(ev.src_lon_deg) || !std::isfinite(ev.src_lat_deg)) return SAT_EINVAL;
+    for (int r = 0; r < ev.nrec; r++) {
+      if (rec_ev[ev.rec_off + r] != -1) return SAT_EINVAL;  // receivers belong to one event
+      rec_ev[ev.rec_off + r] = e;
+    }
   }
   int rc = device_check();
   if (rc) return rc;
@@ -825,6 +971,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
     }                                                                         \
   } while (0)
   b->precision = d->precision;
+  const bool f32 = b->precision == SAT_PREC_F32;
   b->calc_adj = d->calc_adj ? 1 : 0;
   b->nevent = d->nevent;
   b->nrec = d->nrec_total;
@@ -835,6 +982,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(hipEventCreate(&b->evA));
   b->grids.resize(d->ngrid);
   size_t max_seg = 0, max_plane = 0;
+  int max_nx = 0;
   for (int gi = 0; gi < d->ngrid; gi++) {
     const sat_grid_desc& gd = d->grids[gi];
     rc = grid_setup(b->grids[gi], b->precision, gd.nx, gd.ny, gd.nperiod, gd.lon_deg, gd.lat_deg);
@@ -843,55 +991,49 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
     if (rc) return fail(rc);
     max_seg = std::max(max_seg, b->grids[gi].g.seg_plane);
     max_plane = std::max(max_plane, b->grids[gi].g.plane);
+    max_nx = std::max(max_nx, gd.nx);
   }
-  std::vector<GridDev> hg(d->ngrid), hg64(d->ngrid);
-  for (int gi = 0; gi < d->ngrid; gi++) {
-    hg[gi] = b->grids[gi].g;
-    hg64[gi] = b->grids[gi].g64;
-  }
+  b->max_seg = max_seg;
+  b->max_nx = max_nx;
   B_CHECK(dalloc(&b->d_grids, d->ngrid));
-  B_CHECK(hipMemcpy(b->d_grids, hg.data(), sizeof(GridDev) * d->ngrid, hipMemcpyHostToDevice));
-  if (b->precision == SAT_PREC_F32) {
-    B_CHECK(dalloc(&b->d_grids64, d->ngrid));
-    B_CHECK(hipMemcpy(b->d_grids64, hg64.data(), sizeof(GridDev) * d->ngrid, hipMemcpyHostToDevice));
-  }
+  if (f32) B_CHECK(dalloc(&b->d_grids64, d->ngrid));
 
-  // events and their Ta planes
+  // events: per-event plane offsets, index boxes, base flags
   const int ne = d->nevent;
-  std::vector<EventDev> he(ne);
-  size_t ta_total = 0;
+  b->he.resize(ne);
+  b->hx.resize(ne);
+  b->base_flags.assign(std::max(ne, 1), 0);
+  size_t seg_total = 0, pl_total = 0, col_total = 0;
   for (int e = 0; e < ne; e++) {
     const sat_event_desc& ev = d->events[e];
-    he[e] = EventDev{ev.grid, ev.iper, ev.src_lon_deg, ev.src_lat_deg, ev.rec_off, ev.nrec, ta_total};
-    ta_total += b->calc_adj ? hg[ev.grid].plane : 0;
+    const sat_grid_desc& gd = d->grids[ev.grid];
+    const GridDev& g = b->grids[ev.grid].g;
+    b->he[e] = EventDev{ev.grid, ev.iper, ev.src_lon_deg, ev.src_lat_deg, ev.rec_off, ev.nrec, pl_total};
+    EventX& x = b->hx[e];
+    x.grid = ev.grid; x.iper = ev.iper; x.rec_off = ev.rec_off; x.nrec = ev.nrec;
+    x.lon = ev.src_lon_deg; x.lat = ev.src_lat_deg;
+    x.seg_off = seg_total; x.pl_off = pl_total; x.col_off = col_total;
+    std::vector<double> xx(gd.nx), yy(gd.ny);
+    for (int i = 0; i < gd.nx; i++) xx[i] = gd.lon_deg[i] * kDEG2RAD;
+    for (int j = 0; j < gd.ny; j++) yy[j] = gd.lat_deg[j] * kDEG2RAD;
+    event_boxes(b->grids[ev.grid], xx, yy, gd.lon_deg, gd.lat_deg, ev.src_lon_deg, ev.src_lat_deg, x,
+                b->base_flags[e]);
+    seg_total += round_up(g.seg_plane, 64);
+    pl_total += round_up(g.plane, 64);
+    col_total += round_up(g.nx, 64);
   }
-  b->ta_total = ta_total;
+  b->seg_total = seg_total;
+  b->pl_total = pl_total;
+  b->col_total = col_total;
   B_CHECK(dalloc(&b->d_events, ne));
-  if (ne) B_CHECK(hipMemcpy(b->d_events, he.data(), sizeof(EventDev) * ne, hipMemcpyHostToDevice));
-
-  // queue order: grouped by C, largest grids first inside a group
-  std::vector<int> order(ne);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-    const GridDev& ga = hg[he[a].grid];
-    const GridDev& gc = hg[he[c].grid];
-    if (ga.C != gc.C) return ga.C < gc.C;
-    return ga.plane > gc.plane;
-  });
-  for (int i = 0; i < ne;) {
-    const int C = hg[he[order[i]].grid].C;
-    int j = i;
-    while (j < ne && hg[he[order[j]].grid].C == C) j++;
-    b->group_C.push_back(C);
-    b->group_first.push_back(i);
-    b->group_count.push_back(j - i);
-    i = j;
+  B_CHECK(dalloc(&b->d_evx, ne));
+  if (ne) {
+    B_CHECK(hipMemcpy(b->d_events, b->he.data(), sizeof(EventDev) * ne, hipMemcpyHostToDevice));
+    B_CHECK(hipMemcpy(b->d_evx, b->hx.data(), sizeof(EventX) * ne, hipMemcpyHostToDevice));
   }
+  B_CHECK(dalloc(&b->d_base_flags, std::max(ne, 1)));
+  B_CHECK(hipMemcpy(b->d_base_flags, b->base_flags.data(), sizeof(int) * std::max(ne, 1), hipMemcpyHostToDevice));
   B_CHECK(dalloc(&b->d_order, ne));
-  if (ne) B_CHECK(hipMemcpy(b->d_order, order.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
-  B_CHECK(dalloc(&b->d_counter, 2 * std::max<size_t>(b->group_C.size(), 1)));
-  B_CHECK(dalloc(&b->d_conv, 2 * (size_t)std::max(ne, 1)));
-  B_CHECK(dalloc(&b->d_fb, (size_t)std::max(ne, 1) + std::max<size_t>(b->group_C.size(), 1)));
 
   // receivers
   const size_t nr = (size_t)d->nrec_total;
@@ -903,20 +1045,46 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
     B_CHECK(hipMemcpy(b->d_rec + 2 * nr, d->tt_obs, sizeof(double) * nr, hipMemcpyHostToDevice));
     B_CHECK(hipMemcpy(b->d_rec + 3 * nr, d->weight, sizeof(double) * nr, hipMemcpyHostToDevice));
   }
+  B_CHECK(dalloc(&b->d_rec_ev, std::max<size_t>(nr, 1)));
+  B_CHECK(hipMemcpy(b->d_rec_ev, rec_ev.data(), sizeof(int) * rec_ev.size(), hipMemcpyHostToDevice));
+  B_CHECK(dalloc(&b->d_rcell, std::max<size_t>(nr, 1)));
+  B_CHECK(dalloc(&b->d_rwt, 3 * std::max<size_t>(nr, 1)));
   B_CHECK(dalloc(&b->d_chi_ev, std::max(ne, 1)));
+  B_CHECK(hipMemset(b->d_chi_ev, 0, sizeof(double) * std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_chi, 1));
   B_CHECK(dalloc(&b->d_iters, 3 * (size_t)std::max(ne, 1)));
-  B_CHECK(dalloc(&b->d_ta, std::max<size_t>(ta_total, 1) * elem_size(b->precision)));
+  B_CHECK(dalloc(&b->d_conv, 2 * (size_t)std::max(ne, 1)));
+  B_CHECK(dalloc(&b->d_skip, std::max(ne, 1)));
+  B_CHECK(dalloc(&b->d_fb, (size_t)std::max(ne, 1)));
+  B_CHECK(dalloc(&b->d_counter, 5 * (size_t)std::max(ne, 1)));  // groups <= events
 
-  // slots: enough persistent waves to fill the device
+  // per-event planes of the phase pipeline
+  const size_t es = elem_size(b->precision);
+  B_CHECK(dalloc(&b->d_evs, 3 * (size_t)std::max(ne, 1)));
+  B_CHECK(dalloc(&b->d_coltab, std::max<size_t>(col_total, 1)));
+  B_CHECK(dalloc(&b->d_t0, 3 * std::max<size_t>(seg_total, 1) * es));
+  if (f32) B_CHECK(dalloc(&b->d_T0d, std::max<size_t>(pl_total, 1)));
+  B_CHECK(dalloc(&b->d_T, std::max<size_t>(pl_total, 1)));
+  if (b->calc_adj) {
+    B_CHECK(dalloc(&b->d_co, (f32 ? 5 : 6) * std::max<size_t>(seg_total, 1) * es));
+    B_CHECK(dalloc(&b->d_ta, std::max<size_t>(pl_total, 1) * es));
+  }
+
+  // resident waves of the sweep kernels: 3 swept planes each (FP64 sigma/tau sized)
   hipDeviceProp_t prop;
   B_CHECK(hipGetDeviceProperties(&prop, 0));
   const int waves_per_cu = 8;
-  b->nslots = std::max(1, std::min(ne, prop.multiProcessorCount * waves_per_cu));
+  b->nslots_phase = std::max(1, std::min(ne, prop.multiProcessorCount * waves_per_cu));
+  b->slot_plane = round_up(max_seg * sizeof(double), 256);
+  b->slot_bytes = 3 * b->slot_plane;
+  B_CHECK(dalloc(&b->d_slots, b->slot_bytes * (size_t)b->nslots_phase));
+  B_CHECK(hipMemset(b->d_slots, 0, b->slot_bytes * (size_t)b->nslots_phase));
+
+  // fused path: one slot for per-solve runs; FP32 batches also FP64 slots for re-solves
   b->sg = slot_geom(b->precision, max_seg, max_plane);
-  B_CHECK(dalloc(&b->d_ws, b->sg.slot * (size_t)b->nslots));
-  B_CHECK(hipMemset(b->d_ws, 0, b->sg.slot * (size_t)b->nslots));  // also the all-zero FP32 delta planes
-  if (b->precision == SAT_PREC_F32) {  // FP64 slots for the re-solves (a fraction of the pairs)
+  B_CHECK(dalloc(&b->d_ws, b->sg.slot));
+  B_CHECK(hipMemset(b->d_ws, 0, b->sg.slot));  // also the all-zero FP32 delta plane
+  if (f32) {
     b->nslots64 = std::max(1, std::min(ne, prop.multiProcessorCount));
     b->sg64 = slot_geom(SAT_PREC_F64, max_seg, max_plane);
     B_CHECK(dalloc(&b->d_ws64, b->sg64.slot * (size_t)b->nslots64));
@@ -929,14 +1097,15 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   std::vector<int> plane_n, ev_off{0}, ev_ids;
   size_t K_total = 0;
   for (int gi = 0; gi < d->ngrid; gi++) {
-    for (int ip = 0; ip < hg[gi].nperiod; ip++) {
+    const GridDev& g = b->grids[gi].g;
+    for (int ip = 0; ip < g.nperiod; ip++) {
       plane_off.push_back(K_total);
-      vinv_ptr.push_back(b->grids[gi].d_v3 + (size_t)ip * hg[gi].plane);
-      plane_n.push_back((int)hg[gi].plane);
+      vinv_ptr.push_back(b->grids[gi].d_v3 + (size_t)ip * g.plane);
+      plane_n.push_back((int)g.plane);
       for (int e = 0; e < ne; e++)
-        if (he[e].grid == gi && he[e].iper == ip) ev_ids.push_back(e);
+        if (b->he[e].grid == gi && b->he[e].iper == ip) ev_ids.push_back(e);
       ev_off.push_back((int)ev_ids.size());
-      K_total += hg[gi].plane;
+      K_total += g.plane;
       node_prefix.push_back(K_total);
     }
   }
@@ -969,6 +1138,8 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   rd.K = b->d_K;
   rd.total_nodes = K_total;
   B_CHECK(hipStreamSynchronize(b->stream));
+  rc = build_groups(b);
+  if (rc) return fail(rc);
   *out = b;
   return SAT_OK;
 #undef B_CHECK
@@ -977,9 +1148,11 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
 int sat_batch_update_media(sat_batch* b, int grid, const double* svel, const double* m11, const double* m12,
                            const double* m22) {
   if (!b || grid < 0 || grid >= (int)b->grids.size()) return SAT_EINVAL;
+  const int iso0 = b->grids[grid].g.iso;
   const int rc = grid_media(b->grids[grid], 0, svel, m11, m12, m22, b->stream);
   if (rc) return rc;
   SAT_CHECK(hipStreamSynchronize(b->stream));  // host planes may be released on return
+  if (b->grids[grid].g.iso != iso0) b->groups_dirty = true;
   return SAT_OK;
 }
 
@@ -996,7 +1169,21 @@ int sat_batch_update_obs(sat_batch* b, const double* tt_obs, const double* weigh
 
 int sat_batch_run(sat_batch* b, int mode) {
   if (!b || (mode != SAT_RUN_BATCHED && mode != SAT_RUN_PER_SOLVE)) return SAT_EINVAL;
+  if (b->groups_dirty) {
+    const int rc = build_groups(b);
+    if (rc) return rc;
+  }
+  const bool f32 = b->precision == SAT_PREC_F32;
+  const int ne = b->nevent;
   const size_t nr = (size_t)std::max(b->nrec, 1);
+  const size_t ne1 = (size_t)std::max(ne, 1);
+  const size_t ng = b->group_C.size();
+  int* fb_count = b->d_counter;
+  int* fwd_heads = b->d_counter + ng;
+  int* adj_heads = b->d_counter + 2 * ng;
+  int* res_heads = b->d_counter + 3 * ng;
+  int* solo_heads = b->d_counter + 4 * ng;
+
   BatchDev bd{};
   bd.grids = b->d_grids;
   bd.events = b->d_events;
@@ -1008,47 +1195,123 @@ int sat_batch_run(sat_batch* b, int mode) {
   bd.syn = b->d_rec + 4 * nr;
   bd.adjsrc = b->d_rec + 5 * nr;
   bd.chi_ev = b->d_chi_ev;
-  const size_t ne = (size_t)std::max(b->nevent, 1);
   bd.iters_f = b->d_iters;
-  bd.iters_a = b->d_iters + ne;
-  bd.flags = b->d_iters + 2 * ne;
+  bd.iters_a = b->d_iters + ne1;
+  bd.flags = b->d_iters + 2 * ne1;
   bd.ta_out = b->d_ta;
-  bd.ws = b->d_ws;
-  bd.ws_slot = b->sg.slot;
-  bd.ws_seg = b->sg.seg;
-  bd.ws_plain = b->sg.plain;
   bd.calc_adj = b->calc_adj;
   bd.fwd_maxiter = kFwdMaxIter;
   bd.adj_maxiter = kAdjMaxIter;
   bd.conv = b->d_conv;
-  const size_t ng = b->group_C.size();
   bd.fb_list = b->d_fb;
-  bd.fb_count = b->d_fb + std::max(b->nevent, 1);
+  bd.fb_count = fb_count;
   bd.f32_band = b->f32_band;
-  const bool f32 = b->precision == SAT_PREC_F32;
-  auto launch = [&](const BatchDev& a, int C, dim3 grid, int first, int count, int group, const int* list,
-                    const int* dcount) -> hipError_t {
-    if (!f32) return dispatch_C1<PairK64, 0>(C, grid, b->stream, a, first, count, group, list, dcount);
-    if (!list) return dispatch_C1<PairK32, 0>(C, grid, b->stream, a, first, count, group, list, dcount);
-    return dispatch_C1<PairK64to32, 0>(C, grid, b->stream, a, first, count, group, list, dcount);
-  };
+
   SAT_CHECK(hipEventRecord(b->ev0, b->stream));
-  if (ng) {
-    SAT_CHECK(hipMemsetAsync(b->d_counter, 0, sizeof(int) * 2 * ng, b->stream));
-    SAT_CHECK(hipMemsetAsync(bd.fb_count, 0, sizeof(int) * ng, b->stream));
-  }
-  for (size_t gi = 0; gi < ng; gi++) {
-    BatchDev a = bd;
-    a.counter = b->d_counter + gi;
-    a.fb_list = bd.fb_list + b->group_first[gi];
-    const int first = b->group_first[gi], count = b->group_count[gi];
-    if (mode == SAT_RUN_BATCHED) {
-      const int nblk = std::min(count, b->nslots);
-      SAT_CHECK(launch(a, b->group_C[gi], dim3(nblk), first, count, (int)gi, nullptr, nullptr));
-    } else {
-      for (int i = 0; i < count; i++) {
-        SAT_CHECK(hipMemsetAsync(a.counter, 0, sizeof(int), b->stream));
-        SAT_CHECK(launch(a, b->group_C[gi], dim3(1), first + i, 1, (int)gi, nullptr, nullptr));
+  SAT_CHECK(hipMemsetAsync(b->d_counter, 0, sizeof(int) * 5 * ne1, b->stream));
+  SAT_CHECK(hipMemsetAsync(b->d_skip, 0, ne1, b->stream));
+  SAT_CHECK(hipMemsetAsync(b->d_iters, 0, sizeof(int) * 2 * ne1, b->stream));
+  SAT_CHECK(hipMemcpyAsync(bd.flags, b->d_base_flags, sizeof(int) * ne1, hipMemcpyDeviceToDevice, b->stream));
+
+  if (mode == SAT_RUN_PER_SOLVE) {
+    // one launch per pair of the fused kernel (the plain per-solve path)
+    bd.ws = b->d_ws;
+    bd.ws_slot = b->sg.slot;
+    bd.ws_seg = b->sg.seg;
+    bd.ws_plain = b->sg.plain;
+    for (size_t gi = 0; gi < ng; gi++)
+      for (int i = 0; i < b->group_count[gi]; i++) {
+        const int first = b->group_first[gi] + i;
+        BatchDev a = bd;
+        a.counter = solo_heads + first;
+        a.fb_list = bd.fb_list + b->group_first[gi];
+        if (!f32)
+          SAT_CHECK((dispatch_C1<PairK64, 0>(b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi,
+                                              (const int*)nullptr, (const int*)nullptr)));
+        else
+          SAT_CHECK((dispatch_C1<PairK32, 0>(b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi,
+                                              (const int*)nullptr, (const int*)nullptr)));
+      }
+  } else {
+    PhaseDev pd{};
+    pd.grids = b->d_grids;
+    pd.ev = b->d_evx;
+    pd.nevent = ne;
+    pd.evs = b->d_evs;
+    pd.coltab = b->d_coltab;
+    const size_t es = elem_size(b->precision);
+    pd.t0 = b->d_t0;
+    pd.t0x = b->d_t0 + b->seg_total * es;
+    pd.t0y = b->d_t0 + 2 * b->seg_total * es;
+    pd.T0d = b->d_T0d;
+    pd.T = b->d_T;
+    for (int i = 0; i < 6; i++) pd.co[i] = (b->d_co && i < (f32 ? 5 : 6)) ? b->d_co + i * b->seg_total * es : nullptr;
+    pd.ta_out = b->d_ta;
+    pd.rec_lon = bd.rec_lon;
+    pd.rec_lat = bd.rec_lat;
+    pd.tt_obs = bd.tt_obs;
+    pd.weight = bd.weight;
+    pd.rec_ev = b->d_rec_ev;
+    pd.syn = bd.syn;
+    pd.adjsrc = bd.adjsrc;
+    pd.rcell = b->d_rcell;
+    pd.rwt = b->d_rwt;
+    pd.chi_ev = b->d_chi_ev;
+    pd.iters_f = bd.iters_f;
+    pd.iters_a = bd.iters_a;
+    pd.flags = bd.flags;
+    pd.conv = b->d_conv;
+    pd.skip = b->d_skip;
+    pd.fb_list = b->d_fb;
+    pd.fb_count = fb_count;
+    pd.f32_band = b->f32_band;
+    pd.slots = b->d_slots;
+    pd.slot_bytes = b->slot_bytes;
+    pd.slot_plane = b->slot_plane;
+    pd.order = b->d_order;
+    pd.fwd_maxiter = kFwdMaxIter;
+    pd.adj_maxiter = kAdjMaxIter;
+    const int max_nodes = (int)b->max_seg;
+    if (ne) {
+      if (!f32) hipLaunchKernelGGL(trig_kernel<SAT_PREC_F64>, dim3((b->max_nx + 63) / 64, ne), dim3(64), 0, b->stream, pd);
+      else hipLaunchKernelGGL(trig_kernel<SAT_PREC_F32>, dim3((b->max_nx + 63) / 64, ne), dim3(64), 0, b->stream, pd);
+      SAT_CHECK(hipGetLastError());
+      if (!f32) hipLaunchKernelGGL(init_kernel<SAT_PREC_F64>, dim3((max_nodes + 255) / 256, ne), dim3(256), 0, b->stream, pd);
+      else hipLaunchKernelGGL(init_kernel<SAT_PREC_F32>, dim3((max_nodes + 255) / 256, ne), dim3(256), 0, b->stream, pd);
+      SAT_CHECK(hipGetLastError());
+    }
+    for (size_t gi = 0; gi < ng; gi++) {
+      PhaseDev a = pd;
+      a.counter = fwd_heads + gi;
+      a.fb_list = pd.fb_list + b->group_first[gi];
+      const int nblk = std::min(b->group_count[gi], b->nslots_phase);
+      const int C = b->group_C[gi];
+      hipError_t err;
+      if (!f32) err = dispatch_C1<FwdSweepGenK, SAT_PREC_F64>(C, dim3(nblk), b->stream, a, b->group_first[gi], b->group_count[gi], (int)gi);
+      else if (b->group_iso[gi]) err = dispatch_C1<FwdSweepIsoK, SAT_PREC_F32>(C, dim3(nblk), b->stream, a, b->group_first[gi], b->group_count[gi], (int)gi);
+      else err = dispatch_C1<FwdSweepGenK, SAT_PREC_F32>(C, dim3(nblk), b->stream, a, b->group_first[gi], b->group_count[gi], (int)gi);
+      SAT_CHECK(err);
+    }
+    if (b->nrec) {
+      const int blocks = (int)std::min<size_t>((b->nrec + 255) / 256, 4096);
+      hipLaunchKernelGGL(recv_kernel, dim3(blocks), dim3(256), 0, b->stream, pd, b->nrec);
+      SAT_CHECK(hipGetLastError());
+    }
+    if (b->calc_adj && ne) {
+      if (!f32) hipLaunchKernelGGL(adj_setup_kernel<SAT_PREC_F64>, dim3((max_nodes + 255) / 256, ne), dim3(256), 0, b->stream, pd);
+      else hipLaunchKernelGGL(adj_setup_kernel<SAT_PREC_F32>, dim3((max_nodes + 255) / 256, ne), dim3(256), 0, b->stream, pd);
+      SAT_CHECK(hipGetLastError());
+      if (!f32) hipLaunchKernelGGL(delta_kernel<SAT_PREC_F64>, dim3((ne + 63) / 64), dim3(64), 0, b->stream, pd);
+      else hipLaunchKernelGGL(delta_kernel<SAT_PREC_F32>, dim3((ne + 63) / 64), dim3(64), 0, b->stream, pd);
+      SAT_CHECK(hipGetLastError());
+      for (size_t gi = 0; gi < ng; gi++) {
+        PhaseDev a = pd;
+        a.counter = adj_heads + gi;
+        const int nblk = std::min(b->group_count[gi], b->nslots_phase);
+        hipError_t err;
+        if (!f32) err = dispatch_C1<AdjSweepK, SAT_PREC_F64>(b->group_C[gi], dim3(nblk), b->stream, a, b->group_first[gi], b->group_count[gi]);
+        else err = dispatch_C1<AdjSweepK, SAT_PREC_F32>(b->group_C[gi], dim3(nblk), b->stream, a, b->group_first[gi], b->group_count[gi]);
+        SAT_CHECK(err);
       }
     }
   }
@@ -1061,25 +1324,23 @@ int sat_batch_run(sat_batch* b, int mode) {
       a.ws_slot = b->sg64.slot;
       a.ws_seg = b->sg64.seg;
       a.ws_plain = b->sg64.plain;
-      a.counter = b->d_counter + ng + gi;
+      a.counter = res_heads + gi;
       a.f32_band = 0.0;
       const int* list = bd.fb_list + b->group_first[gi];
       const int nblk = std::min(b->group_count[gi], b->nslots64);
-      SAT_CHECK(launch(a, b->group_C[gi], dim3(nblk), 0, 0, (int)gi, list, bd.fb_count + gi));
+      SAT_CHECK((dispatch_C1<PairK64to32, 0>(b->group_C[gi], dim3(nblk), b->stream, a, 0, 0, (int)gi, list,
+                                             (const int*)(fb_count + gi))));
     }
   }
   SAT_CHECK(hipEventRecord(b->ev1, b->stream));
   if (b->calc_adj && b->K_total) {
     const int threads = 256;
     const int blocks = (int)std::min<size_t>((b->K_total + threads - 1) / threads, 65535);
-    if (b->precision == SAT_PREC_F64)
-      hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks), dim3(threads), 0, b->stream, b->rd);
-    else
-      hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks), dim3(threads), 0, b->stream, b->rd);
+    if (!f32) hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks), dim3(threads), 0, b->stream, b->rd);
+    else hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks), dim3(threads), 0, b->stream, b->rd);
     SAT_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(chi_kernel, dim3(1), dim3(1), 0, b->stream, b->d_chi_ev, b->calc_adj ? b->nevent : 0,
-                     b->d_chi);
+  hipLaunchKernelGGL(chi_kernel, dim3(1), dim3(1), 0, b->stream, b->d_chi_ev, b->calc_adj ? ne : 0, b->d_chi);
   SAT_CHECK(hipGetLastError());
   SAT_CHECK(hipEventRecord(b->ev2, b->stream));
   return SAT_OK;
@@ -1141,7 +1402,7 @@ int sat_batch_conv(sat_batch* b, double* linf_last, double* linf_prev, int* n_re
     *n_resolved = 0;
     if (b->precision == SAT_PREC_F32 && !b->group_C.empty()) {
       std::vector<int> cnt(b->group_C.size());
-      SAT_CHECK(hipMemcpy(cnt.data(), b->d_fb + std::max(ne, 1), sizeof(int) * cnt.size(), hipMemcpyDeviceToHost));
+      SAT_CHECK(hipMemcpy(cnt.data(), b->d_counter, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost));
       for (int c : cnt) *n_resolved += c;
     }
   }

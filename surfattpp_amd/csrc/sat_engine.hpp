@@ -17,60 +17,77 @@
 // `tin` and writes `tout`, so every update sees the same values as the
 // sequential loop and the result is identical iterate for iterate.
 //
-// Rows flow through a 3-deep register pipeline (cur, nxt, nn): the row two
-// steps ahead is loaded while the current one is computed.
+// Register pipeline.  A row is split into a head (swept value u and the
+// neighbour auxiliary aux) and a body (the node's own coefficients).  At step
+// t the lane computes row r from cur (head + body of row r) and the head of
+// row r+1, while the body of row r+1 and the head of row r+2 are in flight.
 #pragma once
 #include "sat_device.hpp"
 
 namespace sat {
 
 // Policy P provides:
-//   using V;                                  swept value type (float / double)
-//   using A;                                  neighbour auxiliary value type (T0 of the forward)
-//   template <int C> struct Row               per-row register data; has V u[C], A aux[C], V st[C]
-//   template <int C> static void load(const Ctx&, Row<C>&, size_t base, const V* tin, const V* tst)
-//   static bool fixed(const Ctx&, int ix, int j)     node keeps its value
-//   template <int C> static V update(const Ctx&, const Row<C>&, int k, int ix, int j,
+//   using V;  using A;  static constexpr bool kAux;
+//   template <int C> struct Head { V u[C]; A aux[C]; };
+//   template <int C> struct Body { ...; V st[C]; };      (st: iterate at the round start)
+//   template <int C> static void load_head(const Ctx&, Head<C>&, int base, int j, const V* tin)
+//   template <int C> static void load_body(const Ctx&, Body<C>&, int base, int j, const V* tst)
+//   static bool fixed(const Ctx&, int ix, int j)
+//   template <int C> static V update(const Ctx&, const Head<C>&, const Body<C>&, int k, int j,
 //                                    V uW, A aW, bool hW, V uE, A aE, bool hE,
 //                                    V uS, A aS, bool hS, V uN, A aN, bool hN)
-//   template <int C> static double norm(const Row<C>&, int k, V newv)   convergence term
+//   template <int C> static double norm(const Head<C>&, const Body<C>&, int k, V newv)
 template <class Pol, int C, int XD, int YD, bool NORM>
 __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typename Pol::V* __restrict__ tin,
                                       typename Pol::V* __restrict__ tout,
                                       const typename Pol::V* __restrict__ tst, double& lmax, double& lsum) {
   using V = typename Pol::V;
   using A = typename Pol::A;
-  using Row = typename Pol::template Row<C>;
+  using Head = typename Pol::template Head<C>;
+  using Body = typename Pol::template Body<C>;
   const int lane = threadIdx.x;
   const int La = cx.g->La, nx = cx.g->nx, ny = cx.g->ny, P = cx.g->P;
   const bool lane_ok = lane < La;
   const int lag = (XD > 0) ? lane : (La - 1 - lane);
   const int nsteps = ny + La - 1;
-
-  auto load = [&](Row& R, int r) {
-    if (lane_ok && r >= 0 && r < ny) {
-      const int j = (YD > 0) ? r : (ny - 1 - r);
-      Pol::template load<C>(cx, R, (size_t)j * P + (size_t)lane * C, tin, NORM ? tst : nullptr);
+  auto rowj = [&](int r) { return (YD > 0) ? r : (ny - 1 - r); };
+  auto in_range = [&](int r) { return lane_ok && r >= 0 && r < ny; };
+  auto load_head = [&](Head& H, int r) {
+    if (in_range(r)) {
+      const int j = rowj(r);
+      Pol::template load_head<C>(cx, H, j * P + lane * C, j, tin);
+    }
+  };
+  auto load_body = [&](Body& B, int r) {
+    if (in_range(r)) {
+      const int j = rowj(r);
+      Pol::template load_body<C>(cx, B, j * P + lane * C, j, NORM ? tst : nullptr);
     }
   };
 
-  Row cur, nxt, nn;
+  Head cur_h, nxt_h, nn_h;
+  Body cur_b, nxt_b;
   V prev_u[C];
   A prev_a[C];
 #pragma unroll
   for (int k = 0; k < C; k++) { prev_u[k] = V(0); prev_a[k] = A(0); }
-  load(cur, -lag);
-  load(nxt, 1 - lag);
+  load_head(cur_h, -lag);
+  load_body(cur_b, -lag);
+  load_head(nxt_h, 1 - lag);
 
   for (int t = 0; t < nsteps; ++t) {
     const int r = t - lag;
-    load(nn, r + 2);
+    load_body(nxt_b, r + 1);
+    load_head(nn_h, r + 2);
     const V nw_u = __shfl((XD > 0) ? prev_u[C - 1] : prev_u[0], lane - XD);
-    const A nw_a = Pol::kAux ? __shfl((XD > 0) ? prev_a[C - 1] : prev_a[0], lane - XD) : A(0);
-    const V od_u = __shfl((XD > 0) ? nxt.u[0] : nxt.u[C - 1], lane + XD);
-    const A od_a = Pol::kAux ? __shfl((XD > 0) ? nxt.aux[0] : nxt.aux[C - 1], lane + XD) : A(0);
-    if (lane_ok && r >= 0 && r < ny) {
-      const int j = (YD > 0) ? r : (ny - 1 - r);
+    const V od_u = __shfl((XD > 0) ? nxt_h.u[0] : nxt_h.u[C - 1], lane + XD);
+    A nw_a = A(0), od_a = A(0);
+    if constexpr (Pol::kAux) {
+      nw_a = __shfl((XD > 0) ? prev_a[C - 1] : prev_a[0], lane - XD);
+      od_a = __shfl((XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1], lane + XD);
+    }
+    if (in_range(r)) {
+      const int j = rowj(r);
       const bool hS = j > 0, hN = j < ny - 1;
       V newv[C];
 #pragma unroll
@@ -78,39 +95,40 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
         const int k = (XD > 0) ? kk : (C - 1 - kk);
         const int ix = lane * C + k;
         V uW, uE;
-        A aW, aE;
+        A aW = A(0), aE = A(0);
         if (XD > 0) {
-          if (k > 0) { uW = newv[k - 1]; aW = cur.aux[k - 1]; } else { uW = nw_u; aW = nw_a; }
-          if (k < C - 1) { uE = cur.u[k + 1]; aE = cur.aux[k + 1]; } else { uE = od_u; aE = od_a; }
+          if (k > 0) { uW = newv[k - 1]; aW = cur_h.aux[k - 1]; } else { uW = nw_u; aW = nw_a; }
+          if (k < C - 1) { uE = cur_h.u[k + 1]; aE = cur_h.aux[k + 1]; } else { uE = od_u; aE = od_a; }
         } else {
-          if (k < C - 1) { uE = newv[k + 1]; aE = cur.aux[k + 1]; } else { uE = nw_u; aE = nw_a; }
-          if (k > 0) { uW = cur.u[k - 1]; aW = cur.aux[k - 1]; } else { uW = od_u; aW = od_a; }
+          if (k < C - 1) { uE = newv[k + 1]; aE = cur_h.aux[k + 1]; } else { uE = nw_u; aE = nw_a; }
+          if (k > 0) { uW = cur_h.u[k - 1]; aW = cur_h.aux[k - 1]; } else { uW = od_u; aW = od_a; }
         }
         V uS, uN;
         A aS, aN;
-        if (YD > 0) { uS = prev_u[k]; aS = prev_a[k]; uN = nxt.u[k]; aN = nxt.aux[k]; }
-        else { uN = prev_u[k]; aN = prev_a[k]; uS = nxt.u[k]; aS = nxt.aux[k]; }
+        if (YD > 0) { uS = prev_u[k]; aS = prev_a[k]; uN = nxt_h.u[k]; aN = nxt_h.aux[k]; }
+        else { uN = prev_u[k]; aN = prev_a[k]; uS = nxt_h.u[k]; aS = nxt_h.aux[k]; }
         const bool valid = ix < nx;
         if (!valid || Pol::fixed(cx, ix, j)) {
-          newv[k] = cur.u[k];
+          newv[k] = cur_h.u[k];
         } else {
-          newv[k] = Pol::template update<C>(cx, cur, k, ix, j, uW, aW, ix > 0, uE, aE, ix < nx - 1,
+          newv[k] = Pol::template update<C>(cx, cur_h, cur_b, k, j, uW, aW, ix > 0, uE, aE, ix < nx - 1,
                                             uS, aS, hS, uN, aN, hN);
         }
         if (NORM && valid) {
-          const double d = Pol::template norm<C>(cur, k, newv[k]);
+          const double d = Pol::template norm<C>(cur_h, cur_b, k, newv[k]);
           lsum += d;
           lmax = dmax(lmax, d);
         }
       }
-      const size_t base = (size_t)j * P + (size_t)lane * C;
+      const int base = j * P + lane * C;
 #pragma unroll
       for (int k = 0; k < C; k++) tout[base + k] = newv[k];
 #pragma unroll
-      for (int k = 0; k < C; k++) { prev_u[k] = newv[k]; prev_a[k] = cur.aux[k]; }
+      for (int k = 0; k < C; k++) { prev_u[k] = newv[k]; prev_a[k] = cur_h.aux[k]; }
     }
-    cur = nxt;
-    nxt = nn;
+    cur_h = nxt_h;
+    cur_b = nxt_b;
+    nxt_h = nn_h;
   }
 }
 
@@ -122,7 +140,7 @@ struct IterOut {
 };
 
 // Iterate 4-sweep rounds until L1 < tol and Linf < tol (src/eikonal_solver.cpp:146-314,
-// :471-509).  Three rotating planes: b[0] holds the iterate at the start of a round.
+// :471-509).  Three rotating planes: B0 holds the iterate at the start of a round.
 // Returns the plane holding the result.
 template <class Pol, int C>
 __device__ typename Pol::V* iterate(const typename Pol::Ctx& cx, typename Pol::V* b0, typename Pol::V* b1,

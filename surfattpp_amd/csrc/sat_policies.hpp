@@ -44,38 +44,39 @@ struct FwdF64 {
     FixedBox fb;
     double dx, dy;
   };
+  template <int C> struct Head { double u[C], aux[C]; };
+  template <int C> struct Body { double t0x[C], t0y[C], a[C], b[C], c[C], f[C], st[C]; };
   template <int C>
-  struct Row {
-    double u[C], aux[C], t0x[C], t0y[C], a[C], b[C], c[C], f[C], st[C];
-  };
+  __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
+#pragma unroll
+    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[base + k]; }
+  }
   template <int C>
-  __device__ static void load(const Ctx& cx, Row<C>& R, size_t base, const double* tin, const double* tst) {
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      R.u[k] = tin[base + k];
-      R.aux[k] = cx.t0[base + k];
-      R.t0x[k] = cx.t0x[base + k];
-      R.t0y[k] = cx.t0y[base + k];
-      R.a[k] = cx.a[base + k];
-      R.b[k] = cx.b[base + k];
-      R.c[k] = cx.c[base + k];
-      R.f[k] = cx.f[base + k];
-      R.st[k] = tst ? tst[base + k] : 0.0;
+      B.t0x[k] = cx.t0x[base + k];
+      B.t0y[k] = cx.t0y[base + k];
+      B.a[k] = cx.a[base + k];
+      B.b[k] = cx.b[base + k];
+      B.c[k] = cx.c[base + k];
+      B.f[k] = cx.f[base + k];
+      B.st[k] = tst ? tst[base + k] : 0.0;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
     return j >= cx.fb.y0 && j <= cx.fb.y1 && ix >= cx.fb.x0 && ix <= cx.fb.x1;
   }
   template <int C>
-  __device__ static double update(const Ctx& cx, const Row<C>& r, int k, int, int, double uW, double aW, bool hW,
-                                  double uE, double aE, bool hE, double uS, double aS, bool hS, double uN,
-                                  double aN, bool hN) {
-    return fwd_update_f64(r.u[k], r.aux[k], r.t0x[k], r.t0y[k], r.a[k], r.b[k], r.c[k], r.f[k], cx.dx, cx.dy, hW,
+  __device__ static double update(const Ctx& cx, const Head<C>& h, const Body<C>& r, int k, int, double uW,
+                                  double aW, bool hW, double uE, double aE, bool hE, double uS, double aS, bool hS,
+                                  double uN, double aN, bool hN) {
+    return fwd_update_f64(h.u[k], h.aux[k], r.t0x[k], r.t0y[k], r.a[k], r.b[k], r.c[k], r.f[k], cx.dx, cx.dy, hW,
                           uW, aW, hE, uE, aE, hS, uS, aS, hN, uN, aN);
   }
   template <int C>
-  __device__ static double norm(const Row<C>& r, int k, double v) {
-    return fabs(v - r.st[k]) * r.aux[k];  // :295
+  __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, double v) {
+    return fabs(v - r.st[k]) * h.aux[k];  // :295
   }
 };
 
@@ -88,35 +89,36 @@ struct AdjF64 {
     const double *a1p, *a2m, *b1p, *b2m, *dl, *d;
     double dx, dy;
   };
+  template <int C> struct Head { double u[C], aux[C]; };
+  template <int C> struct Body { double a1p[C], a2m[C], b1p[C], b2m[C], dl[C], d[C], st[C]; };
   template <int C>
-  struct Row {
-    double u[C], aux[C], a1p[C], a2m[C], b1p[C], b2m[C], dl[C], d[C], st[C];
-  };
+  __device__ static void load_head(const Ctx&, Head<C>& H, int base, int, const double* tin) {
+#pragma unroll
+    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = 0.0; }
+  }
   template <int C>
-  __device__ static void load(const Ctx& cx, Row<C>& R, size_t base, const double* tin, const double* tst) {
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      R.u[k] = tin[base + k];
-      R.aux[k] = 0.0;
-      R.a1p[k] = cx.a1p[base + k];
-      R.a2m[k] = cx.a2m[base + k];
-      R.b1p[k] = cx.b1p[base + k];
-      R.b2m[k] = cx.b2m[base + k];
-      R.dl[k] = cx.dl[base + k];
-      R.d[k] = cx.d[base + k];
-      R.st[k] = tst ? tst[base + k] : 0.0;
+      B.a1p[k] = cx.a1p[base + k];
+      B.a2m[k] = cx.a2m[base + k];
+      B.b1p[k] = cx.b1p[base + k];
+      B.b2m[k] = cx.b2m[base + k];
+      B.dl[k] = cx.dl[base + k];
+      B.d[k] = cx.d[base + k];
+      B.st[k] = tst ? tst[base + k] : 0.0;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
     return ix == 0 || ix >= cx.g->nx - 1 || j == 0 || j == cx.g->ny - 1;
   }
   template <int C>
-  __device__ static double update(const Ctx& cx, const Row<C>& r, int k, int, int, double uW, double, bool,
-                                  double uE, double, bool, double uS, double, bool, double uN, double, bool) {
+  __device__ static double update(const Ctx& cx, const Head<C>&, const Body<C>& r, int k, int, double uW, double,
+                                  bool, double uE, double, bool, double uS, double, bool, double uN, double, bool) {
     return adj_update_f64(r.a1p[k], r.a2m[k], r.b1p[k], r.b2m[k], r.dl[k], r.d[k], uW, uE, uS, uN, cx.dx, cx.dy);
   }
   template <int C>
-  __device__ static double norm(const Row<C>& r, int k, double v) {
+  __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, double v) {
     return fabs(v - r.st[k]);  // :501
   }
 };
@@ -133,7 +135,10 @@ struct AdjF64 {
 //     quadratic formula; accepted if -sx*(a*Tx - c*Ty) >= 0 and -sy*(b*Ty - c*Tx) >= 0.
 //   one-point (W): tau = tau_W + w, w = (+-dis - t0x*tau_W)/(t0x + h), accepted if
 //     tau*T0 >= tau_W*T0_W and tau > tau_W/2 (E, S, N alike).
-// Neighbour sigmas and the result are FP64; w is computed in FP32.
+// Neighbour sigmas and the result are FP64; w is computed in FP32.  ISO: the
+// isotropic metric (c == 0, a and b constant along a row), dsx = f/sqrt(a),
+// dsy = f/sqrt(b).
+template <bool ISO>
 __device__ __forceinline__ double fwd_update_f32(double u_self, float t0, float t0x, float t0y, float a, float b,
                                                  float c, float f2, float dsx, float dsy, float h, float kk,
                                                  bool hW, double uW, float T0W, bool hE, double uE, float T0E,
@@ -155,9 +160,16 @@ __device__ __forceinline__ double fwd_update_f32(double u_self, float t0, float 
     const float tr = (float)(ur + 1.0);
     const float Tx0 = t0x * tr + sx * h * (float)(uX - ur);
     const float Ty0 = t0y * tr + sy * kk * (float)(uY - ur);
-    const float Aq = a * px * px + b * py * py - 2.0f * c * px * py;
-    const float Bh = a * px * Tx0 + b * py * Ty0 - c * (px * Ty0 + py * Tx0);
-    const float Cq = a * Tx0 * Tx0 + b * Ty0 * Ty0 - 2.0f * c * Tx0 * Ty0 - f2;
+    float Aq, Bh, Cq;
+    if constexpr (ISO) {
+      Aq = a * px * px + b * py * py;
+      Bh = a * px * Tx0 + b * py * Ty0;
+      Cq = a * Tx0 * Tx0 + b * Ty0 * Ty0 - f2;
+    } else {
+      Aq = a * px * px + b * py * py - 2.0f * c * px * py;
+      Bh = a * px * Tx0 + b * py * Ty0 - c * (px * Ty0 + py * Tx0);
+      Cq = a * Tx0 * Tx0 + b * Ty0 * Ty0 - 2.0f * c * Tx0 * Ty0 - f2;
+    }
     const float D = Bh * Bh - Aq * Cq;
     if (D < 0.0f || fabsf(Aq) < 1e-30f) continue;
     const float sq = __builtin_amdgcn_sqrtf(D);
@@ -169,8 +181,8 @@ __device__ __forceinline__ double fwd_update_f32(double u_self, float t0, float 
       const float w = g ? w1 : w0;
       const float Tx = Tx0 + px * w;
       const float Ty = Ty0 + py * w;
-      const float cxv = a * Tx - c * Ty;
-      const float cyv = b * Ty - c * Tx;
+      const float cxv = ISO ? a * Tx : a * Tx - c * Ty;
+      const float cyv = ISO ? b * Ty : b * Ty - c * Tx;
       if (-sx * cxv >= 0.0f && -sy * cyv >= 0.0f) {
         const double cand = ur + (double)w;
         un = (cand < un) ? cand : un;
@@ -208,46 +220,97 @@ struct FwdF32 {
   static constexpr bool kAux = true;
   struct Ctx {
     const GridDev* g;
-    const float *t0, *t0x, *t0y, *a, *b, *c, *f2, *dsx, *dsy;
+    const float *t0, *t0x, *t0y, *a, *b, *c, *f, *dsx, *dsy;
     FixedBox fb;
     float idx, idy;  // 1/dx, 1/dy (radians)
   };
+  template <int C> struct Head { double u[C]; float aux[C]; };
+  template <int C> struct Body { double st[C]; float t0x[C], t0y[C], a[C], b[C], c[C], f[C], dsx[C], dsy[C]; };
   template <int C>
-  struct Row {
-    double u[C], st[C];
-    float aux[C], t0x[C], t0y[C], a[C], b[C], c[C], f2[C], dsx[C], dsy[C];
-  };
+  __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
+#pragma unroll
+    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[base + k]; }
+  }
   template <int C>
-  __device__ static void load(const Ctx& cx, Row<C>& R, size_t base, const double* tin, const double* tst) {
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      R.u[k] = tin[base + k];
-      R.aux[k] = cx.t0[base + k];
-      R.t0x[k] = cx.t0x[base + k];
-      R.t0y[k] = cx.t0y[base + k];
-      R.a[k] = cx.a[base + k];
-      R.b[k] = cx.b[base + k];
-      R.c[k] = cx.c[base + k];
-      R.f2[k] = cx.f2[base + k];
-      R.dsx[k] = cx.dsx[base + k];
-      R.dsy[k] = cx.dsy[base + k];
-      R.st[k] = tst ? tst[base + k] : 0.0;
+      B.t0x[k] = cx.t0x[base + k];
+      B.t0y[k] = cx.t0y[base + k];
+      B.a[k] = cx.a[base + k];
+      B.b[k] = cx.b[base + k];
+      B.c[k] = cx.c[base + k];
+      B.f[k] = cx.f[base + k];
+      B.dsx[k] = cx.dsx[base + k];
+      B.dsy[k] = cx.dsy[base + k];
+      B.st[k] = tst ? tst[base + k] : 0.0;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
     return j >= cx.fb.y0 && j <= cx.fb.y1 && ix >= cx.fb.x0 && ix <= cx.fb.x1;
   }
   template <int C>
-  __device__ static double update(const Ctx& cx, const Row<C>& r, int k, int, int, double uW, float aW, bool hW,
-                                  double uE, float aE, bool hE, double uS, float aS, bool hS, double uN, float aN,
-                                  bool hN) {
-    const float t0 = r.aux[k];
-    return fwd_update_f32(r.u[k], t0, r.t0x[k], r.t0y[k], r.a[k], r.b[k], r.c[k], r.f2[k], r.dsx[k], r.dsy[k],
-                          t0 * cx.idx, t0 * cx.idy, hW, uW, aW, hE, uE, aE, hS, uS, aS, hN, uN, aN);
+  __device__ static double update(const Ctx& cx, const Head<C>& hd, const Body<C>& r, int k, int, double uW,
+                                  float aW, bool hW, double uE, float aE, bool hE, double uS, float aS, bool hS,
+                                  double uN, float aN, bool hN) {
+    const float t0 = hd.aux[k];
+    return fwd_update_f32<false>(hd.u[k], t0, r.t0x[k], r.t0y[k], r.a[k], r.b[k], r.c[k], r.f[k] * r.f[k], r.dsx[k],
+                                 r.dsy[k], t0 * cx.idx, t0 * cx.idy, hW, uW, aW, hE, uE, aE, hS, uS, aS, hN, uN,
+                                 aN);
   }
   template <int C>
-  __device__ static double norm(const Row<C>& r, int k, double v) {
-    return fabs(v - r.st[k]) * (double)r.aux[k];
+  __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, double v) {
+    return fabs(v - r.st[k]) * (double)h.aux[k];
+  }
+};
+
+// Isotropic metric (spha = sphb = 1, sphc = 0 on every node): a, b per row from a
+// small table {a, b, 1/sqrt(a), 1/sqrt(b)}, only the slowness per node.
+struct FwdF32Iso {
+  using V = double;
+  using A = float;
+  static constexpr bool kAux = true;
+  struct Ctx {
+    const GridDev* g;
+    const float *t0, *t0x, *t0y, *f;
+    const float4* rowtab;
+    FixedBox fb;
+    float idx, idy;
+  };
+  template <int C> struct Head { double u[C]; float aux[C]; };
+  template <int C> struct Body { double st[C]; float t0x[C], t0y[C], f[C]; float4 row; };
+  template <int C>
+  __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
+#pragma unroll
+    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[base + k]; }
+  }
+  template <int C>
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int j, const double* tst) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      B.t0x[k] = cx.t0x[base + k];
+      B.t0y[k] = cx.t0y[base + k];
+      B.f[k] = cx.f[base + k];
+      B.st[k] = tst ? tst[base + k] : 0.0;
+    }
+    B.row = cx.rowtab[j];
+  }
+  __device__ static bool fixed(const Ctx& cx, int ix, int j) {
+    return j >= cx.fb.y0 && j <= cx.fb.y1 && ix >= cx.fb.x0 && ix <= cx.fb.x1;
+  }
+  template <int C>
+  __device__ static double update(const Ctx& cx, const Head<C>& hd, const Body<C>& r, int k, int, double uW,
+                                  float aW, bool hW, double uE, float aE, bool hE, double uS, float aS, bool hS,
+                                  double uN, float aN, bool hN) {
+    const float t0 = hd.aux[k];
+    const float f = r.f[k];
+    return fwd_update_f32<true>(hd.u[k], t0, r.t0x[k], r.t0y[k], r.row.x, r.row.y, 0.0f, f * f, f * r.row.z,
+                                f * r.row.w, t0 * cx.idx, t0 * cx.idy, hW, uW, aW, hE, uE, aE, hS, uS, aS, hN, uN,
+                                aN);
+  }
+  template <int C>
+  __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, double v) {
+    return fabs(v - r.st[k]) * (double)h.aux[k];
   }
 };
 
@@ -259,35 +322,36 @@ struct AdjF32 {
     const GridDev* g;
     const float *cW, *cE, *cS, *cN, *cD;
   };
+  template <int C> struct Head { float u[C], aux[C]; };
+  template <int C> struct Body { float cW[C], cE[C], cS[C], cN[C], cD[C], st[C]; };
   template <int C>
-  struct Row {
-    float u[C], aux[C], cW[C], cE[C], cS[C], cN[C], cD[C], st[C];
-  };
+  __device__ static void load_head(const Ctx&, Head<C>& H, int base, int, const float* tin) {
+#pragma unroll
+    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = 0.0f; }
+  }
   template <int C>
-  __device__ static void load(const Ctx& cx, Row<C>& R, size_t base, const float* tin, const float* tst) {
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const float* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      R.u[k] = tin[base + k];
-      R.aux[k] = 0.0f;
-      R.cW[k] = cx.cW[base + k];
-      R.cE[k] = cx.cE[base + k];
-      R.cS[k] = cx.cS[base + k];
-      R.cN[k] = cx.cN[base + k];
-      R.cD[k] = cx.cD[base + k];
-      R.st[k] = tst ? tst[base + k] : 0.0f;
+      B.cW[k] = cx.cW[base + k];
+      B.cE[k] = cx.cE[base + k];
+      B.cS[k] = cx.cS[base + k];
+      B.cN[k] = cx.cN[base + k];
+      B.cD[k] = cx.cD[base + k];
+      B.st[k] = tst ? tst[base + k] : 0.0f;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
     return ix == 0 || ix >= cx.g->nx - 1 || j == 0 || j == cx.g->ny - 1;
   }
   template <int C>
-  __device__ static float update(const Ctx&, const Row<C>& r, int k, int, int, float uW, float, bool, float uE,
-                                 float, bool, float uS, float, bool, float uN, float, bool) {
+  __device__ static float update(const Ctx&, const Head<C>&, const Body<C>& r, int k, int, float uW, float, bool,
+                                 float uE, float, bool, float uS, float, bool, float uN, float, bool) {
 #pragma clang fp contract(fast)
     return r.cD[k] + r.cW[k] * uW + r.cE[k] * uE + r.cS[k] * uS + r.cN[k] * uN;
   }
   template <int C>
-  __device__ static double norm(const Row<C>& r, int k, float v) {
+  __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, float v) {
     return fabs((double)v - (double)r.st[k]);
   }
 };
