@@ -146,8 +146,14 @@ int sat_batch_set_f32_band(sat_batch* b, double band);
  * that produced the result), and the number of FP64 re-solves of the last run. */
 int sat_batch_conv(sat_batch* b, double* linf_last, double* linf_prev, int* n_resolved);
 /* Duration in ms of the last sat_batch_run, measured with HIP events on the
- * batch's stream; also the ms of its main solve kernel launches alone. */
+ * batch's streams; solve_ms: from the start to the end of the adjoint sweeps. */
 int sat_batch_last_ms(sat_batch* b, double* total_ms, double* solve_ms);
+/* Per-phase ms of the last SAT_RUN_BATCHED run (HIP events on the streams the
+ * kernels run on), first min(n, 7) of: [0] analytic factor T0, [1] forward
+ * sweeps, [2] receivers + adjoint set-up + delta, [3] adjoint sweeps, [4] FP64
+ * re-solves (second stream, concurrent with [2]-[3]), [5] kernel reduction +
+ * chi, [6] total.  Timing aid with no counterpart in the reference. */
+int sat_batch_phase_ms(sat_batch* b, double* ms, int n);
 int sat_batch_destroy(sat_batch* b);
 
 /* Library identity: device name, build arch; returns SAT_EDEVICE without a GPU. */

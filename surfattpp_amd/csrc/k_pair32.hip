@@ -1,0 +1,5 @@
+// k_pair32.hip -- FP32 instantiations of the per-pair and per-solve kernels (sat_fused.hpp).
+#include "sat_launch_impl.hpp"
+
+using namespace sat;
+SAT_INSTANTIATE_PAIR(SAT_PREC_F32)

@@ -31,6 +31,7 @@ constexpr int kAdjMaxIter = 1000;                               // :339
 constexpr double kAdjTol = 1e-9;                                // :338
 constexpr double kAdjEps = 1e-14;                               // :337
 constexpr int kWave = 64;
+constexpr int kMaxC = 8;  // columns per lane: nx <= 512
 
 constexpr int FLAG_NOT_CONVERGED = 1;
 constexpr int FLAG_SRC_ON_BOUNDARY = 2;
@@ -65,10 +66,47 @@ struct GridDev {
   const double2* trig_yy;  // per row: sin(yy), cos(yy), correctly rounded
   int iso;                 // spha = sphb = 1, sphc = 0 on every node of every period
   const float4* rowtab;    // iso grids, per row: a, b, 1/sqrt(a), 1/sqrt(b) (FP32 sweeps)
+  // the forward media again in wavefront-major order (batched sweeps), one copy
+  // per orientation, [nperiod] planes of wplane elements each
+  size_t wplane;           // (ny + La - 1) * La * C
+  const void* wmed[6][2];
 };
 
 __device__ inline int seg_index(const GridDev& g, int ix, int iy) {
   return iy * g.P + ix;  // lane l, column k -> l*C + k == ix
+}
+
+// ----------------------------------------------------------------------------
+// Wavefront-major storage.  In a sweep with directions (XD, YD) lane l handles
+// row j at step t with j + l constant (orientation 0: XD*YD > 0) or j - l
+// constant (orientation 1: XD*YD < 0) across the wave.  Storing a plane ordered
+// by that diagonal, lane-minor, makes every load and store of a step one
+// contiguous run of La*C elements.  Diagonal index q = j + l (o = 0) or
+// j - l + La - 1 (o = 1), q in [0, ny + La - 1); element (q*La + l)*C + k.
+// Entries with j outside [0, ny) or column l*C + k >= nx are padding.
+// ----------------------------------------------------------------------------
+template <int C, int O>
+__device__ __forceinline__ int wave_base(const GridDev& g, int j, int lane) {
+  const int q = (O == 0) ? j + lane : j - lane + g.La - 1;
+  return (q * g.La + lane) * C;
+}
+__host__ __device__ inline size_t wave_index(int C, int La, int o, int ix, int iy) {
+  const int l = ix / C, k = ix - l * C;
+  const int q = (o == 0) ? iy + l : iy - l + La - 1;
+  return ((size_t)q * La + l) * C + k;
+}
+__device__ inline size_t wave_index(const GridDev& g, int o, int ix, int iy) {
+  return wave_index(g.C, g.La, o, ix, iy);
+}
+
+// Cross-lane reads of the swept value and its auxiliary (ds_bpermute).
+__device__ __forceinline__ float shfl_any(float v, int src) { return __shfl(v, src); }
+__device__ __forceinline__ double shfl_any(double v, int src) { return __shfl(v, src); }
+__device__ __forceinline__ float2 shfl_any(float2 v, int src) {
+  return make_float2(__shfl(v.x, src), __shfl(v.y, src));
+}
+__device__ __forceinline__ double2 shfl_any(double2 v, int src) {
+  return make_double2(__shfl(v.x, src), __shfl(v.y, src));
 }
 
 // ----------------------------------------------------------------------------

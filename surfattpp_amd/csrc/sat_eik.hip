@@ -1,15 +1,14 @@
 // sat_eik.hip -- kernels and C ABI of the gfx950 eikonal forward + adjoint path.
 //
-// Kernels
+// Kernels here (the sweep kernels are in sat_fused.hpp / sat_phases.hpp,
+// instantiated by k_*.hip)
 //   media_kernel      per-period metric/slowness planes (src/eikonal_solver.cpp:55-65,
-//                     :380-389; src/preproc.cpp:97-103 and :29)
-//   pair_kernel<C>    persistent: one 64-lane wave drains a queue of (source, period)
-//                     pairs; per pair forward solve -> receiver interp2d -> residual ->
-//                     adjoint solve -> mask -> masked Ta out (src/preproc.cpp:79-143)
+//                     :380-389; src/preproc.cpp:97-103 and :29), segmented and
+//                     wavefront-major
+//   row_kernel        per-row latitude trig and the isotropic FP32 row metric
 //   reduce_kernel     per-period kernel planes, adj_s[iper] = sum_e Ta_e * svel^-3 in
 //                     event order (src/preproc.cpp:28-31)
 //   chi_kernel        chi = sum_e 0.5*sum_r w r^2 in event order (src/preproc.cpp:142)
-//   solo_*_kernel<C>  the per-solve entry points (FSM_UW_PS_lonlat_2d, FSM_O1_JSE_lonlat_2d)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,79 +22,9 @@
 #include <vector>
 
 #include "../../include/surfatt_eik.h"
-#include "sat_device.hpp"
-#include "sat_engine.hpp"
-#include "sat_phases.hpp"
-#include "sat_policies.hpp"
+#include "sat_launch.hpp"
 
 namespace sat {
-
-constexpr int kMaxC = 8;  // nx <= 512
-
-template <int PREC> struct Prec;
-template <> struct Prec<SAT_PREC_F64> {
-  using Fwd = FwdF64;
-  using Adj = AdjF64;
-  using V = double;
-  static constexpr int kSeg = 9;    // T0 T0x T0y u0 u1 u2 | a1p a2m b1p b2m dl d ta0 ta1 ta2
-  static constexpr int kPlain = 1;  // T
-};
-template <> struct Prec<SAT_PREC_F32> {
-  using Fwd = FwdF32;
-  using Adj = AdjF32;
-  using V = float;
-  static constexpr int kSeg = 8;    // t0 t0x t0y u0 u1 u2 | cW cE cS cN cD ta0 ta1 ta2
-  static constexpr int kPlain = 4;  // T, T0 (FP64), d, dl (kept all-zero between solves)
-};
-
-// ----------------------------------------------------------------------------
-// Device descriptors
-// ----------------------------------------------------------------------------
-struct EventDev {
-  int grid, iper;
-  double lon, lat;
-  int rec_off, nrec;
-  size_t ta_off;  // element offset of this event's masked Ta plane in the Ta output
-};
-
-struct BatchDev {
-  const GridDev* grids;
-  const EventDev* events;
-  const int* order;      // event ids, processed in this order
-  int* counter;          // work-queue head (one per launch group)
-  const double* rec_lon;
-  const double* rec_lat;
-  const double* tt_obs;
-  const double* weight;
-  double* syn;
-  double* adjsrc;
-  double* chi_ev;
-  int* iters_f;
-  int* iters_a;
-  int* flags;
-  void* ta_out;          // masked Ta planes (the batch's sweep precision)
-  double* conv;          // per event: Linf of the last two forward rounds
-  int* fb_list;          // FP32 batches: events re-solved in FP64 (decision near the tolerance)
-  int* fb_count;         // one counter per launch group
-  double f32_band;       // relative band around the forward tolerance that triggers the re-solve
-  char* ws;              // slot workspaces
-  size_t ws_slot;        // bytes per slot
-  size_t ws_seg;         // bytes per segmented plane
-  size_t ws_plain;       // bytes per plain FP64 plane
-  int calc_adj;
-  int fwd_maxiter, adj_maxiter;
-};
-
-__device__ inline Slot make_slot(char* base, int nseg, int nplain, size_t seg_bytes, size_t plain_bytes) {
-  Slot s;
-#pragma unroll
-  for (int i = 0; i < 9; i++) s.seg[i] = (i < nseg) ? (void*)(base + (size_t)i * seg_bytes) : nullptr;
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-    s.pl[i] = (i < nplain) ? reinterpret_cast<double*>(base + (size_t)nseg * seg_bytes + (size_t)i * plain_bytes)
-                           : nullptr;
-  return s;
-}
 
 // ----------------------------------------------------------------------------
 // Media planes.  mode 0: from (svel, m11, m12, m22) as forward_for_event passes
@@ -144,19 +73,28 @@ __global__ void media_kernel(GridDev g, int mode, int prec, const double* __rest
       c_plain[po] = sphc / (R2 * cos_y);           // :387
       if (vinv3) vinv3[po] = (mode == 0) ? cr::pow_m3_cr(sv) : 0.0;  // src/preproc.cpp:29
     }
+    void* sp[6] = {s0, s1, s2, s3, s4, s5};
+    const bool wave = q < nx && g.wmed[0][0] != nullptr;
+    size_t wi[2] = {0, 0};
+    if (wave)
+      for (int o = 0; o < 2; o++) wi[o] = (size_t)ip * g.wplane + wave_index(g, o, q, iy);
     if (prec == SAT_PREC_F64) {
-      static_cast<double*>(s0)[so] = va;
-      static_cast<double*>(s1)[so] = vb;
-      static_cast<double*>(s2)[so] = vc;
-      static_cast<double*>(s3)[so] = vf;
+      const double v[4] = {va, vb, vc, vf};
+      for (int i = 0; i < 4; i++) {
+        static_cast<double*>(sp[i])[so] = v[i];
+        if (wave)
+          for (int o = 0; o < 2; o++) static_cast<double*>(const_cast<void*>(g.wmed[i][o]))[wi[o]] = v[i];
+      }
     } else {
       const double det = va * vb - vc * vc;
-      static_cast<float*>(s0)[so] = (float)va;
-      static_cast<float*>(s1)[so] = (float)vb;
-      static_cast<float*>(s2)[so] = (float)vc;
-      static_cast<float*>(s3)[so] = (float)vf;
-      static_cast<float*>(s4)[so] = (det > 0.0) ? (float)sqrt(vf * vf * vb / det) : -1.0f;
-      static_cast<float*>(s5)[so] = (det > 0.0) ? (float)sqrt(vf * vf * va / det) : -1.0f;
+      const float v[6] = {(float)va, (float)vb, (float)vc, (float)vf,
+                          (det > 0.0) ? (float)sqrt(vf * vf * vb / det) : -1.0f,
+                          (det > 0.0) ? (float)sqrt(vf * vf * va / det) : -1.0f};
+      for (int i = 0; i < 6; i++) {
+        static_cast<float*>(sp[i])[so] = v[i];
+        if (wave)
+          for (int o = 0; o < 2; o++) static_cast<float*>(const_cast<void*>(g.wmed[i][o]))[wi[o]] = v[i];
+      }
     }
   }
 }
@@ -174,193 +112,6 @@ __global__ void row_kernel(GridDev g, double2* trig_yy, float4* rowtab) {
     const double R2 = kR * kR;
     const double a = 1.0 / (R2 * (c * c)), b = 1.0 / R2;
     rowtab[iy] = make_float4((float)a, (float)b, (float)(1.0 / sqrt(a)), (float)(1.0 / sqrt(b)));
-  }
-}
-
-template <int PREC>
-__device__ inline typename Prec<PREC>::Fwd::Ctx fwd_ctx(const GridDev& g, int iper, const Slot& ws,
-                                                         const FixedBox& fb) {
-  using V = typename Prec<PREC>::V;
-  typename Prec<PREC>::Fwd::Ctx cx;
-  const size_t po = (size_t)iper * g.seg_plane;
-  cx.g = &g;
-  cx.t0 = static_cast<const V*>(ws.seg[0]);
-  cx.t0x = static_cast<const V*>(ws.seg[1]);
-  cx.t0y = static_cast<const V*>(ws.seg[2]);
-  cx.a = static_cast<const V*>(g.seg[0]) + po;
-  cx.b = static_cast<const V*>(g.seg[1]) + po;
-  cx.c = static_cast<const V*>(g.seg[2]) + po;
-  if constexpr (PREC == SAT_PREC_F64) {
-    cx.f = static_cast<const V*>(g.seg[3]) + po;
-    cx.dx = g.dx;
-    cx.dy = g.dy;
-  } else {
-    cx.f = static_cast<const V*>(g.seg[3]) + po;
-    cx.dsx = static_cast<const V*>(g.seg[4]) + po;
-    cx.dsy = static_cast<const V*>(g.seg[5]) + po;
-    cx.idx = (float)(1.0 / g.dx);
-    cx.idy = (float)(1.0 / g.dy);
-  }
-  cx.fb = fb;
-  return cx;
-}
-
-template <int PREC>
-__device__ inline typename Prec<PREC>::Adj::Ctx adj_ctx(const GridDev& g, const Slot& ws) {
-  using V = typename Prec<PREC>::V;
-  typename Prec<PREC>::Adj::Ctx cx;
-  cx.g = &g;
-  if constexpr (PREC == SAT_PREC_F64) {
-    cx.a1p = static_cast<const V*>(ws.seg[0]);
-    cx.a2m = static_cast<const V*>(ws.seg[1]);
-    cx.b1p = static_cast<const V*>(ws.seg[2]);
-    cx.b2m = static_cast<const V*>(ws.seg[3]);
-    cx.dl = static_cast<const V*>(ws.seg[4]);
-    cx.d = static_cast<const V*>(ws.seg[5]);
-    cx.dx = g.dx;
-    cx.dy = g.dy;
-  } else {
-    cx.cW = static_cast<const V*>(ws.seg[0]);
-    cx.cE = static_cast<const V*>(ws.seg[1]);
-    cx.cS = static_cast<const V*>(ws.seg[2]);
-    cx.cN = static_cast<const V*>(ws.seg[3]);
-    cx.cD = static_cast<const V*>(ws.seg[4]);
-  }
-  return cx;
-}
-
-// Forward solve (src/eikonal_solver.cpp:22-317) into the plain FP64 plane T = pl[0].
-template <int PREC, int C>
-__device__ inline IterOut solve_forward(const GridDev& g, int iper, double lon, double lat, const Slot& ws,
-                                        int maxiter, int& flags) {
-  using P = Prec<PREC>;
-  using V = typename P::V;            // T0 planes
-  using U = typename P::Fwd::V;       // swept value (tau, or sigma in FP64 for the FP32 path)
-  V* S[3];
-  U* Q[3];
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    S[i] = static_cast<V*>(ws.seg[i]);
-    Q[i] = static_cast<U*>(ws.seg[3 + i]);
-  }
-  const FixedBox fb = fwd_init<V, U>(g, iper, lon, lat, S[0], S[1], S[2], Q[0], ws.pl[1], flags);
-  __syncthreads();
-  const auto cx = fwd_ctx<PREC>(g, iper, ws, fb);
-  IterOut io;
-  const U* u = iterate<typename P::Fwd, C>(cx, Q[0], Q[1], Q[2], maxiter, kFwdTol, io);
-  if (!io.converged) flags |= FLAG_NOT_CONVERGED;
-  fwd_finish<V, U>(g, u, S[0], ws.pl[1], ws.pl[0]);
-  __syncthreads();
-  return io;
-}
-
-// Adjoint solve (src/eikonal_solver.cpp:323-511) from T (plain FP64); returns Ta plane.
-template <int PREC, int C>
-__device__ inline const typename Prec<PREC>::V* solve_adjoint(const GridDev& g, int iper, const double* T, int nr,
-                                                               const double* rlon, const double* rlat,
-                                                               const double* adj, const Slot& ws, int maxiter,
-                                                               IterOut& io, int& flags) {
-  using P = Prec<PREC>;
-  using V = typename P::V;
-  if constexpr (PREC == SAT_PREC_F64) adj_setup_f64(g, iper, T, ws);
-  else adj_setup_f32(g, iper, T, ws);
-  __syncthreads();
-  adj_delta<V>(g, nr, rlon, rlat, adj, ws);
-  __syncthreads();
-  const auto cx = adj_ctx<PREC>(g, ws);
-  const int t0 = (PREC == SAT_PREC_F64) ? 6 : 5;
-  const V* ta = iterate<typename P::Adj, C>(cx, static_cast<V*>(ws.seg[t0]), static_cast<V*>(ws.seg[t0 + 1]),
-                                            static_cast<V*>(ws.seg[t0 + 2]), maxiter, kAdjTol, io);
-  if (!io.converged) flags |= FLAG_ADJ_MAXITER;
-  return ta;
-}
-
-// ----------------------------------------------------------------------------
-// One (source, period) pair: src/preproc.cpp:88-143.
-// ----------------------------------------------------------------------------
-template <int PREC, int C, typename TaT>
-__device__ void run_pair(const BatchDev& bd, int e, const Slot& ws, int group) {
-  using V = typename Prec<PREC>::V;
-  const int lane = threadIdx.x;
-  const EventDev ev = bd.events[e];
-  const GridDev& g = bd.grids[ev.grid];
-  int flags = 0;
-  const IterOut fo = solve_forward<PREC, C>(g, ev.iper, ev.lon, ev.lat, ws, bd.fwd_maxiter, flags);
-  if (lane == 0) {
-    bd.conv[2 * (size_t)e] = fo.linf;
-    bd.conv[2 * (size_t)e + 1] = fo.linf_prev;
-  }
-  if constexpr (PREC == SAT_PREC_F32) {
-    // The FP32 iterate's convergence norm differs from the FP64 one by its
-    // rounding; when a stop/continue decision lies within f32_band of the
-    // tolerance the FP32 run may stop one round apart from the reference.
-    // Such pairs are re-solved by the FP64 path after this launch.
-    const double b = bd.f32_band;
-    const bool near_last = fo.converged && fo.linf >= (1.0 - b) * kFwdTol;
-    const bool near_prev = fo.linf_prev >= 0.0 && fo.linf_prev < (1.0 + b) * kFwdTol;
-    if (b > 0.0 && (near_last || near_prev)) {
-      if (lane == 0) {
-        const int slot = atomicAdd(bd.fb_count + group, 1);
-        bd.fb_list[slot] = e;
-      }
-      __syncthreads();
-      return;
-    }
-  }
-  const double* T = ws.pl[0];
-  // :105-115 receivers, residuals, adjoint source
-  int outside = 0;
-  for (int r = lane; r < ev.nrec; r += kWave) {
-    const int k = ev.rec_off + r;
-    const double s = interp2d_point(g.lon, g.nx, g.lat, g.ny, T, bd.rec_lon[k], bd.rec_lat[k]);
-    bd.syn[k] = s;
-    bd.adjsrc[k] = bd.weight[k] * (s - bd.tt_obs[k]);
-    outside |= (s != s) ? 1 : 0;
-  }
-  if (wave_or_i(outside)) flags |= FLAG_REC_OUTSIDE;
-  __syncthreads();
-  int aiters = 0;
-  if (bd.calc_adj) {
-    if (lane == 0) {  // :142 chi term of this event, receiver order
-      double s = 0.0;
-      for (int r = 0; r < ev.nrec; r++) {
-        const int k = ev.rec_off + r;
-        const double d = bd.syn[k] - bd.tt_obs[k];
-        s += bd.weight[k] * (d * d);
-      }
-      bd.chi_ev[e] = 0.5 * s;
-    }
-    IterOut ao;
-    const V* ta = solve_adjoint<PREC, C>(g, ev.iper, T, ev.nrec, bd.rec_lon + ev.rec_off, bd.rec_lat + ev.rec_off,
-                                         bd.adjsrc + ev.rec_off, ws, bd.adj_maxiter, ao, flags);
-    aiters = ao.iters;
-    mask_store<V, TaT>(g, ev.lon, ev.lat, ta, static_cast<TaT*>(bd.ta_out) + ev.ta_off);  // :125
-  } else if (lane == 0) {
-    bd.chi_ev[e] = 0.0;
-  }
-  if (lane == 0) {
-    bd.iters_f[e] = fo.iters;
-    bd.iters_a[e] = aiters;
-    bd.flags[e] = flags;
-  }
-  __syncthreads();
-}
-
-// Persistent: each wave takes the next queue entry until the queue is empty.
-// The queue is order[first, first+count), or list[0, *dcount) when list != nullptr
-// (the FP64 re-solves of an FP32 batch, written by the preceding launch).
-template <int PREC, int C, typename TaT>
-__global__ __launch_bounds__(64) void pair_kernel(BatchDev bd, int first, int count, int group, const int* list,
-                                                  const int* dcount) {
-  using P = Prec<PREC>;
-  const Slot ws = make_slot(bd.ws + (size_t)blockIdx.x * bd.ws_slot, P::kSeg, P::kPlain, bd.ws_seg, bd.ws_plain);
-  const int n = list ? *dcount : count;
-  for (;;) {
-    int idx = 0;
-    if (threadIdx.x == 0) idx = atomicAdd(bd.counter, 1);
-    idx = __shfl(idx, 0);
-    if (idx >= n) break;
-    run_pair<PREC, C, TaT>(bd, list ? list[idx] : bd.order[first + idx], ws, group);
   }
 }
 
@@ -407,52 +158,6 @@ __global__ void chi_kernel(const double* chi_ev, int n, double* chi) {
     double c = 0.0;
     for (int e = 0; e < n; e++) c += chi_ev[e];
     *chi = c;
-  }
-}
-
-// ----------------------------------------------------------------------------
-// Per-solve kernels (one wave, one solve)
-// ----------------------------------------------------------------------------
-struct SoloDev {
-  GridDev g;
-  double src_lon, src_lat;
-  const double* T_in;      // backward: forward travel time, column-major
-  int nr;
-  const double* rlon;
-  const double* rlat;
-  const double* adj;
-  double* out;             // T or Ta, column-major
-  char* ws;
-  size_t ws_seg, ws_plain;
-  int maxiter;
-  double* info;            // iters, flags, l1, linf
-};
-
-template <int PREC, int C>
-__global__ __launch_bounds__(64) void solo_forward(SoloDev sd) {
-  using P = Prec<PREC>;
-  const Slot ws = make_slot(sd.ws, P::kSeg, P::kPlain, sd.ws_seg, sd.ws_plain);
-  int flags = 0;
-  const IterOut io = solve_forward<PREC, C>(sd.g, 0, sd.src_lon, sd.src_lat, ws, sd.maxiter, flags);
-  for (size_t p = threadIdx.x; p < sd.g.plane; p += kWave) sd.out[p] = ws.pl[0][p];
-  if (threadIdx.x == 0) {
-    sd.info[0] = io.iters; sd.info[1] = flags; sd.info[2] = io.l1; sd.info[3] = io.linf;
-  }
-}
-
-template <int PREC, int C>
-__global__ __launch_bounds__(64) void solo_backward(SoloDev sd) {
-  using P = Prec<PREC>;
-  using V = typename P::V;
-  const Slot ws = make_slot(sd.ws, P::kSeg, P::kPlain, sd.ws_seg, sd.ws_plain);
-  int flags = 0;
-  IterOut io;
-  const V* ta = solve_adjoint<PREC, C>(sd.g, 0, sd.T_in, sd.nr, sd.rlon, sd.rlat, sd.adj, ws, sd.maxiter, io, flags);
-  for (int iy = 0; iy < sd.g.ny; iy++)
-    for (int ix = threadIdx.x; ix < sd.g.nx; ix += kWave)
-      sd.out[ix + (size_t)iy * sd.g.nx] = (double)ta[seg_index(sd.g, ix, iy)];
-  if (threadIdx.x == 0) {
-    sd.info[0] = io.iters; sd.info[1] = flags; sd.info[2] = io.l1; sd.info[3] = io.linf;
   }
 }
 
@@ -515,9 +220,11 @@ struct GridHost {
   GridDev g64{};              // FP32 batches: g with the FP64 media
   double2* d_trig_yy = nullptr;
   float4* d_rowtab = nullptr;
+  char* d_wmed = nullptr;     // batched sweeps: wavefront-major media, per orientation
 
   void release() {
-    void* ps[] = {d_lon, d_lat, d_xx, d_yy, d_model, d_fp, d_aA, d_b, d_c, d_v3, d_seg, d_seg64, d_trig_yy, d_rowtab};
+    void* ps[] = {d_lon, d_lat, d_xx, d_yy, d_model, d_fp, d_aA, d_b, d_c, d_v3, d_seg, d_seg64, d_trig_yy, d_rowtab,
+                  d_wmed};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = GridHost();
@@ -526,7 +233,8 @@ struct GridHost {
 
 static size_t elem_size(int prec) { return prec == SAT_PREC_F64 ? sizeof(double) : sizeof(float); }
 
-static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const double* lon, const double* lat) {
+static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const double* lon, const double* lat,
+                      bool wave) {
   if (nx < 3 || ny < 3 || nper < 1 || !lon || !lat) return SAT_EINVAL;
   if (!uniform_axis(lon, nx) || !uniform_axis(lat, ny)) return SAT_EINVAL;
   const Layout L = layout_for(nx);
@@ -571,9 +279,19 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
   hipLaunchKernelGGL(row_kernel, dim3((ny + 63) / 64), dim3(64), 0, nullptr, g, G.d_trig_yy, G.d_rowtab);
   SAT_CHECK(hipGetLastError());
   SAT_CHECK(hipDeviceSynchronize());
+  g.wplane = (size_t)(ny + L.La - 1) * L.La * L.C;
+  for (int i = 0; i < 6; i++) g.wmed[i][0] = g.wmed[i][1] = nullptr;
   G.g64 = g;
   if (G.d_seg64)
     for (int i = 0; i < 6; i++) G.g64.seg[i] = G.d_seg64 + (size_t)i * seg64_bytes;
+  if (wave) {  // padding entries stay zero
+    const int nmed = prec == SAT_PREC_F64 ? 4 : 6;
+    const size_t pb = (size_t)nper * g.wplane * elem_size(prec);
+    SAT_CHECK(dalloc(&G.d_wmed, (size_t)nmed * 2 * pb));
+    SAT_CHECK(hipMemset(G.d_wmed, 0, (size_t)nmed * 2 * pb));
+    for (int i = 0; i < nmed; i++)
+      for (int o = 0; o < 2; o++) g.wmed[i][o] = G.d_wmed + (size_t)(2 * i + o) * pb;
+  }
   return SAT_OK;
 }
 
@@ -636,45 +354,6 @@ using namespace sat;
 
 namespace {
 
-template <int PREC, int C> struct SoloFwd {
-  static void go(dim3 g, hipStream_t s, SoloDev a) { hipLaunchKernelGGL((solo_forward<PREC, C>), g, dim3(kWave), 0, s, a); }
-};
-template <int PREC, int C> struct SoloBwd {
-  static void go(dim3 g, hipStream_t s, SoloDev a) { hipLaunchKernelGGL((solo_backward<PREC, C>), g, dim3(kWave), 0, s, a); }
-};
-template <int PREC, typename TaT> struct PairKT {
-  template <int C> struct K {
-    static void go(dim3 g, hipStream_t s, BatchDev a, int first, int count, int group, const int* list,
-                   const int* dcount) {
-      hipLaunchKernelGGL((pair_kernel<PREC, C, TaT>), g, dim3(kWave), 0, s, a, first, count, group, list, dcount);
-    }
-  };
-};
-template <int PREC, int C> struct PairK64 : PairKT<SAT_PREC_F64, double>::K<C> {};
-template <int PREC, int C> struct PairK32 : PairKT<SAT_PREC_F32, float>::K<C> {};
-template <int PREC, int C> struct PairK64to32 : PairKT<SAT_PREC_F64, float>::K<C> {};
-
-template <template <int, int> class K, int PREC, class... A>
-hipError_t dispatch_C1(int C, A... args) {
-  switch (C) {
-    case 1: K<PREC, 1>::go(args...); break;
-    case 2: K<PREC, 2>::go(args...); break;
-    case 3: K<PREC, 3>::go(args...); break;
-    case 4: K<PREC, 4>::go(args...); break;
-    case 5: K<PREC, 5>::go(args...); break;
-    case 6: K<PREC, 6>::go(args...); break;
-    case 7: K<PREC, 7>::go(args...); break;
-    case 8: K<PREC, 8>::go(args...); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-template <template <int, int> class K, class... A>
-hipError_t dispatch(int prec, int C, A... args) {
-  return prec == SAT_PREC_F64 ? dispatch_C1<K, SAT_PREC_F64>(C, args...) : dispatch_C1<K, SAT_PREC_F32>(C, args...);
-}
-
 int solo_run(bool forward, int precision, int nx, int ny, const double* lon, const double* lat,
              const double* spha, const double* sphb, const double* sphc, const double* fun_or_T, double x0,
              double y0, int nr, const double* rlon, const double* rlat, const double* adj, double* out,
@@ -686,7 +365,7 @@ int solo_run(bool forward, int precision, int nx, int ny, const double* lon, con
   int rc = device_check();
   if (rc) return rc;
   GridHost G;
-  rc = grid_setup(G, precision, nx, ny, 1, lon, lat);
+  rc = grid_setup(G, precision, nx, ny, 1, lon, lat, false);
   if (rc) { G.release(); return rc; }
   const size_t N = (size_t)nx * ny;
   char* d_ws = nullptr;
@@ -724,7 +403,8 @@ int solo_run(bool forward, int precision, int nx, int ny, const double* lon, con
   sd.out = d_out; sd.ws = d_ws; sd.ws_seg = sg.seg; sd.ws_plain = sg.plain; sd.info = d_info;
   if (forward) {
     sd.maxiter = kFwdMaxIter;
-    SOLO_CHECK((dispatch<SoloFwd>(precision, G.g.C, dim3(1), stream, sd)));
+    SOLO_CHECK(precision == SAT_PREC_F64 ? launch_solo<SAT_PREC_F64>(1, G.g.C, stream, sd)
+                                         : launch_solo<SAT_PREC_F32>(1, G.g.C, stream, sd));
   } else {
     SOLO_CHECK(dalloc(&d_T, N));
     SOLO_CHECK(hipMemcpy(d_T, fun_or_T, sizeof(double) * N, hipMemcpyHostToDevice));
@@ -736,7 +416,8 @@ int solo_run(bool forward, int precision, int nx, int ny, const double* lon, con
     }
     sd.T_in = d_T; sd.nr = nr; sd.rlon = d_r; sd.rlat = d_r + nr; sd.adj = d_r + 2 * nr;
     sd.maxiter = kAdjMaxIter;
-    SOLO_CHECK((dispatch<SoloBwd>(precision, G.g.C, dim3(1), stream, sd)));
+    SOLO_CHECK(precision == SAT_PREC_F64 ? launch_solo<SAT_PREC_F64>(0, G.g.C, stream, sd)
+                                         : launch_solo<SAT_PREC_F32>(0, G.g.C, stream, sd));
   }
   SOLO_CHECK(hipDeviceSynchronize());
   SOLO_CHECK(hipMemcpy(out, d_out, sizeof(double) * N, hipMemcpyDeviceToHost));
@@ -780,13 +461,13 @@ struct sat_batch {
   int* d_fb = nullptr;             // re-solve lists, group g at group_first[g]
   double f32_band = 0.02;
   // per-event planes of the phase pipeline
-  size_t seg_total = 0, pl_total = 0, col_total = 0;
+  size_t w_total = 0, pl_total = 0, col_total = 0;
   double* d_evs = nullptr;
   double2* d_coltab = nullptr;
-  char* d_t0 = nullptr;            // 3 planes (t0, t0x, t0y), sweep precision
+  char* d_work = nullptr;          // wavefront-major: t0, t0x, t0y x 2 orientations, then the
+                                   // adjoint's face fluxes and delta x 2 in the same bytes
   double* d_T0d = nullptr;
   double* d_T = nullptr;
-  char* d_co = nullptr;            // 6 (FP64) / 5 (FP32) coefficient planes
   char* d_ta = nullptr;            // masked Ta, sweep precision
   char* d_slots = nullptr;         // phase sweep slots
   size_t slot_plane = 0, slot_bytes = 0;
@@ -803,26 +484,30 @@ struct sat_batch {
   std::vector<int> group_C, group_iso, group_first, group_count;
   bool groups_dirty = true;
   int max_nx = 0;
-  size_t max_seg = 0;
+  size_t max_seg = 0, max_plane = 0, max_wplane = 0;
   int nplanes = 0;
   size_t* d_red_sz = nullptr;      // plane_off | node_prefix
   int* d_red_i = nullptr;          // plane_n | ev_off | ev_ids
   const double** d_red_v3 = nullptr;
   ReduceDev rd{};
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, evA = nullptr;
+  hipStream_t stream2 = nullptr;   // FP64 re-solves, concurrent with the FP32 adjoint phase
+  // time marks: start, init done, forward sweeps done, adjoint set-up done,
+  // adjoint sweeps done, re-solves start / end (stream2), end
+  static constexpr int kMarks = 8;
+  hipEvent_t mk[kMarks] = {};
 
   void release() {
     for (auto& G : grids) G.release();
     void* ps[] = {d_grids, d_grids64, d_events, d_evx, d_order, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
-                  d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_evs, d_coltab, d_t0, d_T0d, d_T, d_co, d_ta,
+                  d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
                   d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3};
     for (void* p : ps)
       if (p) (void)hipFree(p);
-    hipEvent_t es[] = {ev0, ev1, ev2, evA};
-    for (hipEvent_t e : es)
+    for (hipEvent_t e : mk)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+    if (stream2) (void)hipStreamDestroy(stream2);
   }
 };
 
@@ -866,19 +551,6 @@ int build_groups(sat_batch* b) {
   b->groups_dirty = false;
   return SAT_OK;
 }
-
-template <int PREC, int C, bool ISO> struct FwdSweepK {
-  static void go(dim3 g, hipStream_t s, PhaseDev a, int first, int count, int group) {
-    hipLaunchKernelGGL((fwd_sweep_kernel<PREC, C, ISO>), g, dim3(kWave), 0, s, a, first, count, group);
-  }
-};
-template <int PREC, int C> struct FwdSweepIsoK : FwdSweepK<PREC, C, true> {};
-template <int PREC, int C> struct FwdSweepGenK : FwdSweepK<PREC, C, false> {};
-template <int PREC, int C> struct AdjSweepK {
-  static void go(dim3 g, hipStream_t s, PhaseDev a, int first, int count) {
-    hipLaunchKernelGGL((adj_sweep_kernel<PREC, C>), g, dim3(kWave), 0, s, a, first, count);
-  }
-};
 
 // Host copy of the index boxes, computed with the same IEEE double operations
 // as the device (src/eikonal_solver.cpp:119-129 and :528-531).
@@ -976,24 +648,25 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   b->nevent = d->nevent;
   b->nrec = d->nrec_total;
   B_CHECK(hipStreamCreate(&b->stream));
-  B_CHECK(hipEventCreate(&b->ev0));
-  B_CHECK(hipEventCreate(&b->ev1));
-  B_CHECK(hipEventCreate(&b->ev2));
-  B_CHECK(hipEventCreate(&b->evA));
+  B_CHECK(hipStreamCreate(&b->stream2));
+  for (int i = 0; i < sat_batch::kMarks; i++) B_CHECK(hipEventCreate(&b->mk[i]));
   b->grids.resize(d->ngrid);
-  size_t max_seg = 0, max_plane = 0;
+  size_t max_seg = 0, max_plane = 0, max_wplane = 0;
   int max_nx = 0;
   for (int gi = 0; gi < d->ngrid; gi++) {
     const sat_grid_desc& gd = d->grids[gi];
-    rc = grid_setup(b->grids[gi], b->precision, gd.nx, gd.ny, gd.nperiod, gd.lon_deg, gd.lat_deg);
+    rc = grid_setup(b->grids[gi], b->precision, gd.nx, gd.ny, gd.nperiod, gd.lon_deg, gd.lat_deg, true);
     if (rc) return fail(rc);
     rc = grid_media(b->grids[gi], 0, gd.svel, gd.m11, gd.m12, gd.m22, b->stream);
     if (rc) return fail(rc);
     max_seg = std::max(max_seg, b->grids[gi].g.seg_plane);
     max_plane = std::max(max_plane, b->grids[gi].g.plane);
+    max_wplane = std::max(max_wplane, b->grids[gi].g.wplane);
     max_nx = std::max(max_nx, gd.nx);
   }
   b->max_seg = max_seg;
+  b->max_plane = max_plane;
+  b->max_wplane = max_wplane;
   b->max_nx = max_nx;
   B_CHECK(dalloc(&b->d_grids, d->ngrid));
   if (f32) B_CHECK(dalloc(&b->d_grids64, d->ngrid));
@@ -1003,7 +676,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   b->he.resize(ne);
   b->hx.resize(ne);
   b->base_flags.assign(std::max(ne, 1), 0);
-  size_t seg_total = 0, pl_total = 0, col_total = 0;
+  size_t w_total = 0, pl_total = 0, col_total = 0;
   for (int e = 0; e < ne; e++) {
     const sat_event_desc& ev = d->events[e];
     const sat_grid_desc& gd = d->grids[ev.grid];
@@ -1012,17 +685,17 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
     EventX& x = b->hx[e];
     x.grid = ev.grid; x.iper = ev.iper; x.rec_off = ev.rec_off; x.nrec = ev.nrec;
     x.lon = ev.src_lon_deg; x.lat = ev.src_lat_deg;
-    x.seg_off = seg_total; x.pl_off = pl_total; x.col_off = col_total;
+    x.w_off = w_total; x.pl_off = pl_total; x.col_off = col_total;
     std::vector<double> xx(gd.nx), yy(gd.ny);
     for (int i = 0; i < gd.nx; i++) xx[i] = gd.lon_deg[i] * kDEG2RAD;
     for (int j = 0; j < gd.ny; j++) yy[j] = gd.lat_deg[j] * kDEG2RAD;
     event_boxes(b->grids[ev.grid], xx, yy, gd.lon_deg, gd.lat_deg, ev.src_lon_deg, ev.src_lat_deg, x,
                 b->base_flags[e]);
-    seg_total += round_up(g.seg_plane, 64);
+    w_total += round_up(g.wplane, 64);
     pl_total += round_up(g.plane, 64);
     col_total += round_up(g.nx, 64);
   }
-  b->seg_total = seg_total;
+  b->w_total = w_total;
   b->pl_total = pl_total;
   b->col_total = col_total;
   B_CHECK(dalloc(&b->d_events, ne));
@@ -1062,20 +735,22 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   const size_t es = elem_size(b->precision);
   B_CHECK(dalloc(&b->d_evs, 3 * (size_t)std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_coltab, std::max<size_t>(col_total, 1)));
-  B_CHECK(dalloc(&b->d_t0, 3 * std::max<size_t>(seg_total, 1) * es));
+  // 6 forward planes of es bytes, or 2 face-flux planes of 2*es and 2 delta planes
+  const size_t work_bytes = 6 * std::max<size_t>(w_total, 1) * es;
+  B_CHECK(dalloc(&b->d_work, work_bytes));
+  B_CHECK(hipMemset(b->d_work, 0, work_bytes));
   if (f32) B_CHECK(dalloc(&b->d_T0d, std::max<size_t>(pl_total, 1)));
   B_CHECK(dalloc(&b->d_T, std::max<size_t>(pl_total, 1)));
   if (b->calc_adj) {
-    B_CHECK(dalloc(&b->d_co, (f32 ? 5 : 6) * std::max<size_t>(seg_total, 1) * es));
     B_CHECK(dalloc(&b->d_ta, std::max<size_t>(pl_total, 1) * es));
   }
 
   // resident waves of the sweep kernels: 3 swept planes each (FP64 sigma/tau sized)
   hipDeviceProp_t prop;
   B_CHECK(hipGetDeviceProperties(&prop, 0));
-  const int waves_per_cu = 8;
+  const int waves_per_cu = 16;
   b->nslots_phase = std::max(1, std::min(ne, prop.multiProcessorCount * waves_per_cu));
-  b->slot_plane = round_up(max_seg * sizeof(double), 256);
+  b->slot_plane = round_up(max_wplane * sizeof(double), 256);
   b->slot_bytes = 3 * b->slot_plane;
   B_CHECK(dalloc(&b->d_slots, b->slot_bytes * (size_t)b->nslots_phase));
   B_CHECK(hipMemset(b->d_slots, 0, b->slot_bytes * (size_t)b->nslots_phase));
@@ -1207,7 +882,33 @@ int sat_batch_run(sat_batch* b, int mode) {
   bd.fb_count = fb_count;
   bd.f32_band = b->f32_band;
 
-  SAT_CHECK(hipEventRecord(b->ev0, b->stream));
+  hipEvent_t* mk = b->mk;
+  // FP64 re-solves of the FP32 pairs whose stop decision was within the band,
+  // on stream2 once the forward sweeps (which list them) are done
+  auto launch_resolves = [&]() -> int {
+    SAT_CHECK(hipEventRecord(mk[2], b->stream));
+    SAT_CHECK(hipStreamWaitEvent(b->stream2, mk[2], 0));
+    SAT_CHECK(hipEventRecord(mk[5], b->stream2));
+    if (f32) {
+      for (size_t gi = 0; gi < ng; gi++) {
+        BatchDev a = bd;
+        a.grids = b->d_grids64;
+        a.ws = b->d_ws64;
+        a.ws_slot = b->sg64.slot;
+        a.ws_seg = b->sg64.seg;
+        a.ws_plain = b->sg64.plain;
+        a.counter = res_heads + gi;
+        a.f32_band = 0.0;
+        const int* list = bd.fb_list + b->group_first[gi];
+        const int nblk = std::min(b->group_count[gi], b->nslots64);
+        SAT_CHECK(launch_pair<SAT_PREC_F64>(1, b->group_C[gi], dim3(nblk), b->stream2, a, 0, 0, (int)gi, list,
+                                            fb_count + gi));
+      }
+    }
+    SAT_CHECK(hipEventRecord(mk[6], b->stream2));
+    return SAT_OK;
+  };
+  SAT_CHECK(hipEventRecord(mk[0], b->stream));
   SAT_CHECK(hipMemsetAsync(b->d_counter, 0, sizeof(int) * 5 * ne1, b->stream));
   SAT_CHECK(hipMemsetAsync(b->d_skip, 0, ne1, b->stream));
   SAT_CHECK(hipMemsetAsync(b->d_iters, 0, sizeof(int) * 2 * ne1, b->stream));
@@ -1226,12 +927,17 @@ int sat_batch_run(sat_batch* b, int mode) {
         a.counter = solo_heads + first;
         a.fb_list = bd.fb_list + b->group_first[gi];
         if (!f32)
-          SAT_CHECK((dispatch_C1<PairK64, 0>(b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi,
-                                              (const int*)nullptr, (const int*)nullptr)));
+          SAT_CHECK(launch_pair<SAT_PREC_F64>(0, b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi, nullptr,
+                                              nullptr));
         else
-          SAT_CHECK((dispatch_C1<PairK32, 0>(b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi,
-                                              (const int*)nullptr, (const int*)nullptr)));
+          SAT_CHECK(launch_pair<SAT_PREC_F32>(1, b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi, nullptr,
+                                              nullptr));
       }
+    SAT_CHECK(hipEventRecord(mk[1], b->stream));
+    const int rc = launch_resolves();
+    if (rc) return rc;
+    SAT_CHECK(hipEventRecord(mk[3], b->stream));
+    SAT_CHECK(hipEventRecord(mk[4], b->stream));
   } else {
     PhaseDev pd{};
     pd.grids = b->d_grids;
@@ -1240,12 +946,16 @@ int sat_batch_run(sat_batch* b, int mode) {
     pd.evs = b->d_evs;
     pd.coltab = b->d_coltab;
     const size_t es = elem_size(b->precision);
-    pd.t0 = b->d_t0;
-    pd.t0x = b->d_t0 + b->seg_total * es;
-    pd.t0y = b->d_t0 + 2 * b->seg_total * es;
+    const size_t wb = b->w_total * es;  // bytes of one sweep-precision plane set
+    for (int o = 0; o < 2; o++) {
+      pd.t0[o] = b->d_work + (0 + o) * wb;
+      pd.t0x[o] = b->d_work + (2 + o) * wb;
+      pd.t0y[o] = b->d_work + (4 + o) * wb;
+      pd.fxy[o] = b->d_work + 2 * o * wb;
+      pd.dl[o] = b->d_work + (4 + o) * wb;
+    }
     pd.T0d = b->d_T0d;
     pd.T = b->d_T;
-    for (int i = 0; i < 6; i++) pd.co[i] = (b->d_co && i < (f32 ? 5 : 6)) ? b->d_co + i * b->seg_total * es : nullptr;
     pd.ta_out = b->d_ta;
     pd.rec_lon = bd.rec_lon;
     pd.rec_lat = bd.rec_lat;
@@ -1271,68 +981,53 @@ int sat_batch_run(sat_batch* b, int mode) {
     pd.order = b->d_order;
     pd.fwd_maxiter = kFwdMaxIter;
     pd.adj_maxiter = kAdjMaxIter;
-    const int max_nodes = (int)b->max_seg;
+    const dim3 node_grid((unsigned)((b->max_plane + 255) / 256), (unsigned)std::max(ne, 1));
     if (ne) {
-      if (!f32) hipLaunchKernelGGL(trig_kernel<SAT_PREC_F64>, dim3((b->max_nx + 63) / 64, ne), dim3(64), 0, b->stream, pd);
-      else hipLaunchKernelGGL(trig_kernel<SAT_PREC_F32>, dim3((b->max_nx + 63) / 64, ne), dim3(64), 0, b->stream, pd);
-      SAT_CHECK(hipGetLastError());
-      if (!f32) hipLaunchKernelGGL(init_kernel<SAT_PREC_F64>, dim3((max_nodes + 255) / 256, ne), dim3(256), 0, b->stream, pd);
-      else hipLaunchKernelGGL(init_kernel<SAT_PREC_F32>, dim3((max_nodes + 255) / 256, ne), dim3(256), 0, b->stream, pd);
-      SAT_CHECK(hipGetLastError());
+      const dim3 col_grid((unsigned)((b->max_nx + 63) / 64), (unsigned)ne);
+      SAT_CHECK(f32 ? launch_trig<SAT_PREC_F32>(col_grid, b->stream, pd) : launch_trig<SAT_PREC_F64>(col_grid, b->stream, pd));
+      SAT_CHECK(f32 ? launch_init<SAT_PREC_F32>(node_grid, b->stream, pd) : launch_init<SAT_PREC_F64>(node_grid, b->stream, pd));
     }
+    SAT_CHECK(hipEventRecord(mk[1], b->stream));
     for (size_t gi = 0; gi < ng; gi++) {
       PhaseDev a = pd;
       a.counter = fwd_heads + gi;
       a.fb_list = pd.fb_list + b->group_first[gi];
       const int nblk = std::min(b->group_count[gi], b->nslots_phase);
-      const int C = b->group_C[gi];
-      hipError_t err;
-      if (!f32) err = dispatch_C1<FwdSweepGenK, SAT_PREC_F64>(C, dim3(nblk), b->stream, a, b->group_first[gi], b->group_count[gi], (int)gi);
-      else if (b->group_iso[gi]) err = dispatch_C1<FwdSweepIsoK, SAT_PREC_F32>(C, dim3(nblk), b->stream, a, b->group_first[gi], b->group_count[gi], (int)gi);
-      else err = dispatch_C1<FwdSweepGenK, SAT_PREC_F32>(C, dim3(nblk), b->stream, a, b->group_first[gi], b->group_count[gi], (int)gi);
-      SAT_CHECK(err);
+      const int C = b->group_C[gi], iso = b->group_iso[gi];
+      SAT_CHECK(f32 ? launch_fwd_sweep<SAT_PREC_F32>(iso, C, dim3(nblk), b->stream, a, b->group_first[gi],
+                                                     b->group_count[gi], (int)gi)
+                    : launch_fwd_sweep<SAT_PREC_F64>(iso, C, dim3(nblk), b->stream, a, b->group_first[gi],
+                                                     b->group_count[gi], (int)gi));
+    }
+    {
+      const int rc = launch_resolves();
+      if (rc) return rc;
     }
     if (b->nrec) {
       const int blocks = (int)std::min<size_t>((b->nrec + 255) / 256, 4096);
-      hipLaunchKernelGGL(recv_kernel, dim3(blocks), dim3(256), 0, b->stream, pd, b->nrec);
-      SAT_CHECK(hipGetLastError());
+      SAT_CHECK(launch_recv(dim3(blocks), b->stream, pd, b->nrec));
     }
     if (b->calc_adj && ne) {
-      if (!f32) hipLaunchKernelGGL(adj_setup_kernel<SAT_PREC_F64>, dim3((max_nodes + 255) / 256, ne), dim3(256), 0, b->stream, pd);
-      else hipLaunchKernelGGL(adj_setup_kernel<SAT_PREC_F32>, dim3((max_nodes + 255) / 256, ne), dim3(256), 0, b->stream, pd);
-      SAT_CHECK(hipGetLastError());
-      if (!f32) hipLaunchKernelGGL(delta_kernel<SAT_PREC_F64>, dim3((ne + 63) / 64), dim3(64), 0, b->stream, pd);
-      else hipLaunchKernelGGL(delta_kernel<SAT_PREC_F32>, dim3((ne + 63) / 64), dim3(64), 0, b->stream, pd);
-      SAT_CHECK(hipGetLastError());
+      SAT_CHECK(f32 ? launch_adj_setup<SAT_PREC_F32>(node_grid, b->stream, pd)
+                    : launch_adj_setup<SAT_PREC_F64>(node_grid, b->stream, pd));
+      const dim3 ev_grid((unsigned)((ne + 63) / 64));
+      SAT_CHECK(f32 ? launch_delta<SAT_PREC_F32>(ev_grid, b->stream, pd) : launch_delta<SAT_PREC_F64>(ev_grid, b->stream, pd));
+    }
+    SAT_CHECK(hipEventRecord(mk[3], b->stream));
+    if (b->calc_adj && ne) {
       for (size_t gi = 0; gi < ng; gi++) {
         PhaseDev a = pd;
         a.counter = adj_heads + gi;
         const int nblk = std::min(b->group_count[gi], b->nslots_phase);
-        hipError_t err;
-        if (!f32) err = dispatch_C1<AdjSweepK, SAT_PREC_F64>(b->group_C[gi], dim3(nblk), b->stream, a, b->group_first[gi], b->group_count[gi]);
-        else err = dispatch_C1<AdjSweepK, SAT_PREC_F32>(b->group_C[gi], dim3(nblk), b->stream, a, b->group_first[gi], b->group_count[gi]);
-        SAT_CHECK(err);
+        SAT_CHECK(f32 ? launch_adj_sweep<SAT_PREC_F32>(b->group_C[gi], dim3(nblk), b->stream, a, b->group_first[gi],
+                                                       b->group_count[gi])
+                      : launch_adj_sweep<SAT_PREC_F64>(b->group_C[gi], dim3(nblk), b->stream, a, b->group_first[gi],
+                                                       b->group_count[gi]));
       }
     }
+    SAT_CHECK(hipEventRecord(mk[4], b->stream));
   }
-  SAT_CHECK(hipEventRecord(b->evA, b->stream));
-  if (f32) {  // FP64 re-solves of the pairs whose FP32 stop decision was within the band
-    for (size_t gi = 0; gi < ng; gi++) {
-      BatchDev a = bd;
-      a.grids = b->d_grids64;
-      a.ws = b->d_ws64;
-      a.ws_slot = b->sg64.slot;
-      a.ws_seg = b->sg64.seg;
-      a.ws_plain = b->sg64.plain;
-      a.counter = res_heads + gi;
-      a.f32_band = 0.0;
-      const int* list = bd.fb_list + b->group_first[gi];
-      const int nblk = std::min(b->group_count[gi], b->nslots64);
-      SAT_CHECK((dispatch_C1<PairK64to32, 0>(b->group_C[gi], dim3(nblk), b->stream, a, 0, 0, (int)gi, list,
-                                             (const int*)(fb_count + gi))));
-    }
-  }
-  SAT_CHECK(hipEventRecord(b->ev1, b->stream));
+  SAT_CHECK(hipStreamWaitEvent(b->stream, mk[6], 0));
   if (b->calc_adj && b->K_total) {
     const int threads = 256;
     const int blocks = (int)std::min<size_t>((b->K_total + threads - 1) / threads, 65535);
@@ -1342,7 +1037,7 @@ int sat_batch_run(sat_batch* b, int mode) {
   }
   hipLaunchKernelGGL(chi_kernel, dim3(1), dim3(1), 0, b->stream, b->d_chi_ev, b->calc_adj ? ne : 0, b->d_chi);
   SAT_CHECK(hipGetLastError());
-  SAT_CHECK(hipEventRecord(b->ev2, b->stream));
+  SAT_CHECK(hipEventRecord(mk[7], b->stream));
   return SAT_OK;
 }
 
@@ -1355,10 +1050,22 @@ int sat_batch_sync(sat_batch* b) {
 int sat_batch_last_ms(sat_batch* b, double* total_ms, double* solve_ms) {
   if (!b) return SAT_EINVAL;
   float t = 0.f, s = 0.f;
-  SAT_CHECK(hipEventElapsedTime(&t, b->ev0, b->ev2));
-  SAT_CHECK(hipEventElapsedTime(&s, b->ev0, b->evA));
+  SAT_CHECK(hipEventElapsedTime(&t, b->mk[0], b->mk[7]));
+  SAT_CHECK(hipEventElapsedTime(&s, b->mk[0], b->mk[4]));
   if (total_ms) *total_ms = t;
   if (solve_ms) *solve_ms = s;
+  return SAT_OK;
+}
+
+int sat_batch_phase_ms(sat_batch* b, double* ms, int n) {
+  if (!b || !ms || n < 1) return SAT_EINVAL;
+  static const int from[7] = {0, 1, 2, 3, 5, 4, 0};
+  static const int to[7] = {1, 2, 3, 4, 6, 7, 7};
+  for (int i = 0; i < n && i < 7; i++) {
+    float t = 0.f;
+    SAT_CHECK(hipEventElapsedTime(&t, b->mk[from[i]], b->mk[to[i]]));
+    ms[i] = t;
+  }
   return SAT_OK;
 }
 

@@ -1,0 +1,49 @@
+// sat_launch.hpp -- host-side launchers of the kernel templates.  Each is
+// defined, with its instantiations over the columns-per-lane C, in one of the
+// k_*.hip translation units, so the kernels build in parallel.
+#pragma once
+#include "sat_fused.hpp"
+#include "sat_phases.hpp"
+
+namespace sat {
+
+// pair_kernel<PREC, C, TaT>; ta_f32: masked Ta stored as float (FP32 batches)
+template <int PREC>
+hipError_t launch_pair(int ta_f32, int C, dim3 grid, hipStream_t s, const BatchDev& bd, int first, int count,
+                       int group, const int* list, const int* dcount);
+template <int PREC>
+hipError_t launch_solo(int forward, int C, hipStream_t s, const SoloDev& sd);
+
+template <int PREC>
+hipError_t launch_trig(dim3 grid, hipStream_t s, const PhaseDev& pd);
+template <int PREC>
+hipError_t launch_init(dim3 grid, hipStream_t s, const PhaseDev& pd);
+template <int PREC>
+hipError_t launch_fwd_sweep(int iso, int C, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count,
+                            int group);
+hipError_t launch_recv(dim3 grid, hipStream_t s, const PhaseDev& pd, int nrec);
+template <int PREC>
+hipError_t launch_adj_setup(dim3 grid, hipStream_t s, const PhaseDev& pd);
+template <int PREC>
+hipError_t launch_delta(dim3 grid, hipStream_t s, const PhaseDev& pd);
+template <int PREC>
+hipError_t launch_adj_sweep(int C, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count);
+
+// Run f.template operator()<C>() for C in [1, kMaxC].
+template <class F>
+hipError_t with_C(int C, F&& f) {
+  switch (C) {
+    case 1: f.template operator()<1>(); break;
+    case 2: f.template operator()<2>(); break;
+    case 3: f.template operator()<3>(); break;
+    case 4: f.template operator()<4>(); break;
+    case 5: f.template operator()<5>(); break;
+    case 6: f.template operator()<6>(); break;
+    case 7: f.template operator()<7>(); break;
+    case 8: f.template operator()<8>(); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace sat

@@ -8,14 +8,15 @@
 //                      (:146-317); FP32: flags pairs for the FP64 re-solve
 //   recv_kernel        per receiver: interp2d, residual, adjoint source
 //                      (src/preproc.cpp:105-115)
-//   adj_setup_kernel   per (event, node): face fluxes and the sweep coefficients
-//                      (src/eikonal_solver.cpp:377-441)
+//   adj_setup_kernel   per (event, node): x- and y-face fluxes (src/eikonal_solver.cpp:377-441)
 //   delta_kernel       per event: receivers spread in receiver order (:356-375),
 //                      chi term (src/preproc.cpp:142)
 //   adj_sweep_kernel   persistent, one wave per event: adjoint iterations, mask,
 //                      masked Ta (src/eikonal_solver.cpp:471-536)
 // Node-parallel work runs at full occupancy; the wave-per-event kernels hold only
-// the sweep pipeline in registers.
+// the sweep pipeline in registers.  Every plane a sweep streams is stored
+// wavefront-major, one copy per sweep orientation (sat_device.hpp), so each
+// step of a sweep reads one contiguous run per plane.
 #pragma once
 #include <type_traits>
 
@@ -30,7 +31,7 @@ struct EventX {
   int fb[4];      // fixed near-source box x0, x1, y0, y1 (src/eikonal_solver.cpp:119-129)
   int mb[4];      // mask box x0, x1, y0, y1 (:528-531)
   double lon, lat;
-  size_t seg_off; // element offset of this event's segmented planes
+  size_t w_off;   // element offset of this event's wavefront-major planes
   size_t pl_off;  // element offset of this event's column-major planes
   size_t col_off; // offset into the column trig table
 };
@@ -41,12 +42,15 @@ struct PhaseDev {
   int nevent;
   double* evs;          // per event: fun0, sin(y0), cos(y0)
   double2* coltab;      // per (event, column): sin, cos of xx - x0
-  void* t0;             // per event segmented planes, sweep precision
-  void* t0x;
-  void* t0y;
+  // per event, wavefront-major, per orientation; the adjoint planes reuse the
+  // forward planes' memory once the forward sweeps are done
+  void* t0[2];          // T0, T0x, T0y (sweep precision)
+  void* t0x[2];
+  void* t0y[2];
+  void* fxy[2];         // face fluxes: FP64 (a1, b1); FP32 (a1/dx, b1/dy)
+  void* dl[2];          // delta (sweep precision)
   double* T0d;          // FP32 path: FP64 T0, column-major, per event
   double* T;            // travel time, column-major, per event
-  void* co[6];          // adjoint coefficients, segmented, per event
   void* ta_out;         // masked Ta, column-major, per event
   const double* rec_lon;
   const double* rec_lat;
@@ -77,17 +81,29 @@ template <int PREC> struct PhaseT;
 template <> struct PhaseT<SAT_PREC_F64> {
   using V = double;     // T0 planes and adjoint
   using U = double;     // forward swept value (tau)
+  using F = double2;    // face fluxes
   using Fwd = FwdF64;
   using FwdIso = FwdF64;
-  using Adj = AdjF64;
+  using Adj = AdjF64Face;
 };
 template <> struct PhaseT<SAT_PREC_F32> {
   using V = float;
   using U = double;     // sigma = tau - 1 in FP64
+  using F = float2;
   using Fwd = FwdF32;
   using FwdIso = FwdF32Iso;
-  using Adj = AdjF32;
+  using Adj = AdjF32Face;
 };
+
+// Decode element q of an orientation-0 wavefront-major plane: column ix, row j;
+// false for padding.
+__device__ inline bool wave_decode0(const GridDev& g, int q, int& ix, int& j) {
+  const int s = q / g.C, k = q - s * g.C;
+  const int dg = s / g.La, l = s - dg * g.La;
+  j = dg - l;
+  ix = l * g.C + k;
+  return j >= 0 && j < g.ny && ix < g.nx;
+}
 
 // ---------------------------------------------------------------------------
 template <int PREC>
@@ -119,23 +135,16 @@ __global__ void trig_kernel(PhaseDev pd) {
   }
 }
 
-// src/eikonal_solver.cpp:96-117; acos correctly rounded in the FP64 path.
+// src/eikonal_solver.cpp:96-117 per node; acos correctly rounded in the FP64 path.
 template <int PREC>
 __global__ void init_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
   const int e = blockIdx.y;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= g.P * g.ny) return;
-  const int iy = q / g.P, ix = q - iy * g.P;
-  V* t0 = static_cast<V*>(pd.t0) + ev.seg_off;
-  V* t0x = static_cast<V*>(pd.t0x) + ev.seg_off;
-  V* t0y = static_cast<V*>(pd.t0y) + ev.seg_off;
-  if (ix >= g.nx) {
-    t0[q] = V(0); t0x[q] = V(0); t0y[q] = V(0);
-    return;
-  }
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int)g.plane) return;
+  const int iy = p / g.nx, ix = p - iy * g.nx;
   const double fun0 = pd.evs[3 * (size_t)e], sy0 = pd.evs[3 * (size_t)e + 1], cy0 = pd.evs[3 * (size_t)e + 2];
   const double2 rw = g.trig_yy[iy];
   const double2 cl = pd.coltab[ev.col_off + ix];
@@ -150,10 +159,14 @@ __global__ void init_kernel(PhaseDev pd) {
     T0x = cyy * cy0 * sdl / sin_d * kR * fun0;
     T0y = -(cyy * sy0 - syy * cy0 * cdl) / sin_d * kR * fun0;
   }
-  t0[q] = (V)T0;
-  t0x[q] = (V)T0x;
-  t0y[q] = (V)T0y;
-  if (PREC == SAT_PREC_F32) pd.T0d[ev.pl_off + ix + (size_t)iy * g.nx] = T0;
+#pragma unroll
+  for (int o = 0; o < 2; o++) {
+    const size_t w = ev.w_off + wave_index(g, o, ix, iy);
+    static_cast<V*>(pd.t0[o])[w] = (V)T0;
+    static_cast<V*>(pd.t0x[o])[w] = (V)T0x;
+    static_cast<V*>(pd.t0y[o])[w] = (V)T0y;
+  }
+  if (PREC == SAT_PREC_F32) pd.T0d[ev.pl_off + p] = T0;
 }
 
 template <int PREC, bool ISO>
@@ -162,31 +175,30 @@ __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const
   using V = typename PT::V;
   using Pol = std::conditional_t<ISO, typename PT::FwdIso, typename PT::Fwd>;
   typename Pol::Ctx cx;
-  const size_t po = (size_t)ev.iper * g.seg_plane;
+  const size_t po = (size_t)ev.iper * g.wplane;
   cx.g = &g;
-  cx.t0 = static_cast<const V*>(pd.t0) + ev.seg_off;
-  cx.t0x = static_cast<const V*>(pd.t0x) + ev.seg_off;
-  cx.t0y = static_cast<const V*>(pd.t0y) + ev.seg_off;
   cx.fb = FixedBox{ev.fb[0], ev.fb[1], ev.fb[2], ev.fb[3]};
+#pragma unroll
+  for (int o = 0; o < 2; o++) {
+    cx.t0[o] = static_cast<const V*>(pd.t0[o]) + ev.w_off;
+    cx.t0x[o] = static_cast<const V*>(pd.t0x[o]) + ev.w_off;
+    cx.t0y[o] = static_cast<const V*>(pd.t0y[o]) + ev.w_off;
+    cx.f[o] = static_cast<const V*>(g.wmed[3][o]) + po;
+    if constexpr (!(PREC == SAT_PREC_F32 && ISO)) {
+      cx.a[o] = static_cast<const V*>(g.wmed[0][o]) + po;
+      cx.b[o] = static_cast<const V*>(g.wmed[1][o]) + po;
+      cx.c[o] = static_cast<const V*>(g.wmed[2][o]) + po;
+    }
+    if constexpr (PREC == SAT_PREC_F32 && !ISO) {
+      cx.dsx[o] = static_cast<const V*>(g.wmed[4][o]) + po;
+      cx.dsy[o] = static_cast<const V*>(g.wmed[5][o]) + po;
+    }
+  }
   if constexpr (PREC == SAT_PREC_F64) {
-    cx.a = static_cast<const V*>(g.seg[0]) + po;
-    cx.b = static_cast<const V*>(g.seg[1]) + po;
-    cx.c = static_cast<const V*>(g.seg[2]) + po;
-    cx.f = static_cast<const V*>(g.seg[3]) + po;
     cx.dx = g.dx;
     cx.dy = g.dy;
-  } else if constexpr (ISO) {
-    cx.f = static_cast<const V*>(g.seg[3]) + po;
-    cx.rowtab = g.rowtab;
-    cx.idx = (float)(1.0 / g.dx);
-    cx.idy = (float)(1.0 / g.dy);
   } else {
-    cx.a = static_cast<const V*>(g.seg[0]) + po;
-    cx.b = static_cast<const V*>(g.seg[1]) + po;
-    cx.c = static_cast<const V*>(g.seg[2]) + po;
-    cx.f = static_cast<const V*>(g.seg[3]) + po;
-    cx.dsx = static_cast<const V*>(g.seg[4]) + po;
-    cx.dsy = static_cast<const V*>(g.seg[5]) + po;
+    if constexpr (ISO) cx.rowtab = g.rowtab;
     cx.idx = (float)(1.0 / g.dx);
     cx.idy = (float)(1.0 / g.dy);
   }
@@ -213,16 +225,17 @@ __global__ __launch_bounds__(64) void fwd_sweep_kernel(PhaseDev pd, int first, i
     const int e = pd.order[first + idx];
     const EventX ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];
-    const int P = g.P, nx = g.nx, n = P * g.ny;
-    for (int q = lane; q < n; q += kWave) {  // :119-129 initial tau (sigma)
-      const int iy = q / P, ix = q - iy * P;
-      const bool fixed = ix < nx && ix >= ev.fb[0] && ix <= ev.fb[1] && iy >= ev.fb[2] && iy <= ev.fb[3];
-      Q0[q] = kSigma ? (fixed ? U(0) : U(9)) : (fixed ? U(1) : U(10));
+    const int nx = g.nx, n = (int)g.wplane;
+    for (int q = lane; q < n; q += kWave) {  // :119-129 initial tau (sigma), orientation 0
+      int ix, iy;
+      const bool real = wave_decode0(g, q, ix, iy);
+      const bool fixed = real && ix >= ev.fb[0] && ix <= ev.fb[1] && iy >= ev.fb[2] && iy <= ev.fb[3];
+      Q0[q] = !real ? U(0) : kSigma ? (fixed ? U(0) : U(9)) : (fixed ? U(1) : U(10));
     }
     __syncthreads();
     const auto cx = phase_fwd_ctx<PREC, ISO>(pd, g, ev);
     IterOut io;
-    const U* u = iterate<Pol, C>(cx, Q0, Q1, Q2, pd.fwd_maxiter, kFwdTol, io);
+    const U* u = iterate<Pol, WaveLayout, C>(cx, Q0, Q1, Q2, pd.fwd_maxiter, kFwdTol, io);
     if (lane == 0) {
       pd.conv[2 * (size_t)e] = io.linf;
       pd.conv[2 * (size_t)e + 1] = io.linf_prev;
@@ -246,10 +259,10 @@ __global__ __launch_bounds__(64) void fwd_sweep_kernel(PhaseDev pd, int first, i
     }
     // :317 T = tau .* T0 (FP64) or T0 + sigma*T0 (FP32 sweeps), column-major
     double* T = pd.T + ev.pl_off;
-    const typename PT::V* t0 = static_cast<const typename PT::V*>(pd.t0) + ev.seg_off;
+    const typename PT::V* t0 = static_cast<const typename PT::V*>(pd.t0[0]) + ev.w_off;
     for (int iy = 0; iy < g.ny; iy++)
       for (int ix = lane; ix < nx; ix += kWave) {
-        const int o = iy * P + ix;
+        const size_t o = wave_index(g, 0, ix, iy);
         const size_t p = ix + (size_t)iy * nx;
         if (kSigma) T[p] = pd.T0d[ev.pl_off + p] + (double)u[o] * pd.T0d[ev.pl_off + p];
         else T[p] = (double)u[o] * (double)t0[o];
@@ -258,87 +271,54 @@ __global__ __launch_bounds__(64) void fwd_sweep_kernel(PhaseDev pd, int first, i
   }
 }
 
-// src/preproc.cpp:105-115 per receiver, and the receiver's delta cell and
-// weight (src/eikonal_solver.cpp:357-369).
-__global__ void recv_kernel(PhaseDev pd, int nrec) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nrec; k += gridDim.x * blockDim.x) {
-    const int e = pd.rec_ev[k];
-    if (e < 0 || pd.skip[e]) continue;
-    const EventX& ev = pd.ev[e];
-    const GridDev& g = pd.grids[ev.grid];
-    const double s = interp2d_point(g.lon, g.nx, g.lat, g.ny, pd.T + ev.pl_off, pd.rec_lon[k], pd.rec_lat[k]);
-    pd.syn[k] = s;
-    const double adj = pd.weight[k] * (s - pd.tt_obs[k]);
-    pd.adjsrc[k] = adj;
-    if (s != s) atomicOr(pd.flags + e, FLAG_REC_OUTSIDE);
-    const double xrec = pd.rec_lon[k] * kDEG2RAD;
-    const double yrec = pd.rec_lat[k] * kDEG2RAD;
-    int idx0 = (int)floor((xrec - g.xx[0]) / g.dx);
-    int idy0 = (int)floor((yrec - g.yy[0]) / g.dy);
-    idx0 = imax(0, imin(idx0, g.nx - 2));
-    idy0 = imax(0, imin(idy0, g.ny - 2));
-    const double r1 = dmin(1.0, (xrec - g.xx[idx0]) / g.dx);
-    const double r2 = dmin(1.0, (yrec - g.yy[idy0]) / g.dy);
-    const double area = g.dx * kR * cr::cos_cr(yrec) * g.dy * kR;
-    pd.rcell[k] = make_int2(idx0, idy0);
-    pd.rwt[3 * (size_t)k] = adj / area;
-    pd.rwt[3 * (size_t)k + 1] = r1;
-    pd.rwt[3 * (size_t)k + 2] = r2;
-  }
-}
+// recv_kernel (k_phase64.hip): src/preproc.cpp:105-115 per receiver, and the
+// receiver's delta cell and weight (src/eikonal_solver.cpp:357-369).
+__global__ void recv_kernel(PhaseDev pd, int nrec);
 
-// src/eikonal_solver.cpp:407-441, :446-449 per node.  FP64: a1p, a2m, b1p, b2m,
-// dl = 0, d.  FP32: cW, cE, cS, cN (the update divided through by d), cD = 0.
+// Per node: x-face flux a1 on the columns 1..nx-1 of the interior rows, y-face
+// flux b1 on the rows 1..ny-1 of the interior columns (0 elsewhere), delta = 0.
+// FP32 stores (a1/dx, b1/dy) rounded once.
 template <int PREC>
 __global__ void adj_setup_kernel(PhaseDev pd) {
+  using V = typename PhaseT<PREC>::V;
+  using F = typename PhaseT<PREC>::F;
   const int e = blockIdx.y;
   if (pd.skip[e]) return;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= g.P * g.ny) return;
-  const int iy = q / g.P, ix = q - iy * g.P;
-  const bool interior = ix >= 1 && ix < g.nx - 1 && iy >= 1 && iy < g.ny - 1;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (int)g.plane) return;
+  const int nx = g.nx, ny = g.ny;
+  const int iy = p / nx, ix = p - iy * nx;
+  const double* T = pd.T + ev.pl_off;
   const double* a = g.aA_plain + (size_t)ev.iper * g.plane;
   const double* b = g.b_plain + (size_t)ev.iper * g.plane;
   const double* c = g.c_plain + (size_t)ev.iper * g.plane;
-  const size_t o = ev.seg_off + q;
-  if constexpr (PREC == SAT_PREC_F64) {
-    double v[6] = {0, 0, 0, 0, 0, 0};
-    if (interior) {
-      const Faces F = faces_at(g, pd.T + ev.pl_off, a, b, c, ix, iy);
-      v[0] = F.a1p; v[1] = F.a2m; v[2] = F.b1p; v[3] = F.b2m;
-      v[5] = (F.a2p - F.a1m) / g.dx + (F.b2p - F.b1m) / g.dy;  // :479-481
-    }
+  double fx = 0.0, fy = 0.0;
+  if (ix >= 1 && iy >= 1 && iy < ny - 1) fx = face_a1(g, T, a, c, ix, iy);
+  if (iy >= 1 && ix >= 1 && ix < nx - 1) fy = face_b1(g, T, b, c, ix, iy);
+  F v;
+  if constexpr (PREC == SAT_PREC_F64) v = make_double2(fx, fy);
+  else v = make_float2((float)(fx / g.dx), (float)(fy / g.dy));
 #pragma unroll
-    for (int i = 0; i < 6; i++) static_cast<double*>(pd.co[i])[o] = v[i];
-  } else {
-    float v[5] = {0, 0, 0, 0, 0};
-    if (interior) {
-      const Faces F = faces_at(g, pd.T + ev.pl_off, a, b, c, ix, iy);
-      const double d = (F.a2p - F.a1m) / g.dx + (F.b2p - F.b1m) / g.dy;
-      if (!(fabs(d) < kAdjEps)) {
-        const double rdx = 1.0 / (g.dx * d), rdy = 1.0 / (g.dy * d);
-        v[0] = (float)(F.a1p * rdx);
-        v[1] = (float)(-F.a2m * rdx);
-        v[2] = (float)(F.b1p * rdy);
-        v[3] = (float)(-F.b2m * rdy);
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 5; i++) static_cast<float*>(pd.co[i])[o] = v[i];
+  for (int o = 0; o < 2; o++) {
+    const size_t w = ev.w_off + wave_index(g, o, ix, iy);
+    static_cast<F*>(pd.fxy[o])[w] = v;
+    static_cast<V*>(pd.dl[o])[w] = V(0);
   }
 }
 
-// src/eikonal_solver.cpp:356-375 and src/preproc.cpp:142, one thread per event.
-// delta(node) is the receiver-order sum of the contributions to that node.
+// src/eikonal_solver.cpp:356-375 and src/preproc.cpp:142, one thread per event,
+// receivers in order.  FP64: delta accumulated in the orientation-0 plane, then
+// copied.  FP32: accumulated in FP64 in the event's T0d plane (no longer needed
+// once T is formed), then rounded once into both planes.
 template <int PREC>
 __global__ void delta_kernel(PhaseDev pd) {
+  using V = typename PhaseT<PREC>::V;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= pd.nevent || pd.skip[e]) return;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
-  const int nx = g.nx, ny = g.ny;
   double s = 0.0;
   for (int r = 0; r < ev.nrec; r++) {
     const int k = ev.rec_off + r;
@@ -346,48 +326,39 @@ __global__ void delta_kernel(PhaseDev pd) {
     s += pd.weight[k] * (d * d);
   }
   pd.chi_ev[e] = 0.5 * s;
-  // contribution m of receiver k to node (idx0 + (m>>1), idy0 + (m&1)), order of :371-374
-  auto contrib = [&](int k, int m) {
-    const double w = pd.rwt[3 * (size_t)k], r1 = pd.rwt[3 * (size_t)k + 1], r2 = pd.rwt[3 * (size_t)k + 2];
-    return (m == 0) ? w * (1 - r1) * (1 - r2) : (m == 1) ? w * (1 - r1) * r2 : (m == 2) ? w * r1 * (1 - r2)
-                                                                                       : w * r1 * r2;
-  };
   const int k0 = ev.rec_off, k1 = ev.rec_off + ev.nrec;
-  if constexpr (PREC == SAT_PREC_F64) {
-    double* dl = static_cast<double*>(pd.co[4]) + ev.seg_off;
+  V* dl0 = static_cast<V*>(pd.dl[0]) + ev.w_off;
+  V* dl1 = static_cast<V*>(pd.dl[1]) + ev.w_off;
+  double* acc = pd.T0d + ev.pl_off;  // FP32 only
+  for (int pass = 0; pass < ((PREC == SAT_PREC_F64) ? 2 : 3); pass++) {
     for (int k = k0; k < k1; k++) {
-      const int2 c = pd.rcell[k];
-      for (int m = 0; m < 4; m++) dl[(c.y + (m & 1)) * g.P + c.x + (m >> 1)] += contrib(k, m);
-    }
-  } else {
-    float* cD = static_cast<float*>(pd.co[4]) + ev.seg_off;
-    const double* a = g.aA_plain + (size_t)ev.iper * g.plane;
-    const double* b = g.b_plain + (size_t)ev.iper * g.plane;
-    const double* cc = g.c_plain + (size_t)ev.iper * g.plane;
-    for (int k = k0; k < k1; k++) {
-      const int2 c = pd.rcell[k];
-      for (int m = 0; m < 4; m++) {
-        const int ni = c.x + (m >> 1), nj = c.y + (m & 1);
-        bool seen = false;  // an earlier receiver touched this node: already summed
-        for (int kk = k0; kk < k && !seen; kk++) {
-          const int2 c2 = pd.rcell[kk];
-          seen = (ni - c2.x == 0 || ni - c2.x == 1) && (nj - c2.y == 0 || nj - c2.y == 1);
+      const int2 cc = pd.rcell[k];
+      const double w = pd.rwt[3 * (size_t)k], r1 = pd.rwt[3 * (size_t)k + 1], r2 = pd.rwt[3 * (size_t)k + 2];
+      for (int m = 0; m < 4; m++) {  // order of :371-374
+        const int i = cc.x + (m >> 1), j = cc.y + (m & 1);
+        const size_t w0 = wave_index(g, 0, i, j), w1 = wave_index(g, 1, i, j);
+        const size_t pp = i + (size_t)j * g.nx;
+        if constexpr (PREC == SAT_PREC_F64) {
+          if (pass == 0) {
+            const double v = (m == 0) ? w * (1 - r1) * (1 - r2) : (m == 1) ? w * (1 - r1) * r2
+                           : (m == 2) ? w * r1 * (1 - r2) : w * r1 * r2;
+            dl0[w0] += v;
+          } else {
+            dl1[w1] = dl0[w0];
+          }
+        } else {
+          if (pass == 0) {
+            acc[pp] = 0.0;
+          } else if (pass == 1) {
+            const double v = (m == 0) ? w * (1 - r1) * (1 - r2) : (m == 1) ? w * (1 - r1) * r2
+                           : (m == 2) ? w * r1 * (1 - r2) : w * r1 * r2;
+            acc[pp] += v;
+          } else {
+            const float f = (float)acc[pp];
+            dl0[w0] = f;
+            dl1[w1] = f;
+          }
         }
-        if (seen) continue;
-        double acc = 0.0;
-        for (int kk = k; kk < k1; kk++) {
-          const int2 c2 = pd.rcell[kk];
-          for (int mm = 0; mm < 4; mm++)
-            if (c2.x + (mm >> 1) == ni && c2.y + (mm & 1) == nj) acc += contrib(kk, mm);
-        }
-        const bool interior = ni >= 1 && ni < nx - 1 && nj >= 1 && nj < ny - 1;
-        float v = 0.0f;
-        if (interior) {
-          const Faces F = faces_at(g, pd.T + ev.pl_off, a, b, cc, ni, nj);
-          const double d = (F.a2p - F.a1m) / g.dx + (F.b2p - F.b1m) / g.dy;
-          if (!(fabs(d) < kAdjEps)) v = (float)(acc / d);
-        }
-        cD[nj * g.P + ni] = v;
       }
     }
   }
@@ -398,6 +369,7 @@ template <int PREC, int C>
 __global__ __launch_bounds__(64) void adj_sweep_kernel(PhaseDev pd, int first, int count) {
   using PT = PhaseT<PREC>;
   using V = typename PT::V;
+  using F = typename PT::F;
   using Pol = typename PT::Adj;
   const int lane = threadIdx.x;
   char* sb = pd.slots + (size_t)blockIdx.x * pd.slot_bytes;
@@ -413,32 +385,26 @@ __global__ __launch_bounds__(64) void adj_sweep_kernel(PhaseDev pd, int first, i
     if (pd.skip[e]) continue;
     const EventX ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];
-    const int P = g.P, nx = g.nx, ny = g.ny, n = P * ny;
-    for (int q = lane; q < n; q += kWave) {  // :446-449 boundary 0, interior 100
-      const int iy = q / P, ix = q - iy * P;
-      Q0[q] = (ix >= 1 && ix < nx - 1 && iy >= 1 && iy < ny - 1) ? V(100) : V(0);
+    const int nx = g.nx, ny = g.ny, n = (int)g.wplane;
+    for (int q = lane; q < n; q += kWave) {  // :446-449 boundary 0, interior 100, orientation 0
+      int ix, iy;
+      const bool real = wave_decode0(g, q, ix, iy);
+      Q0[q] = (real && ix >= 1 && ix < nx - 1 && iy >= 1 && iy < ny - 1) ? V(100) : V(0);
     }
     __syncthreads();
     typename Pol::Ctx cx;
     cx.g = &g;
+#pragma unroll
+    for (int o = 0; o < 2; o++) {
+      cx.fxy[o] = static_cast<const F*>(pd.fxy[o]) + ev.w_off;
+      cx.dl[o] = static_cast<const V*>(pd.dl[o]) + ev.w_off;
+    }
     if constexpr (PREC == SAT_PREC_F64) {
-      cx.a1p = static_cast<const double*>(pd.co[0]) + ev.seg_off;
-      cx.a2m = static_cast<const double*>(pd.co[1]) + ev.seg_off;
-      cx.b1p = static_cast<const double*>(pd.co[2]) + ev.seg_off;
-      cx.b2m = static_cast<const double*>(pd.co[3]) + ev.seg_off;
-      cx.dl = static_cast<const double*>(pd.co[4]) + ev.seg_off;
-      cx.d = static_cast<const double*>(pd.co[5]) + ev.seg_off;
       cx.dx = g.dx;
       cx.dy = g.dy;
-    } else {
-      cx.cW = static_cast<const float*>(pd.co[0]) + ev.seg_off;
-      cx.cE = static_cast<const float*>(pd.co[1]) + ev.seg_off;
-      cx.cS = static_cast<const float*>(pd.co[2]) + ev.seg_off;
-      cx.cN = static_cast<const float*>(pd.co[3]) + ev.seg_off;
-      cx.cD = static_cast<const float*>(pd.co[4]) + ev.seg_off;
     }
     IterOut io;
-    const V* ta = iterate<Pol, C>(cx, Q0, Q1, Q2, pd.adj_maxiter, kAdjTol, io);
+    const V* ta = iterate<Pol, WaveLayout, C>(cx, Q0, Q1, Q2, pd.adj_maxiter, kAdjTol, io);
     if (lane == 0) {
       pd.iters_a[e] = io.iters;
       if (!io.converged) atomicOr(pd.flags + e, FLAG_ADJ_MAXITER);
@@ -447,7 +413,7 @@ __global__ __launch_bounds__(64) void adj_sweep_kernel(PhaseDev pd, int first, i
     for (int iy = 0; iy < ny; iy++)
       for (int ix = lane; ix < nx; ix += kWave) {
         const bool m = ix >= ev.mb[0] && ix <= ev.mb[1] && iy >= ev.mb[2] && iy <= ev.mb[3];
-        out[ix + (size_t)iy * nx] = m ? V(0) : ta[iy * P + ix];
+        out[ix + (size_t)iy * nx] = m ? V(0) : ta[wave_index(g, 0, ix, iy)];
       }
     __syncthreads();
   }

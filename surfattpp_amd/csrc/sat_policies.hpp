@@ -32,7 +32,8 @@ struct Slot {
 };
 
 // ============================================================================
-// FP64, reference operation order
+// FP64, reference operation order.  Plane pointers are per sweep orientation
+// (the per-solve kernels set both to the same segmented plane).
 // ============================================================================
 struct FwdF64 {
   using V = double;
@@ -40,27 +41,27 @@ struct FwdF64 {
   static constexpr bool kAux = true;
   struct Ctx {
     const GridDev* g;
-    const double *t0, *t0x, *t0y, *a, *b, *c, *f;
+    const double *t0[2], *t0x[2], *t0y[2], *a[2], *b[2], *c[2], *f[2];
     FixedBox fb;
     double dx, dy;
   };
   template <int C> struct Head { double u[C], aux[C]; };
   template <int C> struct Body { double t0x[C], t0y[C], a[C], b[C], c[C], f[C], st[C]; };
-  template <int C>
+  template <int C, int O>
   __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
 #pragma unroll
-    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[base + k]; }
+    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[O][base + k]; }
   }
-  template <int C>
+  template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      B.t0x[k] = cx.t0x[base + k];
-      B.t0y[k] = cx.t0y[base + k];
-      B.a[k] = cx.a[base + k];
-      B.b[k] = cx.b[base + k];
-      B.c[k] = cx.c[base + k];
-      B.f[k] = cx.f[base + k];
+      B.t0x[k] = cx.t0x[O][base + k];
+      B.t0y[k] = cx.t0y[O][base + k];
+      B.a[k] = cx.a[O][base + k];
+      B.b[k] = cx.b[O][base + k];
+      B.c[k] = cx.c[O][base + k];
+      B.f[k] = cx.f[O][base + k];
       B.st[k] = tst ? tst[base + k] : 0.0;
     }
   }
@@ -80,6 +81,7 @@ struct FwdF64 {
   }
 };
 
+// Per-solve adjoint: precomputed a1p, a2m, b1p, b2m, dl, d (one segmented plane each).
 struct AdjF64 {
   using V = double;
   using A = double;
@@ -91,12 +93,12 @@ struct AdjF64 {
   };
   template <int C> struct Head { double u[C], aux[C]; };
   template <int C> struct Body { double a1p[C], a2m[C], b1p[C], b2m[C], dl[C], d[C], st[C]; };
-  template <int C>
+  template <int C, int O>
   __device__ static void load_head(const Ctx&, Head<C>& H, int base, int, const double* tin) {
 #pragma unroll
     for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = 0.0; }
   }
-  template <int C>
+  template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
@@ -123,43 +125,109 @@ struct AdjF64 {
   }
 };
 
+// Batched adjoint, face form.  a2 at node (ix, iy) is the same expression, in
+// the same operation order, as a1 at (ix+1, iy), and b2 at (ix, iy) as b1 at
+// (ix, iy+1) (src/eikonal_solver.cpp:407-441), so one x-face and one y-face
+// flux per node, F = (a1, b1), carry all four: the node reads its own F and,
+// through the neighbour auxiliaries, a2 = F_E.x and b2 = F_N.y.  The splits,
+// d (:479-481) and the update (:483-490) are evaluated per visit in the
+// reference's order, so the result is the same bits as the precomputed form.
+struct AdjF64Face {
+  using V = double;
+  using A = double2;
+  static constexpr bool kAux = true;
+  struct Ctx {
+    const GridDev* g;
+    const double2* fxy[2];  // (a1, b1) per node
+    const double* dl[2];    // delta
+    double dx, dy;
+  };
+  template <int C> struct Head { double u[C]; double2 aux[C]; };
+  template <int C> struct Body { double dl[C], st[C]; };
+  template <int C, int O>
+  __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
+#pragma unroll
+    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.fxy[O][base + k]; }
+  }
+  template <int C, int O>
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      B.dl[k] = cx.dl[O][base + k];
+      B.st[k] = tst ? tst[base + k] : 0.0;
+    }
+  }
+  __device__ static bool fixed(const Ctx& cx, int ix, int j) {
+    return ix == 0 || ix >= cx.g->nx - 1 || j == 0 || j == cx.g->ny - 1;
+  }
+  template <int C>
+  __device__ static double update(const Ctx& cx, const Head<C>& h, const Body<C>& r, int k, int, double uW, double2,
+                                  bool, double uE, double2 aE, bool, double uS, double2, bool, double uN, double2 aN,
+                                  bool) {
+    const double a1 = h.aux[k].x, b1 = h.aux[k].y, a2 = aE.x, b2 = aN.y;
+    const double a1m = (a1 - fabs(a1)) / 2.0, a1p = (a1 + fabs(a1)) / 2.0;
+    const double a2m = (a2 - fabs(a2)) / 2.0, a2p = (a2 + fabs(a2)) / 2.0;
+    const double b1m = (b1 - fabs(b1)) / 2.0, b1p = (b1 + fabs(b1)) / 2.0;
+    const double b2m = (b2 - fabs(b2)) / 2.0, b2p = (b2 + fabs(b2)) / 2.0;
+    const double d = (a2p - a1m) / cx.dx + (b2p - b1m) / cx.dy;
+    return adj_update_f64(a1p, a2m, b1p, b2m, r.dl[k], d, uW, uE, uS, uN, cx.dx, cx.dy);
+  }
+  template <int C>
+  __device__ static double norm(const Head<C>&, const Body<C>& r, int k, double v) {
+    return fabs(v - r.st[k]);  // :501
+  }
+};
+
 // ============================================================================
 // FP32 sweeps on sigma = tau - 1
 // ============================================================================
 
 // One forward node in increment form.  Same candidate set and acceptance tests
-// as src/eikonal_solver.cpp:173-283, written for sigma:
-//   two-point (sx, sy): with sr = min(sX, sY), tau = 1 + sr + w,
-//     Tx = Tx0 + px*w,  Tx0 = t0x*(1+sr) + sx*h*(sX - sr),  px = t0x - sx*h
+// as src/eikonal_solver.cpp:173-283, written for sigma relative to the smallest
+// present neighbour value u_ref (so every difference below is the variation of
+// sigma over about a cell, exact to FP32 relative precision):
+//   dX = sigma_X - u_ref, tau_ref = 1 + u_ref,
+//   two-point (sx, sy): dr = min(dX, dY), tau_r = tau_ref + dr, tau = tau_r + w,
+//     Tx = Tx0 + px*w,  Tx0 = t0x*tau_r + sx*h*(dX - dr),  px = t0x - sx*h
 //     (y alike), a*Tx^2 + b*Ty^2 - 2c*Tx*Ty = f^2 solved for w with the stable
-//     quadratic formula; accepted if -sx*(a*Tx - c*Ty) >= 0 and -sy*(b*Ty - c*Tx) >= 0.
-//   one-point (W): tau = tau_W + w, w = (+-dis - t0x*tau_W)/(t0x + h), accepted if
-//     tau*T0 >= tau_W*T0_W and tau > tau_W/2 (E, S, N alike).
-// Neighbour sigmas and the result are FP64; w is computed in FP32.  ISO: the
-// isotropic metric (c == 0, a and b constant along a row), dsx = f/sqrt(a),
-// dsy = f/sqrt(b).
+//     quadratic formula; accepted if -sx*(a*Tx - c*Ty) >= 0 and -sy*(b*Ty - c*Tx) >= 0;
+//     candidate dr + w.
+//   one-point (W): tau_W = tau_ref + dW, w = (+-dis - t0x*tau_W)/(t0x + h), accepted
+//     if (tau_W + w)*T0 >= tau_W*T0_W and tau_W + w > tau_W/2; candidate dW + w.
+// Candidates are compared with sigma_self - u_ref; the smallest, if below it, is
+// the new value u_ref + candidate formed in FP64 (the reference keeps the
+// smaller of the old and the new tau).  Values only ever decrease, so the
+// rounding of a large change would persist: measuring from u_ref keeps the
+// rounded quantities small.  ISO: the isotropic metric (c == 0, a and b constant
+// along a row), dsx = f/sqrt(a), dsy = f/sqrt(b).
 template <bool ISO>
 __device__ __forceinline__ double fwd_update_f32(double u_self, float t0, float t0x, float t0y, float a, float b,
                                                  float c, float f2, float dsx, float dsy, float h, float kk,
                                                  bool hW, double uW, float T0W, bool hE, double uE, float T0E,
                                                  bool hS, double uS, float T0S, bool hN, double uN, float T0N) {
 #pragma clang fp contract(fast)
-  double un = u_self;
+  // Straight-line: every candidate is evaluated on every lane and masked, so the
+  // wave runs one instruction stream (a rejected or absent candidate is +inf).
+  constexpr double kAbsent = 1e300;
+  const float kInf = __builtin_huge_valf();
+  const double u_ref = fmin(fmin(hW ? uW : kAbsent, hE ? uE : kAbsent), fmin(hS ? uS : kAbsent, hN ? uN : kAbsent));
+  const float ts = (float)(1.0 + u_ref);
+  const float dW = (float)(uW - u_ref), dE = (float)(uE - u_ref);
+  const float dS = (float)(uS - u_ref), dN = (float)(uN - u_ref);
+  float m[4];  // best candidate per quadrant / direction group
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const float sx = (q < 2) ? -1.0f : 1.0f;
     const float sy = (q & 1) ? 1.0f : -1.0f;
-    const bool hx = (q < 2) ? hW : hE;
-    const bool hy = (q & 1) ? hN : hS;
-    if (!(hx && hy)) continue;
-    const double uX = (q < 2) ? uW : uE;
-    const double uY = (q & 1) ? uN : uS;
-    const double ur = (uY < uX) ? uY : uX;
+    const bool ok = ((q < 2) ? hW : hE) & ((q & 1) ? hN : hS);
+    const float dX = (q < 2) ? dW : dE;
+    const float dY = (q & 1) ? dN : dS;
+    const float dr = fminf(dX, dY);
     const float px = t0x - sx * h;
     const float py = t0y - sy * kk;
-    const float tr = (float)(ur + 1.0);
-    const float Tx0 = t0x * tr + sx * h * (float)(uX - ur);
-    const float Ty0 = t0y * tr + sy * kk * (float)(uY - ur);
+    const float tr = ts + dr;
+    const float Tx0 = t0x * tr + sx * h * (dX - dr);
+    const float Ty0 = t0y * tr + sy * kk * (dY - dr);
     float Aq, Bh, Cq;
     if constexpr (ISO) {
       Aq = a * px * px + b * py * py;
@@ -171,11 +239,12 @@ __device__ __forceinline__ double fwd_update_f32(double u_self, float t0, float 
       Cq = a * Tx0 * Tx0 + b * Ty0 * Ty0 - 2.0f * c * Tx0 * Ty0 - f2;
     }
     const float D = Bh * Bh - Aq * Cq;
-    if (D < 0.0f || fabsf(Aq) < 1e-30f) continue;
-    const float sq = __builtin_amdgcn_sqrtf(D);
+    const bool okq = ok & (D >= 0.0f) & (fabsf(Aq) >= 1e-30f);
+    const float sq = __builtin_amdgcn_sqrtf(fmaxf(D, 0.0f));
     const float sv = -(Bh + ((Bh >= 0.0f) ? sq : -sq));
     const float w0 = sv * __builtin_amdgcn_rcpf(Aq);
     const float w1 = (sv != 0.0f) ? Cq * __builtin_amdgcn_rcpf(sv) : w0;
+    float mq = kInf;
 #pragma unroll
     for (int g = 0; g < 2; g++) {
       const float w = g ? w1 : w0;
@@ -183,35 +252,37 @@ __device__ __forceinline__ double fwd_update_f32(double u_self, float t0, float 
       const float Ty = Ty0 + py * w;
       const float cxv = ISO ? a * Tx : a * Tx - c * Ty;
       const float cyv = ISO ? b * Ty : b * Ty - c * Tx;
-      if (-sx * cxv >= 0.0f && -sy * cyv >= 0.0f) {
-        const double cand = ur + (double)w;
-        un = (cand < un) ? cand : un;
-      }
+      const bool acc = okq & (-sx * cxv >= 0.0f) & (-sy * cyv >= 0.0f);
+      mq = fminf(mq, acc ? dr + w : kInf);
     }
+    m[q] = mq;
   }
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const bool hq = (q == 0) ? hW : (q == 1) ? hE : (q == 2) ? hS : hN;
     const float dis = (q < 2) ? dsx : dsy;
-    if (!hq || dis < 0.0f) continue;
-    const double un_ = (q == 0) ? uW : (q == 1) ? uE : (q == 2) ? uS : uN;
+    const bool ok = hq & (dis >= 0.0f);
+    const float dq = (q == 0) ? dW : (q == 1) ? dE : (q == 2) ? dS : dN;
     const float T0n = (q == 0) ? T0W : (q == 1) ? T0E : (q == 2) ? T0S : T0N;
     const float td = (q < 2) ? t0x : t0y;
     const float hh = (q < 2) ? h : kk;
     const float p = (q == 0 || q == 2) ? td + hh : td - hh;
-    const double tnd = un_ + 1.0;
-    const float Td0 = td * (float)tnd;
+    const float tnd = ts + dq;
+    const float Td0 = td * tnd;
     const float rp = __builtin_amdgcn_rcpf(p);
-    const double Tn = tnd * (double)T0n;
+    const float Tn = tnd * T0n;
+    const float half = 0.5f * tnd;
 #pragma unroll
     for (int g = 0; g < 2; g++) {
       const float w = ((g == 0) ? dis - Td0 : -dis - Td0) * rp;
-      const double cu = un_ + (double)w;
-      const double tc = cu + 1.0;
-      if (tc * (double)t0 >= Tn && tc > tnd * 0.5) un = (cu < un) ? cu : un;
+      const float tc = tnd + w;
+      const bool acc = ok & (tc * t0 >= Tn) & (tc > half);
+      m[q] = fminf(m[q], acc ? dq + w : kInf);
     }
   }
-  return un;
+  const float best = fminf(fminf(m[0], m[1]), fminf(m[2], m[3]));
+  const double v = u_ref + (double)best;  // +inf when no candidate was accepted
+  return ((best < (float)(u_self - u_ref)) & (v < u_self)) ? v : u_self;
 }
 
 struct FwdF32 {
@@ -220,29 +291,29 @@ struct FwdF32 {
   static constexpr bool kAux = true;
   struct Ctx {
     const GridDev* g;
-    const float *t0, *t0x, *t0y, *a, *b, *c, *f, *dsx, *dsy;
+    const float *t0[2], *t0x[2], *t0y[2], *a[2], *b[2], *c[2], *f[2], *dsx[2], *dsy[2];
     FixedBox fb;
     float idx, idy;  // 1/dx, 1/dy (radians)
   };
   template <int C> struct Head { double u[C]; float aux[C]; };
   template <int C> struct Body { double st[C]; float t0x[C], t0y[C], a[C], b[C], c[C], f[C], dsx[C], dsy[C]; };
-  template <int C>
+  template <int C, int O>
   __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
 #pragma unroll
-    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[base + k]; }
+    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[O][base + k]; }
   }
-  template <int C>
+  template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      B.t0x[k] = cx.t0x[base + k];
-      B.t0y[k] = cx.t0y[base + k];
-      B.a[k] = cx.a[base + k];
-      B.b[k] = cx.b[base + k];
-      B.c[k] = cx.c[base + k];
-      B.f[k] = cx.f[base + k];
-      B.dsx[k] = cx.dsx[base + k];
-      B.dsy[k] = cx.dsy[base + k];
+      B.t0x[k] = cx.t0x[O][base + k];
+      B.t0y[k] = cx.t0y[O][base + k];
+      B.a[k] = cx.a[O][base + k];
+      B.b[k] = cx.b[O][base + k];
+      B.c[k] = cx.c[O][base + k];
+      B.f[k] = cx.f[O][base + k];
+      B.dsx[k] = cx.dsx[O][base + k];
+      B.dsy[k] = cx.dsy[O][base + k];
       B.st[k] = tst ? tst[base + k] : 0.0;
     }
   }
@@ -272,25 +343,25 @@ struct FwdF32Iso {
   static constexpr bool kAux = true;
   struct Ctx {
     const GridDev* g;
-    const float *t0, *t0x, *t0y, *f;
+    const float *t0[2], *t0x[2], *t0y[2], *f[2];
     const float4* rowtab;
     FixedBox fb;
     float idx, idy;
   };
   template <int C> struct Head { double u[C]; float aux[C]; };
   template <int C> struct Body { double st[C]; float t0x[C], t0y[C], f[C]; float4 row; };
-  template <int C>
+  template <int C, int O>
   __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
 #pragma unroll
-    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[base + k]; }
+    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[O][base + k]; }
   }
-  template <int C>
+  template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int j, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      B.t0x[k] = cx.t0x[base + k];
-      B.t0y[k] = cx.t0y[base + k];
-      B.f[k] = cx.f[base + k];
+      B.t0x[k] = cx.t0x[O][base + k];
+      B.t0y[k] = cx.t0y[O][base + k];
+      B.f[k] = cx.f[O][base + k];
       B.st[k] = tst ? tst[base + k] : 0.0;
     }
     B.row = cx.rowtab[j];
@@ -314,6 +385,7 @@ struct FwdF32Iso {
   }
 };
 
+// Per-solve FP32 adjoint: coefficients divided through by d (one segmented plane each).
 struct AdjF32 {
   using V = float;
   using A = float;
@@ -324,12 +396,12 @@ struct AdjF32 {
   };
   template <int C> struct Head { float u[C], aux[C]; };
   template <int C> struct Body { float cW[C], cE[C], cS[C], cN[C], cD[C], st[C]; };
-  template <int C>
+  template <int C, int O>
   __device__ static void load_head(const Ctx&, Head<C>& H, int base, int, const float* tin) {
 #pragma unroll
     for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = 0.0f; }
   }
-  template <int C>
+  template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const float* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
@@ -352,6 +424,57 @@ struct AdjF32 {
   }
   template <int C>
   __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, float v) {
+    return fabs((double)v - (double)r.st[k]);
+  }
+};
+
+// Batched FP32 adjoint, face form (see AdjF64Face).  The face fluxes are stored
+// pre-divided by dx and dy: g = (a1/dx, b1/dy), so d = max(gE.x,0) - min(g.x,0)
+// + max(gN.y,0) - min(g.y,0) and Ta = (delta + max(g.x,0)*Ta_W - min(gE.x,0)*Ta_E
+// + max(g.y,0)*Ta_S - min(gN.y,0)*Ta_N) / d.
+struct AdjF32Face {
+  using V = float;
+  using A = float2;
+  static constexpr bool kAux = true;
+  struct Ctx {
+    const GridDev* g;
+    const float2* fxy[2];
+    const float* dl[2];
+  };
+  template <int C> struct Head { float u[C]; float2 aux[C]; };
+  template <int C> struct Body { float dl[C], st[C]; };
+  template <int C, int O>
+  __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const float* tin) {
+#pragma unroll
+    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.fxy[O][base + k]; }
+  }
+  template <int C, int O>
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const float* tst) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      B.dl[k] = cx.dl[O][base + k];
+      B.st[k] = tst ? tst[base + k] : 0.0f;
+    }
+  }
+  __device__ static bool fixed(const Ctx& cx, int ix, int j) {
+    return ix == 0 || ix >= cx.g->nx - 1 || j == 0 || j == cx.g->ny - 1;
+  }
+  template <int C>
+  __device__ static float update(const Ctx&, const Head<C>& h, const Body<C>& r, int k, int, float uW, float2, bool,
+                                 float uE, float2 aE, bool, float uS, float2, bool, float uN, float2 aN, bool) {
+#pragma clang fp contract(fast)
+    const float2 s = h.aux[k];
+    const float a1p = fmaxf(s.x, 0.0f), a1m = fminf(s.x, 0.0f);
+    const float a2p = fmaxf(aE.x, 0.0f), a2m = fminf(aE.x, 0.0f);
+    const float b1p = fmaxf(s.y, 0.0f), b1m = fminf(s.y, 0.0f);
+    const float b2p = fmaxf(aN.y, 0.0f), b2m = fminf(aN.y, 0.0f);
+    const float d = (a2p - a1m) + (b2p - b1m);
+    if (fabsf(d) < (float)kAdjEps) return 0.0f;
+    const float e = r.dl[k] + (uW * a1p - uE * a2m) + (uS * b1p - uN * b2m);
+    return e * __builtin_amdgcn_rcpf(d);
+  }
+  template <int C>
+  __device__ static double norm(const Head<C>&, const Body<C>& r, int k, float v) {
     return fabs((double)v - (double)r.st[k]);
   }
 };
@@ -495,6 +618,36 @@ __device__ inline Faces faces_at(const GridDev& g, const double* __restrict__ T,
 #undef BB
 #undef CC
   return F;
+}
+
+// The x-face flux a1 at (ix, iy) and the y-face flux b1 (src/eikonal_solver.cpp:409-413,
+// :423-427), operation for operation; a1 at (ix+1, iy) is a2 at (ix, iy) and b1 at
+// (ix, iy+1) is b2 at (ix, iy) (:416-420, :430-434).
+__device__ inline double face_a1(const GridDev& g, const double* __restrict__ T, const double* __restrict__ a,
+                                 const double* __restrict__ c, int ix, int iy) {
+  const int nx = g.nx;
+#define TT(i, j) T[(i) + (size_t)(j) * nx]
+#define AA(i, j) a[(i) + (size_t)(j) * nx]
+#define CC(i, j) c[(i) + (size_t)(j) * nx]
+  return -(TT(ix, iy) - TT(ix - 1, iy)) / g.dx * (AA(ix, iy) + AA(ix - 1, iy)) / 2.0
+      + (TT(ix, iy + 1) - TT(ix, iy - 1) + TT(ix - 1, iy + 1) - TT(ix - 1, iy - 1)) / (4.0 * g.dy)
+        * (CC(ix, iy) + CC(ix - 1, iy)) / 2.0;
+#undef TT
+#undef AA
+#undef CC
+}
+__device__ inline double face_b1(const GridDev& g, const double* __restrict__ T, const double* __restrict__ b,
+                                 const double* __restrict__ c, int ix, int iy) {
+  const int nx = g.nx;
+#define TT(i, j) T[(i) + (size_t)(j) * nx]
+#define BB(i, j) b[(i) + (size_t)(j) * nx]
+#define CC(i, j) c[(i) + (size_t)(j) * nx]
+  return -(TT(ix, iy) - TT(ix, iy - 1)) / g.dy * (BB(ix, iy) + BB(ix, iy - 1)) / 2.0
+      + (TT(ix + 1, iy) - TT(ix - 1, iy) + TT(ix + 1, iy - 1) - TT(ix - 1, iy - 1)) / (4.0 * g.dx)
+        * (CC(ix, iy) + CC(ix, iy - 1)) / 2.0;
+#undef TT
+#undef BB
+#undef CC
 }
 
 // FP64 adjoint set-up: a1p, a2m, b1p, b2m, dl = 0, d, ta0 (segmented, seg[0..6]).

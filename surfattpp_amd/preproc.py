@@ -99,6 +99,14 @@ class EikonalBatch:
         _lib.check("sat_batch_last_ms", _lib.load().sat_batch_last_ms(self._h, C.byref(t), C.byref(s)))
         return t.value, s.value
 
+    PHASES = ("init", "fwd_sweeps", "adj_setup", "adj_sweeps", "fp64_resolves", "reduce", "total")
+
+    def phase_ms(self) -> dict:
+        """Per-phase ms of the last batched run (HIP events on the kernels' streams)."""
+        buf = (C.c_double * len(self.PHASES))()
+        _lib.check("sat_batch_phase_ms", _lib.load().sat_batch_phase_ms(self._h, buf, len(self.PHASES)))
+        return dict(zip(self.PHASES, list(buf)))
+
     def update_media(self, grid: int, svel, m11, m12, m22):
         planes = [_c64(x) for x in (svel, m11, m12, m22)]
         for p in planes:
