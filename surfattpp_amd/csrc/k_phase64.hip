@@ -11,7 +11,7 @@ namespace sat {
 __global__ void recv_kernel(PhaseDev pd, int nrec) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nrec; k += gridDim.x * blockDim.x) {
     const int e = pd.rec_ev[k];
-    if (e < 0 || pd.skip[e]) continue;
+    if (e < 0 || pd.skip[e] != pd.want_skip) continue;
     const EventX& ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];
     const double s = interp2d_point(g.lon, g.nx, g.lat, g.ny, pd.T + ev.pl_off, pd.rec_lon[k], pd.rec_lat[k]);

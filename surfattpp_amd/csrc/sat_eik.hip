@@ -221,10 +221,11 @@ struct GridHost {
   double2* d_trig_yy = nullptr;
   float4* d_rowtab = nullptr;
   char* d_wmed = nullptr;     // batched sweeps: wavefront-major media, per orientation
+  char* d_wmed64 = nullptr;   // FP32 batches: the FP64 wavefront-major media (re-solves)
 
   void release() {
     void* ps[] = {d_lon, d_lat, d_xx, d_yy, d_model, d_fp, d_aA, d_b, d_c, d_v3, d_seg, d_seg64, d_trig_yy, d_rowtab,
-                  d_wmed};
+                  d_wmed, d_wmed64};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = GridHost();
@@ -291,6 +292,13 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
     SAT_CHECK(hipMemset(G.d_wmed, 0, (size_t)nmed * 2 * pb));
     for (int i = 0; i < nmed; i++)
       for (int o = 0; o < 2; o++) g.wmed[i][o] = G.d_wmed + (size_t)(2 * i + o) * pb;
+    if (prec == SAT_PREC_F32) {
+      const size_t pb64 = (size_t)nper * g.wplane * sizeof(double);
+      SAT_CHECK(dalloc(&G.d_wmed64, (size_t)4 * 2 * pb64));
+      SAT_CHECK(hipMemset(G.d_wmed64, 0, (size_t)4 * 2 * pb64));
+      for (int i = 0; i < 4; i++)
+        for (int o = 0; o < 2; o++) G.g64.wmed[i][o] = G.d_wmed64 + (size_t)(2 * i + o) * pb64;
+    }
   }
   return SAT_OK;
 }
@@ -458,7 +466,13 @@ struct sat_batch {
   int* d_base_flags = nullptr;
   double* d_conv = nullptr;        // 2 per event
   unsigned char* d_skip = nullptr;
-  int* d_fb = nullptr;             // re-solve lists, group g at group_first[g]
+  int* d_fb = nullptr;             // re-solve lists (FP64 phase kernels), group g at group_first[g]
+  int* d_ov = nullptr;             // re-solves beyond the pool (per-pair kernel), same layout
+  EventX* d_evx64 = nullptr;       // FP32 batches: events with the re-solve pool offsets
+  char* d_work64 = nullptr;        // FP32 batches: FP64 re-solve pool (pool_size entries)
+  char* d_slots64 = nullptr;       // FP32 batches: sweep slots of the re-solve kernels
+  int pool_size = 0, nslots64p = 0;
+  size_t pool_stride = 0;
   double f32_band = 0.02;
   // per-event planes of the phase pipeline
   size_t w_total = 0, pl_total = 0, col_total = 0;
@@ -500,7 +514,7 @@ struct sat_batch {
   void release() {
     for (auto& G : grids) G.release();
     void* ps[] = {d_grids, d_grids64, d_events, d_evx, d_order, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
-                  d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
+                  d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_ov, d_evx64, d_work64, d_slots64, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
                   d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3};
     for (void* p : ps)
       if (p) (void)hipFree(p);
@@ -729,7 +743,8 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(dalloc(&b->d_conv, 2 * (size_t)std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_skip, std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_fb, (size_t)std::max(ne, 1)));
-  B_CHECK(dalloc(&b->d_counter, 5 * (size_t)std::max(ne, 1)));  // groups <= events
+  B_CHECK(dalloc(&b->d_ov, (size_t)std::max(ne, 1)));
+  B_CHECK(dalloc(&b->d_counter, 8 * (size_t)std::max(ne, 1) + 2));  // groups <= events
 
   // per-event planes of the phase pipeline
   const size_t es = elem_size(b->precision);
@@ -755,12 +770,26 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(dalloc(&b->d_slots, b->slot_bytes * (size_t)b->nslots_phase));
   B_CHECK(hipMemset(b->d_slots, 0, b->slot_bytes * (size_t)b->nslots_phase));
 
+  // FP32 batches: a pool of FP64 phase-kernel planes for the re-solves
+  if (f32) {
+    b->pool_size = std::max(1, std::min(ne, std::max(64, (ne + 15) / 16)));
+    b->pool_stride = round_up(max_wplane, 64);
+    const size_t pool_bytes = 6 * b->pool_stride * sizeof(double) * (size_t)b->pool_size;
+    B_CHECK(dalloc(&b->d_work64, pool_bytes));
+    B_CHECK(hipMemset(b->d_work64, 0, pool_bytes));
+    B_CHECK(dalloc(&b->d_evx64, std::max(ne, 1)));
+    if (ne) B_CHECK(hipMemcpy(b->d_evx64, b->hx.data(), sizeof(EventX) * ne, hipMemcpyHostToDevice));
+    b->nslots64p = std::min(b->pool_size, 2 * prop.multiProcessorCount);
+    B_CHECK(dalloc(&b->d_slots64, b->slot_bytes * (size_t)b->nslots64p));
+    B_CHECK(hipMemset(b->d_slots64, 0, b->slot_bytes * (size_t)b->nslots64p));
+  }
+
   // fused path: one slot for per-solve runs; FP32 batches also FP64 slots for re-solves
   b->sg = slot_geom(b->precision, max_seg, max_plane);
   B_CHECK(dalloc(&b->d_ws, b->sg.slot));
   B_CHECK(hipMemset(b->d_ws, 0, b->sg.slot));  // also the all-zero FP32 delta plane
   if (f32) {
-    b->nslots64 = std::max(1, std::min(ne, prop.multiProcessorCount));
+    b->nslots64 = std::max(1, std::min(ne, 64));
     b->sg64 = slot_geom(SAT_PREC_F64, max_seg, max_plane);
     B_CHECK(dalloc(&b->d_ws64, b->sg64.slot * (size_t)b->nslots64));
     B_CHECK(hipMemset(b->d_ws64, 0, b->sg64.slot * (size_t)b->nslots64));
@@ -853,11 +882,17 @@ int sat_batch_run(sat_batch* b, int mode) {
   const size_t nr = (size_t)std::max(b->nrec, 1);
   const size_t ne1 = (size_t)std::max(ne, 1);
   const size_t ng = b->group_C.size();
-  int* fb_count = b->d_counter;
+  // work-queue heads and list lengths, zeroed at the start of every run
+  int* fb_count = b->d_counter;               // pool re-solves per group
   int* fwd_heads = b->d_counter + ng;
   int* adj_heads = b->d_counter + 2 * ng;
-  int* res_heads = b->d_counter + 3 * ng;
-  int* solo_heads = b->d_counter + 4 * ng;
+  int* resf_heads = b->d_counter + 3 * ng;    // re-solve forward sweeps
+  int* resa_heads = b->d_counter + 4 * ng;    // re-solve adjoint sweeps
+  int* ov_count = b->d_counter + 5 * ng;      // re-solves beyond the pool per group
+  int* ov_heads = b->d_counter + 6 * ng;
+  int* pool_counter = b->d_counter + 7 * ng;
+  int* solo_heads = b->d_counter + 7 * ng + 1;
+  const size_t n_counters = 8 * ne1 + 2;
 
   BatchDev bd{};
   bd.grids = b->d_grids;
@@ -882,14 +917,117 @@ int sat_batch_run(sat_batch* b, int mode) {
   bd.fb_count = fb_count;
   bd.f32_band = b->f32_band;
 
-  hipEvent_t* mk = b->mk;
+  PhaseDev pd{};
+  pd.grids = b->d_grids;
+  pd.ev = b->d_evx;
+  pd.nevent = ne;
+  pd.evs = b->d_evs;
+  pd.coltab = b->d_coltab;
+  const size_t es = elem_size(b->precision);
+  const size_t wb = b->w_total * es;  // bytes of one sweep-precision plane set
+  for (int o = 0; o < 2; o++) {
+    pd.t0[o] = b->d_work + (0 + o) * wb;
+    pd.t0x[o] = b->d_work + (2 + o) * wb;
+    pd.t0y[o] = b->d_work + (4 + o) * wb;
+    pd.fxy[o] = b->d_work + 2 * o * wb;
+    pd.dl[o] = b->d_work + (4 + o) * wb;
+  }
+  pd.T0d = b->d_T0d;
+  pd.T = b->d_T;
+  pd.ta_out = b->d_ta;
+  pd.rec_lon = bd.rec_lon;
+  pd.rec_lat = bd.rec_lat;
+  pd.tt_obs = bd.tt_obs;
+  pd.weight = bd.weight;
+  pd.rec_ev = b->d_rec_ev;
+  pd.syn = bd.syn;
+  pd.adjsrc = bd.adjsrc;
+  pd.rcell = b->d_rcell;
+  pd.rwt = b->d_rwt;
+  pd.chi_ev = b->d_chi_ev;
+  pd.iters_f = bd.iters_f;
+  pd.iters_a = bd.iters_a;
+  pd.flags = bd.flags;
+  pd.conv = b->d_conv;
+  pd.skip = b->d_skip;
+  pd.want_skip = 0;
+  pd.ov_list = b->d_ov;
+  pd.ov_count = ov_count;
+  pd.pool_counter = pool_counter;
+  pd.pool_size = b->pool_size;
+  pd.pool_stride = b->pool_stride;
+  pd.ev64 = b->d_evx64;
+  pd.list = nullptr;
+  pd.list_count = nullptr;
+  pd.ta_f32 = 0;
+  pd.fb_list = b->d_fb;
+  pd.fb_count = fb_count;
+  pd.f32_band = b->f32_band;
+  pd.slots = b->d_slots;
+  pd.slot_bytes = b->slot_bytes;
+  pd.slot_plane = b->slot_plane;
+  pd.order = b->d_order;
+  pd.fwd_maxiter = kFwdMaxIter;
+  pd.adj_maxiter = kAdjMaxIter;
+  const dim3 node_grid((unsigned)((b->max_plane + 255) / 256), (unsigned)std::max(ne, 1));
+  const int rec_blocks = (int)std::max<size_t>(1, std::min<size_t>((b->nrec + 255) / 256, 4096));
+
   // FP64 re-solves of the FP32 pairs whose stop decision was within the band,
-  // on stream2 once the forward sweeps (which list them) are done
-  auto launch_resolves = [&]() -> int {
+  // on stream2 once the forward sweeps (which list them) are done.  Batched
+  // runs: the FP64 phase kernels over the pool lists, the per-pair kernel over
+  // the overflow lists; per-solve runs: the per-pair kernel.
+  hipEvent_t* mk = b->mk;
+  auto launch_resolves = [&](bool phase) -> int {
     SAT_CHECK(hipEventRecord(mk[2], b->stream));
     SAT_CHECK(hipStreamWaitEvent(b->stream2, mk[2], 0));
     SAT_CHECK(hipEventRecord(mk[5], b->stream2));
-    if (f32) {
+    if (f32 && ne) {
+      hipStream_t s2 = b->stream2;
+      if (phase) {
+        PhaseDev q = pd;
+        q.grids = b->d_grids64;
+        q.ev = b->d_evx64;
+        const size_t wb = b->pool_stride * (size_t)b->pool_size * sizeof(double);
+        for (int o = 0; o < 2; o++) {
+          q.t0[o] = b->d_work64 + (0 + o) * wb;
+          q.t0x[o] = b->d_work64 + (2 + o) * wb;
+          q.t0y[o] = b->d_work64 + (4 + o) * wb;
+          q.fxy[o] = b->d_work64 + 2 * o * wb;
+          q.dl[o] = b->d_work64 + (4 + o) * wb;
+        }
+        q.slots = b->d_slots64;
+        q.want_skip = 1;
+        q.ta_f32 = 1;
+        for (size_t gi = 0; gi < ng; gi++) {
+          const int ub = std::min(b->group_count[gi], b->pool_size);
+          PhaseDev a = q;
+          a.list = b->d_fb + b->group_first[gi];
+          a.list_count = fb_count + gi;
+          SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(node_grid.x, ub), s2, a));
+          a.counter = resf_heads + gi;
+          SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(0, b->group_C[gi], dim3(std::min(ub, b->nslots64p)), s2, a, 0, 0,
+                                                   (int)gi));
+        }
+        if (b->nrec) SAT_CHECK(launch_recv(dim3(rec_blocks), s2, q, b->nrec));
+        if (b->calc_adj) {
+          for (size_t gi = 0; gi < ng; gi++) {
+            const int ub = std::min(b->group_count[gi], b->pool_size);
+            PhaseDev a = q;
+            a.list = b->d_fb + b->group_first[gi];
+            a.list_count = fb_count + gi;
+            SAT_CHECK(launch_adj_setup<SAT_PREC_F64>(dim3(node_grid.x, ub), s2, a));
+          }
+          SAT_CHECK(launch_delta<SAT_PREC_F64>(dim3((ne + 63) / 64), s2, q));
+          for (size_t gi = 0; gi < ng; gi++) {
+            const int ub = std::min(b->group_count[gi], b->pool_size);
+            PhaseDev a = q;
+            a.list = b->d_fb + b->group_first[gi];
+            a.list_count = fb_count + gi;
+            a.counter = resa_heads + gi;
+            SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(b->group_C[gi], dim3(std::min(ub, b->nslots64p)), s2, a, 0, 0));
+          }
+        }
+      }
       for (size_t gi = 0; gi < ng; gi++) {
         BatchDev a = bd;
         a.grids = b->d_grids64;
@@ -897,19 +1035,20 @@ int sat_batch_run(sat_batch* b, int mode) {
         a.ws_slot = b->sg64.slot;
         a.ws_seg = b->sg64.seg;
         a.ws_plain = b->sg64.plain;
-        a.counter = res_heads + gi;
+        a.counter = phase ? ov_heads + gi : resf_heads + gi;
         a.f32_band = 0.0;
-        const int* list = bd.fb_list + b->group_first[gi];
+        const int* list = (phase ? b->d_ov : b->d_fb) + b->group_first[gi];
+        const int* cnt = phase ? ov_count + gi : fb_count + gi;
         const int nblk = std::min(b->group_count[gi], b->nslots64);
-        SAT_CHECK(launch_pair<SAT_PREC_F64>(1, b->group_C[gi], dim3(nblk), b->stream2, a, 0, 0, (int)gi, list,
-                                            fb_count + gi));
+        SAT_CHECK(launch_pair<SAT_PREC_F64>(1, b->group_C[gi], dim3(nblk), s2, a, 0, 0, (int)gi, list, cnt));
       }
     }
     SAT_CHECK(hipEventRecord(mk[6], b->stream2));
     return SAT_OK;
   };
+
   SAT_CHECK(hipEventRecord(mk[0], b->stream));
-  SAT_CHECK(hipMemsetAsync(b->d_counter, 0, sizeof(int) * 5 * ne1, b->stream));
+  SAT_CHECK(hipMemsetAsync(b->d_counter, 0, sizeof(int) * n_counters, b->stream));
   SAT_CHECK(hipMemsetAsync(b->d_skip, 0, ne1, b->stream));
   SAT_CHECK(hipMemsetAsync(b->d_iters, 0, sizeof(int) * 2 * ne1, b->stream));
   SAT_CHECK(hipMemcpyAsync(bd.flags, b->d_base_flags, sizeof(int) * ne1, hipMemcpyDeviceToDevice, b->stream));
@@ -934,54 +1073,11 @@ int sat_batch_run(sat_batch* b, int mode) {
                                               nullptr));
       }
     SAT_CHECK(hipEventRecord(mk[1], b->stream));
-    const int rc = launch_resolves();
+    const int rc = launch_resolves(false);
     if (rc) return rc;
     SAT_CHECK(hipEventRecord(mk[3], b->stream));
     SAT_CHECK(hipEventRecord(mk[4], b->stream));
   } else {
-    PhaseDev pd{};
-    pd.grids = b->d_grids;
-    pd.ev = b->d_evx;
-    pd.nevent = ne;
-    pd.evs = b->d_evs;
-    pd.coltab = b->d_coltab;
-    const size_t es = elem_size(b->precision);
-    const size_t wb = b->w_total * es;  // bytes of one sweep-precision plane set
-    for (int o = 0; o < 2; o++) {
-      pd.t0[o] = b->d_work + (0 + o) * wb;
-      pd.t0x[o] = b->d_work + (2 + o) * wb;
-      pd.t0y[o] = b->d_work + (4 + o) * wb;
-      pd.fxy[o] = b->d_work + 2 * o * wb;
-      pd.dl[o] = b->d_work + (4 + o) * wb;
-    }
-    pd.T0d = b->d_T0d;
-    pd.T = b->d_T;
-    pd.ta_out = b->d_ta;
-    pd.rec_lon = bd.rec_lon;
-    pd.rec_lat = bd.rec_lat;
-    pd.tt_obs = bd.tt_obs;
-    pd.weight = bd.weight;
-    pd.rec_ev = b->d_rec_ev;
-    pd.syn = bd.syn;
-    pd.adjsrc = bd.adjsrc;
-    pd.rcell = b->d_rcell;
-    pd.rwt = b->d_rwt;
-    pd.chi_ev = b->d_chi_ev;
-    pd.iters_f = bd.iters_f;
-    pd.iters_a = bd.iters_a;
-    pd.flags = bd.flags;
-    pd.conv = b->d_conv;
-    pd.skip = b->d_skip;
-    pd.fb_list = b->d_fb;
-    pd.fb_count = fb_count;
-    pd.f32_band = b->f32_band;
-    pd.slots = b->d_slots;
-    pd.slot_bytes = b->slot_bytes;
-    pd.slot_plane = b->slot_plane;
-    pd.order = b->d_order;
-    pd.fwd_maxiter = kFwdMaxIter;
-    pd.adj_maxiter = kAdjMaxIter;
-    const dim3 node_grid((unsigned)((b->max_plane + 255) / 256), (unsigned)std::max(ne, 1));
     if (ne) {
       const dim3 col_grid((unsigned)((b->max_nx + 63) / 64), (unsigned)ne);
       SAT_CHECK(f32 ? launch_trig<SAT_PREC_F32>(col_grid, b->stream, pd) : launch_trig<SAT_PREC_F64>(col_grid, b->stream, pd));
@@ -992,6 +1088,7 @@ int sat_batch_run(sat_batch* b, int mode) {
       PhaseDev a = pd;
       a.counter = fwd_heads + gi;
       a.fb_list = pd.fb_list + b->group_first[gi];
+      a.ov_list = pd.ov_list + b->group_first[gi];
       const int nblk = std::min(b->group_count[gi], b->nslots_phase);
       const int C = b->group_C[gi], iso = b->group_iso[gi];
       SAT_CHECK(f32 ? launch_fwd_sweep<SAT_PREC_F32>(iso, C, dim3(nblk), b->stream, a, b->group_first[gi],
@@ -1000,12 +1097,11 @@ int sat_batch_run(sat_batch* b, int mode) {
                                                      b->group_count[gi], (int)gi));
     }
     {
-      const int rc = launch_resolves();
+      const int rc = launch_resolves(true);
       if (rc) return rc;
     }
     if (b->nrec) {
-      const int blocks = (int)std::min<size_t>((b->nrec + 255) / 256, 4096);
-      SAT_CHECK(launch_recv(dim3(blocks), b->stream, pd, b->nrec));
+      SAT_CHECK(launch_recv(dim3(rec_blocks), b->stream, pd, b->nrec));
     }
     if (b->calc_adj && ne) {
       SAT_CHECK(f32 ? launch_adj_setup<SAT_PREC_F32>(node_grid, b->stream, pd)
@@ -1109,7 +1205,10 @@ int sat_batch_conv(sat_batch* b, double* linf_last, double* linf_prev, int* n_re
     *n_resolved = 0;
     if (b->precision == SAT_PREC_F32 && !b->group_C.empty()) {
       std::vector<int> cnt(b->group_C.size());
-      SAT_CHECK(hipMemcpy(cnt.data(), b->d_counter, sizeof(int) * cnt.size(), hipMemcpyDeviceToHost));
+      const size_t ng = b->group_C.size();
+      SAT_CHECK(hipMemcpy(cnt.data(), b->d_counter, sizeof(int) * ng, hipMemcpyDeviceToHost));
+      for (int c : cnt) *n_resolved += c;
+      SAT_CHECK(hipMemcpy(cnt.data(), b->d_counter + 5 * ng, sizeof(int) * ng, hipMemcpyDeviceToHost));
       for (int c : cnt) *n_resolved += c;
     }
   }

@@ -66,10 +66,21 @@ struct PhaseDev {
   int* iters_a;
   int* flags;
   double* conv;
-  unsigned char* skip;  // FP32: events re-solved in FP64
-  int* fb_list;
+  unsigned char* skip;  // FP32 batches: 0 FP32 result, 1 FP64 re-solve by these kernels,
+                        // 2 FP64 re-solve by the per-pair kernel (pool full)
+  int want_skip;        // the skip value of the events this pass processes
+  int* fb_list;         // FP32 forward: pool re-solves, per launch group
   int* fb_count;
+  int* ov_list;         // FP32 forward: re-solves beyond the pool, per launch group
+  int* ov_count;
+  int* pool_counter;
+  int pool_size;
+  size_t pool_stride;   // elements of one pool entry of the FP64 planes
+  EventX* ev64;         // FP32 forward: the events as the FP64 re-solve sees them
   double f32_band;
+  const int* list;      // non-null: events list[0, *list_count) (FP64 re-solve pass)
+  const int* list_count;
+  int ta_f32;           // masked Ta stored as float (FP64 re-solve of an FP32 batch)
   char* slots;          // per resident wave: 3 swept planes
   size_t slot_bytes, slot_plane;
   const int* order;
@@ -94,6 +105,12 @@ template <> struct PhaseT<SAT_PREC_F32> {
   using FwdIso = FwdF32Iso;
   using Adj = AdjF32Face;
 };
+
+// Event of a node-parallel block: blockIdx.y, or the list entry blockIdx.y; -1 if none.
+__device__ inline int block_event(const PhaseDev& pd) {
+  if (!pd.list) return blockIdx.y;
+  return ((int)blockIdx.y < *pd.list_count) ? pd.list[blockIdx.y] : -1;
+}
 
 // Decode element q of an orientation-0 wavefront-major plane: column ix, row j;
 // false for padding.
@@ -139,7 +156,8 @@ __global__ void trig_kernel(PhaseDev pd) {
 template <int PREC>
 __global__ void init_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
-  const int e = blockIdx.y;
+  const int e = block_event(pd);
+  if (e < 0) return;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -217,12 +235,13 @@ __global__ __launch_bounds__(64) void fwd_sweep_kernel(PhaseDev pd, int first, i
   U* Q1 = reinterpret_cast<U*>(sb + pd.slot_plane);
   U* Q2 = reinterpret_cast<U*>(sb + 2 * pd.slot_plane);
   constexpr bool kSigma = PREC == SAT_PREC_F32;
+  const int n_work = pd.list ? *pd.list_count : count;
   for (;;) {
     int idx = 0;
     if (lane == 0) idx = atomicAdd(pd.counter, 1);
     idx = __shfl(idx, 0);
-    if (idx >= count) break;
-    const int e = pd.order[first + idx];
+    if (idx >= n_work) break;
+    const int e = pd.list ? pd.list[idx] : pd.order[first + idx];
     const EventX ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];
     const int nx = g.nx, n = (int)g.wplane;
@@ -249,8 +268,19 @@ __global__ __launch_bounds__(64) void fwd_sweep_kernel(PhaseDev pd, int first, i
       const bool near_prev = io.linf_prev >= 0.0 && io.linf_prev < (1.0 + b) * kFwdTol;
       const bool amb = b > 0.0 && (near_last || near_prev);
       if (lane == 0) {
-        pd.skip[e] = amb ? 1 : 0;
-        if (amb) pd.fb_list[atomicAdd(pd.fb_count + group, 1)] = e;
+        unsigned char sk = 0;
+        if (amb) {
+          const int slot = atomicAdd(pd.pool_counter, 1);
+          if (slot < pd.pool_size) {
+            pd.ev64[e].w_off = (size_t)slot * pd.pool_stride;
+            pd.fb_list[atomicAdd(pd.fb_count + group, 1)] = e;
+            sk = 1;
+          } else {
+            pd.ov_list[atomicAdd(pd.ov_count + group, 1)] = e;
+            sk = 2;
+          }
+        }
+        pd.skip[e] = sk;
       }
       if (amb) {
         __syncthreads();
@@ -282,8 +312,8 @@ template <int PREC>
 __global__ void adj_setup_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
   using F = typename PhaseT<PREC>::F;
-  const int e = blockIdx.y;
-  if (pd.skip[e]) return;
+  const int e = block_event(pd);
+  if (e < 0 || pd.skip[e] != pd.want_skip) return;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -316,7 +346,7 @@ template <int PREC>
 __global__ void delta_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= pd.nevent || pd.skip[e]) return;
+  if (e >= pd.nevent || pd.skip[e] != pd.want_skip) return;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
   double s = 0.0;
@@ -376,13 +406,14 @@ __global__ __launch_bounds__(64) void adj_sweep_kernel(PhaseDev pd, int first, i
   V* Q0 = reinterpret_cast<V*>(sb);
   V* Q1 = reinterpret_cast<V*>(sb + pd.slot_plane);
   V* Q2 = reinterpret_cast<V*>(sb + 2 * pd.slot_plane);
+  const int n_work = pd.list ? *pd.list_count : count;
   for (;;) {
     int idx = 0;
     if (lane == 0) idx = atomicAdd(pd.counter, 1);
     idx = __shfl(idx, 0);
-    if (idx >= count) break;
-    const int e = pd.order[first + idx];
-    if (pd.skip[e]) continue;
+    if (idx >= n_work) break;
+    const int e = pd.list ? pd.list[idx] : pd.order[first + idx];
+    if (pd.skip[e] != pd.want_skip) continue;
     const EventX ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];
     const int nx = g.nx, ny = g.ny, n = (int)g.wplane;
@@ -409,11 +440,13 @@ __global__ __launch_bounds__(64) void adj_sweep_kernel(PhaseDev pd, int first, i
       pd.iters_a[e] = io.iters;
       if (!io.converged) atomicOr(pd.flags + e, FLAG_ADJ_MAXITER);
     }
-    V* out = static_cast<V*>(pd.ta_out) + ev.pl_off;  // :514-536 mask, column-major
-    for (int iy = 0; iy < ny; iy++)
+    for (int iy = 0; iy < ny; iy++)  // :514-536 mask, column-major
       for (int ix = lane; ix < nx; ix += kWave) {
         const bool m = ix >= ev.mb[0] && ix <= ev.mb[1] && iy >= ev.mb[2] && iy <= ev.mb[3];
-        out[ix + (size_t)iy * nx] = m ? V(0) : ta[wave_index(g, 0, ix, iy)];
+        const V v = m ? V(0) : ta[wave_index(g, 0, ix, iy)];
+        const size_t o = ev.pl_off + ix + (size_t)iy * nx;
+        if (PREC == SAT_PREC_F64 && pd.ta_f32) static_cast<float*>(pd.ta_out)[o] = (float)v;
+        else static_cast<V*>(pd.ta_out)[o] = v;
       }
     __syncthreads();
   }

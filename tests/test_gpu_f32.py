@@ -94,3 +94,30 @@ def test_batch_f32_matches_f64_gpu_full_workload():
     np.testing.assert_array_equal(r32["iters_fwd"], r64["iters_fwd"])
     assert max(errs) <= TOL_K
     assert RC.rel_inf(r32["syn"], r64["syn"]) <= TOL_T
+
+
+def test_batch_f32_resolve_pool_and_overflow():
+    # A band of 0.99 puts most pairs within the band: 64 of them are re-solved
+    # by the FP64 batched kernels (the pool), the rest by the FP64 per-pair
+    # kernel.  A re-solved pair is an FP64 result: its syn equal to the FP64
+    # batch's; the others within the FP32 bar.
+    band = 0.99
+    b = synth.subset_events(synth.config_grf(nperiod=3, nsta=40), list(range(0, 120)))
+    with PP.EikonalBatch(b) as b64, PP.EikonalBatch(b, precision=PREC_F32) as b32:
+        b32.set_f32_band(band)
+        b64.run()
+        b32.run()
+        r64, r32 = b64.results(), b32.results()
+        last, prev, nres = b32.conv()
+    tol = 1e-4
+    amb = ((last >= (1 - band) * tol) & (r32["iters_fwd"] < 2000)) | ((prev >= 0) & (prev < (1 + band) * tol))
+    print("re-solves", nres, "ambiguous", int(amb.sum()), "of", len(b["ev_iper"]), flush=True)
+    assert nres == int(amb.sum()) and nres > 64
+    np.testing.assert_array_equal(r32["iters_fwd"], r64["iters_fwd"])
+    np.testing.assert_array_equal(r32["iters_adj"][amb], r64["iters_adj"][amb])
+    off = np.concatenate([[0], np.cumsum(b["ev_nrec"])])
+    for e in np.nonzero(amb)[0]:
+        np.testing.assert_array_equal(r32["syn"][off[e]:off[e + 1]], r64["syn"][off[e]:off[e + 1]])
+    assert RC.rel_inf(r32["syn"], r64["syn"]) <= TOL_T
+    for p in range(r64["adj_s"].shape[0]):
+        assert RC.rel_inf(r32["adj_s"][p], r64["adj_s"][p]) <= TOL_K
