@@ -13,8 +13,9 @@ accumulation and chi for all 3,000 pairs -- with inputs resident in HBM
 
 `value` is the batched FP32 path.  The line also carries the FP64 batched and
 the plain per-solve FP64 GPU numbers, the FP32 kernel error against the FP64
-path (which equals the FP64 oracle to 1e-12), the roofline of the solve
-kernel and the CPU baseline (the FP64 oracle on the host's cores).
+path (which equals the FP64 oracle to 1e-12), the roofline of the dominant
+kernel (the FP32 forward sweeps), per-phase times and the CPU baseline (the
+FP64 oracle on the host's cores).
 """
 from __future__ import annotations
 
@@ -34,16 +35,21 @@ METRIC = "fwd+adjoint solves/s @201² grid: batched FP32 vs plain FP64, ≤1e-5 
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(nx, ny, iters_f, iters_a, w):
-    """Per-pair algorithmic bytes of the solve kernel (sweeps + set-up):
-    forward 9 words per node per sweep (fun, a, b, c, T0, T0x, T0y, tau read;
-    tau written), adjoint 5 words per interior node per sweep (x-flux, y-flux,
-    delta, Ta read; Ta written), set-up 3 + 6 words per node.  Iteration counts
-    are the FP64 path's (identical to the FP64 oracle's)."""
+# Algorithmic bytes of the forward sweeps per node and sweep: 9 words (slowness,
+# a, b, c, T0, T0x, T0y and tau read; tau written) of w = 4 bytes (FP32 sweeps).
+# This kernel's own streams per node and sweep -- sigma read and written (8 + 8),
+# T0, T0x, T0y, slowness (4 each), the round-start sigma once per round (8) --
+# come to 34 bytes, within 6 % of the same figure.
+FWD_WORDS_PER_NODE_SWEEP = 9
+FWD_WORD_BYTES = 4
+
+
+def fwd_sweep_bytes(nx, ny, iters_f):
+    """Algorithmic HBM bytes of one launch of the FP32 forward sweep kernel over
+    the batch: w*N*(4 sweeps * 9 words * I_f) summed over pairs, I_f the pair's
+    forward rounds (the FP64 path's; the FP32 path's are equal)."""
     N = nx * ny
-    Ni = (nx - 2) * (ny - 2)
-    return (w * N * (4 * 9 * iters_f.astype(np.float64)) + w * Ni * (4 * 5 * iters_a.astype(np.float64))
-            + w * N * (3 + 6))
+    return float(FWD_WORD_BYTES * N * 4 * FWD_WORDS_PER_NODE_SWEEP * iters_f.astype(np.float64).sum())
 
 
 def _cpu_worker(args):
@@ -84,10 +90,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-pairs", type=int, default=30, help="pairs timed in per-solve FP64 mode")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device twice (the plain copy "
+                         "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the (source, period) "
-                                   "pairs shard by period across GPUs, see DESIGN.md)"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
     from surfattpp_amd import PREC_F32, PREC_F64, RUN_PER_SOLVE, _lib, synth
@@ -119,12 +127,12 @@ def main():
     for _ in range(args.warmup):
         bt.run()
     bt.sync()
-    solve_ms = []
+    phases = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         bt.run(sync=False)
         bt.sync()  # per step, so the per-launch event timing is read for every step
-        solve_ms.append(bt.last_ms()[1])
+        phases.append(bt.phase_ms())
     bt.sync()
     wall = time.perf_counter() - t0
     r32 = bt.results()
@@ -137,11 +145,12 @@ def main():
     err_syn = float(np.abs(r32["syn"] - r64["syn"]).max() / np.abs(r64["syn"]).max())
     err_chi = abs(r32["chi"] - r64["chi"]) / abs(r64["chi"])
 
-    kern_ms = float(np.mean(solve_ms))
-    bytes_launch = float(algorithmic_bytes(nx, ny, iters_f, iters_a, 4).sum())
+    phase_mean = {k: round(float(np.mean([p[k] for p in phases])), 3) for k in phases[0]}
+    kern_ms = phase_mean["fwd_sweeps"]  # HIP events around the kernel on its stream
+    bytes_launch = fwd_sweep_bytes(nx, ny, iters_f)
     achieved = bytes_launch / (kern_ms * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_pair_kernel_f32_grf201.json")
+    tf = os.path.join(ROOT, "profiles", "fwd_sweep_f32_grf201_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -170,8 +179,9 @@ def main():
                    "device": dev},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "pair_kernel<SAT_PREC_F32,4>", "kernel_ms": round(kern_ms, 3),
+                     "kernel": "fwd_sweep_kernel<SAT_PREC_F32,4,iso>", "kernel_ms": round(kern_ms, 3),
                      "algorithmic_bytes_per_launch": bytes_launch},
+        "phases_ms": phase_mean,
         "cpu_baseline": cpu,
         "err_vs_fp64": {"kernel_max_per_period": err_k, "syn": err_syn, "chi": err_chi},
         "batched_f64": {"value": round(ne / (f64_ms[0] * 1e-3), 2), "ms_per_step": round(f64_ms[0], 3)},
@@ -182,6 +192,29 @@ def main():
                   "fwd_hist": np.bincount(iters_f).tolist(), "adj_hist": np.bincount(iters_a).tolist()},
     }
     print(json.dumps(line), flush=True)
+    if args.calib_copy_mib > 0:
+        calib_copy(args.calib_copy_mib << 20)
+
+
+def calib_copy(nbytes: int) -> None:
+    """Two device-to-device copies of nbytes (HIP's blit kernel)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMemset failed")
+        for _ in range(2):
+            if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
 
 
 if __name__ == "__main__":

@@ -16,7 +16,7 @@
  * (column-major (ix, iy) -> ix + iy*nx) and its evaluation order of every
  * expression.  It is compiled with -ffp-contract=off so no FMA is formed.
  *
- * Pinning (see tests/test_oracle_*.py): the reference's own known-answer
+ * Pinning (tests/test_oracle.py): the reference's own known-answer
  * tests (tests/eikonal_test/test_eikonal.cpp tests 1-8) restated; closed
  * forms (great-circle travel time, index-set mask, interp2d on planes);
  * invariants; and the reference's Fortran twin
@@ -34,7 +34,19 @@
  * are the same.
  * Parity unpinned: Eigen's reduction order for chi (preproc.cpp:142 uses
  * Eigen's vectorised .sum(); this file sums sequentially).
+ *
+ * Planted mistakes.  Built with -DSAT_ORACLE_MUTANT=n (oracle/Makefile,
+ * liboracle_mutN.so) the file carries one plausible slip, which the pinning
+ * tests must reject: 1 a flipped sign in the two-point causality test (:201),
+ * 2 the cross-metric term dropped from the left-face flux a1 (:409-413),
+ * 3 dx for dy in the two-point y stencil (:181), 4 the near-source fixed box
+ * one cell too small in y (:121).  With the default (0) the expressions are
+ * the reference's.
  */
+#ifndef SAT_ORACLE_MUTANT
+#define SAT_ORACLE_MUTANT 0
+#endif
+#define MUTANT(n) (SAT_ORACLE_MUTANT == (n))
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -170,7 +182,7 @@ int sat_oracle_forward(int nx, int ny, const double* xx_deg, const double* yy_de
                 T0y[IDX(ix, iy)] = -(cos_yy[iy] * sin_y0 - sin_yy[iy] * cos_y0 * cos_dl[ix])
                                    / sin_d * R_EARTH_ * fun0;
             }
-            if (fabs((xx[ix] - x0) / dx) <= 1.0 && fabs((yy[iy] - y0) / dy) <= 1.0) {
+            if (fabs((xx[ix] - x0) / dx) <= 1.0 && fabs((yy[iy] - y0) / dy) <= (MUTANT(4) ? 0.5 : 1.0)) {
                 tau[IDX(ix, iy)] = 1.0;
                 ischange[IDX(ix, iy)] = 0;
                 if (ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1) flags |= SAT_FLAG_SRC_ON_BOUNDARY;
@@ -216,7 +228,7 @@ int sat_oracle_forward(int nx, int ny, const double* xx_deg, const double* yy_de
                         if (nix < 0 || nix >= nx || niy < 0 || niy >= ny) continue;
                         const double px = t0x - sx * (t0 / dx);
                         const double qx = sx * (t0 / dx) * tau[IDX(ix + sx, iy)];
-                        const double py = t0y - sy * (t0 / dy);
+                        const double py = t0y - sy * (t0 / (MUTANT(3) ? dx : dy));
                         const double qy = sy * (t0 / dy) * tau[IDX(ix, iy + sy)];
                         const double eq_a = aij * px * px + bij * py * py - 2.0 * cij * px * py;
                         const double eq_b = 2.0 * (aij * px * qx + bij * py * qy - cij * (px * qy + py * qx));
@@ -231,7 +243,8 @@ int sat_oracle_forward(int nx, int ny, const double* xx_deg, const double* yy_de
                             const double Ty = py * tmp_tau + qy;
                             const double char_x = aij * Tx - cij * Ty;
                             const double char_y = bij * Ty - cij * Tx;
-                            if (-sx * char_x >= 0.0 && -sy * char_y >= 0.0) tnew = dmin(tnew, tmp_tau);
+                            if ((MUTANT(1) ? sx * char_x : -sx * char_x) >= 0.0 && -sy * char_y >= 0.0)
+                                tnew = dmin(tnew, tmp_tau);
                         }
                     }
 
@@ -387,8 +400,9 @@ int sat_oracle_adjoint(int nx, int ny, const double* xx_deg, const double* yy_de
     for (int ix = 1; ix < nx - 1; ix++) {
         for (int iy = 1; iy < ny - 1; iy++) {
             const double a1 = -(TT(ix, iy) - TT(ix - 1, iy)) / dx * (a[IDX(ix, iy)] + a[IDX(ix - 1, iy)]) / 2.0
-                + (TT(ix, iy + 1) - TT(ix, iy - 1) + TT(ix - 1, iy + 1) - TT(ix - 1, iy - 1)) / (4.0 * dy)
-                  * (c[IDX(ix, iy)] + c[IDX(ix - 1, iy)]) / 2.0;
+                + (MUTANT(2) ? 0.0
+                             : (TT(ix, iy + 1) - TT(ix, iy - 1) + TT(ix - 1, iy + 1) - TT(ix - 1, iy - 1)) / (4.0 * dy)
+                                   * (c[IDX(ix, iy)] + c[IDX(ix - 1, iy)]) / 2.0);
             a1m[IDX(ix, iy)] = (a1 - fabs(a1)) / 2.0;
             a1p[IDX(ix, iy)] = (a1 + fabs(a1)) / 2.0;
             const double a2 = -(TT(ix + 1, iy) - TT(ix, iy)) / dx * (a[IDX(ix + 1, iy)] + a[IDX(ix, iy)]) / 2.0

@@ -95,7 +95,15 @@ def _v64(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel())
 
 
-def forward(xx_deg, yy_deg, spha, sphb, sphc, fun, x0_deg, y0_deg, max_iter=0):
+def mutant_lib(n: int):
+    """The restatement with planted mistake n (liboracle_mutN.so, eik_oracle.c)."""
+    path = os.path.join(_HERE, f"liboracle_mut{n}.so")
+    if not os.path.exists(path):
+        build()
+    return _bind(C.CDLL(path))
+
+
+def forward(xx_deg, yy_deg, spha, sphb, sphc, fun, x0_deg, y0_deg, max_iter=0, lib_=None):
     """FSM_UW_PS_lonlat_2d (src/eikonal_solver.cpp:22).  Returns (T, iters, flags)."""
     xx, yy = _v64(xx_deg), _v64(yy_deg)
     nx, ny = xx.size, yy.size
@@ -105,7 +113,7 @@ def forward(xx_deg, yy_deg, spha, sphb, sphc, fun, x0_deg, y0_deg, max_iter=0):
     T = np.zeros((nx, ny), dtype=np.float64, order="F")
     it = np.zeros(1, dtype=np.int32)
     fl = np.zeros(1, dtype=np.int32)
-    rc = lib().sat_oracle_forward(nx, ny, _p(xx), _p(yy), _p(A), _p(B), _p(Cc), _p(F),
+    rc = (lib_ or lib()).sat_oracle_forward(nx, ny, _p(xx), _p(yy), _p(A), _p(B), _p(Cc), _p(F),
                                   float(x0_deg), float(y0_deg), int(max_iter), _p(T), _ip_(it), _ip_(fl))
     if rc == 22:
         raise ValueError("Grid must be at least 3x3")
@@ -114,7 +122,7 @@ def forward(xx_deg, yy_deg, spha, sphb, sphc, fun, x0_deg, y0_deg, max_iter=0):
     return T, int(it[0]), int(fl[0])
 
 
-def adjoint(xx_deg, yy_deg, spha, sphb, sphc, T, xrec_deg, yrec_deg, source_adj, max_iter=0):
+def adjoint(xx_deg, yy_deg, spha, sphb, sphc, T, xrec_deg, yrec_deg, source_adj, max_iter=0, lib_=None):
     """FSM_O1_JSE_lonlat_2d (src/eikonal_solver.cpp:323).  Returns (Ta, iters)."""
     xx, yy = _v64(xx_deg), _v64(yy_deg)
     nx, ny = xx.size, yy.size
@@ -125,7 +133,7 @@ def adjoint(xx_deg, yy_deg, spha, sphb, sphc, T, xrec_deg, yrec_deg, source_adj,
         xr = yr = sa = np.zeros(1)
     Ta = np.zeros((nx, ny), dtype=np.float64, order="F")
     it = np.zeros(1, dtype=np.int32)
-    rc = lib().sat_oracle_adjoint(nx, ny, _p(xx), _p(yy), _p(A), _p(B), _p(Cc), _p(TT), nr,
+    rc = (lib_ or lib()).sat_oracle_adjoint(nx, ny, _p(xx), _p(yy), _p(A), _p(B), _p(Cc), _p(TT), nr,
                                   _p(xr), _p(yr), _p(sa), int(max_iter), _p(Ta), _ip_(it))
     if rc == 22:
         raise ValueError("Grid must be at least 3x3")

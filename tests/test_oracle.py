@@ -86,14 +86,33 @@ def test_twin_as_shipped_differs_only_at_single_precision_level():
     assert 1e-12 < e < 1e-6
 
 
-def test_planted_mistake_is_caught_by_twin():
-    # A plausible slip (source slowness off by one part in 1e9, e.g. a wrong bilinear
-    # weight) must fail the twin comparison at the 1e-12 bar.
+def test_planted_input_slip_is_caught_by_twin():
+    # A slip in an input (source slowness off by one part in 1e9, e.g. a wrong
+    # bilinear weight) must fail the twin comparison at the 1e-12 bar.
     z = np.load(__file__.replace("test_oracle.py", "golden/twin_vectors.npz"))
     g = lambda k: z[f"c1_{k}"]  # noqa: E731
     T, _, _ = O.forward(g("xx"), g("yy"), g("spha"), g("sphb"), g("sphc"), g("fun") * (1 + 1e-9),
                         float(g("x0")), float(g("y0")))
     assert RC.rel_inf(T, g("T")) > 1e-11
+
+
+@pytest.mark.parametrize("mutant", [1, 2, 3, 4])
+def test_planted_oracle_mistakes_are_caught(mutant):
+    # Each liboracle_mutN.so carries one plausible slip in the restatement (a
+    # flipped causality sign, a dropped cross-metric flux term, dx for dy, a
+    # fixed box one cell too small): the golden twin vectors reject every one.
+    L = O.mutant_lib(mutant)
+    z = np.load(__file__.replace("test_oracle.py", "golden/twin_vectors.npz"))
+    worst = 0.0
+    for i in range(int(z["ncase"])):
+        g = lambda k: z[f"c{i}_{k}"]  # noqa: E731
+        T, _, _ = O.forward(g("xx"), g("yy"), g("spha"), g("sphb"), g("sphc"), g("fun"), float(g("x0")),
+                            float(g("y0")), lib_=L)
+        Ta, _ = O.adjoint(g("xx"), g("yy"), g("spha"), g("sphb"), g("sphc"), g("T"), g("xr"), g("yr"), g("adj"),
+                          lib_=L)
+        worst = max(worst, RC.rel_inf(T, g("T")), RC.rel_inf(Ta, g("Ta")))
+    print("mutant", mutant, "worst rel err vs twin", worst)
+    assert worst > 1e-6
 
 
 def test_homogeneous_fixed_nodes_equal_T0():
