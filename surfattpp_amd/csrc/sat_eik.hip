@@ -81,7 +81,7 @@ __global__ void media_kernel(GridDev g, int mode, int prec, const double* __rest
     if (prec == SAT_PREC_F64) {
       const double v[4] = {va, vb, vc, vf};
       for (int i = 0; i < 4; i++) {
-        static_cast<double*>(sp[i])[so] = v[i];
+        if (s0) static_cast<double*>(sp[i])[so] = v[i];
         if (wave)
           for (int o = 0; o < 2; o++) static_cast<double*>(const_cast<void*>(g.wmed[i][o]))[wi[o]] = v[i];
       }
@@ -91,7 +91,7 @@ __global__ void media_kernel(GridDev g, int mode, int prec, const double* __rest
                           (det > 0.0) ? (float)sqrt(vf * vf * vb / det) : -1.0f,
                           (det > 0.0) ? (float)sqrt(vf * vf * va / det) : -1.0f};
       for (int i = 0; i < 6; i++) {
-        static_cast<float*>(sp[i])[so] = v[i];
+        if (s0) static_cast<float*>(sp[i])[so] = v[i];
         if (wave)
           for (int o = 0; o < 2; o++) static_cast<float*>(const_cast<void*>(g.wmed[i][o]))[wi[o]] = v[i];
       }
@@ -221,11 +221,12 @@ struct GridHost {
   double2* d_trig_yy = nullptr;
   float4* d_rowtab = nullptr;
   char* d_wmed = nullptr;     // batched sweeps: wavefront-major media, per orientation
-  char* d_wmed64 = nullptr;   // FP32 batches: the FP64 wavefront-major media (re-solves)
+  char* d_wmedmw = nullptr;   // FP32 batches: FP64 wavefront-major media of the re-solve geometry
+  GridDev gmw{};              // FP32 batches: FP64 media, kMultiWave waves per grid (re-solves)
 
   void release() {
     void* ps[] = {d_lon, d_lat, d_xx, d_yy, d_model, d_fp, d_aA, d_b, d_c, d_v3, d_seg, d_seg64, d_trig_yy, d_rowtab,
-                  d_wmed, d_wmed64};
+                  d_wmed, d_wmedmw};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = GridHost();
@@ -293,11 +294,19 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
     for (int i = 0; i < nmed; i++)
       for (int o = 0; o < 2; o++) g.wmed[i][o] = G.d_wmed + (size_t)(2 * i + o) * pb;
     if (prec == SAT_PREC_F32) {
-      const size_t pb64 = (size_t)nper * g.wplane * sizeof(double);
-      SAT_CHECK(dalloc(&G.d_wmed64, (size_t)4 * 2 * pb64));
-      SAT_CHECK(hipMemset(G.d_wmed64, 0, (size_t)4 * 2 * pb64));
+      // re-solve geometry: kMultiWave waves per grid, C = ceil(nx / (64 kMultiWave))
+      GridDev& m = G.gmw;
+      m = G.g64;
+      m.C = (nx + kWave * kMultiWave - 1) / (kWave * kMultiWave);
+      m.La = (nx + m.C - 1) / m.C;
+      m.P = m.La * m.C;
+      m.wplane = (size_t)(ny + m.La - 1) * m.La * m.C;
+      for (int i = 0; i < 6; i++) m.seg[i] = nullptr;
+      const size_t pbm = (size_t)nper * m.wplane * sizeof(double);
+      SAT_CHECK(dalloc(&G.d_wmedmw, (size_t)4 * 2 * pbm));
+      SAT_CHECK(hipMemset(G.d_wmedmw, 0, (size_t)4 * 2 * pbm));
       for (int i = 0; i < 4; i++)
-        for (int o = 0; o < 2; o++) G.g64.wmed[i][o] = G.d_wmed64 + (size_t)(2 * i + o) * pb64;
+        for (int o = 0; o < 2; o++) m.wmed[i][o] = G.d_wmedmw + (size_t)(2 * i + o) * pbm;
     }
   }
   return SAT_OK;
@@ -318,6 +327,7 @@ static int grid_media(GridHost& G, int mode, const double* p0, const double* p1,
     for (size_t k = 0; k < n && iso; k++) iso = ma[k] == 1.0 && mb[k] == 1.0 && mc[k] == 0.0;
     G.g.iso = iso ? 1 : 0;
     G.g64.iso = G.g.iso;
+    G.gmw.iso = G.g.iso;
   }
   for (int i = 0; i < 4; i++) {
     SAT_CHECK(hipMemcpyAsync(G.d_model + i * n, src[i], sizeof(double) * n, hipMemcpyHostToDevice, stream));
@@ -325,10 +335,11 @@ static int grid_media(GridHost& G, int mode, const double* p0, const double* p1,
   const size_t total = (size_t)G.nper * G.ny * G.g.P;
   const int threads = 256;
   const int blocks = (int)std::min<size_t>((total + threads - 1) / threads, 65535);
-  for (int pass = 0; pass < (G.d_seg64 ? 2 : 1); pass++) {
-    const GridDev& gd = pass ? G.g64 : G.g;
+  const int npass = G.d_wmedmw ? 3 : G.d_seg64 ? 2 : 1;
+  for (int pass = 0; pass < npass; pass++) {
+    const GridDev& gd = (pass == 2) ? G.gmw : pass ? G.g64 : G.g;
     void* sg[6];
-    for (int i = 0; i < 6; i++) sg[i] = const_cast<void*>(gd.seg[i]);
+    for (int i = 0; i < 6; i++) sg[i] = const_cast<void*>(gd.seg[i]);  // null: no segmented planes
     hipLaunchKernelGGL(media_kernel, dim3(blocks), dim3(threads), 0, stream, gd, mode, pass ? SAT_PREC_F64 : G.prec,
                        G.d_model, G.d_model + n, G.d_model + 2 * n, G.d_model + 3 * n, G.d_fp, G.d_aA, G.d_b,
                        G.d_c, (mode == 0) ? G.d_v3 : nullptr, sg[0], sg[1], sg[2], sg[3], sg[4], sg[5]);
@@ -347,7 +358,8 @@ static SlotGeom slot_geom(int prec, size_t max_seg_elems, size_t max_plane) {
   SlotGeom s;
   const int nseg = prec == SAT_PREC_F64 ? Prec<SAT_PREC_F64>::kSeg : Prec<SAT_PREC_F32>::kSeg;
   const int npl = prec == SAT_PREC_F64 ? Prec<SAT_PREC_F64>::kPlain : Prec<SAT_PREC_F32>::kPlain;
-  s.seg = round_up(max_seg_elems * sizeof(double), 256);  // FP32 path: sigma planes are FP64
+  // FP32 path: sigma planes are FP64; every plane carries the sweeps' trash row
+  s.seg = round_up((max_seg_elems + kTrashElems) * sizeof(double), 256);
   s.plain = round_up(max_plane * sizeof(double), 256);
   s.slot = nseg * s.seg + npl * s.plain;
   return s;
@@ -450,7 +462,8 @@ struct sat_batch {
   std::vector<EventX> hx;          // host copy of the phase events
   std::vector<int> base_flags;     // per event, set at creation (source on the boundary)
   GridDev* d_grids = nullptr;
-  GridDev* d_grids64 = nullptr;    // FP32 batches: grids with FP64 media (re-solves)
+  GridDev* d_grids64 = nullptr;    // FP32 batches: grids with FP64 media (per-pair re-solves)
+  GridDev* d_grids64mw = nullptr;  // FP32 batches: FP64 media, re-solve geometry (batched re-solves)
   EventDev* d_events = nullptr;
   EventX* d_evx = nullptr;
   int* d_order = nullptr;
@@ -472,7 +485,7 @@ struct sat_batch {
   char* d_work64 = nullptr;        // FP32 batches: FP64 re-solve pool (pool_size entries)
   char* d_slots64 = nullptr;       // FP32 batches: sweep slots of the re-solve kernels
   int pool_size = 0, nslots64p = 0;
-  size_t pool_stride = 0;
+  size_t pool_stride = 0, slot_plane64 = 0, slot_bytes64 = 0;
   double f32_band = 0.02;
   // per-event planes of the phase pipeline
   size_t w_total = 0, pl_total = 0, col_total = 0;
@@ -513,7 +526,7 @@ struct sat_batch {
 
   void release() {
     for (auto& G : grids) G.release();
-    void* ps[] = {d_grids, d_grids64, d_events, d_evx, d_order, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
+    void* ps[] = {d_grids, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
                   d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_ov, d_evx64, d_work64, d_slots64, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
                   d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3};
     for (void* p : ps)
@@ -555,13 +568,16 @@ int build_groups(sat_batch* b) {
     i = j;
   }
   if (ne) SAT_CHECK(hipMemcpy(b->d_order, order.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
-  std::vector<GridDev> hg(b->grids.size()), hg64(b->grids.size());
+  std::vector<GridDev> hg(b->grids.size()), hg64(b->grids.size()), hgmw(b->grids.size());
   for (size_t gi = 0; gi < b->grids.size(); gi++) {
     hg[gi] = b->grids[gi].g;
     hg64[gi] = b->grids[gi].g64;
+    hgmw[gi] = b->grids[gi].gmw;
   }
   SAT_CHECK(hipMemcpy(b->d_grids, hg.data(), sizeof(GridDev) * hg.size(), hipMemcpyHostToDevice));
   if (b->d_grids64) SAT_CHECK(hipMemcpy(b->d_grids64, hg64.data(), sizeof(GridDev) * hg64.size(), hipMemcpyHostToDevice));
+  if (b->d_grids64mw)
+    SAT_CHECK(hipMemcpy(b->d_grids64mw, hgmw.data(), sizeof(GridDev) * hgmw.size(), hipMemcpyHostToDevice));
   b->groups_dirty = false;
   return SAT_OK;
 }
@@ -684,6 +700,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   b->max_nx = max_nx;
   B_CHECK(dalloc(&b->d_grids, d->ngrid));
   if (f32) B_CHECK(dalloc(&b->d_grids64, d->ngrid));
+  if (f32) B_CHECK(dalloc(&b->d_grids64mw, d->ngrid));
 
   // events: per-event plane offsets, index boxes, base flags
   const int ne = d->nevent;
@@ -765,23 +782,27 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(hipGetDeviceProperties(&prop, 0));
   const int waves_per_cu = 16;
   b->nslots_phase = std::max(1, std::min(ne, prop.multiProcessorCount * waves_per_cu));
-  b->slot_plane = round_up(max_wplane * sizeof(double), 256);
+  b->slot_plane = round_up((max_wplane + kTrashElems) * sizeof(double), 256);  // + the sweeps' trash row
   b->slot_bytes = 3 * b->slot_plane;
   B_CHECK(dalloc(&b->d_slots, b->slot_bytes * (size_t)b->nslots_phase));
   B_CHECK(hipMemset(b->d_slots, 0, b->slot_bytes * (size_t)b->nslots_phase));
 
   // FP32 batches: a pool of FP64 phase-kernel planes for the re-solves
   if (f32) {
+    size_t max_wplane_mw = 0;
+    for (const GridHost& G : b->grids) max_wplane_mw = std::max(max_wplane_mw, G.gmw.wplane);
+    b->slot_plane64 = round_up((max_wplane_mw + kTrashElems) * sizeof(double), 256);
+    b->slot_bytes64 = 3 * b->slot_plane64;
     b->pool_size = std::max(1, std::min(ne, std::max(64, (ne + 15) / 16)));
-    b->pool_stride = round_up(max_wplane, 64);
+    b->pool_stride = round_up(max_wplane_mw, 64);
     const size_t pool_bytes = 6 * b->pool_stride * sizeof(double) * (size_t)b->pool_size;
     B_CHECK(dalloc(&b->d_work64, pool_bytes));
     B_CHECK(hipMemset(b->d_work64, 0, pool_bytes));
     B_CHECK(dalloc(&b->d_evx64, std::max(ne, 1)));
     if (ne) B_CHECK(hipMemcpy(b->d_evx64, b->hx.data(), sizeof(EventX) * ne, hipMemcpyHostToDevice));
     b->nslots64p = std::min(b->pool_size, 2 * prop.multiProcessorCount);
-    B_CHECK(dalloc(&b->d_slots64, b->slot_bytes * (size_t)b->nslots64p));
-    B_CHECK(hipMemset(b->d_slots64, 0, b->slot_bytes * (size_t)b->nslots64p));
+    B_CHECK(dalloc(&b->d_slots64, b->slot_bytes64 * (size_t)b->nslots64p));
+    B_CHECK(hipMemset(b->d_slots64, 0, b->slot_bytes64 * (size_t)b->nslots64p));
   }
 
   // fused path: one slot for per-solve runs; FP32 batches also FP64 slots for re-solves
@@ -977,6 +998,8 @@ int sat_batch_run(sat_batch* b, int mode) {
   // runs: the FP64 phase kernels over the pool lists, the per-pair kernel over
   // the overflow lists; per-solve runs: the per-pair kernel.
   hipEvent_t* mk = b->mk;
+  // columns per lane of the re-solve geometry: ceil(nx / (64 kMultiWave)) = ceil(C / kMultiWave)
+  auto cmw = [&](size_t gi) { return (b->group_C[gi] + kMultiWave - 1) / kMultiWave; };
   auto launch_resolves = [&](bool phase) -> int {
     SAT_CHECK(hipEventRecord(mk[2], b->stream));
     SAT_CHECK(hipStreamWaitEvent(b->stream2, mk[2], 0));
@@ -985,7 +1008,7 @@ int sat_batch_run(sat_batch* b, int mode) {
       hipStream_t s2 = b->stream2;
       if (phase) {
         PhaseDev q = pd;
-        q.grids = b->d_grids64;
+        q.grids = b->d_grids64mw;
         q.ev = b->d_evx64;
         const size_t wb = b->pool_stride * (size_t)b->pool_size * sizeof(double);
         for (int o = 0; o < 2; o++) {
@@ -996,6 +1019,8 @@ int sat_batch_run(sat_batch* b, int mode) {
           q.dl[o] = b->d_work64 + (4 + o) * wb;
         }
         q.slots = b->d_slots64;
+        q.slot_bytes = b->slot_bytes64;
+        q.slot_plane = b->slot_plane64;
         q.want_skip = 1;
         q.ta_f32 = 1;
         for (size_t gi = 0; gi < ng; gi++) {
@@ -1005,8 +1030,8 @@ int sat_batch_run(sat_batch* b, int mode) {
           a.list_count = fb_count + gi;
           SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(node_grid.x, ub), s2, a));
           a.counter = resf_heads + gi;
-          SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(0, b->group_C[gi], dim3(std::min(ub, b->nslots64p)), s2, a, 0, 0,
-                                                   (int)gi));
+          SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(0, cmw(gi), kMultiWave, dim3(std::min(ub, b->nslots64p)), s2, a,
+                                                   0, 0, (int)gi));
         }
         if (b->nrec) SAT_CHECK(launch_recv(dim3(rec_blocks), s2, q, b->nrec));
         if (b->calc_adj) {
@@ -1024,7 +1049,7 @@ int sat_batch_run(sat_batch* b, int mode) {
             a.list = b->d_fb + b->group_first[gi];
             a.list_count = fb_count + gi;
             a.counter = resa_heads + gi;
-            SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(b->group_C[gi], dim3(std::min(ub, b->nslots64p)), s2, a, 0, 0));
+            SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(cmw(gi), kMultiWave, dim3(std::min(ub, b->nslots64p)), s2, a, 0, 0));
           }
         }
       }
@@ -1091,9 +1116,9 @@ int sat_batch_run(sat_batch* b, int mode) {
       a.ov_list = pd.ov_list + b->group_first[gi];
       const int nblk = std::min(b->group_count[gi], b->nslots_phase);
       const int C = b->group_C[gi], iso = b->group_iso[gi];
-      SAT_CHECK(f32 ? launch_fwd_sweep<SAT_PREC_F32>(iso, C, dim3(nblk), b->stream, a, b->group_first[gi],
+      SAT_CHECK(f32 ? launch_fwd_sweep<SAT_PREC_F32>(iso, C, 1, dim3(nblk), b->stream, a, b->group_first[gi],
                                                      b->group_count[gi], (int)gi)
-                    : launch_fwd_sweep<SAT_PREC_F64>(iso, C, dim3(nblk), b->stream, a, b->group_first[gi],
+                    : launch_fwd_sweep<SAT_PREC_F64>(iso, C, 1, dim3(nblk), b->stream, a, b->group_first[gi],
                                                      b->group_count[gi], (int)gi));
     }
     {
@@ -1115,9 +1140,9 @@ int sat_batch_run(sat_batch* b, int mode) {
         PhaseDev a = pd;
         a.counter = adj_heads + gi;
         const int nblk = std::min(b->group_count[gi], b->nslots_phase);
-        SAT_CHECK(f32 ? launch_adj_sweep<SAT_PREC_F32>(b->group_C[gi], dim3(nblk), b->stream, a, b->group_first[gi],
+        SAT_CHECK(f32 ? launch_adj_sweep<SAT_PREC_F32>(b->group_C[gi], 1, dim3(nblk), b->stream, a, b->group_first[gi],
                                                        b->group_count[gi])
-                      : launch_adj_sweep<SAT_PREC_F64>(b->group_C[gi], dim3(nblk), b->stream, a, b->group_first[gi],
+                      : launch_adj_sweep<SAT_PREC_F64>(b->group_C[gi], 1, dim3(nblk), b->stream, a, b->group_first[gi],
                                                        b->group_count[gi]));
       }
     }

@@ -31,20 +31,53 @@
 //               does: sweep 1 (+x,+y) writes orientation 1 for sweep 2
 //               (+x,-y), sweep 3 (-x,+y) writes orientation 0 for sweep 4
 //               (-x,-y), whose output starts the next round in orientation 0.
+//
+// Waves per grid (W).  W = 1: one wave sweeps the grid and the adjacent
+// lanes' values cross by ds_bpermute.  W > 1: a work-group of W waves sweeps
+// one grid, lane l of 64W (C = ceil(nx / 64W) columns each); the values cross
+// through LDS, double-buffered by step parity with one barrier per step (a
+// lane reads slot t&1 at step t; it is written again at step t+2, after the
+// barrier of step t+1).  Fewer columns per lane shorten a grid's latency.
 #pragma once
 #include "sat_device.hpp"
 
 namespace sat {
 
+// Every swept plane has kTrashElems elements past its end: the stores of lanes
+// outside the grid at a step go there, so that every load and store of a step
+// is issued unconditionally (conditional memory operations make the compiler's
+// wait counts conservative, which would wait for the prefetches issued in the
+// same step).
+constexpr int kTrashElems = kWave * kMaxC;
+
+// The geometry a sweep needs, read from the grid descriptor once before the
+// step loop (inside the loop the compiler could not keep it in registers: the
+// stores to the swept plane might alias the descriptor, and every reload would
+// wait for all loads in flight).
+struct SweepGeom {
+  int nx, ny, La, P, trash0;
+};
+
 struct SegLayout {
+  __device__ static __forceinline__ SweepGeom geom(const GridDev& g) {
+    return SweepGeom{g.nx, g.ny, g.La, g.P, (int)g.seg_plane};
+  }
   template <int C, int O>
-  __device__ static __forceinline__ int base(const GridDev& g, int j, int lane) { return j * g.P + lane * C; }
+  __device__ static __forceinline__ int base(const SweepGeom& s, int j, int lane) { return j * s.P + lane * C; }
+  template <int C>
+  __device__ static __forceinline__ int trash(const SweepGeom& s, int lane) { return s.trash0 + lane * C; }
 };
 struct WaveLayout {
-  template <int C, int O>
-  __device__ static __forceinline__ int base(const GridDev& g, int j, int lane) {
-    return wave_base<C, O>(g, j, lane);
+  __device__ static __forceinline__ SweepGeom geom(const GridDev& g) {
+    return SweepGeom{g.nx, g.ny, g.La, g.P, (int)g.wplane};
   }
+  template <int C, int O>
+  __device__ static __forceinline__ int base(const SweepGeom& s, int j, int lane) {
+    const int q = (O == 0) ? j + lane : j - lane + s.La - 1;
+    return (q * s.La + lane) * C;
+  }
+  template <int C>
+  __device__ static __forceinline__ int trash(const SweepGeom& s, int lane) { return s.trash0 + lane * C; }
 };
 
 // Policy P provides:
@@ -59,16 +92,32 @@ struct WaveLayout {
 //                                    V uS, A aS, bool hS, V uN, A aN, bool hN)
 //   template <int C> static double norm(const Head<C>&, const Body<C>&, int k, V newv)
 // O is the orientation of the sweep (0: XD*YD > 0, 1: XD*YD < 0).
-template <class Pol, class Lay, int C, int XD, int YD, bool NORM, int OOUT>
+// Work-group barrier that orders LDS only: the published values cross in LDS,
+// and a full __syncthreads() would also wait for every global load in flight
+// (the prefetched rows), stalling each step for a memory round trip.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Per-lane values published to the neighbouring lanes each step (W > 1).
+template <class Pol> struct Publish {
+  typename Pol::V pu, nu;  // previous-row edge value, next-row edge value
+  typename Pol::A pa, na;
+};
+
+template <class Pol, class Lay, int C, int XD, int YD, bool NORM, int OOUT, int W = 1>
 __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typename Pol::V* __restrict__ tin,
                                       typename Pol::V* __restrict__ tout,
-                                      const typename Pol::V* __restrict__ tst, double& lmax, double& lsum) {
+                                      const typename Pol::V* __restrict__ tst, double& lmax, double& lsum,
+                                      Publish<Pol>* __restrict__ xch = nullptr) {
   using V = typename Pol::V;
   using A = typename Pol::A;
   using Head = typename Pol::template Head<C>;
   using Body = typename Pol::template Body<C>;
   constexpr int O = (XD * YD > 0) ? 0 : 1;
-  const GridDev& g = *cx.g;
+  const SweepGeom g = Lay::geom(*cx.g);
   const int lane = threadIdx.x;
   const int La = g.La, nx = g.nx, ny = g.ny;
   const bool lane_ok = lane < La;
@@ -76,17 +125,16 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
   const int nsteps = ny + La - 1;
   auto rowj = [&](int r) { return (YD > 0) ? r : (ny - 1 - r); };
   auto in_range = [&](int r) { return lane_ok && r >= 0 && r < ny; };
+  // loads are issued for every lane and step; rows and lanes outside the grid
+  // read a clamped (in-bounds) location whose values are not used
+  const int lane_c = imin(lane, La - 1);
   auto load_head = [&](Head& H, int r) {
-    if (in_range(r)) {
-      const int j = rowj(r);
-      Pol::template load_head<C, O>(cx, H, Lay::template base<C, O>(g, j, lane), j, tin);
-    }
+    const int j = rowj(imin(imax(r, 0), ny - 1));
+    Pol::template load_head<C, O>(cx, H, Lay::template base<C, O>(g, j, lane_c), j, tin);
   };
   auto load_body = [&](Body& B, int r) {
-    if (in_range(r)) {
-      const int j = rowj(r);
-      Pol::template load_body<C, O>(cx, B, Lay::template base<C, O>(g, j, lane), j, NORM ? tst : nullptr);
-    }
+    const int j = rowj(imin(imax(r, 0), ny - 1));
+    Pol::template load_body<C, O>(cx, B, Lay::template base<C, O>(g, j, lane_c), j, NORM ? tst : nullptr);
   };
 
   Head cur_h, nxt_h, nn_h;
@@ -103,15 +151,40 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
     const int r = t - lag;
     load_body(nxt_b, r + 1);
     load_head(nn_h, r + 2);
-    const V nw_u = shfl_any((XD > 0) ? prev_u[C - 1] : prev_u[0], lane - XD);
-    const V od_u = shfl_any((XD > 0) ? nxt_h.u[0] : nxt_h.u[C - 1], lane + XD);
+    V nw_u, od_u;
     A nw_a{}, od_a{};
-    if constexpr (Pol::kAux) {
-      nw_a = shfl_any((XD > 0) ? prev_a[C - 1] : prev_a[0], lane - XD);
-      od_a = shfl_any((XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1], lane + XD);
+    if constexpr (W == 1) {
+      nw_u = shfl_any((XD > 0) ? prev_u[C - 1] : prev_u[0], lane - XD);
+      od_u = shfl_any((XD > 0) ? nxt_h.u[0] : nxt_h.u[C - 1], lane + XD);
+      if constexpr (Pol::kAux) {
+        nw_a = shfl_any((XD > 0) ? prev_a[C - 1] : prev_a[0], lane - XD);
+        od_a = shfl_any((XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1], lane + XD);
+      }
+    } else {
+      Publish<Pol>* buf = xch + (t & 1) * (kWave * W);
+      Publish<Pol> me;
+      me.pu = (XD > 0) ? prev_u[C - 1] : prev_u[0];
+      me.nu = (XD > 0) ? nxt_h.u[0] : nxt_h.u[C - 1];
+      me.pa = (XD > 0) ? prev_a[C - 1] : prev_a[0];
+      me.na = (XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1];
+      buf[lane] = me;
+      lds_barrier();
+      // lanes at the ends read a clamped slot; those values are never used
+      // (the node is on the grid edge or outside the active lanes)
+      const int src_w = imin(imax(lane - XD, 0), kWave * W - 1);
+      const int src_o = imin(imax(lane + XD, 0), kWave * W - 1);
+      nw_u = buf[src_w].pu;
+      od_u = buf[src_o].nu;
+      if constexpr (Pol::kAux) {
+        nw_a = buf[src_w].pa;
+        od_a = buf[src_o].na;
+      }
     }
-    if (in_range(r)) {
-      const int j = rowj(r);
+    {
+      // every lane evaluates and stores at every step: rows outside the grid
+      // compute on clamped inputs and store to the plane's trash row
+      const bool live = in_range(r);
+      const int j = rowj(imin(imax(r, 0), ny - 1));
       const bool hS = j > 0, hN = j < ny - 1;
       V newv[C];
 #pragma unroll
@@ -131,8 +204,8 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
         A aS, aN;
         if (YD > 0) { uS = prev_u[k]; aS = prev_a[k]; uN = nxt_h.u[k]; aN = nxt_h.aux[k]; }
         else { uN = prev_u[k]; aN = prev_a[k]; uS = nxt_h.u[k]; aS = nxt_h.aux[k]; }
-        // every lane evaluates the update; padding columns and fixed nodes keep their value
-        const bool valid = ix < nx;
+        // padding columns and fixed nodes keep their value
+        const bool valid = live && ix < nx;
         const V upd = Pol::template update<C>(cx, cur_h, cur_b, k, j, uW, aW, ix > 0, uE, aE, ix < nx - 1, uS, aS,
                                               hS, uN, aN, hN);
         newv[k] = (!valid || Pol::fixed(cx, ix, j)) ? cur_h.u[k] : upd;
@@ -142,7 +215,7 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
           lmax = dmax(lmax, d);
         }
       }
-      const int base = Lay::template base<C, OOUT>(g, j, lane);
+      const int base = live ? Lay::template base<C, OOUT>(g, j, lane) : Lay::template trash<C>(g, lane);
 #pragma unroll
       for (int k = 0; k < C; k++) tout[base + k] = newv[k];
 #pragma unroll
@@ -152,6 +225,138 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
     cur_b = nxt_b;
     nxt_h = nn_h;
   }
+}
+
+// The same sweep with a ring of D+1 rows in registers (D >= 2): at step t the
+// lane computes row r from ring slot t % (D+1), reads row r+1 from the next
+// slot, and issues the loads of row r + D into the slot row r-1 left, so each
+// row's loads have D-1 to D steps to arrive.  The step loop is unrolled by
+// D+1 (slot indices are compile-time); the steps past the end are no-ops for
+// every lane (r >= ny), which keeps the per-step barrier uniform.  Used where
+// few columns per lane leave registers for the ring (W > 1).
+template <class Pol, class Lay, int C, int XD, int YD, bool NORM, int OOUT, int W, int D>
+__device__ __forceinline__ void sweep_ring(const typename Pol::Ctx& cx, const typename Pol::V* __restrict__ tin,
+                                           typename Pol::V* __restrict__ tout,
+                                           const typename Pol::V* __restrict__ tst, double& lmax, double& lsum,
+                                           Publish<Pol>* __restrict__ xch) {
+  static_assert(D >= 2, "the ring needs two rows in flight");
+  using V = typename Pol::V;
+  using A = typename Pol::A;
+  using Head = typename Pol::template Head<C>;
+  using Body = typename Pol::template Body<C>;
+  constexpr int O = (XD * YD > 0) ? 0 : 1;
+  constexpr int R = D + 1;
+  const SweepGeom g = Lay::geom(*cx.g);
+  const int lane = threadIdx.x;
+  const int La = g.La, nx = g.nx, ny = g.ny;
+  const bool lane_ok = lane < La;
+  const int lag = (XD > 0) ? lane : (La - 1 - lane);
+  const int nsteps = ny + La - 1;
+  auto rowj = [&](int r) { return (YD > 0) ? r : (ny - 1 - r); };
+  auto in_range = [&](int r) { return lane_ok && r >= 0 && r < ny; };
+  const int lane_c = imin(lane, La - 1);
+  auto load = [&](Head& H, Body& B, int r) {  // unconditional, clamped (see kTrashElems)
+    const int j = rowj(imin(imax(r, 0), ny - 1));
+    const int base = Lay::template base<C, O>(g, j, lane_c);
+    Pol::template load_head<C, O>(cx, H, base, j, tin);
+    Pol::template load_body<C, O>(cx, B, base, j, NORM ? tst : nullptr);
+  };
+
+  Head H[R];
+  Body B[R];
+  V prev_u[C];
+  A prev_a[C];
+#pragma unroll
+  for (int k = 0; k < C; k++) { prev_u[k] = V(0); prev_a[k] = A{}; }
+#pragma unroll
+  for (int s = 0; s < D; s++) load(H[s], B[s], s - lag);
+
+  for (int t0 = 0; t0 < nsteps; t0 += R) {
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+      const int t = t0 + s;
+      const int r = t - lag;
+      Head& cur_h = H[s];
+      Body& cur_b = B[s];
+      Head& nxt_h = H[(s + 1) % R];
+      load(H[(s + D) % R], B[(s + D) % R], r + D);
+      V nw_u, od_u;
+      A nw_a{}, od_a{};
+      if constexpr (W == 1) {
+        nw_u = shfl_any((XD > 0) ? prev_u[C - 1] : prev_u[0], lane - XD);
+        od_u = shfl_any((XD > 0) ? nxt_h.u[0] : nxt_h.u[C - 1], lane + XD);
+        if constexpr (Pol::kAux) {
+          nw_a = shfl_any((XD > 0) ? prev_a[C - 1] : prev_a[0], lane - XD);
+          od_a = shfl_any((XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1], lane + XD);
+        }
+      } else {
+        Publish<Pol>* buf = xch + (t & 1) * (kWave * W);
+        Publish<Pol> me;
+        me.pu = (XD > 0) ? prev_u[C - 1] : prev_u[0];
+        me.nu = (XD > 0) ? nxt_h.u[0] : nxt_h.u[C - 1];
+        me.pa = (XD > 0) ? prev_a[C - 1] : prev_a[0];
+        me.na = (XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1];
+        buf[lane] = me;
+        lds_barrier();
+        const int src_w = imin(imax(lane - XD, 0), kWave * W - 1);
+        const int src_o = imin(imax(lane + XD, 0), kWave * W - 1);
+        nw_u = buf[src_w].pu;
+        od_u = buf[src_o].nu;
+        if constexpr (Pol::kAux) {
+          nw_a = buf[src_w].pa;
+          od_a = buf[src_o].na;
+        }
+      }
+      {
+        const bool live = in_range(r);
+        const int j = rowj(imin(imax(r, 0), ny - 1));
+        const bool hS = j > 0, hN = j < ny - 1;
+        V newv[C];
+#pragma unroll
+        for (int kk = 0; kk < C; kk++) {
+          const int k = (XD > 0) ? kk : (C - 1 - kk);
+          const int ix = lane * C + k;
+          V uW, uE;
+          A aW{}, aE{};
+          if (XD > 0) {
+            if (k > 0) { uW = newv[k - 1]; aW = cur_h.aux[k - 1]; } else { uW = nw_u; aW = nw_a; }
+            if (k < C - 1) { uE = cur_h.u[k + 1]; aE = cur_h.aux[k + 1]; } else { uE = od_u; aE = od_a; }
+          } else {
+            if (k < C - 1) { uE = newv[k + 1]; aE = cur_h.aux[k + 1]; } else { uE = nw_u; aE = nw_a; }
+            if (k > 0) { uW = cur_h.u[k - 1]; aW = cur_h.aux[k - 1]; } else { uW = od_u; aW = od_a; }
+          }
+          V uS, uN;
+          A aS, aN;
+          if (YD > 0) { uS = prev_u[k]; aS = prev_a[k]; uN = nxt_h.u[k]; aN = nxt_h.aux[k]; }
+          else { uN = prev_u[k]; aN = prev_a[k]; uS = nxt_h.u[k]; aS = nxt_h.aux[k]; }
+          const bool valid = live && ix < nx;
+          const V upd = Pol::template update<C>(cx, cur_h, cur_b, k, j, uW, aW, ix > 0, uE, aE, ix < nx - 1, uS,
+                                                aS, hS, uN, aN, hN);
+          newv[k] = (!valid || Pol::fixed(cx, ix, j)) ? cur_h.u[k] : upd;
+          if (NORM) {
+            const double d = valid ? Pol::template norm<C>(cur_h, cur_b, k, newv[k]) : 0.0;
+            lsum += d;
+            lmax = dmax(lmax, d);
+          }
+        }
+        const int base = live ? Lay::template base<C, OOUT>(g, j, lane) : Lay::template trash<C>(g, lane);
+#pragma unroll
+        for (int k = 0; k < C; k++) tout[base + k] = newv[k];
+#pragma unroll
+        for (int k = 0; k < C; k++) { prev_u[k] = newv[k]; prev_a[k] = cur_h.aux[k]; }
+      }
+    }
+  }
+}
+
+// One sweep: the head/body pipeline (D = 1) or the register ring (D >= 2).
+template <class Pol, class Lay, int C, int XD, int YD, bool NORM, int OOUT, int W, int D>
+__device__ __forceinline__ void sweep_any(const typename Pol::Ctx& cx, const typename Pol::V* __restrict__ tin,
+                                          typename Pol::V* __restrict__ tout,
+                                          const typename Pol::V* __restrict__ tst, double& lmax, double& lsum,
+                                          Publish<Pol>* __restrict__ xch) {
+  if constexpr (D == 1) sweep<Pol, Lay, C, XD, YD, NORM, OOUT, W>(cx, tin, tout, tst, lmax, lsum, xch);
+  else sweep_ring<Pol, Lay, C, XD, YD, NORM, OOUT, W, D>(cx, tin, tout, tst, lmax, lsum, xch);
 }
 
 struct IterOut {
@@ -165,9 +370,42 @@ struct IterOut {
 // :471-509).  Three rotating planes: B0 holds the iterate at the start of a round
 // (orientation 0 under WaveLayout).  Returns the plane holding the result
 // (orientation 0).
-template <class Pol, class Lay, int C>
+// Work-group reductions in a fixed order (W > 1: per wave, then across waves).
+template <int W>
+__device__ inline double block_max(double v, double* red) {
+  v = wave_max(v);
+  if constexpr (W == 1) {
+    return v;
+  } else {
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int w = 1; w < W; w++) r = dmax(r, red[w]);
+    return r;
+  }
+}
+template <int W>
+__device__ inline double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  if constexpr (W == 1) {
+    return v;
+  } else {
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int w = 1; w < W; w++) r += red[w];
+    return r;
+  }
+}
+
+template <class Pol, class Lay, int C, int W = 1, int D = 1>
 __device__ typename Pol::V* iterate(const typename Pol::Ctx& cx, typename Pol::V* b0, typename Pol::V* b1,
-                                    typename Pol::V* b2, int maxiter, double tol, IterOut& io) {
+                                    typename Pol::V* b2, int maxiter, double tol, IterOut& io,
+                                    Publish<Pol>* xch = nullptr, double* red = nullptr) {
   using V = typename Pol::V;
   V* B0 = b0;
   V* B1 = b1;
@@ -180,16 +418,16 @@ __device__ typename Pol::V* iterate(const typename Pol::Ctx& cx, typename Pol::V
   const double npts = (double)(cx.g->nx * cx.g->ny);
   for (int it = 0; it < maxiter; ++it) {
     double lmax = 0.0, lsum = 0.0;
-    sweep<Pol, Lay, C, 1, 1, false, 1>(cx, B0, B1, nullptr, lmax, lsum);
+    sweep_any<Pol, Lay, C, 1, 1, false, 1, W, D>(cx, B0, B1, nullptr, lmax, lsum, xch);
     __syncthreads();
-    sweep<Pol, Lay, C, 1, -1, false, 1>(cx, B1, B2, nullptr, lmax, lsum);
+    sweep_any<Pol, Lay, C, 1, -1, false, 1, W, D>(cx, B1, B2, nullptr, lmax, lsum, xch);
     __syncthreads();
-    sweep<Pol, Lay, C, -1, 1, false, 0>(cx, B2, B1, nullptr, lmax, lsum);
+    sweep_any<Pol, Lay, C, -1, 1, false, 0, W, D>(cx, B2, B1, nullptr, lmax, lsum, xch);
     __syncthreads();
-    sweep<Pol, Lay, C, -1, -1, true, 0>(cx, B1, B2, B0, lmax, lsum);
+    sweep_any<Pol, Lay, C, -1, -1, true, 0, W, D>(cx, B1, B2, B0, lmax, lsum, xch);
     __syncthreads();
-    const double Linf = wave_max(lmax);
-    const double L1 = wave_sum(lsum) / npts;
+    const double Linf = block_max<W>(lmax, red);
+    const double L1 = block_sum<W>(lsum, red) / npts;
     io.l1 = L1;
     io.linf_prev = io.linf;
     io.linf = Linf;

@@ -118,6 +118,8 @@ __device__ inline typename Prec<PREC>::Adj::Ctx adj_ctx(const GridDev& g, const 
   using V = typename Prec<PREC>::V;
   typename Prec<PREC>::Adj::Ctx cx;
   cx.g = &g;
+  cx.nx = g.nx;
+  cx.ny = g.ny;
   if constexpr (PREC == SAT_PREC_F64) {
     cx.a1p = static_cast<const V*>(ws.seg[0]);
     cx.a2m = static_cast<const V*>(ws.seg[1]);

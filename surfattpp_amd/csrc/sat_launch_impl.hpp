@@ -39,16 +39,31 @@ hipError_t launch_init(dim3 grid, hipStream_t s, const PhaseDev& pd) {
 }
 
 template <int PREC>
-hipError_t launch_fwd_sweep(int iso, int C, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count,
-                            int group) {
+hipError_t launch_fwd_sweep(int iso, int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first,
+                            int count, int group) {
+  if (W == kMultiWave) {
+    if constexpr (PREC == SAT_PREC_F64) {
+      if (C == 1)
+        hipLaunchKernelGGL((fwd_sweep_kernel<PREC, 1, false, kMultiWave>), grid, dim3(kWave * kMultiWave), 0, s,
+                           pd, first, count, group);
+      else if (C == 2)
+        hipLaunchKernelGGL((fwd_sweep_kernel<PREC, 2, false, kMultiWave>), grid, dim3(kWave * kMultiWave), 0, s,
+                           pd, first, count, group);
+      else
+        return hipErrorInvalidValue;
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+  }
+  if (W != 1) return hipErrorInvalidValue;
   return with_C(C, [&]<int CC>() {
     if constexpr (PREC == SAT_PREC_F32) {
       if (iso) {
-        hipLaunchKernelGGL((fwd_sweep_kernel<PREC, CC, true>), grid, dim3(kWave), 0, s, pd, first, count, group);
+        hipLaunchKernelGGL((fwd_sweep_kernel<PREC, CC, true, 1>), grid, dim3(kWave), 0, s, pd, first, count, group);
         return;
       }
     }
-    hipLaunchKernelGGL((fwd_sweep_kernel<PREC, CC, false>), grid, dim3(kWave), 0, s, pd, first, count, group);
+    hipLaunchKernelGGL((fwd_sweep_kernel<PREC, CC, false, 1>), grid, dim3(kWave), 0, s, pd, first, count, group);
   });
 }
 
@@ -65,9 +80,24 @@ hipError_t launch_delta(dim3 grid, hipStream_t s, const PhaseDev& pd) {
 }
 
 template <int PREC>
-hipError_t launch_adj_sweep(int C, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count) {
+hipError_t launch_adj_sweep(int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count) {
+  if (W == kMultiWave) {
+    if constexpr (PREC == SAT_PREC_F64) {
+      if (C == 1)
+        hipLaunchKernelGGL((adj_sweep_kernel<PREC, 1, kMultiWave>), grid, dim3(kWave * kMultiWave), 0, s, pd, first,
+                           count);
+      else if (C == 2)
+        hipLaunchKernelGGL((adj_sweep_kernel<PREC, 2, kMultiWave>), grid, dim3(kWave * kMultiWave), 0, s, pd, first,
+                           count);
+      else
+        return hipErrorInvalidValue;
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+  }
+  if (W != 1) return hipErrorInvalidValue;
   return with_C(C, [&]<int CC>() {
-    hipLaunchKernelGGL((adj_sweep_kernel<PREC, CC>), grid, dim3(kWave), 0, s, pd, first, count);
+    hipLaunchKernelGGL((adj_sweep_kernel<PREC, CC, 1>), grid, dim3(kWave), 0, s, pd, first, count);
   });
 }
 
@@ -81,7 +111,7 @@ hipError_t launch_adj_sweep(int C, dim3 grid, hipStream_t s, const PhaseDev& pd,
 #define SAT_INSTANTIATE_PHASE(P)                                                                           \
   template hipError_t sat::launch_trig<P>(dim3, hipStream_t, const PhaseDev&);                             \
   template hipError_t sat::launch_init<P>(dim3, hipStream_t, const PhaseDev&);                             \
-  template hipError_t sat::launch_fwd_sweep<P>(int, int, dim3, hipStream_t, const PhaseDev&, int, int, int); \
+  template hipError_t sat::launch_fwd_sweep<P>(int, int, int, dim3, hipStream_t, const PhaseDev&, int, int, int); \
   template hipError_t sat::launch_adj_setup<P>(dim3, hipStream_t, const PhaseDev&);                        \
   template hipError_t sat::launch_delta<P>(dim3, hipStream_t, const PhaseDev&);                            \
-  template hipError_t sat::launch_adj_sweep<P>(int, dim3, hipStream_t, const PhaseDev&, int, int);
+  template hipError_t sat::launch_adj_sweep<P>(int, int, dim3, hipStream_t, const PhaseDev&, int, int);

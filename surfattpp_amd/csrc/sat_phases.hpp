@@ -20,6 +20,7 @@
 #pragma once
 #include <type_traits>
 
+
 #include "sat_crmath.hpp"
 #include "sat_engine.hpp"
 #include "sat_policies.hpp"
@@ -88,6 +89,22 @@ struct PhaseDev {
   int fwd_maxiter, adj_maxiter;
 };
 
+// Rows in flight per lane (register ring depth, sat_engine.hpp): multi-wave
+// sweeps, and the single-wave forward and adjoint sweeps (1 = the head/body
+// pipeline, fewer registers; >= 2 = the ring, no wait on the prefetches).
+constexpr int kRingDepth = 4;
+constexpr int kFwdDepth1 = 1;
+constexpr int kAdjDepth1 = 2;
+
+// The multi-wave sweeps carry the latency-critical FP64 re-solves, which share
+// the CUs with a throughput kernel on the other stream: their waves take the
+// highest instruction-issue priority on their SIMDs (s_setprio; the streams
+// themselves keep the default priority).
+template <int W>
+__device__ __forceinline__ void latency_critical_priority() {
+  if constexpr (W > 1) __builtin_amdgcn_s_setprio(3);
+}
+
 template <int PREC> struct PhaseT;
 template <> struct PhaseT<SAT_PREC_F64> {
   using V = double;     // T0 planes and adjoint
@@ -105,6 +122,21 @@ template <> struct PhaseT<SAT_PREC_F32> {
   using FwdIso = FwdF32Iso;
   using Adj = AdjF32Face;
 };
+
+// Next work-queue entry for the whole work-group (thread 0 pops it).
+template <int W>
+__device__ inline int pop_work(int* counter, int* s_idx) {
+  if constexpr (W == 1) {
+    int idx = 0;
+    if (threadIdx.x == 0) idx = atomicAdd(counter, 1);
+    return __shfl(idx, 0);
+  } else {
+    __syncthreads();  // the previous entry's readers are done with *s_idx
+    if (threadIdx.x == 0) *s_idx = atomicAdd(counter, 1);
+    __syncthreads();
+    return *s_idx;
+  }
+}
 
 // Event of a node-parallel block: blockIdx.y, or the list entry blockIdx.y; -1 if none.
 __device__ inline int block_event(const PhaseDev& pd) {
@@ -223,13 +255,18 @@ __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const
   return cx;
 }
 
-// Persistent forward sweeps, one wave per event.
-template <int PREC, int C, bool ISO>
-__global__ __launch_bounds__(64) void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
+// Persistent forward sweeps, one work-group of W waves per event.
+template <int PREC, int C, bool ISO, int W>
+__global__ __launch_bounds__(64 * W) void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
   using PT = PhaseT<PREC>;
   using U = typename PT::U;
   using Pol = std::conditional_t<ISO, typename PT::FwdIso, typename PT::Fwd>;
   const int lane = threadIdx.x;
+  const int nthr = kWave * W;
+  __shared__ Publish<Pol> xch[(W > 1) ? 2 * kWave * W : 1];
+  __shared__ double red[W];
+  __shared__ int s_idx;
+  latency_critical_priority<W>();
   char* sb = pd.slots + (size_t)blockIdx.x * pd.slot_bytes;
   U* Q0 = reinterpret_cast<U*>(sb);
   U* Q1 = reinterpret_cast<U*>(sb + pd.slot_plane);
@@ -237,15 +274,13 @@ __global__ __launch_bounds__(64) void fwd_sweep_kernel(PhaseDev pd, int first, i
   constexpr bool kSigma = PREC == SAT_PREC_F32;
   const int n_work = pd.list ? *pd.list_count : count;
   for (;;) {
-    int idx = 0;
-    if (lane == 0) idx = atomicAdd(pd.counter, 1);
-    idx = __shfl(idx, 0);
+    const int idx = pop_work<W>(pd.counter, &s_idx);
     if (idx >= n_work) break;
     const int e = pd.list ? pd.list[idx] : pd.order[first + idx];
     const EventX ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];
     const int nx = g.nx, n = (int)g.wplane;
-    for (int q = lane; q < n; q += kWave) {  // :119-129 initial tau (sigma), orientation 0
+    for (int q = lane; q < n; q += nthr) {  // :119-129 initial tau (sigma), orientation 0
       int ix, iy;
       const bool real = wave_decode0(g, q, ix, iy);
       const bool fixed = real && ix >= ev.fb[0] && ix <= ev.fb[1] && iy >= ev.fb[2] && iy <= ev.fb[3];
@@ -254,7 +289,8 @@ __global__ __launch_bounds__(64) void fwd_sweep_kernel(PhaseDev pd, int first, i
     __syncthreads();
     const auto cx = phase_fwd_ctx<PREC, ISO>(pd, g, ev);
     IterOut io;
-    const U* u = iterate<Pol, WaveLayout, C>(cx, Q0, Q1, Q2, pd.fwd_maxiter, kFwdTol, io);
+    const U* u = iterate<Pol, WaveLayout, C, W, (W > 1) ? kRingDepth : kFwdDepth1>(cx, Q0, Q1, Q2, pd.fwd_maxiter, kFwdTol,
+                                                                          io, xch, red);
     if (lane == 0) {
       pd.conv[2 * (size_t)e] = io.linf;
       pd.conv[2 * (size_t)e + 1] = io.linf_prev;
@@ -291,7 +327,7 @@ __global__ __launch_bounds__(64) void fwd_sweep_kernel(PhaseDev pd, int first, i
     double* T = pd.T + ev.pl_off;
     const typename PT::V* t0 = static_cast<const typename PT::V*>(pd.t0[0]) + ev.w_off;
     for (int iy = 0; iy < g.ny; iy++)
-      for (int ix = lane; ix < nx; ix += kWave) {
+      for (int ix = lane; ix < nx; ix += nthr) {
         const size_t o = wave_index(g, 0, ix, iy);
         const size_t p = ix + (size_t)iy * nx;
         if (kSigma) T[p] = pd.T0d[ev.pl_off + p] + (double)u[o] * pd.T0d[ev.pl_off + p];
@@ -394,30 +430,33 @@ __global__ void delta_kernel(PhaseDev pd) {
   }
 }
 
-// Persistent adjoint sweeps, one wave per event; mask and store.
-template <int PREC, int C>
-__global__ __launch_bounds__(64) void adj_sweep_kernel(PhaseDev pd, int first, int count) {
+// Persistent adjoint sweeps, one work-group of W waves per event; mask and store.
+template <int PREC, int C, int W>
+__global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int first, int count) {
   using PT = PhaseT<PREC>;
   using V = typename PT::V;
   using F = typename PT::F;
   using Pol = typename PT::Adj;
   const int lane = threadIdx.x;
+  const int nthr = kWave * W;
+  __shared__ Publish<Pol> xch[(W > 1) ? 2 * kWave * W : 1];
+  __shared__ double red[W];
+  __shared__ int s_idx;
+  latency_critical_priority<W>();
   char* sb = pd.slots + (size_t)blockIdx.x * pd.slot_bytes;
   V* Q0 = reinterpret_cast<V*>(sb);
   V* Q1 = reinterpret_cast<V*>(sb + pd.slot_plane);
   V* Q2 = reinterpret_cast<V*>(sb + 2 * pd.slot_plane);
   const int n_work = pd.list ? *pd.list_count : count;
   for (;;) {
-    int idx = 0;
-    if (lane == 0) idx = atomicAdd(pd.counter, 1);
-    idx = __shfl(idx, 0);
+    const int idx = pop_work<W>(pd.counter, &s_idx);
     if (idx >= n_work) break;
     const int e = pd.list ? pd.list[idx] : pd.order[first + idx];
     if (pd.skip[e] != pd.want_skip) continue;
     const EventX ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];
     const int nx = g.nx, ny = g.ny, n = (int)g.wplane;
-    for (int q = lane; q < n; q += kWave) {  // :446-449 boundary 0, interior 100, orientation 0
+    for (int q = lane; q < n; q += nthr) {  // :446-449 boundary 0, interior 100, orientation 0
       int ix, iy;
       const bool real = wave_decode0(g, q, ix, iy);
       Q0[q] = (real && ix >= 1 && ix < nx - 1 && iy >= 1 && iy < ny - 1) ? V(100) : V(0);
@@ -425,6 +464,8 @@ __global__ __launch_bounds__(64) void adj_sweep_kernel(PhaseDev pd, int first, i
     __syncthreads();
     typename Pol::Ctx cx;
     cx.g = &g;
+    cx.nx = g.nx;
+    cx.ny = g.ny;
 #pragma unroll
     for (int o = 0; o < 2; o++) {
       cx.fxy[o] = static_cast<const F*>(pd.fxy[o]) + ev.w_off;
@@ -435,13 +476,14 @@ __global__ __launch_bounds__(64) void adj_sweep_kernel(PhaseDev pd, int first, i
       cx.dy = g.dy;
     }
     IterOut io;
-    const V* ta = iterate<Pol, WaveLayout, C>(cx, Q0, Q1, Q2, pd.adj_maxiter, kAdjTol, io);
+    const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? kRingDepth : kAdjDepth1>(cx, Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,
+                                                                           io, xch, red);
     if (lane == 0) {
       pd.iters_a[e] = io.iters;
       if (!io.converged) atomicOr(pd.flags + e, FLAG_ADJ_MAXITER);
     }
     for (int iy = 0; iy < ny; iy++)  // :514-536 mask, column-major
-      for (int ix = lane; ix < nx; ix += kWave) {
+      for (int ix = lane; ix < nx; ix += nthr) {
         const bool m = ix >= ev.mb[0] && ix <= ev.mb[1] && iy >= ev.mb[2] && iy <= ev.mb[3];
         const V v = m ? V(0) : ta[wave_index(g, 0, ix, iy)];
         const size_t o = ev.pl_off + ix + (size_t)iy * nx;

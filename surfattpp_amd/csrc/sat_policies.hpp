@@ -88,6 +88,7 @@ struct AdjF64 {
   static constexpr bool kAux = false;
   struct Ctx {
     const GridDev* g;
+    int nx, ny;
     const double *a1p, *a2m, *b1p, *b2m, *dl, *d;
     double dx, dy;
   };
@@ -112,7 +113,7 @@ struct AdjF64 {
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
-    return ix == 0 || ix >= cx.g->nx - 1 || j == 0 || j == cx.g->ny - 1;
+    return ix == 0 || ix >= cx.nx - 1 || j == 0 || j == cx.ny - 1;
   }
   template <int C>
   __device__ static double update(const Ctx& cx, const Head<C>&, const Body<C>& r, int k, int, double uW, double,
@@ -138,6 +139,7 @@ struct AdjF64Face {
   static constexpr bool kAux = true;
   struct Ctx {
     const GridDev* g;
+    int nx, ny;
     const double2* fxy[2];  // (a1, b1) per node
     const double* dl[2];    // delta
     double dx, dy;
@@ -158,7 +160,7 @@ struct AdjF64Face {
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
-    return ix == 0 || ix >= cx.g->nx - 1 || j == 0 || j == cx.g->ny - 1;
+    return ix == 0 || ix >= cx.nx - 1 || j == 0 || j == cx.ny - 1;
   }
   template <int C>
   __device__ static double update(const Ctx& cx, const Head<C>& h, const Body<C>& r, int k, int, double uW, double2,
@@ -392,6 +394,7 @@ struct AdjF32 {
   static constexpr bool kAux = false;
   struct Ctx {
     const GridDev* g;
+    int nx, ny;
     const float *cW, *cE, *cS, *cN, *cD;
   };
   template <int C> struct Head { float u[C], aux[C]; };
@@ -414,7 +417,7 @@ struct AdjF32 {
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
-    return ix == 0 || ix >= cx.g->nx - 1 || j == 0 || j == cx.g->ny - 1;
+    return ix == 0 || ix >= cx.nx - 1 || j == 0 || j == cx.ny - 1;
   }
   template <int C>
   __device__ static float update(const Ctx&, const Head<C>&, const Body<C>& r, int k, int, float uW, float, bool,
@@ -438,6 +441,7 @@ struct AdjF32Face {
   static constexpr bool kAux = true;
   struct Ctx {
     const GridDev* g;
+    int nx, ny;
     const float2* fxy[2];
     const float* dl[2];
   };
@@ -457,7 +461,7 @@ struct AdjF32Face {
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
-    return ix == 0 || ix >= cx.g->nx - 1 || j == 0 || j == cx.g->ny - 1;
+    return ix == 0 || ix >= cx.nx - 1 || j == 0 || j == cx.ny - 1;
   }
   template <int C>
   __device__ static float update(const Ctx&, const Head<C>& h, const Body<C>& r, int k, int, float uW, float2, bool,
