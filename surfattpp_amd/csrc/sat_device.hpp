@@ -99,6 +99,62 @@ __device__ inline size_t wave_index(const GridDev& g, int o, int ix, int iy) {
   return wave_index(g.C, g.La, o, ix, iy);
 }
 
+// Load through a global-memory pointer.  A pointer the compiler cannot prove
+// global (one read from device memory, such as a GridDev plane) is loaded with a
+// flat instruction, which also counts on the LDS counter: each wait for the
+// step's cross-lane exchange would then also wait for the prefetched rows.
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+// HIP's vector types are classes whose copy goes through a generic reference:
+// load them as the compiler's own vector types.
+__device__ __forceinline__ float2 gld(const float2* p) {
+  typedef float v2 __attribute__((ext_vector_type(2)));
+  const v2 v = *(const __attribute__((address_space(1))) v2*)p;
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float4 gld(const float4* p) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = *(const __attribute__((address_space(1))) v4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ double2 gld(const double2* p) {
+  typedef double v2 __attribute__((ext_vector_type(2)));
+  const v2 v = *(const __attribute__((address_space(1))) v2*)p;
+  return make_double2(v.x, v.y);
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
+// Element i of a plane, addressed as a wave-uniform base plus a 32-bit byte
+// offset (planes are < 4 GiB): the load takes the base from scalar registers
+// and needs no 64-bit vector address.  (Stores keep pointer arithmetic: with a
+// 32-bit offset that may wrap, consecutive stores are not merged.)
+template <class T>
+__device__ __forceinline__ T gld(const T* base, unsigned i) {
+  const unsigned off = i * (unsigned)sizeof(T);
+  return gld(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off));
+}
+
+// A wave-uniform copy of v (readfirstlane of every 32-bit word): for descriptors
+// read from device memory that the compiler cannot prove unchanged by the
+// kernel's own stores, so it loads them into vector registers.  Scalar copies
+// leave the vector registers to the sweeps' row rings.  Every lane must hold
+// the same v.
+template <class T>
+__device__ __forceinline__ T uniform(const T& v) {
+  static_assert(sizeof(T) % 4 == 0, "32-bit words");
+  int w[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); i++) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
+
 // Cross-lane reads of the swept value and its auxiliary (ds_bpermute).
 __device__ __forceinline__ float shfl_any(float v, int src) { return __shfl(v, src); }
 __device__ __forceinline__ double shfl_any(double v, int src) { return __shfl(v, src); }
@@ -187,13 +243,24 @@ __device__ inline double interp2d_point(const double* xg, int nx, const double* 
 // One forward node (src/eikonal_solver.cpp:155-283).  W/E/S/N = (ix-1,iy),
 // (ix+1,iy), (ix,iy-1), (ix,iy+1); h* says the neighbour exists; t* its tau,
 // T0* its factor T0.
+//
+// Straight-line: every candidate of the reference is evaluated with the
+// reference's operations and operand order, whether or not the reference
+// would reach it, and an acceptance mask (the reference's skip and acceptance
+// conditions) decides whether it enters the running minimum.  The minimum is
+// taken in the reference's candidate order, so the result is the same bits;
+// without branches the compiler interleaves the independent candidates, which
+// hides the latency of the FP64 divide and square-root sequences.  A rejected
+// candidate may be NaN or infinite (square root of a negative discriminant,
+// division by a vanishing coefficient); it never reaches the minimum.
 __device__ inline double fwd_update_f64(double tau_self, double t0, double t0x, double t0y,
                                         double a, double b, double c, double f,
                                         double dx, double dy,
                                         bool hW, double tW, double T0W, bool hE, double tE, double T0E,
                                         bool hS, double tS, double T0S, bool hN, double tN, double T0N) {
   const double det = a * b - c * c;
-  double tn = tau_self;
+  double cand[16];
+  bool acc[16];
   // :173-219 two-point quadrants (sx, sy) = (-1,-1), (-1,+1), (+1,-1), (+1,+1)
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -201,7 +268,6 @@ __device__ inline double fwd_update_f64(double tau_self, double t0, double t0x, 
     const int sy = (k & 1) ? 1 : -1;
     const bool hx = (sx < 0) ? hW : hE;
     const bool hy = (sy < 0) ? hS : hN;
-    if (!(hx && hy)) continue;
     const double tX = (sx < 0) ? tW : tE;
     const double tY = (sy < 0) ? tS : tN;
     const double px = t0x - sx * (t0 / dx);
@@ -212,7 +278,7 @@ __device__ inline double fwd_update_f64(double tau_self, double t0, double t0x, 
     const double eq_b = 2.0 * (a * px * qx + b * py * qy - c * (px * qy + py * qx));
     const double eq_c = a * qx * qx + b * qy * qy - 2.0 * c * qx * qy - f * f;
     const double Delta = eq_b * eq_b - 4.0 * eq_a * eq_c;
-    if (Delta < 0.0 || fabs(eq_a) < 1e-30) continue;
+    const bool ok = hx & hy & !(Delta < 0.0 || fabs(eq_a) < 1e-30);
     const double sq = sqrt(Delta);
     const double two_a = 2.0 * eq_a;
 #pragma unroll
@@ -222,58 +288,35 @@ __device__ inline double fwd_update_f64(double tau_self, double t0, double t0x, 
       const double Ty = py * tmp_tau + qy;
       const double char_x = a * Tx - c * Ty;
       const double char_y = b * Ty - c * Tx;
-      if (-sx * char_x >= 0.0 && -sy * char_y >= 0.0) tn = dmin(tn, tmp_tau);
+      cand[2 * k + sgn] = tmp_tau;
+      acc[2 * k + sgn] = ok & (-sx * char_x >= 0.0) & (-sy * char_y >= 0.0);
     }
   }
-  // :224-279 one-point stencils
-  if (hW) {
-    const double px = t0x + t0 / dx;
-    const double qx = -(t0 / dx) * tW;
-    if (det > 0.0) {
-      const double dis = sqrt(f * f * b / det);
+  // :224-279 one-point stencils W, E (dis from b), S, N (dis from a)
+  const double dis_x = sqrt(f * f * b / det);
+  const double dis_y = sqrt(f * f * a / det);
 #pragma unroll
-      for (int sgn = 0; sgn < 2; sgn++) {
-        const double tmp_tau = ((sgn == 0 ? dis : -dis) - qx) / px;
-        if (tmp_tau * t0 >= tW * T0W && tmp_tau > tW * 0.5) tn = dmin(tn, tmp_tau);
-      }
-    }
-  }
-  if (hE) {
-    const double px = t0x - t0 / dx;
-    const double qx = (t0 / dx) * tE;
-    if (det > 0.0) {
-      const double dis = sqrt(f * f * b / det);
+  for (int d = 0; d < 4; d++) {
+    const bool hd = (d == 0) ? hW : (d == 1) ? hE : (d == 2) ? hS : hN;
+    const double tD = (d == 0) ? tW : (d == 1) ? tE : (d == 2) ? tS : tN;
+    const double T0D = (d == 0) ? T0W : (d == 1) ? T0E : (d == 2) ? T0S : T0N;
+    const double g = (d < 2) ? t0x : t0y;
+    const double h = (d < 2) ? dx : dy;
+    // W, S: p = g + t0/h, q = -(t0/h)*t;  E, N: p = g - t0/h, q = (t0/h)*t
+    const double p = (d == 0 || d == 2) ? g + t0 / h : g - t0 / h;
+    const double q = (d == 0 || d == 2) ? -(t0 / h) * tD : (t0 / h) * tD;
+    const double dis = (d < 2) ? dis_x : dis_y;
+    const bool ok = hd & (det > 0.0);
 #pragma unroll
-      for (int sgn = 0; sgn < 2; sgn++) {
-        const double tmp_tau = ((sgn == 0 ? dis : -dis) - qx) / px;
-        if (tmp_tau * t0 >= tE * T0E && tmp_tau > tE * 0.5) tn = dmin(tn, tmp_tau);
-      }
+    for (int sgn = 0; sgn < 2; sgn++) {
+      const double tmp_tau = ((sgn == 0 ? dis : -dis) - q) / p;
+      cand[8 + 2 * d + sgn] = tmp_tau;
+      acc[8 + 2 * d + sgn] = ok & (tmp_tau * t0 >= tD * T0D) & (tmp_tau > tD * 0.5);
     }
   }
-  if (hS) {
-    const double py = t0y + t0 / dy;
-    const double qy = -(t0 / dy) * tS;
-    if (det > 0.0) {
-      const double dis = sqrt(f * f * a / det);
+  double tn = tau_self;
 #pragma unroll
-      for (int sgn = 0; sgn < 2; sgn++) {
-        const double tmp_tau = ((sgn == 0 ? dis : -dis) - qy) / py;
-        if (tmp_tau * t0 >= tS * T0S && tmp_tau > tS * 0.5) tn = dmin(tn, tmp_tau);
-      }
-    }
-  }
-  if (hN) {
-    const double py = t0y - t0 / dy;
-    const double qy = (t0 / dy) * tN;
-    if (det > 0.0) {
-      const double dis = sqrt(f * f * a / det);
-#pragma unroll
-      for (int sgn = 0; sgn < 2; sgn++) {
-        const double tmp_tau = ((sgn == 0 ? dis : -dis) - qy) / py;
-        if (tmp_tau * t0 >= tN * T0N && tmp_tau > tN * 0.5) tn = dmin(tn, tmp_tau);
-      }
-    }
-  }
+  for (int i = 0; i < 16; i++) tn = acc[i] ? dmin(tn, cand[i]) : tn;
   return tn;
 }
 

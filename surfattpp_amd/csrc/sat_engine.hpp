@@ -117,7 +117,7 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
   using Head = typename Pol::template Head<C>;
   using Body = typename Pol::template Body<C>;
   constexpr int O = (XD * YD > 0) ? 0 : 1;
-  const SweepGeom g = Lay::geom(*cx.g);
+  const SweepGeom g = uniform(Lay::geom(*cx.g));
   const int lane = threadIdx.x;
   const int La = g.La, nx = g.nx, ny = g.ny;
   const bool lane_ok = lane < La;
@@ -217,7 +217,7 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
       }
       const int base = live ? Lay::template base<C, OOUT>(g, j, lane) : Lay::template trash<C>(g, lane);
 #pragma unroll
-      for (int k = 0; k < C; k++) tout[base + k] = newv[k];
+      for (int k = 0; k < C; k++) gst(tout + base + k, newv[k]);
 #pragma unroll
       for (int k = 0; k < C; k++) { prev_u[k] = newv[k]; prev_a[k] = cur_h.aux[k]; }
     }
@@ -246,7 +246,7 @@ __device__ __forceinline__ void sweep_ring(const typename Pol::Ctx& cx, const ty
   using Body = typename Pol::template Body<C>;
   constexpr int O = (XD * YD > 0) ? 0 : 1;
   constexpr int R = D + 1;
-  const SweepGeom g = Lay::geom(*cx.g);
+  const SweepGeom g = uniform(Lay::geom(*cx.g));
   const int lane = threadIdx.x;
   const int La = g.La, nx = g.nx, ny = g.ny;
   const bool lane_ok = lane < La;
@@ -341,7 +341,7 @@ __device__ __forceinline__ void sweep_ring(const typename Pol::Ctx& cx, const ty
         }
         const int base = live ? Lay::template base<C, OOUT>(g, j, lane) : Lay::template trash<C>(g, lane);
 #pragma unroll
-        for (int k = 0; k < C; k++) tout[base + k] = newv[k];
+        for (int k = 0; k < C; k++) gst(tout + base + k, newv[k]);
 #pragma unroll
         for (int k = 0; k < C; k++) { prev_u[k] = newv[k]; prev_a[k] = cur_h.aux[k]; }
       }

@@ -20,7 +20,6 @@
 #pragma once
 #include <type_traits>
 
-
 #include "sat_crmath.hpp"
 #include "sat_engine.hpp"
 #include "sat_policies.hpp"
@@ -92,7 +91,10 @@ struct PhaseDev {
 // Rows in flight per lane (register ring depth, sat_engine.hpp): multi-wave
 // sweeps, and the single-wave forward and adjoint sweeps (1 = the head/body
 // pipeline, fewer registers; >= 2 = the ring, no wait on the prefetches).
-constexpr int kRingDepth = 4;
+#ifndef SAT_RING_DEPTH
+#define SAT_RING_DEPTH 4
+#endif
+constexpr int kRingDepth = SAT_RING_DEPTH;
 constexpr int kFwdDepth1 = 1;
 constexpr int kAdjDepth1 = 2;
 
@@ -123,18 +125,21 @@ template <> struct PhaseT<SAT_PREC_F32> {
   using Adj = AdjF32Face;
 };
 
-// Next work-queue entry for the whole work-group (thread 0 pops it).
+// Next work-queue entry for the whole work-group (thread 0 pops it).  The
+// entry is returned as a wave-uniform value (readfirstlane): the event's
+// descriptor and plane pointers derived from it then live in scalar registers,
+// which leaves the vector registers to the sweep's row ring.
 template <int W>
 __device__ inline int pop_work(int* counter, int* s_idx) {
   if constexpr (W == 1) {
     int idx = 0;
     if (threadIdx.x == 0) idx = atomicAdd(counter, 1);
-    return __shfl(idx, 0);
+    return __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
   } else {
     __syncthreads();  // the previous entry's readers are done with *s_idx
     if (threadIdx.x == 0) *s_idx = atomicAdd(counter, 1);
     __syncthreads();
-    return *s_idx;
+    return __builtin_amdgcn_readfirstlane(*s_idx);
   }
 }
 
@@ -219,11 +224,10 @@ __global__ void init_kernel(PhaseDev pd) {
   if (PREC == SAT_PREC_F32) pd.T0d[ev.pl_off + p] = T0;
 }
 
-template <int PREC, bool ISO>
+template <int PREC, bool ISO, class Pol>
 __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const EventX& ev) {
   using PT = PhaseT<PREC>;
   using V = typename PT::V;
-  using Pol = std::conditional_t<ISO, typename PT::FwdIso, typename PT::Fwd>;
   typename Pol::Ctx cx;
   const size_t po = (size_t)ev.iper * g.wplane;
   cx.g = &g;
@@ -247,6 +251,10 @@ __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const
   if constexpr (PREC == SAT_PREC_F64) {
     cx.dx = g.dx;
     cx.dy = g.dy;
+    if constexpr (std::is_same_v<Pol, FwdF64R>) {
+      cx.idx = 1.0 / g.dx;
+      cx.idy = 1.0 / g.dy;
+    }
   } else {
     if constexpr (ISO) cx.rowtab = g.rowtab;
     cx.idx = (float)(1.0 / g.dx);
@@ -255,12 +263,25 @@ __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const
   return cx;
 }
 
+// Waves per SIMD the single-wave FP32 forward sweeps are compiled for: three
+// (<= 168 VGPRs) where the row pipeline fits without spills in the step loop --
+// with 3 x 1024 resident waves a 3,000-pair batch is resident at once and the
+// VALU-bound sweeps of three pairs interleave on each SIMD -- otherwise the
+// compiler's own choice.
+template <int PREC, int C, bool ISO, int W>
+constexpr int fwd_waves_per_eu() {
+  return (PREC == SAT_PREC_F32 && W == 1 && (ISO ? C <= 4 : C <= 2)) ? 3 : 1;
+}
+
 // Persistent forward sweeps, one work-group of W waves per event.
 template <int PREC, int C, bool ISO, int W>
-__global__ __launch_bounds__(64 * W) void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fwd_waves_per_eu<PREC, C, ISO, W>())))
+void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
   using PT = PhaseT<PREC>;
   using U = typename PT::U;
-  using Pol = std::conditional_t<ISO, typename PT::FwdIso, typename PT::Fwd>;
+  // multi-wave FP64 launches are the re-solves of FP32 batches (FwdF64R)
+  using Pol = std::conditional_t<(PREC == SAT_PREC_F64 && W > 1), FwdF64R,
+                                 std::conditional_t<ISO, typename PT::FwdIso, typename PT::Fwd>>;
   const int lane = threadIdx.x;
   const int nthr = kWave * W;
   __shared__ Publish<Pol> xch[(W > 1) ? 2 * kWave * W : 1];
@@ -287,7 +308,7 @@ __global__ __launch_bounds__(64 * W) void fwd_sweep_kernel(PhaseDev pd, int firs
       Q0[q] = !real ? U(0) : kSigma ? (fixed ? U(0) : U(9)) : (fixed ? U(1) : U(10));
     }
     __syncthreads();
-    const auto cx = phase_fwd_ctx<PREC, ISO>(pd, g, ev);
+    const auto cx = uniform(phase_fwd_ctx<PREC, ISO, Pol>(pd, g, ev));
     IterOut io;
     const U* u = iterate<Pol, WaveLayout, C, W, (W > 1) ? kRingDepth : kFwdDepth1>(cx, Q0, Q1, Q2, pd.fwd_maxiter, kFwdTol,
                                                                           io, xch, red);
@@ -476,7 +497,7 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
       cx.dy = g.dy;
     }
     IterOut io;
-    const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? kRingDepth : kAdjDepth1>(cx, Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,
+    const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? kRingDepth : kAdjDepth1>(uniform(cx), Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,
                                                                            io, xch, red);
     if (lane == 0) {
       pd.iters_a[e] = io.iters;

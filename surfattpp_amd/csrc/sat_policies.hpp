@@ -50,19 +50,19 @@ struct FwdF64 {
   template <int C, int O>
   __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
 #pragma unroll
-    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[O][base + k]; }
+    for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.t0[O], base + k); }
   }
   template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      B.t0x[k] = cx.t0x[O][base + k];
-      B.t0y[k] = cx.t0y[O][base + k];
-      B.a[k] = cx.a[O][base + k];
-      B.b[k] = cx.b[O][base + k];
-      B.c[k] = cx.c[O][base + k];
-      B.f[k] = cx.f[O][base + k];
-      B.st[k] = tst ? tst[base + k] : 0.0;
+      B.t0x[k] = gld(cx.t0x[O], base + k);
+      B.t0y[k] = gld(cx.t0y[O], base + k);
+      B.a[k] = gld(cx.a[O], base + k);
+      B.b[k] = gld(cx.b[O], base + k);
+      B.c[k] = gld(cx.c[O], base + k);
+      B.f[k] = gld(cx.f[O], base + k);
+      B.st[k] = tst ? gld(tst, base + k) : 0.0;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
@@ -78,6 +78,122 @@ struct FwdF64 {
   template <int C>
   __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, double v) {
     return fabs(v - r.st[k]) * h.aux[k];  // :295
+  }
+};
+
+// ============================================================================
+// FP64 re-solves of FP32 batch pairs.  A re-solve must reproduce the reference's
+// stop decision, which needs FP64 iterates, not the reference's exact bits: its
+// iterates differ from the reference's by rounding (~1e-16 relative), which moves
+// a convergence norm of ~1e-4 by ~1e-12 relative.  One work-group sweeps one pair
+// (a single wave per SIMD), so each step's latency is the chain of dependent
+// FP64 operations of one node; the IEEE divide and square-root sequences hold
+// VCC and run one after another.  This update shares one reciprocal per
+// denominator and uses the hardware estimates refined in FMA, so the
+// independent candidates interleave.
+// ============================================================================
+
+// 1/x: hardware estimate and two Newton-Raphson steps (about 1 ulp; x normal and
+// finite -- otherwise the result is not finite and the candidate is rejected).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+// sqrt(x) for x > 0: reciprocal square-root estimate, one Goldschmidt step and
+// two residual corrections (about 1 ulp).
+__device__ __forceinline__ double sqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+
+// The candidates and acceptance tests of fwd_update_f64 (src/eikonal_solver.cpp:155-283)
+// with h = t0/dx and k = t0/dy formed as t0 * (1/dx), one reciprocal per
+// quadratic (1/(2A)), per one-point denominator and of det, and FMA contraction.
+__device__ inline double fwd_update_f64r(double tau_self, double t0, double t0x, double t0y, double a, double b,
+                                         double c, double f, double idx, double idy, bool hW, double tW,
+                                         double T0W, bool hE, double tE, double T0E, bool hS, double tS,
+                                         double T0S, bool hN, double tN, double T0N) {
+#pragma clang fp contract(fast)
+  const double h = t0 * idx, k = t0 * idy;
+  const double det = a * b - c * c;
+  const double f2 = f * f;
+  double cand[16];
+  bool acc[16];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const double sx = (q < 2) ? -1.0 : 1.0;
+    const double sy = (q & 1) ? 1.0 : -1.0;
+    const bool ok0 = ((q < 2) ? hW : hE) & ((q & 1) ? hN : hS);
+    const double tX = (q < 2) ? tW : tE;
+    const double tY = (q & 1) ? tN : tS;
+    const double px = t0x - sx * h;
+    const double qx = sx * h * tX;
+    const double py = t0y - sy * k;
+    const double qy = sy * k * tY;
+    const double eq_a = a * px * px + b * py * py - 2.0 * c * px * py;
+    const double eq_b = 2.0 * (a * px * qx + b * py * qy - c * (px * qy + py * qx));
+    const double eq_c = a * qx * qx + b * qy * qy - 2.0 * c * qx * qy - f2;
+    const double Delta = eq_b * eq_b - 4.0 * eq_a * eq_c;
+    const bool ok = ok0 & !(Delta < 0.0 || fabs(eq_a) < 1e-30);
+    const double sq = (Delta > 0.0) ? sqrt_nr(Delta) : 0.0;
+    const double r2a = rcp_nr(2.0 * eq_a);
+#pragma unroll
+    for (int sgn = 0; sgn < 2; sgn++) {
+      const double tmp_tau = (-eq_b + (sgn == 0 ? sq : -sq)) * r2a;
+      const double Tx = px * tmp_tau + qx;
+      const double Ty = py * tmp_tau + qy;
+      cand[2 * q + sgn] = tmp_tau;
+      acc[2 * q + sgn] = ok & (-sx * (a * Tx - c * Ty) >= 0.0) & (-sy * (b * Ty - c * Tx) >= 0.0);
+    }
+  }
+  const double idet = rcp_nr(det);
+  const double dis_x = sqrt_nr(f2 * b * idet);
+  const double dis_y = sqrt_nr(f2 * a * idet);
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const bool hd = (d == 0) ? hW : (d == 1) ? hE : (d == 2) ? hS : hN;
+    const double tD = (d == 0) ? tW : (d == 1) ? tE : (d == 2) ? tS : tN;
+    const double T0D = (d == 0) ? T0W : (d == 1) ? T0E : (d == 2) ? T0S : T0N;
+    const double g = (d < 2) ? t0x : t0y;
+    const double hh = (d < 2) ? h : k;
+    const double p = (d == 0 || d == 2) ? g + hh : g - hh;
+    const double qd = (d == 0 || d == 2) ? -hh * tD : hh * tD;
+    const double dis = (d < 2) ? dis_x : dis_y;
+    const double rp = rcp_nr(p);
+    const bool ok = hd & (det > 0.0);
+#pragma unroll
+    for (int sgn = 0; sgn < 2; sgn++) {
+      const double tmp_tau = ((sgn == 0 ? dis : -dis) - qd) * rp;
+      cand[8 + 2 * d + sgn] = tmp_tau;
+      acc[8 + 2 * d + sgn] = ok & (tmp_tau * t0 >= tD * T0D) & (tmp_tau > tD * 0.5);
+    }
+  }
+  double tn = tau_self;
+#pragma unroll
+  for (int i = 0; i < 16; i++) tn = acc[i] ? dmin(tn, cand[i]) : tn;
+  return tn;
+}
+
+struct FwdF64R : FwdF64 {
+  struct Ctx : FwdF64::Ctx {
+    double idx, idy;  // 1/dx, 1/dy
+  };
+  template <int C>
+  __device__ static double update(const Ctx& cx, const Head<C>& h, const Body<C>& r, int k, int, double uW,
+                                  double aW, bool hW, double uE, double aE, bool hE, double uS, double aS, bool hS,
+                                  double uN, double aN, bool hN) {
+    return fwd_update_f64r(h.u[k], h.aux[k], r.t0x[k], r.t0y[k], r.a[k], r.b[k], r.c[k], r.f[k], cx.idx, cx.idy,
+                           hW, uW, aW, hE, uE, aE, hS, uS, aS, hN, uN, aN);
   }
 };
 
@@ -97,19 +213,19 @@ struct AdjF64 {
   template <int C, int O>
   __device__ static void load_head(const Ctx&, Head<C>& H, int base, int, const double* tin) {
 #pragma unroll
-    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = 0.0; }
+    for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = 0.0; }
   }
   template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      B.a1p[k] = cx.a1p[base + k];
-      B.a2m[k] = cx.a2m[base + k];
-      B.b1p[k] = cx.b1p[base + k];
-      B.b2m[k] = cx.b2m[base + k];
-      B.dl[k] = cx.dl[base + k];
-      B.d[k] = cx.d[base + k];
-      B.st[k] = tst ? tst[base + k] : 0.0;
+      B.a1p[k] = gld(cx.a1p, base + k);
+      B.a2m[k] = gld(cx.a2m, base + k);
+      B.b1p[k] = gld(cx.b1p, base + k);
+      B.b2m[k] = gld(cx.b2m, base + k);
+      B.dl[k] = gld(cx.dl, base + k);
+      B.d[k] = gld(cx.d, base + k);
+      B.st[k] = tst ? gld(tst, base + k) : 0.0;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
@@ -149,14 +265,14 @@ struct AdjF64Face {
   template <int C, int O>
   __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
 #pragma unroll
-    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.fxy[O][base + k]; }
+    for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.fxy[O], base + k); }
   }
   template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      B.dl[k] = cx.dl[O][base + k];
-      B.st[k] = tst ? tst[base + k] : 0.0;
+      B.dl[k] = gld(cx.dl[O], base + k);
+      B.st[k] = tst ? gld(tst, base + k) : 0.0;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
@@ -302,21 +418,21 @@ struct FwdF32 {
   template <int C, int O>
   __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
 #pragma unroll
-    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[O][base + k]; }
+    for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.t0[O], base + k); }
   }
   template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      B.t0x[k] = cx.t0x[O][base + k];
-      B.t0y[k] = cx.t0y[O][base + k];
-      B.a[k] = cx.a[O][base + k];
-      B.b[k] = cx.b[O][base + k];
-      B.c[k] = cx.c[O][base + k];
-      B.f[k] = cx.f[O][base + k];
-      B.dsx[k] = cx.dsx[O][base + k];
-      B.dsy[k] = cx.dsy[O][base + k];
-      B.st[k] = tst ? tst[base + k] : 0.0;
+      B.t0x[k] = gld(cx.t0x[O], base + k);
+      B.t0y[k] = gld(cx.t0y[O], base + k);
+      B.a[k] = gld(cx.a[O], base + k);
+      B.b[k] = gld(cx.b[O], base + k);
+      B.c[k] = gld(cx.c[O], base + k);
+      B.f[k] = gld(cx.f[O], base + k);
+      B.dsx[k] = gld(cx.dsx[O], base + k);
+      B.dsy[k] = gld(cx.dsy[O], base + k);
+      B.st[k] = tst ? gld(tst, base + k) : 0.0;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
@@ -355,18 +471,18 @@ struct FwdF32Iso {
   template <int C, int O>
   __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
 #pragma unroll
-    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.t0[O][base + k]; }
+    for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.t0[O], base + k); }
   }
   template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int j, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      B.t0x[k] = cx.t0x[O][base + k];
-      B.t0y[k] = cx.t0y[O][base + k];
-      B.f[k] = cx.f[O][base + k];
-      B.st[k] = tst ? tst[base + k] : 0.0;
+      B.t0x[k] = gld(cx.t0x[O], base + k);
+      B.t0y[k] = gld(cx.t0y[O], base + k);
+      B.f[k] = gld(cx.f[O], base + k);
+      B.st[k] = tst ? gld(tst, base + k) : 0.0;
     }
-    B.row = cx.rowtab[j];
+    B.row = gld(cx.rowtab, j);
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
     return j >= cx.fb.y0 && j <= cx.fb.y1 && ix >= cx.fb.x0 && ix <= cx.fb.x1;
@@ -402,18 +518,18 @@ struct AdjF32 {
   template <int C, int O>
   __device__ static void load_head(const Ctx&, Head<C>& H, int base, int, const float* tin) {
 #pragma unroll
-    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = 0.0f; }
+    for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = 0.0f; }
   }
   template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const float* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      B.cW[k] = cx.cW[base + k];
-      B.cE[k] = cx.cE[base + k];
-      B.cS[k] = cx.cS[base + k];
-      B.cN[k] = cx.cN[base + k];
-      B.cD[k] = cx.cD[base + k];
-      B.st[k] = tst ? tst[base + k] : 0.0f;
+      B.cW[k] = gld(cx.cW, base + k);
+      B.cE[k] = gld(cx.cE, base + k);
+      B.cS[k] = gld(cx.cS, base + k);
+      B.cN[k] = gld(cx.cN, base + k);
+      B.cD[k] = gld(cx.cD, base + k);
+      B.st[k] = tst ? gld(tst, base + k) : 0.0f;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
@@ -450,14 +566,14 @@ struct AdjF32Face {
   template <int C, int O>
   __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const float* tin) {
 #pragma unroll
-    for (int k = 0; k < C; k++) { H.u[k] = tin[base + k]; H.aux[k] = cx.fxy[O][base + k]; }
+    for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.fxy[O], base + k); }
   }
   template <int C, int O>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const float* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
-      B.dl[k] = cx.dl[O][base + k];
-      B.st[k] = tst ? tst[base + k] : 0.0f;
+      B.dl[k] = gld(cx.dl[O], base + k);
+      B.st[k] = tst ? gld(tst, base + k) : 0.0f;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {

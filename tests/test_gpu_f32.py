@@ -99,8 +99,10 @@ def test_batch_f32_matches_f64_gpu_full_workload():
 def test_batch_f32_resolve_pool_and_overflow():
     # A band of 0.99 puts most pairs within the band: 64 of them are re-solved
     # by the FP64 batched kernels (the pool), the rest by the FP64 per-pair
-    # kernel.  A re-solved pair is an FP64 result: its syn equal to the FP64
-    # batch's; the others within the FP32 bar.
+    # kernel.  A re-solved pair is an FP64 result: its iteration counts equal
+    # the FP64 batch's and its syn agrees to 1e-12 (the pool's update forms
+    # reciprocals in FMA instead of IEEE divisions: rounding-level differences);
+    # the others within the FP32 bar.
     band = 0.99
     b = synth.subset_events(synth.config_grf(nperiod=3, nsta=40), list(range(0, 120)))
     with PP.EikonalBatch(b) as b64, PP.EikonalBatch(b, precision=PREC_F32) as b32:
@@ -117,7 +119,8 @@ def test_batch_f32_resolve_pool_and_overflow():
     np.testing.assert_array_equal(r32["iters_adj"][amb], r64["iters_adj"][amb])
     off = np.concatenate([[0], np.cumsum(b["ev_nrec"])])
     for e in np.nonzero(amb)[0]:
-        np.testing.assert_array_equal(r32["syn"][off[e]:off[e + 1]], r64["syn"][off[e]:off[e + 1]])
+        ref = r64["syn"][off[e]:off[e + 1]]
+        assert RC.rel_inf(r32["syn"][off[e]:off[e + 1]], ref) <= 1e-12, e
     assert RC.rel_inf(r32["syn"], r64["syn"]) <= TOL_T
     for p in range(r64["adj_s"].shape[0]):
         assert RC.rel_inf(r32["adj_s"][p], r64["adj_s"][p]) <= TOL_K
