@@ -155,6 +155,9 @@ __device__ __forceinline__ T uniform(const T& v) {
   return out;
 }
 
+// s_waitcnt vmcnt(0): wait for every vector memory operation of the wave.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Cross-lane reads of the swept value and its auxiliary (ds_bpermute).
 __device__ __forceinline__ float shfl_any(float v, int src) { return __shfl(v, src); }
 __device__ __forceinline__ double shfl_any(double v, int src) { return __shfl(v, src); }

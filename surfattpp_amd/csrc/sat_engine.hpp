@@ -39,6 +39,8 @@
 // lane reads slot t&1 at step t; it is written again at step t+2, after the
 // barrier of step t+1).  Fewer columns per lane shorten a grid's latency.
 #pragma once
+#include <type_traits>
+
 #include "sat_device.hpp"
 
 namespace sat {
@@ -85,7 +87,8 @@ struct WaveLayout {
 //   template <int C> struct Head { V u[C]; A aux[C]; };
 //   template <int C> struct Body { ...; V st[C]; };      (st: iterate at the round start)
 //   template <int C, int O> static void load_head(const Ctx&, Head<C>&, int base, int j, const V* tin)
-//   template <int C, int O> static void load_body(const Ctx&, Body<C>&, int base, int j, const V* tst)
+//   template <int C, int O, bool NORM> static void load_body(const Ctx&, Body<C>&, int base, int j, const V* tst)
+//                                    (st is read from tst only in the sweep that forms the norm)
 //   static bool fixed(const Ctx&, int ix, int j)
 //   template <int C> static V update(const Ctx&, const Head<C>&, const Body<C>&, int k, int j,
 //                                    V uW, A aW, bool hW, V uE, A aE, bool hE,
@@ -101,17 +104,30 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Per-lane values published to the neighbouring lanes each step (W > 1).
-template <class Pol> struct Publish {
-  typename Pol::V pu, nu;  // previous-row edge value, next-row edge value
-  typename Pol::A pa, na;
+// Per-lane values published to the neighbouring lanes each step (W > 1): the
+// previous-row edge value (pu, pa) and the next-row edge value (nu, na), one
+// array per value and step parity.  Each value goes out in its own LDS store:
+// a combined 128-bit store would need a computed value and a just-loaded one in
+// adjacent registers, and the copy that puts them there waits for the load.
+template <class Pol, int W> struct Exchange {
+  typename Pol::V pu[2][kWave * W], nu[2][kWave * W];
+  typename Pol::A pa[2][kWave * W], na[2][kWave * W];
 };
+// The exchange a kernel declares in LDS (none for single-wave sweeps, which
+// cross lanes by ds_bpermute) and the pointer the sweeps take.
+template <class Pol, int W>
+using ExchangeLds = std::conditional_t<(W > 1), Exchange<Pol, W>, char>;
+template <class Pol, int W>
+__device__ __forceinline__ Exchange<Pol, W>* exchange_ptr(ExchangeLds<Pol, W>& x) {
+  if constexpr (W > 1) return &x;
+  else return nullptr;
+}
 
 template <class Pol, class Lay, int C, int XD, int YD, bool NORM, int OOUT, int W = 1>
 __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typename Pol::V* __restrict__ tin,
                                       typename Pol::V* __restrict__ tout,
                                       const typename Pol::V* __restrict__ tst, double& lmax, double& lsum,
-                                      Publish<Pol>* __restrict__ xch = nullptr) {
+                                      Exchange<Pol, W>* __restrict__ xch = nullptr) {
   using V = typename Pol::V;
   using A = typename Pol::A;
   using Head = typename Pol::template Head<C>;
@@ -134,7 +150,7 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
   };
   auto load_body = [&](Body& B, int r) {
     const int j = rowj(imin(imax(r, 0), ny - 1));
-    Pol::template load_body<C, O>(cx, B, Lay::template base<C, O>(g, j, lane_c), j, NORM ? tst : nullptr);
+    Pol::template load_body<C, O, NORM>(cx, B, Lay::template base<C, O>(g, j, lane_c), j, tst);
   };
 
   Head cur_h, nxt_h, nn_h;
@@ -161,23 +177,23 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
         od_a = shfl_any((XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1], lane + XD);
       }
     } else {
-      Publish<Pol>* buf = xch + (t & 1) * (kWave * W);
-      Publish<Pol> me;
-      me.pu = (XD > 0) ? prev_u[C - 1] : prev_u[0];
-      me.nu = (XD > 0) ? nxt_h.u[0] : nxt_h.u[C - 1];
-      me.pa = (XD > 0) ? prev_a[C - 1] : prev_a[0];
-      me.na = (XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1];
-      buf[lane] = me;
+      const int par = t & 1;
+      xch->pu[par][lane] = (XD > 0) ? prev_u[C - 1] : prev_u[0];
+      xch->nu[par][lane] = (XD > 0) ? nxt_h.u[0] : nxt_h.u[C - 1];
+      if constexpr (Pol::kAux) {
+        xch->pa[par][lane] = (XD > 0) ? prev_a[C - 1] : prev_a[0];
+        xch->na[par][lane] = (XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1];
+      }
       lds_barrier();
       // lanes at the ends read a clamped slot; those values are never used
       // (the node is on the grid edge or outside the active lanes)
       const int src_w = imin(imax(lane - XD, 0), kWave * W - 1);
       const int src_o = imin(imax(lane + XD, 0), kWave * W - 1);
-      nw_u = buf[src_w].pu;
-      od_u = buf[src_o].nu;
+      nw_u = xch->pu[par][src_w];
+      od_u = xch->nu[par][src_o];
       if constexpr (Pol::kAux) {
-        nw_a = buf[src_w].pa;
-        od_a = buf[src_o].na;
+        nw_a = xch->pa[par][src_w];
+        od_a = xch->na[par][src_o];
       }
     }
     {
@@ -238,7 +254,7 @@ template <class Pol, class Lay, int C, int XD, int YD, bool NORM, int OOUT, int 
 __device__ __forceinline__ void sweep_ring(const typename Pol::Ctx& cx, const typename Pol::V* __restrict__ tin,
                                            typename Pol::V* __restrict__ tout,
                                            const typename Pol::V* __restrict__ tst, double& lmax, double& lsum,
-                                           Publish<Pol>* __restrict__ xch) {
+                                           Exchange<Pol, W>* __restrict__ xch) {
   static_assert(D >= 2, "the ring needs two rows in flight");
   using V = typename Pol::V;
   using A = typename Pol::A;
@@ -259,7 +275,7 @@ __device__ __forceinline__ void sweep_ring(const typename Pol::Ctx& cx, const ty
     const int j = rowj(imin(imax(r, 0), ny - 1));
     const int base = Lay::template base<C, O>(g, j, lane_c);
     Pol::template load_head<C, O>(cx, H, base, j, tin);
-    Pol::template load_body<C, O>(cx, B, base, j, NORM ? tst : nullptr);
+    Pol::template load_body<C, O, NORM>(cx, B, base, j, tst);
   };
 
   Head H[R];
@@ -270,6 +286,12 @@ __device__ __forceinline__ void sweep_ring(const typename Pol::Ctx& cx, const ty
   for (int k = 0; k < C; k++) { prev_u[k] = V(0); prev_a[k] = A{}; }
 #pragma unroll
   for (int s = 0; s < D; s++) load(H[s], B[s], s - lag);
+  // The ring's first rows are in registers before the loop starts.  Without this
+  // wait the compiler merges, at the loop head, the loads still in flight on
+  // entry with those of the back edge and waits for nearly every load issued
+  // so far at each loop head, which takes the prefetch depth back to one row;
+  // with it the waits in the loop keep D - 1 rows in flight.
+  wait_vmem();
 
   for (int t0 = 0; t0 < nsteps; t0 += R) {
 #pragma unroll
@@ -290,21 +312,21 @@ __device__ __forceinline__ void sweep_ring(const typename Pol::Ctx& cx, const ty
           od_a = shfl_any((XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1], lane + XD);
         }
       } else {
-        Publish<Pol>* buf = xch + (t & 1) * (kWave * W);
-        Publish<Pol> me;
-        me.pu = (XD > 0) ? prev_u[C - 1] : prev_u[0];
-        me.nu = (XD > 0) ? nxt_h.u[0] : nxt_h.u[C - 1];
-        me.pa = (XD > 0) ? prev_a[C - 1] : prev_a[0];
-        me.na = (XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1];
-        buf[lane] = me;
+        const int par = t & 1;
+        xch->pu[par][lane] = (XD > 0) ? prev_u[C - 1] : prev_u[0];
+        xch->nu[par][lane] = (XD > 0) ? nxt_h.u[0] : nxt_h.u[C - 1];
+        if constexpr (Pol::kAux) {
+          xch->pa[par][lane] = (XD > 0) ? prev_a[C - 1] : prev_a[0];
+          xch->na[par][lane] = (XD > 0) ? nxt_h.aux[0] : nxt_h.aux[C - 1];
+        }
         lds_barrier();
         const int src_w = imin(imax(lane - XD, 0), kWave * W - 1);
         const int src_o = imin(imax(lane + XD, 0), kWave * W - 1);
-        nw_u = buf[src_w].pu;
-        od_u = buf[src_o].nu;
+        nw_u = xch->pu[par][src_w];
+        od_u = xch->nu[par][src_o];
         if constexpr (Pol::kAux) {
-          nw_a = buf[src_w].pa;
-          od_a = buf[src_o].na;
+          nw_a = xch->pa[par][src_w];
+          od_a = xch->na[par][src_o];
         }
       }
       {
@@ -354,7 +376,7 @@ template <class Pol, class Lay, int C, int XD, int YD, bool NORM, int OOUT, int 
 __device__ __forceinline__ void sweep_any(const typename Pol::Ctx& cx, const typename Pol::V* __restrict__ tin,
                                           typename Pol::V* __restrict__ tout,
                                           const typename Pol::V* __restrict__ tst, double& lmax, double& lsum,
-                                          Publish<Pol>* __restrict__ xch) {
+                                          Exchange<Pol, W>* __restrict__ xch) {
   if constexpr (D == 1) sweep<Pol, Lay, C, XD, YD, NORM, OOUT, W>(cx, tin, tout, tst, lmax, lsum, xch);
   else sweep_ring<Pol, Lay, C, XD, YD, NORM, OOUT, W, D>(cx, tin, tout, tst, lmax, lsum, xch);
 }
@@ -405,7 +427,7 @@ __device__ inline double block_sum(double v, double* red) {
 template <class Pol, class Lay, int C, int W = 1, int D = 1>
 __device__ typename Pol::V* iterate(const typename Pol::Ctx& cx, typename Pol::V* b0, typename Pol::V* b1,
                                     typename Pol::V* b2, int maxiter, double tol, IterOut& io,
-                                    Publish<Pol>* xch = nullptr, double* red = nullptr) {
+                                    Exchange<Pol, W>* xch = nullptr, double* red = nullptr) {
   using V = typename Pol::V;
   V* B0 = b0;
   V* B1 = b1;

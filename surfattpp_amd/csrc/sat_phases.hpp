@@ -96,7 +96,7 @@ struct PhaseDev {
 #endif
 constexpr int kRingDepth = SAT_RING_DEPTH;
 constexpr int kFwdDepth1 = 1;
-constexpr int kAdjDepth1 = 2;
+constexpr int kAdjDepth1 = 3;
 
 // The multi-wave sweeps carry the latency-critical FP64 re-solves, which share
 // the CUs with a throughput kernel on the other stream: their waves take the
@@ -284,7 +284,7 @@ void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
                                  std::conditional_t<ISO, typename PT::FwdIso, typename PT::Fwd>>;
   const int lane = threadIdx.x;
   const int nthr = kWave * W;
-  __shared__ Publish<Pol> xch[(W > 1) ? 2 * kWave * W : 1];
+  __shared__ ExchangeLds<Pol, W> xch;
   __shared__ double red[W];
   __shared__ int s_idx;
   latency_critical_priority<W>();
@@ -311,7 +311,7 @@ void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
     const auto cx = uniform(phase_fwd_ctx<PREC, ISO, Pol>(pd, g, ev));
     IterOut io;
     const U* u = iterate<Pol, WaveLayout, C, W, (W > 1) ? kRingDepth : kFwdDepth1>(cx, Q0, Q1, Q2, pd.fwd_maxiter, kFwdTol,
-                                                                          io, xch, red);
+                                                                          io, exchange_ptr<Pol, W>(xch), red);
     if (lane == 0) {
       pd.conv[2 * (size_t)e] = io.linf;
       pd.conv[2 * (size_t)e + 1] = io.linf_prev;
@@ -460,7 +460,7 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
   using Pol = typename PT::Adj;
   const int lane = threadIdx.x;
   const int nthr = kWave * W;
-  __shared__ Publish<Pol> xch[(W > 1) ? 2 * kWave * W : 1];
+  __shared__ ExchangeLds<Pol, W> xch;
   __shared__ double red[W];
   __shared__ int s_idx;
   latency_critical_priority<W>();
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
     }
     IterOut io;
     const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? kRingDepth : kAdjDepth1>(uniform(cx), Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,
-                                                                           io, xch, red);
+                                                                           io, exchange_ptr<Pol, W>(xch), red);
     if (lane == 0) {
       pd.iters_a[e] = io.iters;
       if (!io.converged) atomicOr(pd.flags + e, FLAG_ADJ_MAXITER);

@@ -52,7 +52,7 @@ struct FwdF64 {
 #pragma unroll
     for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.t0[O], base + k); }
   }
-  template <int C, int O>
+  template <int C, int O, bool NORM>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
@@ -62,7 +62,7 @@ struct FwdF64 {
       B.b[k] = gld(cx.b[O], base + k);
       B.c[k] = gld(cx.c[O], base + k);
       B.f[k] = gld(cx.f[O], base + k);
-      B.st[k] = tst ? gld(tst, base + k) : 0.0;
+      B.st[k] = NORM ? gld(tst, base + k) : 0.0;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
@@ -215,7 +215,7 @@ struct AdjF64 {
 #pragma unroll
     for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = 0.0; }
   }
-  template <int C, int O>
+  template <int C, int O, bool NORM>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
@@ -225,7 +225,7 @@ struct AdjF64 {
       B.b2m[k] = gld(cx.b2m, base + k);
       B.dl[k] = gld(cx.dl, base + k);
       B.d[k] = gld(cx.d, base + k);
-      B.st[k] = tst ? gld(tst, base + k) : 0.0;
+      B.st[k] = NORM ? gld(tst, base + k) : 0.0;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
@@ -267,12 +267,12 @@ struct AdjF64Face {
 #pragma unroll
     for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.fxy[O], base + k); }
   }
-  template <int C, int O>
+  template <int C, int O, bool NORM>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
       B.dl[k] = gld(cx.dl[O], base + k);
-      B.st[k] = tst ? gld(tst, base + k) : 0.0;
+      B.st[k] = NORM ? gld(tst, base + k) : 0.0;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
@@ -420,7 +420,7 @@ struct FwdF32 {
 #pragma unroll
     for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.t0[O], base + k); }
   }
-  template <int C, int O>
+  template <int C, int O, bool NORM>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
@@ -432,7 +432,7 @@ struct FwdF32 {
       B.f[k] = gld(cx.f[O], base + k);
       B.dsx[k] = gld(cx.dsx[O], base + k);
       B.dsy[k] = gld(cx.dsy[O], base + k);
-      B.st[k] = tst ? gld(tst, base + k) : 0.0;
+      B.st[k] = NORM ? gld(tst, base + k) : 0.0;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
@@ -473,14 +473,14 @@ struct FwdF32Iso {
 #pragma unroll
     for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.t0[O], base + k); }
   }
-  template <int C, int O>
+  template <int C, int O, bool NORM>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int j, const double* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
       B.t0x[k] = gld(cx.t0x[O], base + k);
       B.t0y[k] = gld(cx.t0y[O], base + k);
       B.f[k] = gld(cx.f[O], base + k);
-      B.st[k] = tst ? gld(tst, base + k) : 0.0;
+      B.st[k] = NORM ? gld(tst, base + k) : 0.0;
     }
     B.row = gld(cx.rowtab, j);
   }
@@ -520,7 +520,7 @@ struct AdjF32 {
 #pragma unroll
     for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = 0.0f; }
   }
-  template <int C, int O>
+  template <int C, int O, bool NORM>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const float* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
@@ -529,7 +529,7 @@ struct AdjF32 {
       B.cS[k] = gld(cx.cS, base + k);
       B.cN[k] = gld(cx.cN, base + k);
       B.cD[k] = gld(cx.cD, base + k);
-      B.st[k] = tst ? gld(tst, base + k) : 0.0f;
+      B.st[k] = NORM ? gld(tst, base + k) : 0.0f;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
@@ -568,12 +568,12 @@ struct AdjF32Face {
 #pragma unroll
     for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.fxy[O], base + k); }
   }
-  template <int C, int O>
+  template <int C, int O, bool NORM>
   __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const float* tst) {
 #pragma unroll
     for (int k = 0; k < C; k++) {
       B.dl[k] = gld(cx.dl[O], base + k);
-      B.st[k] = tst ? gld(tst, base + k) : 0.0f;
+      B.st[k] = NORM ? gld(tst, base + k) : 0.0f;
     }
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
