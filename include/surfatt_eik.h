@@ -114,6 +114,12 @@ typedef struct sat_batch_desc {
   const double* weight;         /* receiver weights */
   int precision;                /* SAT_PREC_* */
   int calc_adj;                 /* 0: forward + syn only (preproc.cpp:113 is_calc_adj) */
+  size_t workspace_cap;         /* bytes of per-event device workspace held at once; 0: all
+                                   events when they fit in 3/4 of the free device memory,
+                                   else half of it.  Above the cap a run processes consecutive
+                                   chunks of the event order (forward -> adjoint -> accumulate
+                                   per chunk): a pair's forward field is never kept past its
+                                   adjoint.  Results are the same bits at any cap. */
 } sat_batch_desc;
 
 typedef struct sat_batch sat_batch; /* opaque: device inputs, workspace, results */

@@ -8,6 +8,9 @@
 //   eikonal::mask_uniform_grid     include/eikonal_solver.h:85-89
 //   surfatt::Batch                 the batched form of preproc::forward_for_event
 //                                  (include/preproc.h:14, src/preproc.cpp:71-148)
+//   preproc::forward_for_event     the same call surface over light views of the
+//                                  reference's SrcRec (include/src_rec.h:15-22) and
+//                                  SurfGrid (include/surf_grid.h:63-65) members
 //
 // Fields are surfatt::Field: an owning column-major nx-by-ny matrix of double
 // (element (ix, iy) at ix + iy*nx), the storage of Eigen::MatrixX<real_t>, with
@@ -17,6 +20,7 @@
 // -lsurfatt_eik.
 #pragma once
 #include <cstddef>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -143,3 +147,104 @@ inline void mask_uniform_grid(const std::vector<double>& xx, const std::vector<d
 }
 
 }  // namespace eikonal
+
+namespace preproc {
+
+// The members of the reference's event_info (include/src_rec.h:15-22) that
+// forward_for_event reads and writes.
+struct event_info {
+  double evla = 0.0, evlo = 0.0, period = 0.0;
+  int iper = 0;
+  std::vector<int> rec_indices;  // into SrcRecView::stlo/stla/tt/weight
+  std::vector<double> syn_data;  // written: synthetic travel time per receiver
+};
+
+// The members of SrcRec that forward_for_event uses: per-record station
+// coordinates, observed times and weights, and the events in the order of the
+// reference's events_local map (keys "{evt}_{iper}", src/src_rec.cpp:149-171).
+struct SrcRecView {
+  std::vector<double> stla, stlo, tt, weight;
+  std::map<std::string, event_info> events_local;
+};
+
+// The members of SurfGrid that forward_for_event uses: grid axes (degrees) and
+// the per-period planes in the reference's [ix][iy][iper] layout
+// (surf_idx = ((ix*ny) + iy)*nperiod + iper, include/surf_grid.h:63-65), and the
+// per-period kernel accumulators adj_s_local (nx-by-ny each, added to).
+struct SurfGridView {
+  std::vector<double> xgrids, ygrids;
+  int nperiod = 0;
+  const double *svel = nullptr, *m11 = nullptr, *m12 = nullptr, *m22 = nullptr;
+  std::vector<surfatt::Field> adj_s_local;
+};
+
+// src/preproc.cpp:71-148 for every event of sr in one batched GPU run: the
+// forward field, syn_data, and with is_calc_adj the adjoint, mask, kernel
+// accumulation adj_s_local[iper] += Ta .* svel^-3 (in event order) and chi,
+// which is returned (0 without is_calc_adj, as in the reference).  Precision:
+// surfatt::precision().  The period planes are transposed to the library's
+// period-major layout once per call.
+inline double forward_for_event(SrcRecView& sr, SurfGridView& sg, bool is_calc_adj) {
+  const int nx = (int)sg.xgrids.size(), ny = (int)sg.ygrids.size(), np = sg.nperiod;
+  if (nx < 3 || ny < 3) throw std::invalid_argument("Grid must be at least 3x3");
+  if (np < 1 || !sg.svel || !sg.m11 || !sg.m12 || !sg.m22) throw std::invalid_argument("no period planes");
+  const size_t N = (size_t)nx * ny;
+  std::vector<double> planes[4];
+  const double* src[4] = {sg.svel, sg.m11, sg.m12, sg.m22};
+  for (int k = 0; k < 4; k++) {
+    planes[k].resize((size_t)np * N);
+    for (int ix = 0; ix < nx; ix++)
+      for (int iy = 0; iy < ny; iy++)
+        for (int ip = 0; ip < np; ip++)
+          planes[k][(size_t)ip * N + ix + (size_t)iy * nx] = src[k][((size_t)ix * ny + iy) * np + ip];
+  }
+  sat_grid_desc g{nx, ny, np, sg.xgrids.data(), sg.ygrids.data(),
+                  planes[0].data(), planes[1].data(), planes[2].data(), planes[3].data()};
+  std::vector<sat_event_desc> ev;
+  std::vector<double> rlon, rlat, tt, w;
+  for (auto& kv : sr.events_local) {
+    const event_info& e = kv.second;
+    if (e.iper < 0 || e.iper >= np) throw std::invalid_argument("event period out of range");
+    ev.push_back(sat_event_desc{0, e.iper, e.evlo, e.evla, (int)rlon.size(), (int)e.rec_indices.size()});
+    for (int r : e.rec_indices) {
+      if (r < 0 || r >= (int)sr.stlo.size()) throw std::invalid_argument("receiver index out of range");
+      rlon.push_back(sr.stlo[r]);
+      rlat.push_back(sr.stla[r]);
+      tt.push_back(sr.tt[r]);
+      w.push_back(sr.weight[r]);
+    }
+  }
+  sat_batch_desc d{};
+  d.ngrid = 1;
+  d.grids = &g;
+  d.nevent = (int)ev.size();
+  d.events = ev.data();
+  d.nrec_total = (int)rlon.size();
+  d.rec_lon_deg = rlon.data();
+  d.rec_lat_deg = rlat.data();
+  d.tt_obs = tt.data();
+  d.weight = w.data();
+  d.precision = surfatt::precision();
+  d.calc_adj = is_calc_adj ? 1 : 0;
+  surfatt::Batch batch(d);
+  batch.run();
+  std::vector<double> syn(rlon.size() + 1), K((size_t)np * N);
+  const double chi = batch.results(syn.data(), K.data());
+  size_t off = 0;
+  for (auto& kv : sr.events_local) {
+    event_info& e = kv.second;
+    e.syn_data.assign(syn.begin() + off, syn.begin() + off + e.rec_indices.size());
+    off += e.rec_indices.size();
+  }
+  if (is_calc_adj) {
+    if ((int)sg.adj_s_local.size() != np) sg.adj_s_local.assign(np, surfatt::Field(nx, ny));
+    for (int ip = 0; ip < np; ip++) {
+      surfatt::check_shape(sg.adj_s_local[ip], nx, ny);
+      double* a = sg.adj_s_local[ip].data();
+      for (size_t n = 0; n < N; n++) a[n] += K[(size_t)ip * N + n];
+    }
+  }
+  return chi;
+}
+
+}  // namespace preproc

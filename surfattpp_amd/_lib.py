@@ -61,7 +61,7 @@ class sat_batch_desc(C.Structure):
                 ("nevent", C.c_int), ("events", C.POINTER(sat_event_desc)),
                 ("nrec_total", C.c_int), ("rec_lon_deg", C.c_void_p), ("rec_lat_deg", C.c_void_p),
                 ("tt_obs", C.c_void_p), ("weight", C.c_void_p),
-                ("precision", C.c_int), ("calc_adj", C.c_int)]
+                ("precision", C.c_int), ("calc_adj", C.c_int), ("workspace_cap", C.c_size_t)]
 
 
 _LIB = None

@@ -9,9 +9,10 @@ namespace sat {
 // src/preproc.cpp:105-115 per receiver, and the receiver's delta cell and
 // weight (src/eikonal_solver.cpp:357-369).
 __global__ void recv_kernel(PhaseDev pd, int nrec) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nrec; k += gridDim.x * blockDim.x) {
+  for (int k = pd.rec_lo + (int)(blockIdx.x * blockDim.x + threadIdx.x); k < pd.rec_lo + nrec;
+       k += gridDim.x * blockDim.x) {
     const int e = pd.rec_ev[k];
-    if (e < 0 || pd.skip[e] != pd.want_skip) continue;
+    if (e < pd.ev_base || e >= pd.ev_base + pd.nevent || pd.skip[e] != pd.want_skip) continue;
     const EventX& ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];
     const double s = interp2d_point(g.lon, g.nx, g.lat, g.ny, pd.T + ev.pl_off, pd.rec_lon[k], pd.rec_lat[k]);

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -129,6 +130,8 @@ struct ReduceDev {
   double* K;
   size_t total_nodes;
   const size_t* node_prefix;   // prefix sum of plane_n (size nplanes+1)
+  int ev_lo, ev_hi;            // events of this pass (a chunk of the batch order)
+  int accumulate;              // add to K (a later chunk) instead of overwriting it
 };
 
 template <typename V>
@@ -144,10 +147,13 @@ __global__ void reduce_kernel(ReduceDev rd) {
     const int pl = lo;
     const size_t n = id - rd.node_prefix[pl];
     const double v3 = rd.vinv3[pl][n];
-    double acc = 0.0;
+    // the chunks are consecutive ranges of the batch order: continuing from the
+    // previous chunk's partial sum keeps the reference's summation order
+    double acc = rd.accumulate ? rd.K[rd.plane_off[pl] + n] : 0.0;
     for (int q = rd.ev_off[pl]; q < rd.ev_off[pl + 1]; q++) {
-      const EventDev& ev = rd.events[rd.ev_ids[q]];
-      acc += (double)ta[ev.ta_off + n] * v3;
+      const int e = rd.ev_ids[q];
+      if (e < rd.ev_lo || e >= rd.ev_hi) continue;
+      acc += (double)ta[rd.events[e].ta_off + n] * v3;
     }
     rd.K[rd.plane_off[pl] + n] = acc;
   }
@@ -177,6 +183,9 @@ __global__ void chi_kernel(const double* chi_ev, int n, double* chi) {
 struct Layout {
   int C, La, P;
 };
+
+// Launch groups: one per (columns per lane C, isotropic metric).
+constexpr int kMaxGroups = 2 * kMaxC;
 
 static Layout layout_for(int nx) {
   Layout L;
@@ -509,6 +518,14 @@ struct sat_batch {
   size_t K_total = 0;
   // launch groups: events grouped by (columns per lane C, isotropic grid)
   std::vector<int> group_C, group_iso, group_first, group_count;
+  // per group: sweep slots [group_slot0, group_slot0 + group_nblk); launch order
+  // of the groups (most work first)
+  std::vector<int> group_slot0, group_nblk, launch_order;
+  // Chunks: consecutive ranges of the batch order whose per-event device
+  // workspace fits the cap, run one after the other (one chunk when uncapped):
+  // events, receiver span, groups [chunk_g0[c], chunk_g0[c+1]).
+  std::vector<int> chunk_first, chunk_count, chunk_rec_lo, chunk_rec_n, chunk_g0;
+  size_t ws_cap = 0, ws_event_max = 0;
   bool groups_dirty = true;
   int max_nx = 0;
   size_t max_seg = 0, max_plane = 0, max_wplane = 0;
@@ -519,6 +536,12 @@ struct sat_batch {
   ReduceDev rd{};
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;   // FP64 re-solves, concurrent with the FP32 adjoint phase
+  // Launch groups of different grid shapes run concurrently: group k of the
+  // launch order on `stream` (k % (kSide+1) == 0) or side[k % (kSide+1) - 1].
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev[kSide] = {};
   // time marks: start, init done, forward sweeps done, adjoint set-up done,
   // adjoint sweeps done, re-solves start / end (stream2), end
   static constexpr int kMarks = 8;
@@ -533,6 +556,11 @@ struct sat_batch {
       if (p) (void)hipFree(p);
     for (hipEvent_t e : mk)
       if (e) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    for (int i = 0; i < kSide; i++) {
+      if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
+      if (side[i]) (void)hipStreamDestroy(side[i]);
+    }
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);
   }
@@ -549,23 +577,55 @@ int build_groups(sat_batch* b) {
     const GridDev& g = b->grids[b->he[e].grid].g;
     return std::make_pair(g.C, g.iso);
   };
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-    if (key(a) != key(c)) return key(a) < key(c);
-    return b->grids[b->he[a].grid].g.plane > b->grids[b->he[c].grid].g.plane;
-  });
   b->group_C.clear();
   b->group_iso.clear();
   b->group_first.clear();
   b->group_count.clear();
-  for (int i = 0; i < ne;) {
-    const auto k = key(order[i]);
-    int j = i;
-    while (j < ne && key(order[j]) == k) j++;
-    b->group_C.push_back(k.first);
-    b->group_iso.push_back(k.second);
-    b->group_first.push_back(i);
-    b->group_count.push_back(j - i);
-    i = j;
+  b->group_slot0.clear();
+  b->group_nblk.clear();
+  b->launch_order.clear();
+  b->chunk_g0.assign(1, 0);
+  const int nchunk = (int)b->chunk_first.size();
+  for (int c = 0; c < nchunk; c++) {
+    // within a chunk: events sorted by launch group, largest grids first
+    const int c0 = b->chunk_first[c], cn = b->chunk_count[c];
+    std::stable_sort(order.begin() + c0, order.begin() + c0 + cn, [&](int a, int d) {
+      if (key(a) != key(d)) return key(a) < key(d);
+      return b->grids[b->he[a].grid].g.plane > b->grids[b->he[d].grid].g.plane;
+    });
+    const size_t g_lo = b->group_C.size();
+    for (int i = c0; i < c0 + cn;) {
+      const auto k = key(order[i]);
+      int j = i;
+      while (j < c0 + cn && key(order[j]) == k) j++;
+      b->group_C.push_back(k.first);
+      b->group_iso.push_back(k.second);
+      b->group_first.push_back(i);
+      b->group_count.push_back(j - i);
+      i = j;
+    }
+    const size_t g_hi = b->group_C.size();
+    if (g_hi - g_lo > (size_t)kMaxGroups) return SAT_EINVAL;
+    // Each group's persistent sweep kernels get a share of the sweep slots in
+    // proportion to its events (rounded up: at most nslots_phase + kMaxGroups);
+    // a chunk's groups run concurrently, the one with the most sweep steps first.
+    std::vector<double> work(g_hi - g_lo, 0.0);
+    int slot = 0;
+    for (size_t gi = g_lo; gi < g_hi; gi++) {
+      const long share = ((long)b->nslots_phase * b->group_count[gi] + cn - 1) / std::max(cn, 1);
+      b->group_nblk.push_back((int)std::max(1L, std::min<long>(b->group_count[gi], share)));
+      b->group_slot0.push_back(slot);
+      slot += b->group_nblk.back();
+      for (int k = 0; k < b->group_count[gi]; k++) {
+        const GridDev& g = b->grids[b->he[order[b->group_first[gi] + k]].grid].g;
+        work[gi - g_lo] += (double)(g.ny + g.La - 1) * g.C;
+      }
+    }
+    std::vector<int> lo(g_hi - g_lo);
+    std::iota(lo.begin(), lo.end(), (int)g_lo);
+    std::stable_sort(lo.begin(), lo.end(), [&](int a, int d) { return work[a - g_lo] > work[d - g_lo]; });
+    b->launch_order.insert(b->launch_order.end(), lo.begin(), lo.end());
+    b->chunk_g0.push_back((int)g_hi);
   }
   if (ne) SAT_CHECK(hipMemcpy(b->d_order, order.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
   std::vector<GridDev> hg(b->grids.size()), hg64(b->grids.size()), hgmw(b->grids.size());
@@ -680,6 +740,11 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(hipStreamCreate(&b->stream));
   B_CHECK(hipStreamCreate(&b->stream2));
   for (int i = 0; i < sat_batch::kMarks; i++) B_CHECK(hipEventCreate(&b->mk[i]));
+  B_CHECK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
+  for (int i = 0; i < sat_batch::kSide; i++) {
+    B_CHECK(hipStreamCreate(&b->side[i]));
+    B_CHECK(hipEventCreateWithFlags(&b->join_ev[i], hipEventDisableTiming));
+  }
   b->grids.resize(d->ngrid);
   size_t max_seg = 0, max_plane = 0, max_wplane = 0;
   int max_nx = 0;
@@ -707,24 +772,75 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   b->he.resize(ne);
   b->hx.resize(ne);
   b->base_flags.assign(std::max(ne, 1), 0);
-  size_t w_total = 0, pl_total = 0, col_total = 0;
+  // Per-event workspace, in bytes: wavefront-major work planes, FP64 T (and
+  // the FP64 T0 of FP32 sweeps), the masked Ta, the column trig table.
+  const size_t es = elem_size(b->precision);
+  auto ev_bytes = [&](const GridDev& g) {
+    const size_t w = round_up(g.wplane, 64), pl = round_up(g.plane, 64), col = round_up(g.nx, 64);
+    return 6 * w * es + (f32 ? 2 : 1) * pl * sizeof(double) + (b->calc_adj ? pl * es : 0) +
+           col * sizeof(double2);
+  };
+  size_t ws_all = 0;
+  for (int e = 0; e < ne; e++) {
+    const size_t eb = ev_bytes(b->grids[d->events[e].grid].g);
+    ws_all += eb;
+    b->ws_event_max = std::max(b->ws_event_max, eb);
+  }
+  // The cap: the caller's, or -- when the whole batch would not fit in three
+  // quarters of the free device memory -- half of the free memory.
+  b->ws_cap = d->workspace_cap;
+  if (b->ws_cap == 0) {
+    size_t mfree = 0, mtotal = 0;
+    B_CHECK(hipMemGetInfo(&mfree, &mtotal));
+    if (ws_all > mfree / 4 * 3) b->ws_cap = mfree / 2;
+  }
+  if (b->ws_cap != 0 && b->ws_cap < b->ws_event_max) return fail(SAT_ENOMEM);
+  size_t w_total = 0, pl_total = 0, col_total = 0;   // maxima over the chunks
+  size_t w_c = 0, pl_c = 0, col_c = 0, bytes_c = 0;  // the current chunk
   for (int e = 0; e < ne; e++) {
     const sat_event_desc& ev = d->events[e];
     const sat_grid_desc& gd = d->grids[ev.grid];
     const GridDev& g = b->grids[ev.grid].g;
-    b->he[e] = EventDev{ev.grid, ev.iper, ev.src_lon_deg, ev.src_lat_deg, ev.rec_off, ev.nrec, pl_total};
+    const size_t eb = ev_bytes(g);
+    if (e == 0 || (b->ws_cap != 0 && bytes_c + eb > b->ws_cap)) {  // a new chunk
+      b->chunk_first.push_back(e);
+      b->chunk_count.push_back(0);
+      w_c = pl_c = col_c = bytes_c = 0;
+    }
+    b->chunk_count.back()++;
+    bytes_c += eb;
+    // offsets within the chunk's workspace
+    b->he[e] = EventDev{ev.grid, ev.iper, ev.src_lon_deg, ev.src_lat_deg, ev.rec_off, ev.nrec, pl_c};
     EventX& x = b->hx[e];
     x.grid = ev.grid; x.iper = ev.iper; x.rec_off = ev.rec_off; x.nrec = ev.nrec;
     x.lon = ev.src_lon_deg; x.lat = ev.src_lat_deg;
-    x.w_off = w_total; x.pl_off = pl_total; x.col_off = col_total;
+    x.w_off = w_c; x.pl_off = pl_c; x.col_off = col_c;
     std::vector<double> xx(gd.nx), yy(gd.ny);
     for (int i = 0; i < gd.nx; i++) xx[i] = gd.lon_deg[i] * kDEG2RAD;
     for (int j = 0; j < gd.ny; j++) yy[j] = gd.lat_deg[j] * kDEG2RAD;
     event_boxes(b->grids[ev.grid], xx, yy, gd.lon_deg, gd.lat_deg, ev.src_lon_deg, ev.src_lat_deg, x,
                 b->base_flags[e]);
-    w_total += round_up(g.wplane, 64);
-    pl_total += round_up(g.plane, 64);
-    col_total += round_up(g.nx, 64);
+    w_c += round_up(g.wplane, 64);
+    pl_c += round_up(g.plane, 64);
+    col_c += round_up(g.nx, 64);
+    w_total = std::max(w_total, w_c);
+    pl_total = std::max(pl_total, pl_c);
+    col_total = std::max(col_total, col_c);
+  }
+  if (ne == 0) {
+    b->chunk_first.push_back(0);
+    b->chunk_count.push_back(0);
+  }
+  // each chunk's receiver span
+  for (size_t c = 0; c < b->chunk_first.size(); c++) {
+    int lo = INT_MAX, hi = 0;
+    for (int e = b->chunk_first[c]; e < b->chunk_first[c] + b->chunk_count[c]; e++)
+      if (d->events[e].nrec > 0) {
+        lo = std::min(lo, d->events[e].rec_off);
+        hi = std::max(hi, d->events[e].rec_off + d->events[e].nrec);
+      }
+    b->chunk_rec_lo.push_back(lo == INT_MAX ? 0 : lo);
+    b->chunk_rec_n.push_back(lo == INT_MAX ? 0 : hi - lo);
   }
   b->w_total = w_total;
   b->pl_total = pl_total;
@@ -761,10 +877,10 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(dalloc(&b->d_skip, std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_fb, (size_t)std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_ov, (size_t)std::max(ne, 1)));
-  B_CHECK(dalloc(&b->d_counter, 8 * (size_t)std::max(ne, 1) + 2));  // groups <= events
+  // groups <= events; then one re-solve pool counter per chunk
+  B_CHECK(dalloc(&b->d_counter, 8 * (size_t)std::max(ne, 1) + 2 + b->chunk_first.size()));
 
-  // per-event planes of the phase pipeline
-  const size_t es = elem_size(b->precision);
+  // per-event planes of the phase pipeline (sized for the largest chunk)
   B_CHECK(dalloc(&b->d_evs, 3 * (size_t)std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_coltab, std::max<size_t>(col_total, 1)));
   // 6 forward planes of es bytes, or 2 face-flux planes of 2*es and 2 delta planes
@@ -781,11 +897,14 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   hipDeviceProp_t prop;
   B_CHECK(hipGetDeviceProperties(&prop, 0));
   const int waves_per_cu = 16;
-  b->nslots_phase = std::max(1, std::min(ne, prop.multiProcessorCount * waves_per_cu));
+  const int max_chunk = *std::max_element(b->chunk_count.begin(), b->chunk_count.end());
+  b->nslots_phase = std::max(1, std::min(max_chunk, prop.multiProcessorCount * waves_per_cu));
   b->slot_plane = round_up((max_wplane + kTrashElems) * sizeof(double), 256);  // + the sweeps' trash row
   b->slot_bytes = 3 * b->slot_plane;
-  B_CHECK(dalloc(&b->d_slots, b->slot_bytes * (size_t)b->nslots_phase));
-  B_CHECK(hipMemset(b->d_slots, 0, b->slot_bytes * (size_t)b->nslots_phase));
+  // concurrent launch groups share the slots; each group's share is rounded up
+  const size_t nslots_alloc = (size_t)b->nslots_phase + kMaxGroups;
+  B_CHECK(dalloc(&b->d_slots, b->slot_bytes * nslots_alloc));
+  B_CHECK(hipMemset(b->d_slots, 0, b->slot_bytes * nslots_alloc));
 
   // FP32 batches: a pool of FP64 phase-kernel planes for the re-solves
   if (f32) {
@@ -911,9 +1030,9 @@ int sat_batch_run(sat_batch* b, int mode) {
   int* resa_heads = b->d_counter + 4 * ng;    // re-solve adjoint sweeps
   int* ov_count = b->d_counter + 5 * ng;      // re-solves beyond the pool per group
   int* ov_heads = b->d_counter + 6 * ng;
-  int* pool_counter = b->d_counter + 7 * ng;
   int* solo_heads = b->d_counter + 7 * ng + 1;
-  const size_t n_counters = 8 * ne1 + 2;
+  int* pool_counters = b->d_counter + 8 * ne1 + 2;  // one per chunk
+  const size_t n_counters = 8 * ne1 + 2 + b->chunk_first.size();
 
   BatchDev bd{};
   bd.grids = b->d_grids;
@@ -941,7 +1060,9 @@ int sat_batch_run(sat_batch* b, int mode) {
   PhaseDev pd{};
   pd.grids = b->d_grids;
   pd.ev = b->d_evx;
+  pd.ev_base = 0;
   pd.nevent = ne;
+  pd.rec_lo = 0;
   pd.evs = b->d_evs;
   pd.coltab = b->d_coltab;
   const size_t es = elem_size(b->precision);
@@ -974,7 +1095,7 @@ int sat_batch_run(sat_batch* b, int mode) {
   pd.want_skip = 0;
   pd.ov_list = b->d_ov;
   pd.ov_count = ov_count;
-  pd.pool_counter = pool_counter;
+  pd.pool_counter = pool_counters;
   pd.pool_size = b->pool_size;
   pd.pool_stride = b->pool_stride;
   pd.ev64 = b->d_evx64;
@@ -990,87 +1111,10 @@ int sat_batch_run(sat_batch* b, int mode) {
   pd.order = b->d_order;
   pd.fwd_maxiter = kFwdMaxIter;
   pd.adj_maxiter = kAdjMaxIter;
-  const dim3 node_grid((unsigned)((b->max_plane + 255) / 256), (unsigned)std::max(ne, 1));
-  const int rec_blocks = (int)std::max<size_t>(1, std::min<size_t>((b->nrec + 255) / 256, 4096));
-
-  // FP64 re-solves of the FP32 pairs whose stop decision was within the band,
-  // on stream2 once the forward sweeps (which list them) are done.  Batched
-  // runs: the FP64 phase kernels over the pool lists, the per-pair kernel over
-  // the overflow lists; per-solve runs: the per-pair kernel.
+  const unsigned node_bx = (unsigned)((b->max_plane + 255) / 256);
   hipEvent_t* mk = b->mk;
   // columns per lane of the re-solve geometry: ceil(nx / (64 kMultiWave)) = ceil(C / kMultiWave)
   auto cmw = [&](size_t gi) { return (b->group_C[gi] + kMultiWave - 1) / kMultiWave; };
-  auto launch_resolves = [&](bool phase) -> int {
-    SAT_CHECK(hipEventRecord(mk[2], b->stream));
-    SAT_CHECK(hipStreamWaitEvent(b->stream2, mk[2], 0));
-    SAT_CHECK(hipEventRecord(mk[5], b->stream2));
-    if (f32 && ne) {
-      hipStream_t s2 = b->stream2;
-      if (phase) {
-        PhaseDev q = pd;
-        q.grids = b->d_grids64mw;
-        q.ev = b->d_evx64;
-        const size_t wb = b->pool_stride * (size_t)b->pool_size * sizeof(double);
-        for (int o = 0; o < 2; o++) {
-          q.t0[o] = b->d_work64 + (0 + o) * wb;
-          q.t0x[o] = b->d_work64 + (2 + o) * wb;
-          q.t0y[o] = b->d_work64 + (4 + o) * wb;
-          q.fxy[o] = b->d_work64 + 2 * o * wb;
-          q.dl[o] = b->d_work64 + (4 + o) * wb;
-        }
-        q.slots = b->d_slots64;
-        q.slot_bytes = b->slot_bytes64;
-        q.slot_plane = b->slot_plane64;
-        q.want_skip = 1;
-        q.ta_f32 = 1;
-        for (size_t gi = 0; gi < ng; gi++) {
-          const int ub = std::min(b->group_count[gi], b->pool_size);
-          PhaseDev a = q;
-          a.list = b->d_fb + b->group_first[gi];
-          a.list_count = fb_count + gi;
-          SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(node_grid.x, ub), s2, a));
-          a.counter = resf_heads + gi;
-          SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(0, cmw(gi), kMultiWave, dim3(std::min(ub, b->nslots64p)), s2, a,
-                                                   0, 0, (int)gi));
-        }
-        if (b->nrec) SAT_CHECK(launch_recv(dim3(rec_blocks), s2, q, b->nrec));
-        if (b->calc_adj) {
-          for (size_t gi = 0; gi < ng; gi++) {
-            const int ub = std::min(b->group_count[gi], b->pool_size);
-            PhaseDev a = q;
-            a.list = b->d_fb + b->group_first[gi];
-            a.list_count = fb_count + gi;
-            SAT_CHECK(launch_adj_setup<SAT_PREC_F64>(dim3(node_grid.x, ub), s2, a));
-          }
-          SAT_CHECK(launch_delta<SAT_PREC_F64>(dim3((ne + 63) / 64), s2, q));
-          for (size_t gi = 0; gi < ng; gi++) {
-            const int ub = std::min(b->group_count[gi], b->pool_size);
-            PhaseDev a = q;
-            a.list = b->d_fb + b->group_first[gi];
-            a.list_count = fb_count + gi;
-            a.counter = resa_heads + gi;
-            SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(cmw(gi), kMultiWave, dim3(std::min(ub, b->nslots64p)), s2, a, 0, 0));
-          }
-        }
-      }
-      for (size_t gi = 0; gi < ng; gi++) {
-        BatchDev a = bd;
-        a.grids = b->d_grids64;
-        a.ws = b->d_ws64;
-        a.ws_slot = b->sg64.slot;
-        a.ws_seg = b->sg64.seg;
-        a.ws_plain = b->sg64.plain;
-        a.counter = phase ? ov_heads + gi : resf_heads + gi;
-        a.f32_band = 0.0;
-        const int* list = (phase ? b->d_ov : b->d_fb) + b->group_first[gi];
-        const int* cnt = phase ? ov_count + gi : fb_count + gi;
-        const int nblk = std::min(b->group_count[gi], b->nslots64);
-        SAT_CHECK(launch_pair<SAT_PREC_F64>(1, b->group_C[gi], dim3(nblk), s2, a, 0, 0, (int)gi, list, cnt));
-      }
-    }
-    SAT_CHECK(hipEventRecord(mk[6], b->stream2));
-    return SAT_OK;
-  };
 
   SAT_CHECK(hipEventRecord(mk[0], b->stream));
   SAT_CHECK(hipMemsetAsync(b->d_counter, 0, sizeof(int) * n_counters, b->stream));
@@ -1078,83 +1122,211 @@ int sat_batch_run(sat_batch* b, int mode) {
   SAT_CHECK(hipMemsetAsync(b->d_iters, 0, sizeof(int) * 2 * ne1, b->stream));
   SAT_CHECK(hipMemcpyAsync(bd.flags, b->d_base_flags, sizeof(int) * ne1, hipMemcpyDeviceToDevice, b->stream));
 
-  if (mode == SAT_RUN_PER_SOLVE) {
-    // one launch per pair of the fused kernel (the plain per-solve path)
-    bd.ws = b->d_ws;
-    bd.ws_slot = b->sg.slot;
-    bd.ws_seg = b->sg.seg;
-    bd.ws_plain = b->sg.plain;
-    for (size_t gi = 0; gi < ng; gi++)
-      for (int i = 0; i < b->group_count[gi]; i++) {
-        const int first = b->group_first[gi] + i;
-        BatchDev a = bd;
-        a.counter = solo_heads + first;
-        a.fb_list = bd.fb_list + b->group_first[gi];
-        if (!f32)
-          SAT_CHECK(launch_pair<SAT_PREC_F64>(0, b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi, nullptr,
-                                              nullptr));
-        else
-          SAT_CHECK(launch_pair<SAT_PREC_F32>(1, b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi, nullptr,
-                                              nullptr));
+  // The chunks run one after the other through the same workspace (one chunk
+  // unless the batch exceeds the workspace cap).  The phase marks describe the
+  // last chunk; mk[0] -> mk[7] spans the whole run.
+  const int nchunk = (int)b->chunk_first.size();
+  for (int c = 0; c < nchunk; c++) {
+    const int cn = b->chunk_count[c];
+    const size_t g_lo = (size_t)b->chunk_g0[c], g_hi = (size_t)b->chunk_g0[c + 1];
+    const int nrec_c = b->chunk_rec_n[c];
+    PhaseDev pc = pd;
+    pc.ev_base = b->chunk_first[c];
+    pc.nevent = cn;
+    pc.rec_lo = b->chunk_rec_lo[c];
+    pc.pool_counter = pool_counters + c;
+    const dim3 node_grid(node_bx, (unsigned)std::max(cn, 1));
+    const int rec_blocks = (int)std::max(1, std::min((nrec_c + 255) / 256, 4096));
+
+    // FP64 re-solves of the FP32 pairs whose stop decision was within the band,
+    // on stream2 once the forward sweeps (which list them) are done.  Batched
+    // runs: the FP64 phase kernels over the pool lists, the per-pair kernel over
+    // the overflow lists; per-solve runs: the per-pair kernel.
+    auto launch_resolves = [&](bool phase) -> int {
+      SAT_CHECK(hipEventRecord(mk[2], b->stream));
+      SAT_CHECK(hipStreamWaitEvent(b->stream2, mk[2], 0));
+      SAT_CHECK(hipEventRecord(mk[5], b->stream2));
+      if (f32 && cn) {
+        hipStream_t s2 = b->stream2;
+        if (phase) {
+          PhaseDev q = pc;
+          q.grids = b->d_grids64mw;
+          q.ev = b->d_evx64;
+          const size_t wb = b->pool_stride * (size_t)b->pool_size * sizeof(double);
+          for (int o = 0; o < 2; o++) {
+            q.t0[o] = b->d_work64 + (0 + o) * wb;
+            q.t0x[o] = b->d_work64 + (2 + o) * wb;
+            q.t0y[o] = b->d_work64 + (4 + o) * wb;
+            q.fxy[o] = b->d_work64 + 2 * o * wb;
+            q.dl[o] = b->d_work64 + (4 + o) * wb;
+          }
+          q.slots = b->d_slots64;
+          q.slot_bytes = b->slot_bytes64;
+          q.slot_plane = b->slot_plane64;
+          q.want_skip = 1;
+          q.ta_f32 = 1;
+          for (size_t gi = g_lo; gi < g_hi; gi++) {
+            const int ub = std::min(b->group_count[gi], b->pool_size);
+            PhaseDev a = q;
+            a.list = b->d_fb + b->group_first[gi];
+            a.list_count = fb_count + gi;
+            SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(node_bx, ub), s2, a));
+            a.counter = resf_heads + gi;
+            SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(0, cmw(gi), kMultiWave, dim3(std::min(ub, b->nslots64p)), s2,
+                                                     a, 0, 0, (int)gi));
+          }
+          if (nrec_c) SAT_CHECK(launch_recv(dim3(rec_blocks), s2, q, nrec_c));
+          if (b->calc_adj) {
+            for (size_t gi = g_lo; gi < g_hi; gi++) {
+              const int ub = std::min(b->group_count[gi], b->pool_size);
+              PhaseDev a = q;
+              a.list = b->d_fb + b->group_first[gi];
+              a.list_count = fb_count + gi;
+              SAT_CHECK(launch_adj_setup<SAT_PREC_F64>(dim3(node_bx, ub), s2, a));
+            }
+            SAT_CHECK(launch_delta<SAT_PREC_F64>(dim3((cn + 63) / 64), s2, q));
+            for (size_t gi = g_lo; gi < g_hi; gi++) {
+              const int ub = std::min(b->group_count[gi], b->pool_size);
+              PhaseDev a = q;
+              a.list = b->d_fb + b->group_first[gi];
+              a.list_count = fb_count + gi;
+              a.counter = resa_heads + gi;
+              SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(cmw(gi), kMultiWave, dim3(std::min(ub, b->nslots64p)), s2, a,
+                                                       0, 0));
+            }
+          }
+        }
+        for (size_t gi = g_lo; gi < g_hi; gi++) {
+          BatchDev a = bd;
+          a.grids = b->d_grids64;
+          a.ws = b->d_ws64;
+          a.ws_slot = b->sg64.slot;
+          a.ws_seg = b->sg64.seg;
+          a.ws_plain = b->sg64.plain;
+          a.counter = phase ? ov_heads + gi : resf_heads + gi;
+          a.f32_band = 0.0;
+          const int* list = (phase ? b->d_ov : b->d_fb) + b->group_first[gi];
+          const int* cnt = phase ? ov_count + gi : fb_count + gi;
+          const int nblk = std::min(b->group_count[gi], b->nslots64);
+          SAT_CHECK(launch_pair<SAT_PREC_F64>(1, b->group_C[gi], dim3(nblk), s2, a, 0, 0, (int)gi, list, cnt));
+        }
       }
-    SAT_CHECK(hipEventRecord(mk[1], b->stream));
-    const int rc = launch_resolves(false);
-    if (rc) return rc;
-    SAT_CHECK(hipEventRecord(mk[3], b->stream));
-    SAT_CHECK(hipEventRecord(mk[4], b->stream));
-  } else {
-    if (ne) {
-      const dim3 col_grid((unsigned)((b->max_nx + 63) / 64), (unsigned)ne);
-      SAT_CHECK(f32 ? launch_trig<SAT_PREC_F32>(col_grid, b->stream, pd) : launch_trig<SAT_PREC_F64>(col_grid, b->stream, pd));
-      SAT_CHECK(f32 ? launch_init<SAT_PREC_F32>(node_grid, b->stream, pd) : launch_init<SAT_PREC_F64>(node_grid, b->stream, pd));
-    }
-    SAT_CHECK(hipEventRecord(mk[1], b->stream));
-    for (size_t gi = 0; gi < ng; gi++) {
-      PhaseDev a = pd;
-      a.counter = fwd_heads + gi;
-      a.fb_list = pd.fb_list + b->group_first[gi];
-      a.ov_list = pd.ov_list + b->group_first[gi];
-      const int nblk = std::min(b->group_count[gi], b->nslots_phase);
-      const int C = b->group_C[gi], iso = b->group_iso[gi];
-      SAT_CHECK(f32 ? launch_fwd_sweep<SAT_PREC_F32>(iso, C, 1, dim3(nblk), b->stream, a, b->group_first[gi],
-                                                     b->group_count[gi], (int)gi)
-                    : launch_fwd_sweep<SAT_PREC_F64>(iso, C, 1, dim3(nblk), b->stream, a, b->group_first[gi],
-                                                     b->group_count[gi], (int)gi));
-    }
-    {
-      const int rc = launch_resolves(true);
+      SAT_CHECK(hipEventRecord(mk[6], b->stream2));
+      return SAT_OK;
+    };
+
+    // Launch fn(gi, stream) for every group of the chunk, concurrently across
+    // the side streams.
+    auto fan_out = [&](auto&& fn) -> int {
+      const size_t ngc = g_hi - g_lo;
+      const int nused = (int)std::min<size_t>(ngc, sat_batch::kSide + 1);
+      if (nused > 1) {
+        SAT_CHECK(hipEventRecord(b->fork_ev, b->stream));
+        for (int k = 1; k < nused; k++) SAT_CHECK(hipStreamWaitEvent(b->side[k - 1], b->fork_ev, 0));
+      }
+      for (size_t k = 0; k < ngc; k++) {
+        const int sk = (int)(k % (size_t)nused);
+        const int rc = fn(b->launch_order[g_lo + k], sk == 0 ? b->stream : b->side[sk - 1]);
+        if (rc) return rc;
+      }
+      if (nused > 1) {
+        for (int k = 1; k < nused; k++) {
+          SAT_CHECK(hipEventRecord(b->join_ev[k - 1], b->side[k - 1]));
+          SAT_CHECK(hipStreamWaitEvent(b->stream, b->join_ev[k - 1], 0));
+        }
+      }
+      return SAT_OK;
+    };
+
+    if (mode == SAT_RUN_PER_SOLVE) {
+      // one launch per pair of the fused kernel (the plain per-solve path)
+      bd.ws = b->d_ws;
+      bd.ws_slot = b->sg.slot;
+      bd.ws_seg = b->sg.seg;
+      bd.ws_plain = b->sg.plain;
+      for (size_t gi = g_lo; gi < g_hi; gi++)
+        for (int i = 0; i < b->group_count[gi]; i++) {
+          const int first = b->group_first[gi] + i;
+          BatchDev a = bd;
+          a.counter = solo_heads + first;
+          a.fb_list = bd.fb_list + b->group_first[gi];
+          if (!f32)
+            SAT_CHECK(launch_pair<SAT_PREC_F64>(0, b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi, nullptr,
+                                                nullptr));
+          else
+            SAT_CHECK(launch_pair<SAT_PREC_F32>(1, b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi, nullptr,
+                                                nullptr));
+        }
+      SAT_CHECK(hipEventRecord(mk[1], b->stream));
+      const int rc = launch_resolves(false);
       if (rc) return rc;
-    }
-    if (b->nrec) {
-      SAT_CHECK(launch_recv(dim3(rec_blocks), b->stream, pd, b->nrec));
-    }
-    if (b->calc_adj && ne) {
-      SAT_CHECK(f32 ? launch_adj_setup<SAT_PREC_F32>(node_grid, b->stream, pd)
-                    : launch_adj_setup<SAT_PREC_F64>(node_grid, b->stream, pd));
-      const dim3 ev_grid((unsigned)((ne + 63) / 64));
-      SAT_CHECK(f32 ? launch_delta<SAT_PREC_F32>(ev_grid, b->stream, pd) : launch_delta<SAT_PREC_F64>(ev_grid, b->stream, pd));
-    }
-    SAT_CHECK(hipEventRecord(mk[3], b->stream));
-    if (b->calc_adj && ne) {
-      for (size_t gi = 0; gi < ng; gi++) {
-        PhaseDev a = pd;
-        a.counter = adj_heads + gi;
-        const int nblk = std::min(b->group_count[gi], b->nslots_phase);
-        SAT_CHECK(f32 ? launch_adj_sweep<SAT_PREC_F32>(b->group_C[gi], 1, dim3(nblk), b->stream, a, b->group_first[gi],
-                                                       b->group_count[gi])
-                      : launch_adj_sweep<SAT_PREC_F64>(b->group_C[gi], 1, dim3(nblk), b->stream, a, b->group_first[gi],
-                                                       b->group_count[gi]));
+      SAT_CHECK(hipEventRecord(mk[3], b->stream));
+      SAT_CHECK(hipEventRecord(mk[4], b->stream));
+    } else {
+      if (cn) {
+        const dim3 col_grid((unsigned)((b->max_nx + 63) / 64), (unsigned)cn);
+        SAT_CHECK(f32 ? launch_trig<SAT_PREC_F32>(col_grid, b->stream, pc) : launch_trig<SAT_PREC_F64>(col_grid, b->stream, pc));
+        SAT_CHECK(f32 ? launch_init<SAT_PREC_F32>(node_grid, b->stream, pc) : launch_init<SAT_PREC_F64>(node_grid, b->stream, pc));
       }
+      SAT_CHECK(hipEventRecord(mk[1], b->stream));
+      {
+        const int rc = fan_out([&](int gi, hipStream_t st) -> int {
+          PhaseDev a = pc;
+          a.counter = fwd_heads + gi;
+          a.fb_list = pc.fb_list + b->group_first[gi];
+          a.ov_list = pc.ov_list + b->group_first[gi];
+          a.slots = b->d_slots + (size_t)b->group_slot0[gi] * b->slot_bytes;
+          const int C = b->group_C[gi], iso = b->group_iso[gi];
+          SAT_CHECK(f32 ? launch_fwd_sweep<SAT_PREC_F32>(iso, C, 1, dim3(b->group_nblk[gi]), st, a,
+                                                         b->group_first[gi], b->group_count[gi], gi)
+                        : launch_fwd_sweep<SAT_PREC_F64>(iso, C, 1, dim3(b->group_nblk[gi]), st, a,
+                                                         b->group_first[gi], b->group_count[gi], gi));
+          return SAT_OK;
+        });
+        if (rc) return rc;
+      }
+      {
+        const int rc = launch_resolves(true);
+        if (rc) return rc;
+      }
+      if (nrec_c) SAT_CHECK(launch_recv(dim3(rec_blocks), b->stream, pc, nrec_c));
+      if (b->calc_adj && cn) {
+        SAT_CHECK(f32 ? launch_adj_setup<SAT_PREC_F32>(node_grid, b->stream, pc)
+                      : launch_adj_setup<SAT_PREC_F64>(node_grid, b->stream, pc));
+        const dim3 ev_grid((unsigned)((cn + 63) / 64));
+        SAT_CHECK(f32 ? launch_delta<SAT_PREC_F32>(ev_grid, b->stream, pc) : launch_delta<SAT_PREC_F64>(ev_grid, b->stream, pc));
+      }
+      SAT_CHECK(hipEventRecord(mk[3], b->stream));
+      if (b->calc_adj && cn) {
+        const int rc = fan_out([&](int gi, hipStream_t st) -> int {
+          PhaseDev a = pc;
+          a.counter = adj_heads + gi;
+          a.slots = b->d_slots + (size_t)b->group_slot0[gi] * b->slot_bytes;
+          const dim3 grid(b->group_nblk[gi]);
+          SAT_CHECK(f32 ? launch_adj_sweep<SAT_PREC_F32>(b->group_C[gi], 1, grid, st, a, b->group_first[gi],
+                                                         b->group_count[gi])
+                        : launch_adj_sweep<SAT_PREC_F64>(b->group_C[gi], 1, grid, st, a, b->group_first[gi],
+                                                         b->group_count[gi]));
+          return SAT_OK;
+        });
+        if (rc) return rc;
+      }
+      SAT_CHECK(hipEventRecord(mk[4], b->stream));
     }
-    SAT_CHECK(hipEventRecord(mk[4], b->stream));
-  }
-  SAT_CHECK(hipStreamWaitEvent(b->stream, mk[6], 0));
-  if (b->calc_adj && b->K_total) {
-    const int threads = 256;
-    const int blocks = (int)std::min<size_t>((b->K_total + threads - 1) / threads, 65535);
-    if (!f32) hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks), dim3(threads), 0, b->stream, b->rd);
-    else hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks), dim3(threads), 0, b->stream, b->rd);
-    SAT_CHECK(hipGetLastError());
+    // the chunk's re-solves are done before its Ta is reduced and before the
+    // next chunk reuses the workspace
+    SAT_CHECK(hipStreamWaitEvent(b->stream, mk[6], 0));
+    if (b->calc_adj && b->K_total) {
+      ReduceDev rd = b->rd;
+      rd.ev_lo = b->chunk_first[c];
+      rd.ev_hi = b->chunk_first[c] + cn;
+      rd.accumulate = c > 0;
+      const int threads = 256;
+      const int blocks = (int)std::min<size_t>((b->K_total + threads - 1) / threads, 65535);
+      if (!f32) hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks), dim3(threads), 0, b->stream, rd);
+      else hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks), dim3(threads), 0, b->stream, rd);
+      SAT_CHECK(hipGetLastError());
+    }
   }
   hipLaunchKernelGGL(chi_kernel, dim3(1), dim3(1), 0, b->stream, b->d_chi_ev, b->calc_adj ? ne : 0, b->d_chi);
   SAT_CHECK(hipGetLastError());

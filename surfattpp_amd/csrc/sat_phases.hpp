@@ -39,7 +39,9 @@ struct EventX {
 struct PhaseDev {
   const GridDev* grids;
   const EventX* ev;
+  int ev_base;          // the launch covers events [ev_base, ev_base + nevent) (a chunk)
   int nevent;
+  int rec_lo;           // and receivers [rec_lo, rec_lo + nrec) of the launch's receiver pass
   double* evs;          // per event: fun0, sin(y0), cos(y0)
   double2* coltab;      // per (event, column): sin, cos of xx - x0
   // per event, wavefront-major, per orientation; the adjoint planes reuse the
@@ -145,7 +147,7 @@ __device__ inline int pop_work(int* counter, int* s_idx) {
 
 // Event of a node-parallel block: blockIdx.y, or the list entry blockIdx.y; -1 if none.
 __device__ inline int block_event(const PhaseDev& pd) {
-  if (!pd.list) return blockIdx.y;
+  if (!pd.list) return pd.ev_base + (int)blockIdx.y;
   return ((int)blockIdx.y < *pd.list_count) ? pd.list[blockIdx.y] : -1;
 }
 
@@ -162,7 +164,7 @@ __device__ inline bool wave_decode0(const GridDev& g, int q, int& ix, int& j) {
 // ---------------------------------------------------------------------------
 template <int PREC>
 __global__ void trig_kernel(PhaseDev pd) {
-  const int e = blockIdx.y;
+  const int e = pd.ev_base + (int)blockIdx.y;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
   const double x0 = ev.lon * kDEG2RAD;
@@ -402,8 +404,8 @@ __global__ void adj_setup_kernel(PhaseDev pd) {
 template <int PREC>
 __global__ void delta_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= pd.nevent || pd.skip[e] != pd.want_skip) return;
+  const int e = pd.ev_base + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (e >= pd.ev_base + pd.nevent || pd.skip[e] != pd.want_skip) return;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
   double s = 0.0;

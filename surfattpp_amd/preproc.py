@@ -30,7 +30,11 @@ def _c64(a):
 class EikonalBatch:
     """Device-resident batch of forward+adjoint pairs over one or more grids."""
 
-    def __init__(self, grids: dict | Sequence[dict], precision: int = _lib.PREC_F64, calc_adj: bool = True):
+    def __init__(self, grids: dict | Sequence[dict], precision: int = _lib.PREC_F64, calc_adj: bool = True,
+                 workspace_cap: int = 0):
+        """workspace_cap: bytes of per-event device workspace held at once (0: the
+        library's default); above it each run processes the events in consecutive
+        chunks of the batch order, with the same results."""
         if isinstance(grids, dict):
             grids = [grids]
         self._keep = []
@@ -78,6 +82,7 @@ class EikonalBatch:
         d.rec_lon_deg, d.rec_lat_deg, d.tt_obs, d.weight = (a.ctypes.data if a.size else None for a in self._rec)
         d.precision = int(precision)
         d.calc_adj = 1 if calc_adj else 0
+        d.workspace_cap = int(workspace_cap)
         self.precision = precision
         self.calc_adj = calc_adj
         h = C.c_void_p()

@@ -194,3 +194,177 @@ def subset_events(batch: dict, ev_ids) -> dict:
     for k in ("rec_idx", "rec_lon", "rec_lat", "tt_obs", "weight"):
         out[k] = batch[k][sel].copy()
     return out
+
+
+# ----------------------------------------------------------------------------
+# Topography and its metric (an input producer of the path: the reference
+# builds it on the host once per period, src/surf_grid.cpp:80-163)
+# ----------------------------------------------------------------------------
+RAD2DEG = 180.0 / PI
+
+
+def gaussian_edifices(rng: Rng, lon: np.ndarray, lat: np.ndarray, n: int = 8, h_km=(0.5, 4.0),
+                      w_km=(10.0, 40.0)) -> np.ndarray:
+    """Surface relief z (ny, nx) in km: a sum of n Gaussian edifices
+    h*exp(-r^2/(2 w^2)) with h in U[h_km], w in U[w_km] and centres uniform over
+    the grid; r is the local Cartesian distance in km (the grid is centred on
+    (0, 0), so lon and lat map to km by R_EARTH*DEG2RAD)."""
+    u = rng.uniform(4 * n).reshape(4, n)
+    h = h_km[0] + (h_km[1] - h_km[0]) * u[0]
+    w = w_km[0] + (w_km[1] - w_km[0]) * u[1]
+    cx = lon[0] + (lon[-1] - lon[0]) * u[2]
+    cy = lat[0] + (lat[-1] - lat[0]) * u[3]
+    km = R_EARTH * DEG2RAD
+    X = lon[None, :] * km
+    Y = lat[:, None] * km
+    z = np.zeros((lat.size, lon.size))
+    for k in range(n):
+        r2 = (X - cx[k] * km) ** 2 + (Y - cy[k] * km) ** 2
+        z += h[k] * np.exp(-r2 / (2.0 * w[k] ** 2))
+    return z
+
+
+def gaussian_smooth_geo(z: np.ndarray, lon: np.ndarray, lat: np.ndarray, sigma_deg: float) -> np.ndarray:
+    """Great-circle Gaussian smoothing of z (ny, nx), the operator of
+    include/utils.h:505-573 (gaussian_smooth_geo_2): for node (i, j) the window
+    is |m - i| <= hx = int(3 sigma / dlon) and |n - j| <= hy_j =
+    int(3 sigma / cos(lat_j) / dlat), clipped to the grid; the weight is
+    exp(-delta^2 / (2 sigma^2)) with delta = 2*atan(sqrt(a/(1-a))) in degrees and
+    a = sin^2(dlat/2) + cos(lat_j) cos(lat_n) sin^2(dlon/2) (haversine, a <= 1);
+    the result is the weighted mean over the window.
+
+    The grid is uniform in longitude, so the weights depend on m - i only and
+    each output row j is a sum over window rows n of 1-D convolutions along
+    longitude, evaluated with FFTs (agrees with the direct sum to rounding)."""
+    ny, nx = z.shape
+    s3 = 3.0 * sigma_deg
+    dx, dy = lon[1] - lon[0], lat[1] - lat[0]
+    hx = int(s3 / dx)
+    L = 1 << int(math.ceil(math.log2(nx + 2 * hx + 1)))
+    dm = np.arange(-hx, hx + 1)
+    s2lon = np.sin(dm * dx * (DEG2RAD * 0.5)) ** 2
+    Zf = np.fft.rfft(z, n=L, axis=1)                       # rows n, z at [0, nx)
+    Of = np.fft.rfft(np.ones(nx), n=L)                     # the clipped window's support
+    lat_r = lat * DEG2RAD
+    inv2s2 = 1.0 / (2.0 * sigma_deg * sigma_deg)
+    out = np.empty_like(z)
+    idx = (-dm) % L                                        # kernel kr[t] = K[-t] at t mod L
+    for j in range(ny):
+        c0 = math.cos(lat_r[j])
+        hy = int(s3 / c0 / dy)
+        n1, n2 = max(0, j - hy), min(ny - 1, j + hy)
+        s2lat = np.sin((lat[n1:n2 + 1] - lat[j]) * (DEG2RAD * 0.5)) ** 2
+        c1 = np.cos(lat_r[n1:n2 + 1])
+        a = s2lat[:, None] + (c0 * c1)[:, None] * s2lon[None, :]
+        a = np.minimum(a, 1.0)
+        w = np.exp(-((2.0 * RAD2DEG) * np.arctan(np.sqrt(a / (1.0 - a)))) ** 2 * inv2s2)
+        kr = np.zeros((n2 - n1 + 1, L))
+        kr[:, idx] = w
+        Kf = np.fft.rfft(kr, axis=1)
+        num = np.fft.irfft((Kf * Zf[n1:n2 + 1]).sum(axis=0), n=L)[:nx]
+        den = np.fft.irfft(Kf.sum(axis=0) * Of, n=L)[:nx]
+        out[j] = num / den
+    return out
+
+
+def gradient_geo(f: np.ndarray, lon: np.ndarray, lat: np.ndarray):
+    """Geographic gradient (df/dx, df/dy) in f-units per km of f (ny, nx), the
+    differences of include/utils.h:595-649 (gradient_2_geo): central in the
+    interior, one-sided at the edges; dx_km = R*dlon*DEG2RAD*cos(lat),
+    dy_km = R*dlat*DEG2RAD."""
+    ny, nx = f.shape
+    dx1 = np.zeros(nx)
+    dy1 = np.zeros(ny)
+    dx1[1:] = (lon[1:] - lon[:-1]) * (R_EARTH * DEG2RAD)
+    dy1[1:] = (lat[1:] - lat[:-1]) * (R_EARTH * DEG2RAD)
+    dxm = np.cos(lat * DEG2RAD)[:, None] * dx1[None, :]   # (ny, nx)
+    dym = np.broadcast_to(dy1[:, None], (ny, nx))
+    tx = np.empty_like(f)
+    ty = np.empty_like(f)
+    tx[:, 1:-1] = (f[:, 2:] - f[:, :-2]) / (2.0 * dxm[:, 1:-1])
+    tx[:, 0] = (f[:, 1] - f[:, 0]) / dxm[:, 1]
+    tx[:, -1] = (f[:, -1] - f[:, -2]) / dxm[:, -1]
+    ty[1:-1, :] = (f[2:, :] - f[:-2, :]) / (2.0 * dym[1:-1, :])
+    ty[0, :] = (f[1, :] - f[0, :]) / dym[1, :]
+    ty[-1, :] = (f[-1, :] - f[-2, :]) / dym[-1, :]
+    return tx, ty
+
+
+def topo_metric(z: np.ndarray, lon: np.ndarray, lat: np.ndarray, sigma_deg: float | None):
+    """Per-period metric planes (m11, m12, m22) of the surface z (ny, nx, km),
+    src/surf_grid.cpp:115-145 and :184-190: z smoothed with sigma_deg (None: not
+    smoothed), fx, fy its geographic gradient, D = 1 + fx^2 + fy^2,
+    a = (1 + fy^2)/D, b = (1 + fx^2)/D, c = fx*fy/D; m11 = a, m22 = b, m12 = -c."""
+    zs = z if sigma_deg is None else gaussian_smooth_geo(z, lon, lat, sigma_deg)
+    fx, fy = gradient_geo(zs, lon, lat)
+    fx2, fy2 = fx * fx, fy * fy
+    den = 1.0 + fx2 + fy2
+    return (1.0 + fy2) / den, -(fx * fy / den), (1.0 + fx2) / den
+
+
+def periods_log(n: int, t0: float = 4.0, t1: float = 40.0) -> np.ndarray:
+    """n periods log-spaced from t0 to t1 s."""
+    return t0 * (t1 / t0) ** (np.arange(n) / max(n - 1, 1))
+
+
+def mean_phase_velocity(periods: np.ndarray) -> np.ndarray:
+    """A smooth Rayleigh-like dispersion curve (km/s), 2.9 at 4 s rising to 3.8 at
+    40 s, linear in log-period; stands in for surfdisp96's average velocity."""
+    x = np.log(periods / 4.0) / np.log(10.0)
+    return 2.9 + 0.9 * np.clip(x, 0.0, 1.0)
+
+
+def config_topo_grid(seed: int, n: int, spacing_km: float = 1.5, nsta: int | None = None, nperiod: int = 12,
+                     wavelen_factor: float = 2.5, smooth: bool = True):
+    """One grid of the topography batch: n x n nodes at spacing_km, 8 seeded Gaussian
+    edifices (0.5-4 km high, 10-40 km wide) whose per-period smoothed slopes give the
+    anisotropic metric (topo_metric, sigma = v(T)*T*wavelen_factor/(2 pi) km,
+    src/surf_grid.cpp:115-118); velocity per period is the mean dispersion velocity
+    v(T) times the 2x3-cell +-6 % checkerboard; nsta (default: seeded in 10..60)
+    stations, every one a virtual source; observed times are great-circle times at
+    v(T)."""
+    rng = Rng(seed)
+    d = km_to_deg(spacing_km)
+    lon, lat = centred_axis(n, d), centred_axis(n, d)
+    if nsta is None:
+        nsta = 10 + int(rng.uniform(1)[0] * 51)
+    sta_lon, sta_lat = interior_stations(rng, nsta, lon, lat)
+    z = gaussian_edifices(rng, lon, lat)
+    periods = periods_log(nperiod)
+    vbar = mean_phase_velocity(periods)
+    cb = checkerboard(n, n, 1.0, 0.06, 2, 3)
+    svel = np.stack([vbar[p] * cb for p in range(nperiod)])
+    m11 = np.empty_like(svel)
+    m12 = np.empty_like(svel)
+    m22 = np.empty_like(svel)
+    for p in range(nperiod):
+        sig = km_to_deg(vbar[p] * periods[p] * wavelen_factor / (2.0 * PI)) if smooth else None
+        m11[p], m12[p], m22[p] = topo_metric(z, lon, lat, sig)
+    ev_src, ev_iper, rec_idx, off, nrec = all_pairs_events(sta_lon, sta_lat, nperiod)
+    tt_obs = great_circle_time(sta_lon[rec_idx], sta_lat[rec_idx], sta_lon[ev_src].repeat(nrec),
+                               sta_lat[ev_src].repeat(nrec), vbar[ev_iper].repeat(nrec))
+    weight = np.ones(rec_idx.size)
+    meta = dict(name=f"topo-{n}x{n}", spacing_km=spacing_km, nsta=nsta, nperiod=nperiod, seed=seed,
+                periods=periods.tolist(), vbar=vbar.tolist(), wavelen_factor=wavelen_factor)
+    b = _assemble(lon, lat, svel, m11, m12, m22, sta_lon, sta_lat, ev_src, ev_iper, rec_idx, off, nrec,
+                  tt_obs, weight, meta)
+    b["z"] = z
+    return b
+
+
+TOPO_SIZES = (101, 161, 241, 321)
+
+
+def config_topo(seed: int = 30, sizes=TOPO_SIZES, nperiod: int = 12, **kw):
+    """The uneven topography batch: one grid per size (1.5 km spacing), seeds
+    seed+k, 10-60 stations each, nperiod periods; a list of batch dicts."""
+    return [config_topo_grid(seed + k, n, nperiod=nperiod, **kw) for k, n in enumerate(sizes)]
+
+
+def config_large(seed: int = 5, nx: int = 401, ny: int = 481, nsta: int = 400, nperiod: int = 30, **kw):
+    """401x481 nodes at 1 km, 400 stations x 30 periods: 12,000 forward+adjoint pairs
+    with 399 receivers each; velocity as config_grf (3.2 km/s, independent +-10 %
+    GRF of 15 km correlation per period)."""
+    b = config_grf(seed=seed, nx=nx, ny=ny, nsta=nsta, nperiod=nperiod, **kw)
+    b["meta"]["name"] = f"large-{nx}x{ny}"
+    return b

@@ -13,10 +13,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBDIR = os.path.join(ROOT, "surfattpp_amd")
 
 
-def _build(tmp_path):
-    exe = str(tmp_path / "shim_driver")
+def _build(tmp_path, name="shim_driver"):
+    exe = str(tmp_path / name)
     cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
-           os.path.join(ROOT, "tests", "cpp", "shim_driver.cpp"), "-o", exe,
+           os.path.join(ROOT, "tests", "cpp", name + ".cpp"), "-o", exe,
            "-L", LIBDIR, "-lsurfatt_eik", f"-Wl,-rpath,{LIBDIR}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
@@ -27,6 +27,7 @@ def test_shim_compiles_and_links(tmp_path):
     if not os.path.exists(os.path.join(LIBDIR, "libsurfatt_eik.so")):
         pytest.skip("library not built")
     _build(tmp_path)
+    _build(tmp_path, "shim_batch_driver")
 
 
 @pytest.mark.gpu
@@ -57,3 +58,36 @@ def test_shim_matches_oracle(tmp_path):
     Tao = O.mask(xx, yy, Tao, x0, y0)
     assert RC.rel_inf(T, To) <= 1e-12
     assert RC.rel_inf(Ta, Tao) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_shim_forward_for_event_matches_oracle(tmp_path):
+    # the reference's layout on the caller's side: [ix][iy][iper] planes, map-ordered events
+    from surfattpp_amd import synth
+    b = synth.config_grf(seed=8, nx=57, ny=49, nsta=7, nperiod=3)
+    b = dict(b, m12=b["m12"] + 0.01, m11=b["m11"] * 0.98)  # a non-trivial metric
+    nper, ny, nx = b["svel"].shape
+    ne, nrec = len(b["ev_iper"]), len(b["rec_lon"])
+    inp = tmp_path / "batch.bin"
+    with open(inp, "wb") as f:
+        f.write(np.array([nx, ny, nper, ne, nrec], dtype=np.int32).tobytes())
+        f.write(b["lon"].astype(np.float64).tobytes())
+        f.write(b["lat"].astype(np.float64).tobytes())
+        for k in ("svel", "m11", "m12", "m22"):
+            f.write(np.ascontiguousarray(np.transpose(b[k], (2, 1, 0))).tobytes())  # [ix][iy][iper]
+        for e in range(ne):
+            f.write(np.array([b["ev_iper"][e], b["ev_nrec"][e]], dtype=np.int32).tobytes())
+            f.write(np.array([b["ev_lon"][e], b["ev_lat"][e]], dtype=np.float64).tobytes())
+        for k in ("rec_lon", "rec_lat", "tt_obs", "weight"):
+            f.write(b[k].astype(np.float64).tobytes())
+    out = tmp_path / "out.bin"
+    exe = _build(tmp_path, "shim_batch_driver")
+    r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = np.fromfile(out, dtype=np.float64)
+    chi, syn = res[0], res[1:1 + nrec]
+    K = res[1 + nrec:].reshape(nper, ny, nx)
+    ro = O.forward_for_event(b)
+    assert RC.rel_inf(syn, ro["syn"]) <= 1e-12
+    assert RC.rel_inf(K, ro["adj_s"]) <= 1e-12
+    assert abs(chi - ro["chi"]) <= 1e-12 * abs(ro["chi"])
