@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-pairs", type=int, default=30, help="pairs timed in per-solve FP64 mode")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the uneven-topography and 401x481 workloads (reported under 'workloads')")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device twice (the plain copy "
                          "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -155,6 +157,8 @@ def main():
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
+    extra = None if args.no_extra else extra_workloads()
+
     cpu = None
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
@@ -190,10 +194,62 @@ def main():
                     "vs_cpu": (round(value / cpu["value"], 1) if cpu else None)},
         "iters": {"fwd_mean": float(iters_f.mean()), "adj_mean": float(iters_a.mean()),
                   "fwd_hist": np.bincount(iters_f).tolist(), "adj_hist": np.bincount(iters_a).tolist()},
+        "workloads": extra,
     }
     print(json.dumps(line), flush=True)
     if args.calib_copy_mib > 0:
         calib_copy(args.calib_copy_mib << 20)
+
+
+def _kernel_err(r32, r64, ngrid):
+    k32 = r32["adj_s"] if ngrid > 1 else [r32["adj_s"]]
+    k64 = r64["adj_s"] if ngrid > 1 else [r64["adj_s"]]
+    return max(float(np.abs(a[p] - c[p]).max() / np.abs(c[p]).max())
+               for a, c in zip(k32, k64) for p in range(c.shape[0]))
+
+
+def _time_pair(grids, runs=2):
+    """FP64 (one run after a warm-up) and FP32 (`runs` runs after a warm-up) of one batch:
+    pairs/s of each, kernel and syn error of FP32 against FP64, FP32 phases."""
+    from surfattpp_amd import PREC_F32, PREC_F64
+    from surfattpp_amd import preproc as PP
+    ngrid = len(grids) if isinstance(grids, list) else 1
+    with PP.EikonalBatch(grids, precision=PREC_F64) as b64:
+        b64.run()
+        b64.run()
+        ms64 = b64.last_ms()[0]
+        r64 = b64.results()
+    with PP.EikonalBatch(grids, precision=PREC_F32) as b32:
+        b32.run()
+        t = []
+        for _ in range(runs):
+            b32.run()
+            t.append(b32.last_ms()[0])
+        ph = b32.phase_ms()
+        r32 = b32.results()
+        nres = b32.conv()[2]
+    ne = len(r64["iters_fwd"])
+    ms32 = float(np.mean(t))
+    return {"pairs": ne, "value_f32": round(ne / (ms32 * 1e-3), 2), "ms_f32": round(ms32, 3),
+            "value_f64": round(ne / (ms64 * 1e-3), 2), "ms_f64": round(ms64, 3),
+            "err_vs_fp64": {"kernel_max_per_period": _kernel_err(r32, r64, ngrid),
+                            "syn": float(np.abs(r32["syn"] - r64["syn"]).max() / np.abs(r64["syn"]).max())},
+            "iters_fwd_equal": bool(np.array_equal(r32["iters_fwd"], r64["iters_fwd"])),
+            "fp64_resolves": int(nres), "phases_ms_f32": {k: round(v, 3) for k, v in ph.items()}}
+
+
+def extra_workloads():
+    """The uneven topography batch (101^2, 161^2, 241^2, 321^2 at 1.5 km, 12 periods,
+    anisotropic metric), the 121x141 batch (25 sources x 20 periods) and the 401x481
+    batch (400 sources x 30 periods = 12,000 pairs)."""
+    from surfattpp_amd import synth
+    topo = synth.config_topo()
+    out = {"topo-uneven-101..321-1.5km-12per": dict(
+        _time_pair(topo), grids=[[g["lon"].size, g["lat"].size, len(g["ev_iper"])] for g in topo])}
+    del topo
+    out["disp-121x141-2km-25src-20per"] = _time_pair(synth.config_dispersion_shape())
+    out["large-401x481-1km-400src-30per"] = _time_pair(synth.config_large())
+    return out
 
 
 def calib_copy(nbytes: int) -> None:

@@ -91,6 +91,10 @@ typedef struct sat_grid_desc {
   const double* m11;     /* [nperiod][ny][nx] metric (surf_grid.cpp:184-190) */
   const double* m12;     /* [nperiod][ny][nx]; the solver is called with sphc = -m12 */
   const double* m22;     /* [nperiod][ny][nx] */
+  int waves;             /* waves (64 lanes each) per pair in the batched sweeps of FP32
+                            batches: 1, 2 or 4; 0: chosen by the library (more waves for the
+                            largest grids while the batch stays within three waves per SIMD).
+                            FP64 batches: 0 or 1.  Results do not depend on it. */
 } sat_grid_desc;
 
 typedef struct sat_event_desc {
@@ -133,9 +137,13 @@ int sat_batch_update_media(sat_batch* b, int grid, const double* svel, const dou
 int sat_batch_update_obs(sat_batch* b, const double* tt_obs, const double* weight);
 /* Queue the whole pipeline (forward -> misfit -> adjoint -> mask -> kernel
  * accumulate -> chi) on the batch's stream.  mode: SAT_RUN_BATCHED (one
- * persistent launch) or SAT_RUN_PER_SOLVE (one launch per pair, same kernels). */
+ * persistent launch), SAT_RUN_PER_SOLVE (one launch per pair, same kernels) or
+ * SAT_RUN_GRAPH (the batched run captured once as a hipGraph and replayed: the
+ * same work and results, one launch on the stream; set_f32_band or a change of
+ * a grid's isotropy re-captures it; per-phase times are not available). */
 #define SAT_RUN_BATCHED 0
 #define SAT_RUN_PER_SOLVE 1
+#define SAT_RUN_GRAPH 2
 int sat_batch_run(sat_batch* b, int mode);
 int sat_batch_sync(sat_batch* b);
 /* Copy results to host.  syn[nrec_total]; adj_s: for each grid in order,
@@ -155,11 +163,13 @@ int sat_batch_conv(sat_batch* b, double* linf_last, double* linf_prev, int* n_re
  * batch's streams; solve_ms: from the start to the end of the adjoint sweeps. */
 int sat_batch_last_ms(sat_batch* b, double* total_ms, double* solve_ms);
 /* Per-phase ms of the last SAT_RUN_BATCHED run (HIP events on the streams the
- * kernels run on), first min(n, 7) of: [0] analytic factor T0, [1] forward
+ * kernels run on; NaN except the total after a SAT_RUN_GRAPH run), first min(n, 7) of: [0] analytic factor T0, [1] forward
  * sweeps, [2] receivers + adjoint set-up + delta, [3] adjoint sweeps, [4] FP64
  * re-solves (second stream, concurrent with [2]-[3]), [5] kernel reduction +
  * chi, [6] total.  Timing aid with no counterpart in the reference. */
 int sat_batch_phase_ms(sat_batch* b, double* ms, int n);
+/* Waves per pair of each grid's batched sweeps, as chosen at creation: waves[ngrid]. */
+int sat_batch_waves(sat_batch* b, int* waves);
 int sat_batch_destroy(sat_batch* b);
 
 /* Library identity: device name, build arch; returns SAT_EDEVICE without a GPU. */

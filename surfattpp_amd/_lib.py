@@ -19,6 +19,7 @@ PREC_F64 = 0
 PREC_F32 = 1
 RUN_BATCHED = 0
 RUN_PER_SOLVE = 1
+RUN_GRAPH = 2
 FLAG_NOT_CONVERGED = 1
 FLAG_SRC_ON_BOUNDARY = 2
 FLAG_ADJ_MAXITER = 4
@@ -29,7 +30,7 @@ EXPORTS = (
     "sat_eik_forward", "sat_eik_backward", "sat_eik_mask",
     "sat_batch_create", "sat_batch_update_media", "sat_batch_update_obs", "sat_batch_run",
     "sat_batch_sync", "sat_batch_results", "sat_batch_last_ms", "sat_batch_destroy",
-    "sat_batch_set_f32_band", "sat_batch_conv", "sat_batch_phase_ms",
+    "sat_batch_set_f32_band", "sat_batch_conv", "sat_batch_phase_ms", "sat_batch_waves",
     "sat_device_info",
 )
 
@@ -48,7 +49,8 @@ class sat_info(C.Structure):
 class sat_grid_desc(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nperiod", C.c_int),
                 ("lon_deg", C.c_void_p), ("lat_deg", C.c_void_p),
-                ("svel", C.c_void_p), ("m11", C.c_void_p), ("m12", C.c_void_p), ("m22", C.c_void_p)]
+                ("svel", C.c_void_p), ("m11", C.c_void_p), ("m12", C.c_void_p), ("m22", C.c_void_p),
+                ("waves", C.c_int)]
 
 
 class sat_event_desc(C.Structure):
@@ -93,6 +95,7 @@ def load(path: str = LIB_PATH):
     L.sat_batch_set_f32_band.argtypes = [vp, C.c_double]
     L.sat_batch_conv.argtypes = [vp, vp, vp, vp]
     L.sat_batch_phase_ms.argtypes = [vp, dp, C.c_int]
+    L.sat_batch_waves.argtypes = [vp, vp]
     L.sat_device_info.argtypes = [C.c_char_p, C.c_int]
     for name in EXPORTS:
         getattr(L, name).restype = C.c_int

@@ -20,6 +20,7 @@
 #include <new>
 #include <numeric>
 #include <type_traits>
+#include <tuple>
 #include <vector>
 
 #include "../../include/surfatt_eik.h"
@@ -159,6 +160,25 @@ __global__ void reduce_kernel(ReduceDev rd) {
   }
 }
 
+// Start-of-run state: work-queue heads and list lengths, re-solve marks and
+// iteration counts zeroed, flags from the creation-time flags.  A kernel, not
+// memset/memcpy nodes, so that the run can be captured in a graph and replayed.
+__global__ void reset_kernel(int* counters, size_t n_counters, unsigned char* skip, int* iters, int* flags,
+                             const int* base_flags, size_t ne) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_counters + ne;
+       i += (size_t)gridDim.x * blockDim.x) {
+    if (i < n_counters) {
+      counters[i] = 0;
+    } else {
+      const size_t e = i - n_counters;
+      skip[e] = 0;
+      iters[e] = 0;
+      iters[ne + e] = 0;
+      flags[e] = base_flags[e];
+    }
+  }
+}
+
 __global__ void chi_kernel(const double* chi_ev, int n, double* chi) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     double c = 0.0;
@@ -232,6 +252,8 @@ struct GridHost {
   char* d_wmed = nullptr;     // batched sweeps: wavefront-major media, per orientation
   char* d_wmedmw = nullptr;   // FP32 batches: FP64 wavefront-major media of the re-solve geometry
   GridDev gmw{};              // FP32 batches: FP64 media, kMultiWave waves per grid (re-solves)
+  GridDev gseg{};             // g with one wave per pair (per-pair kernels; equals g when W == 1)
+  int W = 1;                  // waves per pair of the batched sweeps
 
   void release() {
     void* ps[] = {d_lon, d_lat, d_xx, d_yy, d_model, d_fp, d_aA, d_b, d_c, d_v3, d_seg, d_seg64, d_trig_yy, d_rowtab,
@@ -244,8 +266,10 @@ struct GridHost {
 
 static size_t elem_size(int prec) { return prec == SAT_PREC_F64 ? sizeof(double) : sizeof(float); }
 
+// wave: also the wavefront-major media of the batched sweeps, laid out for W
+// waves per pair (W * 64 lanes, ceil(ceil(nx / 64) / W) columns per lane).
 static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const double* lon, const double* lat,
-                      bool wave) {
+                      bool wave, int W = 1) {
   if (nx < 3 || ny < 3 || nper < 1 || !lon || !lat) return SAT_EINVAL;
   if (!uniform_axis(lon, nx) || !uniform_axis(lat, ny)) return SAT_EINVAL;
   const Layout L = layout_for(nx);
@@ -295,6 +319,13 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
   G.g64 = g;
   if (G.d_seg64)
     for (int i = 0; i < 6; i++) G.g64.seg[i] = G.d_seg64 + (size_t)i * seg64_bytes;
+  G.gseg = g;  // one wave per pair: the segmented layout of the per-pair kernels
+  G.W = W;
+  if (wave && W > 1) {  // the batched sweeps' wave geometry (the segmented pitch P stays)
+    g.C = (L.C + W - 1) / W;
+    g.La = (nx + g.C - 1) / g.C;
+    g.wplane = (size_t)(ny + g.La - 1) * g.La * g.C;
+  }
   if (wave) {  // padding entries stay zero
     const int nmed = prec == SAT_PREC_F64 ? 4 : 6;
     const size_t pb = (size_t)nper * g.wplane * elem_size(prec);
@@ -337,6 +368,7 @@ static int grid_media(GridHost& G, int mode, const double* p0, const double* p1,
     G.g.iso = iso ? 1 : 0;
     G.g64.iso = G.g.iso;
     G.gmw.iso = G.g.iso;
+    G.gseg.iso = G.g.iso;
   }
   for (int i = 0; i < 4; i++) {
     SAT_CHECK(hipMemcpyAsync(G.d_model + i * n, src[i], sizeof(double) * n, hipMemcpyHostToDevice, stream));
@@ -471,6 +503,7 @@ struct sat_batch {
   std::vector<EventX> hx;          // host copy of the phase events
   std::vector<int> base_flags;     // per event, set at creation (source on the boundary)
   GridDev* d_grids = nullptr;
+  GridDev* d_grids_seg = nullptr;  // one-wave geometry of the per-pair kernels
   GridDev* d_grids64 = nullptr;    // FP32 batches: grids with FP64 media (per-pair re-solves)
   GridDev* d_grids64mw = nullptr;  // FP32 batches: FP64 media, re-solve geometry (batched re-solves)
   EventDev* d_events = nullptr;
@@ -517,7 +550,9 @@ struct sat_batch {
   double* d_K = nullptr;
   size_t K_total = 0;
   // launch groups: events grouped by (columns per lane C, isotropic grid)
-  std::vector<int> group_C, group_iso, group_first, group_count;
+  // group_C: columns per lane of one wave per pair (per-pair kernels, re-solves);
+  // group_W, group_wC: waves per pair and columns per lane of the batched sweeps
+  std::vector<int> group_C, group_iso, group_first, group_count, group_W, group_wC;
   // per group: sweep slots [group_slot0, group_slot0 + group_nblk); launch order
   // of the groups (most work first)
   std::vector<int> group_slot0, group_nblk, launch_order;
@@ -538,28 +573,38 @@ struct sat_batch {
   hipStream_t stream2 = nullptr;   // FP64 re-solves, concurrent with the FP32 adjoint phase
   // Launch groups of different grid shapes run concurrently: group k of the
   // launch order on `stream` (k % (kSide+1) == 0) or side[k % (kSide+1) - 1].
+  // The FP64 re-solves fan out from stream2 the same way (side2).
   static constexpr int kSide = 3;
-  hipStream_t side[kSide] = {};
-  hipEvent_t fork_ev = nullptr;
-  hipEvent_t join_ev[kSide] = {};
+  hipStream_t side[kSide] = {}, side2[kSide] = {};
+  hipEvent_t fork_ev = nullptr, fork2_ev = nullptr;
+  hipEvent_t join_ev[kSide] = {}, join2_ev[kSide] = {};
   // time marks: start, init done, forward sweeps done, adjoint set-up done,
   // adjoint sweeps done, re-solves start / end (stream2), end
   static constexpr int kMarks = 8;
   hipEvent_t mk[kMarks] = {};
+  // SAT_RUN_GRAPH: the captured batched run and its executable instance
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool last_graph = false;
 
   void release() {
     for (auto& G : grids) G.release();
-    void* ps[] = {d_grids, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
+    void* ps[] = {d_grids, d_grids_seg, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
                   d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_ov, d_evx64, d_work64, d_slots64, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
                   d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3};
     for (void* p : ps)
       if (p) (void)hipFree(p);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
     for (hipEvent_t e : mk)
       if (e) (void)hipEventDestroy(e);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (fork2_ev) (void)hipEventDestroy(fork2_ev);
     for (int i = 0; i < kSide; i++) {
       if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
+      if (join2_ev[i]) (void)hipEventDestroy(join2_ev[i]);
       if (side[i]) (void)hipStreamDestroy(side[i]);
+      if (side2[i]) (void)hipStreamDestroy(side2[i]);
     }
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);
@@ -574,9 +619,11 @@ int build_groups(sat_batch* b) {
   std::vector<int> order(ne);
   std::iota(order.begin(), order.end(), 0);
   auto key = [&](int e) {
-    const GridDev& g = b->grids[b->he[e].grid].g;
-    return std::make_pair(g.C, g.iso);
+    const GridHost& G = b->grids[b->he[e].grid];
+    return std::make_tuple(G.W, G.gseg.C, G.g.iso);
   };
+  b->group_W.clear();
+  b->group_wC.clear();
   b->group_C.clear();
   b->group_iso.clear();
   b->group_first.clear();
@@ -598,8 +645,10 @@ int build_groups(sat_batch* b) {
       const auto k = key(order[i]);
       int j = i;
       while (j < c0 + cn && key(order[j]) == k) j++;
-      b->group_C.push_back(k.first);
-      b->group_iso.push_back(k.second);
+      b->group_W.push_back(std::get<0>(k));
+      b->group_C.push_back(std::get<1>(k));
+      b->group_wC.push_back((std::get<1>(k) + std::get<0>(k) - 1) / std::get<0>(k));
+      b->group_iso.push_back(std::get<2>(k));
       b->group_first.push_back(i);
       b->group_count.push_back(j - i);
       i = j;
@@ -628,17 +677,63 @@ int build_groups(sat_batch* b) {
     b->chunk_g0.push_back((int)g_hi);
   }
   if (ne) SAT_CHECK(hipMemcpy(b->d_order, order.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
-  std::vector<GridDev> hg(b->grids.size()), hg64(b->grids.size()), hgmw(b->grids.size());
+  std::vector<GridDev> hg(b->grids.size()), hg64(b->grids.size()), hgmw(b->grids.size()), hgs(b->grids.size());
   for (size_t gi = 0; gi < b->grids.size(); gi++) {
     hg[gi] = b->grids[gi].g;
     hg64[gi] = b->grids[gi].g64;
     hgmw[gi] = b->grids[gi].gmw;
+    hgs[gi] = b->grids[gi].gseg;
+    hgs[gi].iso = hg[gi].iso;
   }
   SAT_CHECK(hipMemcpy(b->d_grids, hg.data(), sizeof(GridDev) * hg.size(), hipMemcpyHostToDevice));
+  SAT_CHECK(hipMemcpy(b->d_grids_seg, hgs.data(), sizeof(GridDev) * hgs.size(), hipMemcpyHostToDevice));
   if (b->d_grids64) SAT_CHECK(hipMemcpy(b->d_grids64, hg64.data(), sizeof(GridDev) * hg64.size(), hipMemcpyHostToDevice));
   if (b->d_grids64mw)
     SAT_CHECK(hipMemcpy(b->d_grids64mw, hgmw.data(), sizeof(GridDev) * hgmw.size(), hipMemcpyHostToDevice));
   b->groups_dirty = false;
+  return SAT_OK;
+}
+
+// Waves per pair of each grid's batched sweeps.  FP64 batches: one (the
+// reference's operation order is kept by the one-wave kernels).  FP32 batches:
+// the descriptor's value (1, 2 or 4), or for 0 the library's choice: start at
+// one wave per pair and, while the batch's waves stay within three per SIMD
+// (the residency the FP32 forward sweeps are compiled for), double the waves
+// of the grid whose pairs take the most sweep steps
+// ((ny + La - 1) * C per sweep), so that batches too small to fill the device
+// split their longest pairs over more lanes.
+int choose_waves(const sat_batch_desc& d, bool f32, int cus, std::vector<int>& w) {
+  const int ng = d.ngrid;
+  w.assign(ng, 1);
+  std::vector<long> nev(ng, 0);
+  for (int e = 0; e < d.nevent; e++) nev[d.events[e].grid]++;
+  std::vector<bool> autow(ng, false);
+  for (int gi = 0; gi < ng; gi++) {
+    const int v = d.grids[gi].waves;
+    if (v != 0 && v != 1 && v != 2 && v != 4) return SAT_EINVAL;
+    if (v > 1 && !f32) return SAT_EINVAL;
+    if (v == 0) autow[gi] = f32;
+    else w[gi] = v;
+  }
+  auto steps = [&](int gi, int W) {  // sweep steps x columns of one pair
+    const int C = ((d.grids[gi].nx + kWave - 1) / kWave + W - 1) / W;
+    const int La = (d.grids[gi].nx + C - 1) / C;
+    return (double)(d.grids[gi].ny + La - 1) * C;
+  };
+  const long cap = 3L * 4 * cus;  // three waves per SIMD
+  long total = 0;
+  for (int gi = 0; gi < ng; gi++) total += nev[gi] * w[gi];
+  for (;;) {
+    int best = -1;
+    for (int gi = 0; gi < ng; gi++) {
+      if (!autow[gi] || nev[gi] == 0 || w[gi] >= 4 || d.grids[gi].nx <= kWave * w[gi]) continue;
+      if (total + nev[gi] * w[gi] > cap) continue;
+      if (best < 0 || steps(gi, w[gi]) > steps(best, w[best])) best = gi;
+    }
+    if (best < 0) break;
+    total += nev[best] * w[best];
+    w[best] *= 2;
+  }
   return SAT_OK;
 }
 
@@ -741,16 +836,26 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(hipStreamCreate(&b->stream2));
   for (int i = 0; i < sat_batch::kMarks; i++) B_CHECK(hipEventCreate(&b->mk[i]));
   B_CHECK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
+  B_CHECK(hipEventCreateWithFlags(&b->fork2_ev, hipEventDisableTiming));
   for (int i = 0; i < sat_batch::kSide; i++) {
     B_CHECK(hipStreamCreate(&b->side[i]));
+    B_CHECK(hipStreamCreate(&b->side2[i]));
     B_CHECK(hipEventCreateWithFlags(&b->join_ev[i], hipEventDisableTiming));
+    B_CHECK(hipEventCreateWithFlags(&b->join2_ev[i], hipEventDisableTiming));
   }
   b->grids.resize(d->ngrid);
+  std::vector<int> waves;
+  {
+    hipDeviceProp_t prop;
+    B_CHECK(hipGetDeviceProperties(&prop, 0));
+    rc = choose_waves(*d, f32, prop.multiProcessorCount, waves);
+    if (rc) return fail(rc);
+  }
   size_t max_seg = 0, max_plane = 0, max_wplane = 0;
   int max_nx = 0;
   for (int gi = 0; gi < d->ngrid; gi++) {
     const sat_grid_desc& gd = d->grids[gi];
-    rc = grid_setup(b->grids[gi], b->precision, gd.nx, gd.ny, gd.nperiod, gd.lon_deg, gd.lat_deg, true);
+    rc = grid_setup(b->grids[gi], b->precision, gd.nx, gd.ny, gd.nperiod, gd.lon_deg, gd.lat_deg, true, waves[gi]);
     if (rc) return fail(rc);
     rc = grid_media(b->grids[gi], 0, gd.svel, gd.m11, gd.m12, gd.m22, b->stream);
     if (rc) return fail(rc);
@@ -764,6 +869,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   b->max_wplane = max_wplane;
   b->max_nx = max_nx;
   B_CHECK(dalloc(&b->d_grids, d->ngrid));
+  B_CHECK(dalloc(&b->d_grids_seg, d->ngrid));
   if (f32) B_CHECK(dalloc(&b->d_grids64, d->ngrid));
   if (f32) B_CHECK(dalloc(&b->d_grids64mw, d->ngrid));
 
@@ -920,8 +1026,10 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
     B_CHECK(dalloc(&b->d_evx64, std::max(ne, 1)));
     if (ne) B_CHECK(hipMemcpy(b->d_evx64, b->hx.data(), sizeof(EventX) * ne, hipMemcpyHostToDevice));
     b->nslots64p = std::min(b->pool_size, 2 * prop.multiProcessorCount);
-    B_CHECK(dalloc(&b->d_slots64, b->slot_bytes64 * (size_t)b->nslots64p));
-    B_CHECK(hipMemset(b->d_slots64, 0, b->slot_bytes64 * (size_t)b->nslots64p));
+    // concurrent re-solve groups: each group's share rounded up
+    const size_t nslots64_alloc = (size_t)b->nslots64p + kMaxGroups;
+    B_CHECK(dalloc(&b->d_slots64, b->slot_bytes64 * nslots64_alloc));
+    B_CHECK(hipMemset(b->d_slots64, 0, b->slot_bytes64 * nslots64_alloc));
   }
 
   // fused path: one slot for per-solve runs; FP32 batches also FP64 slots for re-solves
@@ -1011,12 +1119,12 @@ int sat_batch_update_obs(sat_batch* b, const double* tt_obs, const double* weigh
   return SAT_OK;
 }
 
-int sat_batch_run(sat_batch* b, int mode) {
-  if (!b || (mode != SAT_RUN_BATCHED && mode != SAT_RUN_PER_SOLVE)) return SAT_EINVAL;
-  if (b->groups_dirty) {
-    const int rc = build_groups(b);
-    if (rc) return rc;
-  }
+namespace {
+
+// Queue one run of the pipeline on the batch's streams.  capturing: the calls
+// are being recorded into a graph; the timing-only marks are left out (the
+// events that order the streams stay).
+int enqueue_run(sat_batch* b, int mode, bool capturing) {
   const bool f32 = b->precision == SAT_PREC_F32;
   const int ne = b->nevent;
   const size_t nr = (size_t)std::max(b->nrec, 1);
@@ -1035,7 +1143,7 @@ int sat_batch_run(sat_batch* b, int mode) {
   const size_t n_counters = 8 * ne1 + 2 + b->chunk_first.size();
 
   BatchDev bd{};
-  bd.grids = b->d_grids;
+  bd.grids = b->d_grids_seg;
   bd.events = b->d_events;
   bd.order = b->d_order;
   bd.rec_lon = b->d_rec;
@@ -1116,11 +1224,12 @@ int sat_batch_run(sat_batch* b, int mode) {
   // columns per lane of the re-solve geometry: ceil(nx / (64 kMultiWave)) = ceil(C / kMultiWave)
   auto cmw = [&](size_t gi) { return (b->group_C[gi] + kMultiWave - 1) / kMultiWave; };
 
-  SAT_CHECK(hipEventRecord(mk[0], b->stream));
-  SAT_CHECK(hipMemsetAsync(b->d_counter, 0, sizeof(int) * n_counters, b->stream));
-  SAT_CHECK(hipMemsetAsync(b->d_skip, 0, ne1, b->stream));
-  SAT_CHECK(hipMemsetAsync(b->d_iters, 0, sizeof(int) * 2 * ne1, b->stream));
-  SAT_CHECK(hipMemcpyAsync(bd.flags, b->d_base_flags, sizeof(int) * ne1, hipMemcpyDeviceToDevice, b->stream));
+  auto mark = [&](int i, hipStream_t st) -> hipError_t {
+    return capturing ? hipSuccess : hipEventRecord(mk[i], st);
+  };
+  hipLaunchKernelGGL(reset_kernel, dim3((unsigned)std::min<size_t>((n_counters + ne1 + 255) / 256, 4096)), dim3(256),
+                     0, b->stream, b->d_counter, n_counters, b->d_skip, b->d_iters, bd.flags, b->d_base_flags, ne1);
+  SAT_CHECK(hipGetLastError());
 
   // The chunks run one after the other through the same workspace (one chunk
   // unless the batch exceeds the workspace cap).  The phase marks describe the
@@ -1138,6 +1247,34 @@ int sat_batch_run(sat_batch* b, int mode) {
     const dim3 node_grid(node_bx, (unsigned)std::max(cn, 1));
     const int rec_blocks = (int)std::max(1, std::min((nrec_c + 255) / 256, 4096));
 
+    // Launch fn(gi, stream) for every group of the chunk, concurrently: from the
+    // base stream (main or stream2) over its side streams, joined back at the end.
+    auto fan_out_on = [&](hipStream_t base, hipStream_t* sides, hipEvent_t fork, hipEvent_t* joins,
+                          auto&& fn) -> int {
+      const size_t ngc = g_hi - g_lo;
+      const int nused = (int)std::min<size_t>(ngc, sat_batch::kSide + 1);
+      if (nused > 1) {
+        SAT_CHECK(hipEventRecord(fork, base));
+        for (int k = 1; k < nused; k++) SAT_CHECK(hipStreamWaitEvent(sides[k - 1], fork, 0));
+      }
+      for (size_t k = 0; k < ngc; k++) {
+        const int sk = (int)(k % (size_t)nused);
+        const int rc = fn(b->launch_order[g_lo + k], sk == 0 ? base : sides[sk - 1]);
+        if (rc) return rc;
+      }
+      if (nused > 1) {
+        for (int k = 1; k < nused; k++) {
+          SAT_CHECK(hipEventRecord(joins[k - 1], sides[k - 1]));
+          SAT_CHECK(hipStreamWaitEvent(base, joins[k - 1], 0));
+        }
+      }
+      return SAT_OK;
+    };
+    auto fan_out = [&](auto&& fn) -> int { return fan_out_on(b->stream, b->side, b->fork_ev, b->join_ev, fn); };
+    auto fan_out2 = [&](auto&& fn) -> int {
+      return fan_out_on(b->stream2, b->side2, b->fork2_ev, b->join2_ev, fn);
+    };
+
     // FP64 re-solves of the FP32 pairs whose stop decision was within the band,
     // on stream2 once the forward sweeps (which list them) are done.  Batched
     // runs: the FP64 phase kernels over the pool lists, the per-pair kernel over
@@ -1145,7 +1282,7 @@ int sat_batch_run(sat_batch* b, int mode) {
     auto launch_resolves = [&](bool phase) -> int {
       SAT_CHECK(hipEventRecord(mk[2], b->stream));
       SAT_CHECK(hipStreamWaitEvent(b->stream2, mk[2], 0));
-      SAT_CHECK(hipEventRecord(mk[5], b->stream2));
+      SAT_CHECK(mark(5, b->stream2));
       if (f32 && cn) {
         hipStream_t s2 = b->stream2;
         if (phase) {
@@ -1165,35 +1302,51 @@ int sat_batch_run(sat_batch* b, int mode) {
           q.slot_plane = b->slot_plane64;
           q.want_skip = 1;
           q.ta_f32 = 1;
-          for (size_t gi = g_lo; gi < g_hi; gi++) {
+          // the groups' re-solves run concurrently; each takes a share of the
+          // re-solve slots in proportion to its events, as the main sweeps do
+          auto res_slots = [&](int gi, PhaseDev& a, int ub) {
+            const int share = std::max(1, (int)(((long)b->nslots64p * b->group_count[gi] + cn - 1) / cn));
+            int base = 0;
+            for (size_t k = g_lo; k < (size_t)gi; k++)
+              base += std::max(1, (int)(((long)b->nslots64p * b->group_count[k] + cn - 1) / cn));
+            a.slots = b->d_slots64 + (size_t)base * b->slot_bytes64;
+            return dim3(std::min(ub, share));
+          };
+          int rc = fan_out2([&](int gi, hipStream_t st) -> int {
             const int ub = std::min(b->group_count[gi], b->pool_size);
             PhaseDev a = q;
             a.list = b->d_fb + b->group_first[gi];
             a.list_count = fb_count + gi;
-            SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(node_bx, ub), s2, a));
+            SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(node_bx, ub), st, a));
             a.counter = resf_heads + gi;
-            SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(0, cmw(gi), kMultiWave, dim3(std::min(ub, b->nslots64p)), s2,
-                                                     a, 0, 0, (int)gi));
-          }
+            const dim3 grid = res_slots(gi, a, ub);
+            SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(0, cmw(gi), kMultiWave, grid, st, a, 0, 0, gi));
+            return SAT_OK;
+          });
+          if (rc) return rc;
           if (nrec_c) SAT_CHECK(launch_recv(dim3(rec_blocks), s2, q, nrec_c));
           if (b->calc_adj) {
-            for (size_t gi = g_lo; gi < g_hi; gi++) {
+            rc = fan_out2([&](int gi, hipStream_t st) -> int {
               const int ub = std::min(b->group_count[gi], b->pool_size);
               PhaseDev a = q;
               a.list = b->d_fb + b->group_first[gi];
               a.list_count = fb_count + gi;
-              SAT_CHECK(launch_adj_setup<SAT_PREC_F64>(dim3(node_bx, ub), s2, a));
-            }
+              SAT_CHECK(launch_adj_setup<SAT_PREC_F64>(dim3(node_bx, ub), st, a));
+              return SAT_OK;
+            });
+            if (rc) return rc;
             SAT_CHECK(launch_delta<SAT_PREC_F64>(dim3((cn + 63) / 64), s2, q));
-            for (size_t gi = g_lo; gi < g_hi; gi++) {
+            rc = fan_out2([&](int gi, hipStream_t st) -> int {
               const int ub = std::min(b->group_count[gi], b->pool_size);
               PhaseDev a = q;
               a.list = b->d_fb + b->group_first[gi];
               a.list_count = fb_count + gi;
               a.counter = resa_heads + gi;
-              SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(cmw(gi), kMultiWave, dim3(std::min(ub, b->nslots64p)), s2, a,
-                                                       0, 0));
-            }
+              const dim3 grid = res_slots(gi, a, ub);
+              SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(cmw(gi), kMultiWave, grid, st, a, 0, 0));
+              return SAT_OK;
+            });
+            if (rc) return rc;
           }
         }
         for (size_t gi = g_lo; gi < g_hi; gi++) {
@@ -1212,29 +1365,6 @@ int sat_batch_run(sat_batch* b, int mode) {
         }
       }
       SAT_CHECK(hipEventRecord(mk[6], b->stream2));
-      return SAT_OK;
-    };
-
-    // Launch fn(gi, stream) for every group of the chunk, concurrently across
-    // the side streams.
-    auto fan_out = [&](auto&& fn) -> int {
-      const size_t ngc = g_hi - g_lo;
-      const int nused = (int)std::min<size_t>(ngc, sat_batch::kSide + 1);
-      if (nused > 1) {
-        SAT_CHECK(hipEventRecord(b->fork_ev, b->stream));
-        for (int k = 1; k < nused; k++) SAT_CHECK(hipStreamWaitEvent(b->side[k - 1], b->fork_ev, 0));
-      }
-      for (size_t k = 0; k < ngc; k++) {
-        const int sk = (int)(k % (size_t)nused);
-        const int rc = fn(b->launch_order[g_lo + k], sk == 0 ? b->stream : b->side[sk - 1]);
-        if (rc) return rc;
-      }
-      if (nused > 1) {
-        for (int k = 1; k < nused; k++) {
-          SAT_CHECK(hipEventRecord(b->join_ev[k - 1], b->side[k - 1]));
-          SAT_CHECK(hipStreamWaitEvent(b->stream, b->join_ev[k - 1], 0));
-        }
-      }
       return SAT_OK;
     };
 
@@ -1257,18 +1387,18 @@ int sat_batch_run(sat_batch* b, int mode) {
             SAT_CHECK(launch_pair<SAT_PREC_F32>(1, b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi, nullptr,
                                                 nullptr));
         }
-      SAT_CHECK(hipEventRecord(mk[1], b->stream));
+      SAT_CHECK(mark(1, b->stream));
       const int rc = launch_resolves(false);
       if (rc) return rc;
-      SAT_CHECK(hipEventRecord(mk[3], b->stream));
-      SAT_CHECK(hipEventRecord(mk[4], b->stream));
+      SAT_CHECK(mark(3, b->stream));
+      SAT_CHECK(mark(4, b->stream));
     } else {
       if (cn) {
         const dim3 col_grid((unsigned)((b->max_nx + 63) / 64), (unsigned)cn);
         SAT_CHECK(f32 ? launch_trig<SAT_PREC_F32>(col_grid, b->stream, pc) : launch_trig<SAT_PREC_F64>(col_grid, b->stream, pc));
         SAT_CHECK(f32 ? launch_init<SAT_PREC_F32>(node_grid, b->stream, pc) : launch_init<SAT_PREC_F64>(node_grid, b->stream, pc));
       }
-      SAT_CHECK(hipEventRecord(mk[1], b->stream));
+      SAT_CHECK(mark(1, b->stream));
       {
         const int rc = fan_out([&](int gi, hipStream_t st) -> int {
           PhaseDev a = pc;
@@ -1276,10 +1406,10 @@ int sat_batch_run(sat_batch* b, int mode) {
           a.fb_list = pc.fb_list + b->group_first[gi];
           a.ov_list = pc.ov_list + b->group_first[gi];
           a.slots = b->d_slots + (size_t)b->group_slot0[gi] * b->slot_bytes;
-          const int C = b->group_C[gi], iso = b->group_iso[gi];
-          SAT_CHECK(f32 ? launch_fwd_sweep<SAT_PREC_F32>(iso, C, 1, dim3(b->group_nblk[gi]), st, a,
+          const int C = b->group_wC[gi], W = b->group_W[gi], iso = b->group_iso[gi];
+          SAT_CHECK(f32 ? launch_fwd_sweep<SAT_PREC_F32>(iso, C, W, dim3(b->group_nblk[gi]), st, a,
                                                          b->group_first[gi], b->group_count[gi], gi)
-                        : launch_fwd_sweep<SAT_PREC_F64>(iso, C, 1, dim3(b->group_nblk[gi]), st, a,
+                        : launch_fwd_sweep<SAT_PREC_F64>(iso, C, W, dim3(b->group_nblk[gi]), st, a,
                                                          b->group_first[gi], b->group_count[gi], gi));
           return SAT_OK;
         });
@@ -1296,22 +1426,21 @@ int sat_batch_run(sat_batch* b, int mode) {
         const dim3 ev_grid((unsigned)((cn + 63) / 64));
         SAT_CHECK(f32 ? launch_delta<SAT_PREC_F32>(ev_grid, b->stream, pc) : launch_delta<SAT_PREC_F64>(ev_grid, b->stream, pc));
       }
-      SAT_CHECK(hipEventRecord(mk[3], b->stream));
+      SAT_CHECK(mark(3, b->stream));
       if (b->calc_adj && cn) {
         const int rc = fan_out([&](int gi, hipStream_t st) -> int {
           PhaseDev a = pc;
           a.counter = adj_heads + gi;
           a.slots = b->d_slots + (size_t)b->group_slot0[gi] * b->slot_bytes;
           const dim3 grid(b->group_nblk[gi]);
-          SAT_CHECK(f32 ? launch_adj_sweep<SAT_PREC_F32>(b->group_C[gi], 1, grid, st, a, b->group_first[gi],
-                                                         b->group_count[gi])
-                        : launch_adj_sweep<SAT_PREC_F64>(b->group_C[gi], 1, grid, st, a, b->group_first[gi],
-                                                         b->group_count[gi]));
+          const int C = b->group_wC[gi], W = b->group_W[gi];
+          SAT_CHECK(f32 ? launch_adj_sweep<SAT_PREC_F32>(C, W, grid, st, a, b->group_first[gi], b->group_count[gi])
+                        : launch_adj_sweep<SAT_PREC_F64>(C, W, grid, st, a, b->group_first[gi], b->group_count[gi]));
           return SAT_OK;
         });
         if (rc) return rc;
       }
-      SAT_CHECK(hipEventRecord(mk[4], b->stream));
+      SAT_CHECK(mark(4, b->stream));
     }
     // the chunk's re-solves are done before its Ta is reduced and before the
     // next chunk reuses the workspace
@@ -1330,7 +1459,53 @@ int sat_batch_run(sat_batch* b, int mode) {
   }
   hipLaunchKernelGGL(chi_kernel, dim3(1), dim3(1), 0, b->stream, b->d_chi_ev, b->calc_adj ? ne : 0, b->d_chi);
   SAT_CHECK(hipGetLastError());
-  SAT_CHECK(hipEventRecord(mk[7], b->stream));
+  return SAT_OK;
+}
+
+void drop_graph(sat_batch* b) {
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+}
+
+}  // namespace
+
+int sat_batch_run(sat_batch* b, int mode) {
+  if (!b || (mode != SAT_RUN_BATCHED && mode != SAT_RUN_PER_SOLVE && mode != SAT_RUN_GRAPH)) return SAT_EINVAL;
+  if (b->groups_dirty) {
+    drop_graph(b);
+    const int rc = build_groups(b);
+    if (rc) return rc;
+  }
+  if (mode != SAT_RUN_GRAPH) {
+    b->last_graph = false;
+    SAT_CHECK(hipEventRecord(b->mk[0], b->stream));
+    const int rc = enqueue_run(b, mode, false);
+    if (rc) return rc;
+    SAT_CHECK(hipEventRecord(b->mk[7], b->stream));
+    return SAT_OK;
+  }
+  if (!b->graph_exec) {
+    // Capture the batched run once: every kernel and stream dependency of the
+    // pipeline, chunks included; replays reuse it (the device inputs change
+    // in place: update_media / update_obs need no new capture).
+    SAT_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_run(b, SAT_RUN_BATCHED, true);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    SAT_CHECK(ec);
+    b->graph = g;
+    SAT_CHECK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+  }
+  b->last_graph = true;
+  SAT_CHECK(hipEventRecord(b->mk[0], b->stream));
+  SAT_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
+  SAT_CHECK(hipEventRecord(b->mk[7], b->stream));
   return SAT_OK;
 }
 
@@ -1344,7 +1519,8 @@ int sat_batch_last_ms(sat_batch* b, double* total_ms, double* solve_ms) {
   if (!b) return SAT_EINVAL;
   float t = 0.f, s = 0.f;
   SAT_CHECK(hipEventElapsedTime(&t, b->mk[0], b->mk[7]));
-  SAT_CHECK(hipEventElapsedTime(&s, b->mk[0], b->mk[4]));
+  if (b->last_graph) s = std::nanf("");  // a graph replay is timed as a whole
+  else SAT_CHECK(hipEventElapsedTime(&s, b->mk[0], b->mk[4]));
   if (total_ms) *total_ms = t;
   if (solve_ms) *solve_ms = s;
   return SAT_OK;
@@ -1355,6 +1531,10 @@ int sat_batch_phase_ms(sat_batch* b, double* ms, int n) {
   static const int from[7] = {0, 1, 2, 3, 5, 4, 0};
   static const int to[7] = {1, 2, 3, 4, 6, 7, 7};
   for (int i = 0; i < n && i < 7; i++) {
+    if (b->last_graph && i < 6) {  // a graph replay is timed as a whole
+      ms[i] = std::nan("");
+      continue;
+    }
     float t = 0.f;
     SAT_CHECK(hipEventElapsedTime(&t, b->mk[from[i]], b->mk[to[i]]));
     ms[i] = t;
@@ -1382,6 +1562,7 @@ int sat_batch_results(sat_batch* b, double* syn, double* adj_s, double* chi, int
 
 int sat_batch_set_f32_band(sat_batch* b, double band) {
   if (!b || !(band >= 0.0 && band < 1.0)) return SAT_EINVAL;
+  if (band != b->f32_band) drop_graph(b);  // the band is a value captured in the graph
   b->f32_band = band;
   return SAT_OK;
 }
@@ -1409,6 +1590,12 @@ int sat_batch_conv(sat_batch* b, double* linf_last, double* linf_prev, int* n_re
       for (int c : cnt) *n_resolved += c;
     }
   }
+  return SAT_OK;
+}
+
+int sat_batch_waves(sat_batch* b, int* waves) {
+  if (!b || !waves) return SAT_EINVAL;
+  for (size_t gi = 0; gi < b->grids.size(); gi++) waves[gi] = b->grids[gi].W;
   return SAT_OK;
 }
 

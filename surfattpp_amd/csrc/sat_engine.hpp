@@ -60,7 +60,14 @@ struct SweepGeom {
   int nx, ny, La, P, trash0;
 };
 
+// kStepRun: every lane's row of a step lies in one contiguous run of the
+// plane, and a row outside the grid (a lane before or after its part of the
+// sweep) addresses the plane's padding inside that run.  Such loads are issued
+// unclamped (for the inactive lanes at the address of the last active lane):
+// they touch only the cache lines the active lanes read anyway.  A clamped row
+// instead sends each idle lane to its own distant line, every step.
 struct SegLayout {
+  static constexpr bool kStepRun = false;
   __device__ static __forceinline__ SweepGeom geom(const GridDev& g) {
     return SweepGeom{g.nx, g.ny, g.La, g.P, (int)g.seg_plane};
   }
@@ -70,6 +77,7 @@ struct SegLayout {
   __device__ static __forceinline__ int trash(const SweepGeom& s, int lane) { return s.trash0 + lane * C; }
 };
 struct WaveLayout {
+  static constexpr bool kStepRun = true;
   __device__ static __forceinline__ SweepGeom geom(const GridDev& g) {
     return SweepGeom{g.nx, g.ny, g.La, g.P, (int)g.wplane};
   }
@@ -142,15 +150,27 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
   auto rowj = [&](int r) { return (YD > 0) ? r : (ny - 1 - r); };
   auto in_range = [&](int r) { return lane_ok && r >= 0 && r < ny; };
   // loads are issued for every lane and step; rows and lanes outside the grid
-  // read a clamped (in-bounds) location whose values are not used
+  // read an in-bounds location whose values are not used (see kStepRun)
   const int lane_c = imin(lane, La - 1);
+  const int lag_c = (XD > 0) ? lane_c : (La - 1 - lane_c);
+  auto row_base = [&](int r, int& j) {  // j: clamped row (row tables); returns the address
+    j = rowj(imin(imax(r, 0), ny - 1));
+    if constexpr (Lay::kStepRun) {
+      const int ta = imin(r + lag, nsteps - 1);  // the step this row belongs to
+      return Lay::template base<C, O>(g, rowj(ta - lag_c), lane_c);
+    } else {
+      return Lay::template base<C, O>(g, j, lane_c);
+    }
+  };
   auto load_head = [&](Head& H, int r) {
-    const int j = rowj(imin(imax(r, 0), ny - 1));
-    Pol::template load_head<C, O>(cx, H, Lay::template base<C, O>(g, j, lane_c), j, tin);
+    int j;
+    const int base = row_base(r, j);
+    Pol::template load_head<C, O>(cx, H, base, j, tin);
   };
   auto load_body = [&](Body& B, int r) {
-    const int j = rowj(imin(imax(r, 0), ny - 1));
-    Pol::template load_body<C, O, NORM>(cx, B, Lay::template base<C, O>(g, j, lane_c), j, tst);
+    int j;
+    const int base = row_base(r, j);
+    Pol::template load_body<C, O, NORM>(cx, B, base, j, tst);
   };
 
   Head cur_h, nxt_h, nn_h;
@@ -271,9 +291,16 @@ __device__ __forceinline__ void sweep_ring(const typename Pol::Ctx& cx, const ty
   auto rowj = [&](int r) { return (YD > 0) ? r : (ny - 1 - r); };
   auto in_range = [&](int r) { return lane_ok && r >= 0 && r < ny; };
   const int lane_c = imin(lane, La - 1);
-  auto load = [&](Head& H, Body& B, int r) {  // unconditional, clamped (see kTrashElems)
+  const int lag_c = (XD > 0) ? lane_c : (La - 1 - lane_c);
+  auto load = [&](Head& H, Body& B, int r) {  // unconditional, in bounds (see kTrashElems, kStepRun)
     const int j = rowj(imin(imax(r, 0), ny - 1));
-    const int base = Lay::template base<C, O>(g, j, lane_c);
+    int base;
+    if constexpr (Lay::kStepRun) {
+      const int ta = imin(r + lag, nsteps - 1);
+      base = Lay::template base<C, O>(g, rowj(ta - lag_c), lane_c);
+    } else {
+      base = Lay::template base<C, O>(g, j, lane_c);
+    }
     Pol::template load_head<C, O>(cx, H, base, j, tin);
     Pol::template load_body<C, O, NORM>(cx, B, base, j, tst);
   };

@@ -18,12 +18,16 @@ template <int PREC>
 hipError_t launch_trig(dim3 grid, hipStream_t s, const PhaseDev& pd);
 template <int PREC>
 hipError_t launch_init(dim3 grid, hipStream_t s, const PhaseDev& pd);
-// Sweep kernels: W = 1 (one wave per event, C in [1, kMaxC]) or, FP64 only,
-// W = kMultiWave (a work-group of kMultiWave waves per event, C in [1, 2]).
+// Sweep kernels: W = 1 (one wave per event, C in [1, kMaxC]) or a work-group
+// of W waves per event: FP64 W = kMultiWave, C in [1, 2] (the re-solves of FP32
+// batches); FP32 W = 2 (C in [1, 4]) or 4 (C in [1, 2]).
 constexpr int kMultiWave = 4;
 template <int PREC>
 hipError_t launch_fwd_sweep(int iso, int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first,
                             int count, int group);
+template <int PREC>
+hipError_t launch_fwd_sweep_mw(int iso, int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first,
+                               int count, int group);
 hipError_t launch_recv(dim3 grid, hipStream_t s, const PhaseDev& pd, int nrec);
 template <int PREC>
 hipError_t launch_adj_setup(dim3 grid, hipStream_t s, const PhaseDev& pd);
@@ -31,6 +35,8 @@ template <int PREC>
 hipError_t launch_delta(dim3 grid, hipStream_t s, const PhaseDev& pd);
 template <int PREC>
 hipError_t launch_adj_sweep(int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count);
+template <int PREC>
+hipError_t launch_adj_sweep_mw(int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count);
 
 // Run f.template operator()<C>() for C in [1, kMaxC].
 template <class F>

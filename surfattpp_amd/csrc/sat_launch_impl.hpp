@@ -41,21 +41,7 @@ hipError_t launch_init(dim3 grid, hipStream_t s, const PhaseDev& pd) {
 template <int PREC>
 hipError_t launch_fwd_sweep(int iso, int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first,
                             int count, int group) {
-  if (W == kMultiWave) {
-    if constexpr (PREC == SAT_PREC_F64) {
-      if (C == 1)
-        hipLaunchKernelGGL((fwd_sweep_kernel<PREC, 1, false, kMultiWave>), grid, dim3(kWave * kMultiWave), 0, s,
-                           pd, first, count, group);
-      else if (C == 2)
-        hipLaunchKernelGGL((fwd_sweep_kernel<PREC, 2, false, kMultiWave>), grid, dim3(kWave * kMultiWave), 0, s,
-                           pd, first, count, group);
-      else
-        return hipErrorInvalidValue;
-      return hipGetLastError();
-    }
-    return hipErrorInvalidValue;
-  }
-  if (W != 1) return hipErrorInvalidValue;
+  if (W != 1) return launch_fwd_sweep_mw<PREC>(iso, C, W, grid, s, pd, first, count, group);
   return with_C(C, [&]<int CC>() {
     if constexpr (PREC == SAT_PREC_F32) {
       if (iso) {
@@ -65,6 +51,38 @@ hipError_t launch_fwd_sweep(int iso, int C, int W, dim3 grid, hipStream_t s, con
     }
     hipLaunchKernelGGL((fwd_sweep_kernel<PREC, CC, false, 1>), grid, dim3(kWave), 0, s, pd, first, count, group);
   });
+}
+
+// Multi-wave sweeps (a work-group of W waves per event): FP64 W = kMultiWave,
+// C <= 2, general metric (the re-solves of FP32 batches); FP32 W = 2 (C <= 4)
+// and W = 4 (C <= 2), both metrics.
+template <int PREC, int W, int C, bool ISO>
+static void launch_fwd_mw1(dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count, int group) {
+  hipLaunchKernelGGL((fwd_sweep_kernel<PREC, C, ISO, W>), grid, dim3(kWave * W), 0, s, pd, first, count, group);
+}
+
+template <int PREC>
+hipError_t launch_fwd_sweep_mw(int iso, int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first,
+                               int count, int group) {
+  if constexpr (PREC == SAT_PREC_F64) {
+    if (W != kMultiWave || iso) return hipErrorInvalidValue;
+    if (C == 1) launch_fwd_mw1<PREC, kMultiWave, 1, false>(grid, s, pd, first, count, group);
+    else if (C == 2) launch_fwd_mw1<PREC, kMultiWave, 2, false>(grid, s, pd, first, count, group);
+    else return hipErrorInvalidValue;
+  } else {
+    auto go = [&]<int WW, int CC>() {
+      if (iso) launch_fwd_mw1<PREC, WW, CC, true>(grid, s, pd, first, count, group);
+      else launch_fwd_mw1<PREC, WW, CC, false>(grid, s, pd, first, count, group);
+    };
+    if (W == 2 && C == 1) go.template operator()<2, 1>();
+    else if (W == 2 && C == 2) go.template operator()<2, 2>();
+    else if (W == 2 && C == 3) go.template operator()<2, 3>();
+    else if (W == 2 && C == 4) go.template operator()<2, 4>();
+    else if (W == 4 && C == 1) go.template operator()<4, 1>();
+    else if (W == 4 && C == 2) go.template operator()<4, 2>();
+    else return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 template <int PREC>
@@ -81,24 +99,32 @@ hipError_t launch_delta(dim3 grid, hipStream_t s, const PhaseDev& pd) {
 
 template <int PREC>
 hipError_t launch_adj_sweep(int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count) {
-  if (W == kMultiWave) {
-    if constexpr (PREC == SAT_PREC_F64) {
-      if (C == 1)
-        hipLaunchKernelGGL((adj_sweep_kernel<PREC, 1, kMultiWave>), grid, dim3(kWave * kMultiWave), 0, s, pd, first,
-                           count);
-      else if (C == 2)
-        hipLaunchKernelGGL((adj_sweep_kernel<PREC, 2, kMultiWave>), grid, dim3(kWave * kMultiWave), 0, s, pd, first,
-                           count);
-      else
-        return hipErrorInvalidValue;
-      return hipGetLastError();
-    }
-    return hipErrorInvalidValue;
-  }
-  if (W != 1) return hipErrorInvalidValue;
+  if (W != 1) return launch_adj_sweep_mw<PREC>(C, W, grid, s, pd, first, count);
   return with_C(C, [&]<int CC>() {
     hipLaunchKernelGGL((adj_sweep_kernel<PREC, CC, 1>), grid, dim3(kWave), 0, s, pd, first, count);
   });
+}
+
+template <int PREC>
+hipError_t launch_adj_sweep_mw(int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count) {
+  auto go = [&]<int WW, int CC>() {
+    hipLaunchKernelGGL((adj_sweep_kernel<PREC, CC, WW>), grid, dim3(kWave * WW), 0, s, pd, first, count);
+  };
+  if constexpr (PREC == SAT_PREC_F64) {
+    if (W != kMultiWave) return hipErrorInvalidValue;
+    if (C == 1) go.template operator()<kMultiWave, 1>();
+    else if (C == 2) go.template operator()<kMultiWave, 2>();
+    else return hipErrorInvalidValue;
+  } else {
+    if (W == 2 && C == 1) go.template operator()<2, 1>();
+    else if (W == 2 && C == 2) go.template operator()<2, 2>();
+    else if (W == 2 && C == 3) go.template operator()<2, 3>();
+    else if (W == 2 && C == 4) go.template operator()<2, 4>();
+    else if (W == 4 && C == 1) go.template operator()<4, 1>();
+    else if (W == 4 && C == 2) go.template operator()<4, 2>();
+    else return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 }  // namespace sat
@@ -108,10 +134,20 @@ hipError_t launch_adj_sweep(int C, int W, dim3 grid, hipStream_t s, const PhaseD
                                           const int*, const int*);                                         \
   template hipError_t sat::launch_solo<P>(int, int, hipStream_t, const SoloDev&);
 
+// the multi-wave launchers are instantiated in their own translation units
 #define SAT_INSTANTIATE_PHASE(P)                                                                           \
+  extern template hipError_t sat::launch_fwd_sweep_mw<P>(int, int, int, dim3, hipStream_t, const PhaseDev&, \
+                                                         int, int, int);                                   \
+  extern template hipError_t sat::launch_adj_sweep_mw<P>(int, int, dim3, hipStream_t, const PhaseDev&, int, \
+                                                         int);                                             \
   template hipError_t sat::launch_trig<P>(dim3, hipStream_t, const PhaseDev&);                             \
   template hipError_t sat::launch_init<P>(dim3, hipStream_t, const PhaseDev&);                             \
   template hipError_t sat::launch_fwd_sweep<P>(int, int, int, dim3, hipStream_t, const PhaseDev&, int, int, int); \
   template hipError_t sat::launch_adj_setup<P>(dim3, hipStream_t, const PhaseDev&);                        \
   template hipError_t sat::launch_delta<P>(dim3, hipStream_t, const PhaseDev&);                            \
   template hipError_t sat::launch_adj_sweep<P>(int, int, dim3, hipStream_t, const PhaseDev&, int, int);
+
+#define SAT_INSTANTIATE_MW(P)                                                                              \
+  template hipError_t sat::launch_fwd_sweep_mw<P>(int, int, int, dim3, hipStream_t, const PhaseDev&, int, int, \
+                                                  int);                                                    \
+  template hipError_t sat::launch_adj_sweep_mw<P>(int, int, dim3, hipStream_t, const PhaseDev&, int, int);

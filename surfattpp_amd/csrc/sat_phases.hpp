@@ -100,13 +100,22 @@ constexpr int kRingDepth = SAT_RING_DEPTH;
 constexpr int kFwdDepth1 = 1;
 constexpr int kAdjDepth1 = 3;
 
-// The multi-wave sweeps carry the latency-critical FP64 re-solves, which share
-// the CUs with a throughput kernel on the other stream: their waves take the
-// highest instruction-issue priority on their SIMDs (s_setprio; the streams
-// themselves keep the default priority).
-template <int W>
+// The multi-wave FP64 sweeps carry the latency-critical re-solves of FP32
+// batches, which share the CUs with a throughput kernel on the other stream:
+// their waves take the highest instruction-issue priority on their SIMDs
+// (s_setprio; the streams themselves keep the default priority).  Multi-wave
+// FP32 sweeps are ordinary throughput work and keep the default.
+template <int PREC, int W>
 __device__ __forceinline__ void latency_critical_priority() {
-  if constexpr (W > 1) __builtin_amdgcn_s_setprio(3);
+  if constexpr (PREC == SAT_PREC_F64 && W > 1) __builtin_amdgcn_s_setprio(3);
+}
+
+// Rows in flight per lane of the multi-wave sweeps: the register ring of
+// kRingDepth rows where it fits, two rows for the FP32 forward at C >= 3
+// (its per-row body is ten registers per column).
+template <int PREC, int C>
+constexpr int mw_fwd_depth() {
+  return (PREC == SAT_PREC_F32 && C >= 3) ? 2 : kRingDepth;
 }
 
 template <int PREC> struct PhaseT;
@@ -289,7 +298,7 @@ void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
   __shared__ ExchangeLds<Pol, W> xch;
   __shared__ double red[W];
   __shared__ int s_idx;
-  latency_critical_priority<W>();
+  latency_critical_priority<PREC, W>();
   char* sb = pd.slots + (size_t)blockIdx.x * pd.slot_bytes;
   U* Q0 = reinterpret_cast<U*>(sb);
   U* Q1 = reinterpret_cast<U*>(sb + pd.slot_plane);
@@ -312,7 +321,7 @@ void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
     __syncthreads();
     const auto cx = uniform(phase_fwd_ctx<PREC, ISO, Pol>(pd, g, ev));
     IterOut io;
-    const U* u = iterate<Pol, WaveLayout, C, W, (W > 1) ? kRingDepth : kFwdDepth1>(cx, Q0, Q1, Q2, pd.fwd_maxiter, kFwdTol,
+    const U* u = iterate<Pol, WaveLayout, C, W, (W > 1) ? mw_fwd_depth<PREC, C>() : kFwdDepth1>(cx, Q0, Q1, Q2, pd.fwd_maxiter, kFwdTol,
                                                                           io, exchange_ptr<Pol, W>(xch), red);
     if (lane == 0) {
       pd.conv[2 * (size_t)e] = io.linf;
@@ -465,7 +474,7 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
   __shared__ ExchangeLds<Pol, W> xch;
   __shared__ double red[W];
   __shared__ int s_idx;
-  latency_critical_priority<W>();
+  latency_critical_priority<PREC, W>();
   char* sb = pd.slots + (size_t)blockIdx.x * pd.slot_bytes;
   V* Q0 = reinterpret_cast<V*>(sb);
   V* Q1 = reinterpret_cast<V*>(sb + pd.slot_plane);

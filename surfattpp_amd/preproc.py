@@ -31,12 +31,19 @@ class EikonalBatch:
     """Device-resident batch of forward+adjoint pairs over one or more grids."""
 
     def __init__(self, grids: dict | Sequence[dict], precision: int = _lib.PREC_F64, calc_adj: bool = True,
-                 workspace_cap: int = 0):
+                 workspace_cap: int = 0, waves=None):
         """workspace_cap: bytes of per-event device workspace held at once (0: the
         library's default); above it each run processes the events in consecutive
-        chunks of the batch order, with the same results."""
+        chunks of the batch order, with the same results.
+        waves: waves per pair of each grid's batched FP32 sweeps (an int for every
+        grid or a sequence, each 1, 2 or 4; None or 0: the library's choice; see
+        surfattpp_amd.tune for measured choices).  Results do not depend on it."""
         if isinstance(grids, dict):
             grids = [grids]
+        if waves is None or isinstance(waves, int):
+            waves = [int(waves or 0)] * len(grids)
+        if len(waves) != len(grids):
+            raise ValueError("one waves value per grid")
         self._keep = []
         gdesc = (_lib.sat_grid_desc * len(grids))()
         events, rl, rb, tt, w = [], [], [], [], []
@@ -53,7 +60,7 @@ class EikonalBatch:
             self._keep += [lon, lat, *planes]
             gdesc[gi] = _lib.sat_grid_desc(lon.size, lat.size, nper, lon.ctypes.data, lat.ctypes.data,
                                            planes[0].ctypes.data, planes[1].ctypes.data,
-                                           planes[2].ctypes.data, planes[3].ctypes.data)
+                                           planes[2].ctypes.data, planes[3].ctypes.data, int(waves[gi]))
             self.shapes.append((nper, lat.size, lon.size))
             n0 = len(events)
             for e in range(len(b["ev_iper"])):
@@ -144,6 +151,12 @@ class EikonalBatch:
             off += n
         return dict(syn=syn[: self.nrec], adj_s=K if len(K) > 1 else K[0], chi=float(chi[0]),
                     iters_fwd=itf[: self.nevent], iters_adj=ita[: self.nevent], flags=fl[: self.nevent])
+
+    def waves(self) -> list:
+        """Waves per pair of each grid's batched sweeps, as chosen at creation."""
+        buf = (C.c_int * len(self.shapes))()
+        _lib.check("sat_batch_waves", _lib.load().sat_batch_waves(self._h, buf))
+        return list(buf)
 
     def set_f32_band(self, band: float):
         _lib.check("sat_batch_set_f32_band", _lib.load().sat_batch_set_f32_band(self._h, float(band)))

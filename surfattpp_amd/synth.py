@@ -368,3 +368,32 @@ def config_large(seed: int = 5, nx: int = 401, ny: int = 481, nsta: int = 400, n
     b = config_grf(seed=seed, nx=nx, ny=ny, nsta=nsta, nperiod=nperiod, **kw)
     b["meta"]["name"] = f"large-{nx}x{ny}"
     return b
+
+
+def config_dispersion_shape(seed: int = 2, nx: int = 121, ny: int = 141, spacing_km: float = 2.0,
+                            nperiod: int = 20, amp: float = 0.05, corr_km: float = 30.0):
+    """121x141 nodes at 2 km with the 25 stations of config_checkerboard (same seed,
+    same degrees) and 20 periods log-spaced 4-40 s: 500 forward+adjoint pairs.
+    The per-period phase-velocity maps are the mean dispersion curve
+    (mean_phase_velocity) times (1 + a seeded +-5 % GRF of 30 km correlation),
+    one realisation shared by all periods -- a stand-in for maps from a layered
+    Vs model through a dispersion solver, which this package does not build.
+    Observed times: great-circle times at the period's mean velocity."""
+    rng = Rng(seed)
+    d = km_to_deg(spacing_km)
+    lon, lat = centred_axis(nx, d), centred_axis(ny, d)
+    cb = config_checkerboard()
+    sta_lon, sta_lat = cb["sta_lon"], cb["sta_lat"]
+    periods = periods_log(nperiod)
+    vbar = mean_phase_velocity(periods)
+    pert = grf(rng, nx, ny, spacing_km, corr_km, amp)
+    svel = np.stack([vbar[p] * (1.0 + pert) for p in range(nperiod)])
+    ones = np.ones((nperiod, ny, nx))
+    ev_src, ev_iper, rec_idx, off, nrec = all_pairs_events(sta_lon, sta_lat, nperiod)
+    tt_obs = great_circle_time(sta_lon[rec_idx], sta_lat[rec_idx], sta_lon[ev_src].repeat(nrec),
+                               sta_lat[ev_src].repeat(nrec), vbar[ev_iper].repeat(nrec))
+    weight = np.ones(rec_idx.size)
+    meta = dict(name=f"disp-{nx}x{ny}", spacing_km=spacing_km, nsta=len(sta_lon), nperiod=nperiod,
+                periods=periods.tolist(), vbar=vbar.tolist(), amp=amp, corr_km=corr_km, seed=seed)
+    return _assemble(lon, lat, svel, ones, 0 * ones, ones, sta_lon, sta_lat, ev_src, ev_iper, rec_idx, off,
+                     nrec, tt_obs, weight, meta)

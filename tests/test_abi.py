@@ -25,11 +25,27 @@ def test_library_exports_every_header_symbol():
         assert hasattr(L, name), name
 
 
-def test_struct_sizes_match_c_layout():
-    # sat_info: 2 ints + 2 doubles; sat_event_desc: 2 ints, 2 doubles, 2 ints
-    assert ctypes.sizeof(_lib.sat_info) == 24
-    assert ctypes.sizeof(_lib.sat_event_desc) == 32
-    assert ctypes.sizeof(_lib.sat_grid_desc) == 8 + 4 + 4 + 6 * 8
+def test_struct_layouts_match_the_header(tmp_path):
+    # sizes and field offsets of the ctypes mirrors against the C compiler's
+    import subprocess
+    structs = {"sat_info": _lib.sat_info, "sat_grid_desc": _lib.sat_grid_desc,
+               "sat_event_desc": _lib.sat_event_desc, "sat_batch_desc": _lib.sat_batch_desc}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "surfatt_eik.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'  printf("{name} %zu\\n", sizeof({name}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{name}.{fname} %zu\\n", offsetof({name}, {fname}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                 check=True).stdout.splitlines())
+    for name, cls in structs.items():
+        assert int(got[name]) == ctypes.sizeof(cls), name
+        for fname, _ in cls._fields_:
+            assert int(got[f"{name}.{fname}"]) == getattr(cls, fname).offset, f"{name}.{fname}"
 
 
 def test_missing_library_fails_loudly(tmp_path):

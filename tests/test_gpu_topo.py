@@ -62,3 +62,25 @@ def test_topo_uneven_full_f32_vs_f64(topo):
     np.testing.assert_array_equal(r32["iters_fwd"], r64["iters_fwd"])
     assert max(errs) <= 1e-5
     assert RC.rel_inf(r32["syn"], r64["syn"]) <= 1e-6
+
+
+def test_dispersion_shape_500_pairs():
+    # 121x141 at 2 km, 25 stations x 20 periods: FP64 = oracle on a subset, FP32 within the bar
+    b = synth.config_dispersion_shape()
+    sub = synth.subset_events(b, [0, 137, 311, 499])
+    with PP.EikonalBatch(sub) as bt:
+        bt.run()
+        r = bt.results()
+    ro = O.forward_for_event(sub)
+    assert RC.rel_inf(r["syn"], ro["syn"]) <= 1e-12
+    np.testing.assert_array_equal(r["iters_fwd"], ro["iters_fwd"])
+    for p in np.unique(sub["ev_iper"]):
+        assert RC.rel_inf(r["adj_s"][p], ro["adj_s"][p]) <= 1e-12
+    with PP.EikonalBatch(b) as b64, PP.EikonalBatch(b, precision=PREC_F32) as b32:
+        b64.run()
+        b32.run()
+        r64, r32 = b64.results(), b32.results()
+    errs = [RC.rel_inf(r32["adj_s"][p], r64["adj_s"][p]) for p in range(20)]
+    print("500 pairs: K err max", max(errs), flush=True)
+    np.testing.assert_array_equal(r32["iters_fwd"], r64["iters_fwd"])
+    assert max(errs) <= 1e-5
