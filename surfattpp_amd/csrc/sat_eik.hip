@@ -1220,6 +1220,10 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   pd.fwd_maxiter = kFwdMaxIter;
   pd.adj_maxiter = kAdjMaxIter;
   const unsigned node_bx = (unsigned)((b->max_plane + 255) / 256);
+  // init_kernel: one thread per wavefront-major element (either geometry) or node
+  size_t init_n = b->max_plane;
+  for (const GridHost& G : b->grids) init_n = std::max({init_n, G.g.wplane, G.gmw.wplane});
+  const unsigned init_bx = (unsigned)((init_n + 255) / 256);
   hipEvent_t* mk = b->mk;
   // columns per lane of the re-solve geometry: ceil(nx / (64 kMultiWave)) = ceil(C / kMultiWave)
   auto cmw = [&](size_t gi) { return (b->group_C[gi] + kMultiWave - 1) / kMultiWave; };
@@ -1317,7 +1321,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
             PhaseDev a = q;
             a.list = b->d_fb + b->group_first[gi];
             a.list_count = fb_count + gi;
-            SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(node_bx, ub), st, a));
+            SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(init_bx, ub, 2), st, a));
             a.counter = resf_heads + gi;
             const dim3 grid = res_slots(gi, a, ub);
             SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(0, cmw(gi), kMultiWave, grid, st, a, 0, 0, gi));
@@ -1396,7 +1400,8 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
       if (cn) {
         const dim3 col_grid((unsigned)((b->max_nx + 63) / 64), (unsigned)cn);
         SAT_CHECK(f32 ? launch_trig<SAT_PREC_F32>(col_grid, b->stream, pc) : launch_trig<SAT_PREC_F64>(col_grid, b->stream, pc));
-        SAT_CHECK(f32 ? launch_init<SAT_PREC_F32>(node_grid, b->stream, pc) : launch_init<SAT_PREC_F64>(node_grid, b->stream, pc));
+        SAT_CHECK(f32 ? launch_init<SAT_PREC_F32>(dim3(init_bx, cn, 3), b->stream, pc)
+                      : launch_init<SAT_PREC_F64>(dim3(init_bx, cn, 2), b->stream, pc));
       }
       SAT_CHECK(mark(1, b->stream));
       {

@@ -200,7 +200,21 @@ __global__ void trig_kernel(PhaseDev pd) {
   }
 }
 
-// src/eikonal_solver.cpp:96-117 per node; acos correctly rounded in the FP64 path.
+// Decode element q of an orientation-o wavefront-major plane: column ix, row j;
+// false for padding.
+__device__ inline bool wave_decode(const GridDev& g, int o, int q, int& ix, int& j) {
+  const int s = q / g.C, k = q - s * g.C;
+  const int dg = s / g.La, l = s - dg * g.La;
+  j = (o == 0) ? dg - l : dg + l - (g.La - 1);
+  ix = l * g.C + k;
+  return j >= 0 && j < g.ny && ix < g.nx;
+}
+
+// src/eikonal_solver.cpp:96-117; acos correctly rounded in the FP64 path.
+// blockIdx.z = pass: 0 and 1 write the orientation-0 / -1 wavefront-major
+// planes (T0, T0x, T0y; thread = plane element, padding skipped), 2 the FP64
+// column-major T0 of FP32 sweeps (thread = node): every store coalesced, the
+// node's arithmetic repeated per pass.
 template <int PREC>
 __global__ void init_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
@@ -208,9 +222,16 @@ __global__ void init_kernel(PhaseDev pd) {
   if (e < 0) return;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (int)g.plane) return;
-  const int iy = p / g.nx, ix = p - iy * g.nx;
+  const int pass = blockIdx.z;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  int ix, iy;
+  if (pass < 2) {
+    if (idx >= (int)g.wplane || !wave_decode(g, pass, idx, ix, iy)) return;
+  } else {
+    if (idx >= (int)g.plane) return;
+    iy = idx / g.nx;
+    ix = idx - iy * g.nx;
+  }
   const double fun0 = pd.evs[3 * (size_t)e], sy0 = pd.evs[3 * (size_t)e + 1], cy0 = pd.evs[3 * (size_t)e + 2];
   const double2 rw = g.trig_yy[iy];
   const double2 cl = pd.coltab[ev.col_off + ix];
@@ -225,14 +246,14 @@ __global__ void init_kernel(PhaseDev pd) {
     T0x = cyy * cy0 * sdl / sin_d * kR * fun0;
     T0y = -(cyy * sy0 - syy * cy0 * cdl) / sin_d * kR * fun0;
   }
-#pragma unroll
-  for (int o = 0; o < 2; o++) {
-    const size_t w = ev.w_off + wave_index(g, o, ix, iy);
-    static_cast<V*>(pd.t0[o])[w] = (V)T0;
-    static_cast<V*>(pd.t0x[o])[w] = (V)T0x;
-    static_cast<V*>(pd.t0y[o])[w] = (V)T0y;
+  if (pass < 2) {
+    const size_t w = ev.w_off + idx;
+    static_cast<V*>(pd.t0[pass])[w] = (V)T0;
+    static_cast<V*>(pd.t0x[pass])[w] = (V)T0x;
+    static_cast<V*>(pd.t0y[pass])[w] = (V)T0y;
+  } else {
+    pd.T0d[ev.pl_off + idx] = T0;
   }
-  if (PREC == SAT_PREC_F32) pd.T0d[ev.pl_off + p] = T0;
 }
 
 template <int PREC, bool ISO, class Pol>
