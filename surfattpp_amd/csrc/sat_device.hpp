@@ -128,6 +128,31 @@ template <class T>
 __device__ __forceinline__ void gst(T* p, T v) {
   *(__attribute__((address_space(1))) T*)p = v;
 }
+
+// Raw buffer stores over one plane: a store whose byte offset lies at or beyond
+// the plane's size is dropped by the hardware (no memory access), so lanes with
+// nothing to store issue the same unconditional instruction with kDropOffset.
+// Word 3 of the descriptor: the gfx9 raw-buffer format (32-bit data format,
+// range-checked against num_records in bytes).
+constexpr int kBufferWord3 = 0x00020000;
+constexpr int kDropOffset = 0x40000000;
+// The descriptor lives in scalar registers: the plane pointer is made
+// wave-uniform explicitly (it is the same on every lane; a pointer the compiler
+// keeps in vector registers would turn each store into a waterfall loop).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(void* p, int bytes) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  void* pu = (void*)(((unsigned long long)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(pu, (short)0, __builtin_amdgcn_readfirstlane(bytes), kBufferWord3);
+}
+typedef unsigned int sat_u32x2 __attribute__((__vector_size__(8)));
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, int off, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(sat_u32x2, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, int off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
 // Element i of a plane, addressed as a wave-uniform base plus a 32-bit byte
 // offset (planes are < 4 GiB): the load takes the base from scalar registers
 // and needs no 64-bit vector address.  (Stores keep pointer arithmetic: with a

@@ -45,11 +45,12 @@
 
 namespace sat {
 
-// Every swept plane has kTrashElems elements past its end: the stores of lanes
-// outside the grid at a step go there, so that every load and store of a step
-// is issued unconditionally (conditional memory operations make the compiler's
-// wait counts conservative, which would wait for the prefetches issued in the
-// same step).
+// Every swept plane has kTrashElems elements past its end: under SegLayout the
+// stores of lanes outside the grid at a step go there, so that every load and
+// store of a step is issued unconditionally (conditional memory operations
+// make the compiler's wait counts conservative, which would wait for the
+// prefetches issued in the same step).  WaveLayout drops those stores instead
+// (store_row).
 constexpr int kTrashElems = kWave * kMaxC;
 
 // The geometry a sweep needs, read from the grid descriptor once before the
@@ -129,6 +130,27 @@ template <class Pol, int W>
 __device__ __forceinline__ Exchange<Pol, W>* exchange_ptr(ExchangeLds<Pol, W>& x) {
   if constexpr (W > 1) return &x;
   else return nullptr;
+}
+
+// A lane's C new values of one step.  Lanes with no live row (outside their
+// part of the sweep, or beyond the active lanes) store nothing: under
+// WaveLayout their raw buffer stores carry an offset past the plane and are
+// dropped; under SegLayout they go to the plane's trash row.  Either way the
+// store instruction is unconditional (a branch around it would make the
+// compiler wait for the prefetches issued in the same step).
+template <class Lay, int C, int OOUT, class V>
+__device__ __forceinline__ void store_row(const SweepGeom& g, V* __restrict__ tout, bool live, int j, int lane,
+                                          const V (&newv)[C]) {
+  if constexpr (Lay::kStepRun) {
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc(tout, g.trash0 * (int)sizeof(V));
+    const int off = live ? Lay::template base<C, OOUT>(g, j, lane) * (int)sizeof(V) : kDropOffset;
+#pragma unroll
+    for (int k = 0; k < C; k++) bst(rs, off + k * (int)sizeof(V), newv[k]);
+  } else {
+    const int base = live ? Lay::template base<C, OOUT>(g, j, lane) : Lay::template trash<C>(g, lane);
+#pragma unroll
+    for (int k = 0; k < C; k++) gst(tout + base + k, newv[k]);
+  }
 }
 
 template <class Pol, class Lay, int C, int XD, int YD, bool NORM, int OOUT, int W = 1>
@@ -251,9 +273,7 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
           lmax = dmax(lmax, d);
         }
       }
-      const int base = live ? Lay::template base<C, OOUT>(g, j, lane) : Lay::template trash<C>(g, lane);
-#pragma unroll
-      for (int k = 0; k < C; k++) gst(tout + base + k, newv[k]);
+      store_row<Lay, C, OOUT>(g, tout, live, j, lane, newv);
 #pragma unroll
       for (int k = 0; k < C; k++) { prev_u[k] = newv[k]; prev_a[k] = cur_h.aux[k]; }
     }
@@ -388,9 +408,7 @@ __device__ __forceinline__ void sweep_ring(const typename Pol::Ctx& cx, const ty
             lmax = dmax(lmax, d);
           }
         }
-        const int base = live ? Lay::template base<C, OOUT>(g, j, lane) : Lay::template trash<C>(g, lane);
-#pragma unroll
-        for (int k = 0; k < C; k++) gst(tout + base + k, newv[k]);
+        store_row<Lay, C, OOUT>(g, tout, live, j, lane, newv);
 #pragma unroll
         for (int k = 0; k < C; k++) { prev_u[k] = newv[k]; prev_a[k] = cur_h.aux[k]; }
       }
