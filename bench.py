@@ -249,7 +249,29 @@ def extra_workloads():
     del topo
     out["disp-121x141-2km-25src-20per"] = _time_pair(synth.config_dispersion_shape())
     out["large-401x481-1km-400src-30per"] = _time_pair(synth.config_large())
+    out["combine-201x201-30z-30per"] = combine_workload()
     return out
+
+
+def combine_workload(nx=201, ny=201, nz=30, nper=30):
+    """The model kernels from 30 per-period kernels (vs-only, src/preproc.cpp:280-301)
+    at 201x201 nodes x 30 depths: device kernel time and its HBM rate (each
+    sensitivity read once, FP64)."""
+    from surfattpp_amd import preproc as PP
+    r = np.random.default_rng(7)
+    adj = r.normal(size=(nper, ny, nx))
+    sens = [r.normal(size=(nx, ny, nz, nper)) for _ in range(3)]
+    vs = 2.5 + 2.0 * r.uniform(size=(nx, ny, nz))
+    vp = 0.9409 + 2.0947 * vs - 0.8206 * vs**2 + 0.2683 * vs**3 - 0.0251 * vs**4
+    ms = []
+    for _ in range(3):
+        t = {}
+        PP.combine_kernels(adj, *sens, vs, vp, mode=0, timing=t)
+        ms.append(t["kernel_ms"])
+    nbytes = 8 * (3 * sens[0].size + adj.size + 2 * vs.size + vs.size)
+    best = min(ms)
+    return {"kernel_ms": round(best, 4), "bytes": nbytes, "GBps": round(nbytes / best / 1e6, 1),
+            "frac_hbm": round(nbytes / best / 1e6 / HBM_PEAK_GBS, 4)}
 
 
 def calib_copy(nbytes: int) -> None:

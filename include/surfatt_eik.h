@@ -172,6 +172,28 @@ int sat_batch_phase_ms(sat_batch* b, double* ms, int n);
 int sat_batch_waves(sat_batch* b, int* waves);
 int sat_batch_destroy(sat_batch* b);
 
+/* ------------------------------------------------------------------------
+ * Model kernels from the per-period kernels.
+ * ------------------------------------------------------------------------ */
+
+/* Replaces the isotropic part of preproc::combine_kernels (include/preproc.h,
+ * src/preproc.cpp:196-342): the per-period kernels adj_s [nper][ny][nx] (as
+ * sat_batch_results returns them) times the depth sensitivities sen_vs, sen_vp,
+ * sen_rho [nx][ny][nz][nper] (the reference's RowMajor Tensor4r, from its
+ * dispersion kernels), summed over the periods in order, into ker_* [nx][ny][nz]
+ * (RowMajor Tensor3r), FP64.  mode 0: vs only (use_alpha_beta_rho false:
+ * ker_vs -= att*(s_vs + s_vp*dvp/dvs + s_rho*drho/dvp*dvp/dvs) with the
+ * relations of include/utils.h:198-212 at vs3d, vp3d [nx][ny][nz]); mode 1:
+ * vs, vp, rho kernels; mode 2: vs, vp and rho = 0.33*vs (rho_scaling).
+ * ker_vp, ker_rho (and sen_rho in mode 2, vs3d / vp3d outside mode 0) may be
+ * NULL where unused.  Host pointers, synchronous; kernel_ms (may be NULL):
+ * the device kernel's time.  nper <= 96.  The anisotropic (Gc, Gs) and density
+ * terms are not built (off in every configuration). */
+int sat_combine_kernels(int nx, int ny, int nz, int nper, int mode, const double* adj_s,
+                        const double* sen_vs, const double* sen_vp, const double* sen_rho,
+                        const double* vs3d, const double* vp3d, double* ker_vs, double* ker_vp,
+                        double* ker_rho, double* kernel_ms);
+
 /* Library identity: device name, build arch; returns SAT_EDEVICE without a GPU. */
 int sat_device_info(char* buf, int buflen);
 

@@ -601,3 +601,56 @@ int sat_oracle_forward_for_event(int nx, int ny, const double* xgrids, const dou
     free(fun); free(msphc); free(T); free(Ta); free(adj);
     return rc;
 }
+
+/* ------------------------------------------------------------------------
+ * preproc::combine_kernels, isotropic part (src/preproc.cpp:196-342): the
+ * per-period 2-D kernels adj_tt (here [nper][ny][nx], ix fastest) combined with
+ * the depth sensitivities sen_* ([nx][ny][nz][nper], the reference's RowMajor
+ * Tensor4r) into 3-D kernels ker_* ([nx][ny][nz], RowMajor Tensor3r), periods
+ * in order (:239-301).  mode 0: vs only (use_alpha_beta_rho false, :280-301,
+ * with dalpha_dbeta / drho_dalpha of include/utils.h:198-212); mode 1: vs, vp,
+ * rho (:258-278, scale = 1 without anisotropy); mode 2: vs, vp and rho =
+ * RHO_SCALING * vs (:276-278).  Outputs are zeroed first (:206-221).
+ * ------------------------------------------------------------------------ */
+static double o_drho_dalpha(double vp) {  /* include/utils.h:198-200 */
+  return 1.6612 - 2*0.4721*vp + 3*0.0671*vp*vp - 4*0.0043*vp*vp*vp + 5*0.000106*vp*vp*vp*vp;
+}
+static double o_dalpha_dbeta(double vs) { /* include/utils.h:210-212 */
+  return 2.0947 - 2*0.8206*vs + 3*0.2683*vs*vs - 4*0.0251*vs*vs*vs;
+}
+
+int sat_oracle_combine_kernels(int nx, int ny, int nz, int nper, int mode, const double* adj_tt,
+                               const double* sen_vs, const double* sen_vp, const double* sen_rho,
+                               const double* vs3d, const double* vp3d, double* ker_vs, double* ker_vp,
+                               double* ker_rho) {
+  if (nx < 1 || ny < 1 || nz < 1 || nper < 1 || mode < 0 || mode > 2) return SAT_ORACLE_EINVAL;
+  const size_t n3 = (size_t)nx * ny * nz;
+  for (size_t i = 0; i < n3; i++) {
+    ker_vs[i] = 0.0;
+    if (mode > 0) { ker_vp[i] = 0.0; ker_rho[i] = 0.0; }
+  }
+  for (int iper = 0; iper < nper; iper++) {
+    for (int ix = 0; ix < nx; ix++) {
+      for (int iy = 0; iy < ny; iy++) {
+        const double att = adj_tt[(size_t)iper * nx * ny + ix + (size_t)iy * nx];
+        for (int k = 0; k < nz; k++) {
+          const size_t o3 = ((size_t)ix * ny + iy) * nz + k;
+          const size_t o4 = o3 * nper + iper;
+          if (mode == 0) {
+            const double dab = o_dalpha_dbeta(vs3d[o3]);
+            const double dra = o_drho_dalpha(vp3d[o3]);
+            ker_vs[o3] -= att * (sen_vs[o4] + sen_vp[o4] * dab + sen_rho[o4] * dra * dab);
+          } else {
+            const double scale = 1.0 / sqrt(1.0 + 0.0 * 0.0 + 0.0 * 0.0);
+            ker_vs[o3] -= att * scale * sen_vs[o4];
+            ker_vp[o3] -= att * scale * sen_vp[o4];
+            if (mode == 1) ker_rho[o3] -= att * scale * sen_rho[o4];
+          }
+        }
+      }
+    }
+    if (mode == 2)  /* :276-278, after every period */
+      for (size_t i = 0; i < n3; i++) ker_rho[i] = ker_vs[i] * 0.33;
+  }
+  return SAT_ORACLE_OK;
+}

@@ -40,6 +40,11 @@ int sat_oracle_forward_for_event(int nx, int ny, const double* xgrids, const dou
                                  int is_calc_adj, double* syn, double* adj_s, double* chi_out,
                                  int* iters_fwd, int* iters_adj, int* flags);
 
+int sat_oracle_combine_kernels(int nx, int ny, int nz, int nper, int mode, const double* adj_tt,
+                               const double* sen_vs, const double* sen_vp, const double* sen_rho,
+                               const double* vs3d, const double* vp3d, double* ker_vs, double* ker_vp,
+                               double* ker_rho);
+
 #ifdef __cplusplus
 }
 #endif

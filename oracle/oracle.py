@@ -66,6 +66,8 @@ def _bind(L):
     L.sat_oracle_mask.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_double]
     L.sat_oracle_interp2d.restype = None
     L.sat_oracle_interp2d.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp]
+    L.sat_oracle_combine_kernels.restype = C.c_int
+    L.sat_oracle_combine_kernels.argtypes = [C.c_int] * 5 + [_dp] * 9
     L.sat_oracle_forward_for_event.restype = C.c_int
     L.sat_oracle_forward_for_event.argtypes = [
         C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp,
@@ -245,3 +247,19 @@ def twin_adjoint(xx_deg, yy_deg, spha, sphb, sphc, T, xrec_deg, yrec_deg, source
     _twin(kind).fsm_o1_jse_lonlat_2d_(_p(xx), _p(yy), C.byref(nx), C.byref(ny), _p(A), _p(B), _p(Cc),
                                       _p(TT), _p(Ta), _p(xr), _p(yr), _p(sa), C.byref(nr))
     return Ta
+
+
+def combine_kernels(adj_s, sen_vs, sen_vp, sen_rho, vs3d, vp3d, mode=0):
+    """preproc::combine_kernels, isotropic part (src/preproc.cpp:196-342): the
+    oracle's loop, periods outermost.  Shapes as surfattpp_amd.preproc.combine_kernels."""
+    adj = np.ascontiguousarray(adj_s, dtype=np.float64)
+    nper, ny, nx = adj.shape
+    s = [np.ascontiguousarray(a, dtype=np.float64) for a in (sen_vs, sen_vp, sen_rho)]
+    nz = s[0].shape[2]
+    v = [np.ascontiguousarray(a, dtype=np.float64) for a in (vs3d, vp3d)]
+    out = [np.zeros((nx, ny, nz)) for _ in range(3)]
+    rc = lib().sat_oracle_combine_kernels(nx, ny, nz, nper, int(mode), _p(adj), _p(s[0]), _p(s[1]), _p(s[2]),
+                                          _p(v[0]), _p(v[1]), _p(out[0]), _p(out[1]), _p(out[2]))
+    if rc:
+        raise RuntimeError(f"oracle combine_kernels failed rc={rc}")
+    return tuple(out)

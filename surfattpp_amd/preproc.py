@@ -194,3 +194,45 @@ def forward_for_event(batch: dict | Sequence[dict], is_calc_adj: bool = True,
     with EikonalBatch(batch, precision=precision, calc_adj=is_calc_adj) as b:
         b.run()
         return b.results()
+
+
+COMBINE_VS, COMBINE_VS_VP_RHO, COMBINE_VS_VP_RHO_SCALED = 0, 1, 2
+
+
+def combine_kernels(adj_s, sen_vs, sen_vp, sen_rho=None, vs3d=None, vp3d=None, mode: int = COMBINE_VS,
+                    timing: dict | None = None):
+    """Isotropic preproc::combine_kernels (src/preproc.cpp:196-342) on the GPU.
+
+    adj_s (nper, ny, nx) per-period kernels (EikonalBatch.results()["adj_s"]);
+    sen_vs, sen_vp, sen_rho (nx, ny, nz, nper) depth sensitivities in the
+    reference's RowMajor layout; vs3d, vp3d (nx, ny, nz) for mode 0 (vs only:
+    the vp and rho sensitivities enter through dvp/dvs and drho/dvp); mode 1:
+    vs, vp, rho kernels; mode 2: rho = 0.33 * vs.  Returns (ker_vs, ker_vp,
+    ker_rho), each (nx, ny, nz) (None where the mode has no such kernel).
+    `timing`, if given, receives the device kernel's milliseconds."""
+    adj = _c64(adj_s)
+    nper, ny, nx = adj.shape
+    s_vs = _c64(sen_vs)
+    nz = s_vs.shape[2]
+    shape4 = (nx, ny, nz, nper)
+    for a in (s_vs, sen_vp) + ((sen_rho,) if mode != COMBINE_VS_VP_RHO_SCALED else ()):
+        if a is None or np.shape(a) != shape4:
+            raise ValueError(f"sensitivities must be {shape4}")
+    s_vp = _c64(sen_vp)
+    s_rho = _c64(sen_rho) if sen_rho is not None else None
+    v_s = _c64(vs3d) if vs3d is not None else None
+    v_p = _c64(vp3d) if vp3d is not None else None
+    if mode == COMBINE_VS and (s_rho is None or v_s is None or v_p is None
+                               or v_s.shape != (nx, ny, nz) or v_p.shape != (nx, ny, nz)):
+        raise ValueError("mode 0 needs sen_rho and vs3d, vp3d of shape (nx, ny, nz)")
+    k_vs = np.empty((nx, ny, nz))
+    k_vp = np.empty((nx, ny, nz)) if mode else None
+    k_rho = np.empty((nx, ny, nz)) if mode else None
+    ms = C.c_double(0.0)
+    ptr = lambda a: a.ctypes.data if a is not None else None  # noqa: E731
+    rc = _lib.load().sat_combine_kernels(nx, ny, nz, nper, int(mode), ptr(adj), ptr(s_vs), ptr(s_vp), ptr(s_rho),
+                                         ptr(v_s), ptr(v_p), ptr(k_vs), ptr(k_vp), ptr(k_rho), C.byref(ms))
+    _lib.check("sat_combine_kernels", rc)
+    if timing is not None:
+        timing["kernel_ms"] = ms.value
+    return k_vs, k_vp, k_rho

@@ -1,0 +1,36 @@
+"""GPU combine_kernels (sat_combine_kernels) against the oracle: the same FP64
+operations in the same order, so the same bits, in all three modes; and the
+201x201 x 30 depths x 30 periods size, timed (device kernel)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from surfattpp_amd import preproc as PP
+from tests.test_combine_oracle import _inputs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_combine_equals_oracle(mode):
+    adj, s, vs, vp = _inputs(23, 17, 9, 6, 11 + mode)
+    got = PP.combine_kernels(adj, *s, vs, vp, mode=mode)
+    want = O.combine_kernels(adj, *s, vs, vp, mode=mode)
+    np.testing.assert_array_equal(got[0], want[0])
+    if mode:
+        np.testing.assert_array_equal(got[1], want[1])
+        np.testing.assert_array_equal(got[2], want[2])
+    else:
+        assert got[1] is None and got[2] is None
+
+
+def test_combine_full_size_timed():
+    nx = ny = 201
+    nz, nper = 30, 30
+    adj, s, vs, vp = _inputs(nx, ny, nz, nper, 5)
+    t = {}
+    k0, _, _ = PP.combine_kernels(adj, *s, vs, vp, mode=0, timing=t)
+    want, _, _ = O.combine_kernels(adj, *s, vs, vp, mode=0)
+    np.testing.assert_array_equal(k0, want)
+    nbytes = 8 * (3 * s[0].size + adj.size + 2 * vs.size + k0.size)
+    print(f"combine 201x201x30x30: {t['kernel_ms']:.3f} ms, {nbytes / t['kernel_ms'] / 1e6:.0f} GB/s", flush=True)
