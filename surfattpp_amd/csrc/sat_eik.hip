@@ -88,10 +88,11 @@ __global__ void media_kernel(GridDev g, int mode, int prec, const double* __rest
           for (int o = 0; o < 2; o++) static_cast<double*>(const_cast<void*>(g.wmed[i][o]))[wi[o]] = v[i];
       }
     } else {
+      // one-point distances; NaN where det <= 0 (the FP32 update drops NaN candidates)
       const double det = va * vb - vc * vc;
       const float v[6] = {(float)va, (float)vb, (float)vc, (float)vf,
-                          (det > 0.0) ? (float)sqrt(vf * vf * vb / det) : -1.0f,
-                          (det > 0.0) ? (float)sqrt(vf * vf * va / det) : -1.0f};
+                          (det > 0.0) ? (float)sqrt(vf * vf * vb / det) : __builtin_nanf(""),
+                          (det > 0.0) ? (float)sqrt(vf * vf * va / det) : __builtin_nanf("")};
       for (int i = 0; i < 6; i++) {
         if (s0) static_cast<float*>(sp[i])[so] = v[i];
         if (wave)

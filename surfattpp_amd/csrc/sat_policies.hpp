@@ -308,97 +308,105 @@ struct AdjF64Face {
 //   two-point (sx, sy): dr = min(dX, dY), tau_r = tau_ref + dr, tau = tau_r + w,
 //     Tx = Tx0 + px*w,  Tx0 = t0x*tau_r + sx*h*(dX - dr),  px = t0x - sx*h
 //     (y alike), a*Tx^2 + b*Ty^2 - 2c*Tx*Ty = f^2 solved for w with the stable
-//     quadratic formula; accepted if -sx*(a*Tx - c*Ty) >= 0 and -sy*(b*Ty - c*Tx) >= 0;
-//     candidate dr + w.
+//     quadratic formula (w0 = s/A, w1 = C/s); accepted if -sx*(a*Tx - c*Ty) >= 0
+//     and -sy*(b*Ty - c*Tx) >= 0; candidate dr + w.
 //   one-point (W): tau_W = tau_ref + dW, w = (+-dis - t0x*tau_W)/(t0x + h), accepted
 //     if (tau_W + w)*T0 >= tau_W*T0_W and tau_W + w > tau_W/2; candidate dW + w.
 // Candidates are compared with sigma_self - u_ref; the smallest, if below it, is
 // the new value u_ref + candidate formed in FP64 (the reference keeps the
 // smaller of the old and the new tau).  Values only ever decrease, so the
 // rounding of a large change would persist: measuring from u_ref keeps the
-// rounded quantities small.  ISO: the isotropic metric (c == 0, a and b constant
-// along a row), dsx = f/sqrt(a), dsy = f/sqrt(b).
+// rounded quantities small.
+//
+// An absent neighbour is passed as NaN, and the tests are stated so that NaN
+// rejects, which leaves no presence masks in the instruction stream:
+//  - the NaN drops out of u_ref (fmin ignores NaN) and turns every candidate
+//    that uses the neighbour into NaN, as does a negative discriminant (sqrt),
+//    |A| < 1e-30 or det <= 0 (dis is NaN then); fminf ignores NaN;
+//  - two-point causality is max(sx*(a*Tx - c*Ty), sy*(b*Ty - c*Tx)) <= 0, and
+//    with c == 0, a, b > 0 (ISO) max(sx*Tx, sy*Ty) <= 0; a NaN root makes both
+//    operands NaN: rejected;
+//  - the second root C/s needs no s != 0 guard: s == 0 only with C == 0, where
+//    C/s is NaN and the first root carries the same value;
+//  - one-point: both tests are lower bounds on w (t0, tau_W > 0):
+//    w >= tau_W*max(T0_W/t0 - 1, -1/2).
+// ISO: the isotropic metric (c == 0, a and b constant along a row, both > 0).
+// dsx = f*sqrt(b/det), dsy = f*sqrt(a/det) (f/sqrt(a), f/sqrt(b) when ISO).
 template <bool ISO>
 __device__ __forceinline__ double fwd_update_f32(double u_self, float t0, float t0x, float t0y, float a, float b,
                                                  float c, float f2, float dsx, float dsy, float h, float kk,
-                                                 bool hW, double uW, float T0W, bool hE, double uE, float T0E,
-                                                 bool hS, double uS, float T0S, bool hN, double uN, float T0N) {
+                                                 double uW, float T0W, double uE, float T0E, double uS, float T0S,
+                                                 double uN, float T0N) {
 #pragma clang fp contract(fast)
-  // Straight-line: every candidate is evaluated on every lane and masked, so the
-  // wave runs one instruction stream (a rejected or absent candidate is +inf).
-  constexpr double kAbsent = 1e300;
   const float kInf = __builtin_huge_valf();
-  const double u_ref = fmin(fmin(hW ? uW : kAbsent, hE ? uE : kAbsent), fmin(hS ? uS : kAbsent, hN ? uN : kAbsent));
+  const float kNaNf = __builtin_nanf("");
+  const double u_ref = fmin(fmin(uW, uE), fmin(uS, uN));
   const float ts = (float)(1.0 + u_ref);
   const float dW = (float)(uW - u_ref), dE = (float)(uE - u_ref);
   const float dS = (float)(uS - u_ref), dN = (float)(uN - u_ref);
-  float m[4];  // best candidate per quadrant / direction group
+  // px = t0x - sx*h, py = t0y - sy*k for s = -1 (m) and +1 (p)
+  const float pxm = t0x + h, pxp = t0x - h, pym = t0y + kk, pyp = t0y - kk;
+  const float apm = a * pxm, app = a * pxp, bpm = b * pym, bpp = b * pyp;
+  const float Axm = apm * pxm, Axp = app * pxp, Bym = bpm * pym, Byp = bpp * pyp;
+  float best = kInf;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    const float sx = (q < 2) ? -1.0f : 1.0f;
-    const float sy = (q & 1) ? 1.0f : -1.0f;
-    const bool ok = ((q < 2) ? hW : hE) & ((q & 1) ? hN : hS);
-    const float dX = (q < 2) ? dW : dE;
-    const float dY = (q & 1) ? dN : dS;
+    const bool xm = q < 2, ym = !(q & 1);
+    const float sxh = xm ? -h : h, syk = ym ? -kk : kk;
+    const float dX = xm ? dW : dE, dY = ym ? dS : dN;
+    const float px = xm ? pxm : pxp, py = ym ? pym : pyp;
+    const float apx = xm ? apm : app, bpy = ym ? bpm : bpp;
     const float dr = fminf(dX, dY);
-    const float px = t0x - sx * h;
-    const float py = t0y - sy * kk;
     const float tr = ts + dr;
-    const float Tx0 = t0x * tr + sx * h * (dX - dr);
-    const float Ty0 = t0y * tr + sy * kk * (dY - dr);
+    const float Tx0 = t0x * tr + sxh * (dX - dr);
+    const float Ty0 = t0y * tr + syk * (dY - dr);
     float Aq, Bh, Cq;
     if constexpr (ISO) {
-      Aq = a * px * px + b * py * py;
-      Bh = a * px * Tx0 + b * py * Ty0;
-      Cq = a * Tx0 * Tx0 + b * Ty0 * Ty0 - f2;
+      Aq = (xm ? Axm : Axp) + (ym ? Bym : Byp);
+      Bh = apx * Tx0 + bpy * Ty0;
+      Cq = Tx0 * (a * Tx0) + Ty0 * (b * Ty0) - f2;
     } else {
-      Aq = a * px * px + b * py * py - 2.0f * c * px * py;
-      Bh = a * px * Tx0 + b * py * Ty0 - c * (px * Ty0 + py * Tx0);
-      Cq = a * Tx0 * Tx0 + b * Ty0 * Ty0 - 2.0f * c * Tx0 * Ty0 - f2;
+      Aq = (xm ? Axm : Axp) + (ym ? Bym : Byp) - 2.0f * c * px * py;
+      Bh = apx * Tx0 + bpy * Ty0 - c * (px * Ty0 + py * Tx0);
+      Cq = Tx0 * (a * Tx0) + Ty0 * (b * Ty0) - 2.0f * c * Tx0 * Ty0 - f2;
     }
-    const float D = Bh * Bh - Aq * Cq;
-    const bool okq = ok & (D >= 0.0f) & (fabsf(Aq) >= 1e-30f);
-    const float sq = __builtin_amdgcn_sqrtf(fmaxf(D, 0.0f));
-    const float sv = -(Bh + ((Bh >= 0.0f) ? sq : -sq));
+    float D = Bh * Bh - Aq * Cq;
+    D = (fabsf(Aq) >= 1e-30f) ? D : kNaNf;
+    const float sq = __builtin_amdgcn_sqrtf(D);
+    const float sv = -(Bh + __builtin_copysignf(sq, Bh));
     const float w0 = sv * __builtin_amdgcn_rcpf(Aq);
-    const float w1 = (sv != 0.0f) ? Cq * __builtin_amdgcn_rcpf(sv) : w0;
-    float mq = kInf;
+    const float w1 = Cq * __builtin_amdgcn_rcpf(sv);
+    float cw[2];
 #pragma unroll
     for (int g = 0; g < 2; g++) {
       const float w = g ? w1 : w0;
       const float Tx = Tx0 + px * w;
       const float Ty = Ty0 + py * w;
-      const float cxv = ISO ? a * Tx : a * Tx - c * Ty;
-      const float cyv = ISO ? b * Ty : b * Ty - c * Tx;
-      const bool acc = okq & (-sx * cxv >= 0.0f) & (-sy * cyv >= 0.0f);
-      mq = fminf(mq, acc ? dr + w : kInf);
+      const float cxv = ISO ? Tx : a * Tx - c * Ty;
+      const float cyv = ISO ? Ty : b * Ty - c * Tx;
+      const float m = fmaxf(xm ? -cxv : cxv, ym ? -cyv : cyv);
+      cw[g] = (m <= 0.0f) ? w : kInf;
     }
-    m[q] = mq;
+    best = fminf(best, dr + fminf(cw[0], cw[1]));
   }
+  const float rt0 = __builtin_amdgcn_rcpf(t0);
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const bool hq = (q == 0) ? hW : (q == 1) ? hE : (q == 2) ? hS : hN;
-    const float dis = (q < 2) ? dsx : dsy;
-    const bool ok = hq & (dis >= 0.0f);
-    const float dq = (q == 0) ? dW : (q == 1) ? dE : (q == 2) ? dS : dN;
-    const float T0n = (q == 0) ? T0W : (q == 1) ? T0E : (q == 2) ? T0S : T0N;
-    const float td = (q < 2) ? t0x : t0y;
-    const float hh = (q < 2) ? h : kk;
-    const float p = (q == 0 || q == 2) ? td + hh : td - hh;
+  for (int d = 0; d < 4; d++) {
+    const float dq = (d == 0) ? dW : (d == 1) ? dE : (d == 2) ? dS : dN;
+    const float T0n = (d == 0) ? T0W : (d == 1) ? T0E : (d == 2) ? T0S : T0N;
+    const float td = (d < 2) ? t0x : t0y;
+    const float p = (d == 0) ? pxm : (d == 1) ? pxp : (d == 2) ? pym : pyp;
+    const float dis = (d < 2) ? dsx : dsy;
+    const float rp = __builtin_amdgcn_rcpf(p);
     const float tnd = ts + dq;
     const float Td0 = td * tnd;
-    const float rp = __builtin_amdgcn_rcpf(p);
-    const float Tn = tnd * T0n;
-    const float half = 0.5f * tnd;
-#pragma unroll
-    for (int g = 0; g < 2; g++) {
-      const float w = ((g == 0) ? dis - Td0 : -dis - Td0) * rp;
-      const float tc = tnd + w;
-      const bool acc = ok & (tc * t0 >= Tn) & (tc > half);
-      m[q] = fminf(m[q], acc ? dq + w : kInf);
-    }
+    const float wp = (dis - Td0) * rp;
+    const float wm = (-dis - Td0) * rp;
+    const float thw = tnd * fmaxf(T0n * rt0 - 1.0f, -0.5f);
+    const float cp = (wp >= thw) ? wp : kInf;
+    const float cm = (wm >= thw) ? wm : kInf;
+    best = fminf(best, dq + fminf(cp, cm));
   }
-  const float best = fminf(fminf(m[0], m[1]), fminf(m[2], m[3]));
   const double v = u_ref + (double)best;  // +inf when no candidate was accepted
   return ((best < (float)(u_self - u_ref)) & (v < u_self)) ? v : u_self;
 }
@@ -443,9 +451,10 @@ struct FwdF32 {
                                   float aW, bool hW, double uE, float aE, bool hE, double uS, float aS, bool hS,
                                   double uN, float aN, bool hN) {
     const float t0 = hd.aux[k];
+    const double kNaN = __builtin_nan("");
     return fwd_update_f32<false>(hd.u[k], t0, r.t0x[k], r.t0y[k], r.a[k], r.b[k], r.c[k], r.f[k] * r.f[k], r.dsx[k],
-                                 r.dsy[k], t0 * cx.idx, t0 * cx.idy, hW, uW, aW, hE, uE, aE, hS, uS, aS, hN, uN,
-                                 aN);
+                                 r.dsy[k], t0 * cx.idx, t0 * cx.idy, hW ? uW : kNaN, aW, hE ? uE : kNaN, aE,
+                                 hS ? uS : kNaN, aS, hN ? uN : kNaN, aN);
   }
   template <int C>
   __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, double v) {
@@ -493,9 +502,10 @@ struct FwdF32Iso {
                                   double uN, float aN, bool hN) {
     const float t0 = hd.aux[k];
     const float f = r.f[k];
+    const double kNaN = __builtin_nan("");
     return fwd_update_f32<true>(hd.u[k], t0, r.t0x[k], r.t0y[k], r.row.x, r.row.y, 0.0f, f * f, f * r.row.z,
-                                f * r.row.w, t0 * cx.idx, t0 * cx.idy, hW, uW, aW, hE, uE, aE, hS, uS, aS, hN, uN,
-                                aN);
+                                f * r.row.w, t0 * cx.idx, t0 * cx.idy, hW ? uW : kNaN, aW, hE ? uE : kNaN, aE,
+                                hS ? uS : kNaN, aS, hN ? uN : kNaN, aN);
   }
   template <int C>
   __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, double v) {
