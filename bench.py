@@ -250,7 +250,30 @@ def extra_workloads():
     out["disp-121x141-2km-25src-20per"] = _time_pair(synth.config_dispersion_shape())
     out["large-401x481-1km-400src-30per"] = _time_pair(synth.config_large())
     out["combine-201x201-30z-30per"] = combine_workload()
+    out["topo-metric-321x321-1.5km-12per"] = topo_metric_workload()
     return out
+
+
+def topo_metric_workload(n=321, nperiod=12):
+    """The anisotropic metric planes of the topography workload's largest grid
+    (321x321 at 1.5 km, 8 Gaussian edifices, 12 per-period smoothing widths,
+    src/surf_grid.cpp:109-145) on the GPU: device time of the smoothing and
+    metric kernels, window nodes visited per second and the FP64 rate of the
+    window sums (3 flops per window node: w*z + num, den + w)."""
+    from surfattpp_amd import synth
+    from surfattpp_amd import topo as TG
+    b = synth.config_topo_grid(33, n, nsta=10, nperiod=nperiod, smooth=False)
+    lon, lat, z = b["lon"], b["lat"], b["z"]
+    sig = TG.sigma_for_periods(b["meta"]["vbar"], b["meta"]["periods"], b["meta"]["wavelen_factor"])
+    ms = []
+    for _ in range(3):
+        t = {}
+        TG.topo_metric(z, lon, lat, sig, timing=t)
+        ms.append(t["kernel_ms"])
+    best = min(ms)
+    nodes = sum(TG.window_nodes(n, n, lon, lat, s) for s in sig)
+    return {"kernel_ms": round(best, 3), "window_nodes": nodes, "Gnodes_per_s": round(nodes / best / 1e6, 1),
+            "fp64_TFLOPs": round(3 * nodes / best / 1e9, 2)}
 
 
 def combine_workload(nx=201, ny=201, nz=30, nper=30):

@@ -194,6 +194,24 @@ int sat_combine_kernels(int nx, int ny, int nz, int nper, int mode, const double
                         const double* vs3d, const double* vp3d, double* ker_vs, double* ker_vp,
                         double* ker_rho, double* kernel_ms);
 
+/* The anisotropic metric of a topographic surface for every period:
+ * SurfGrid::build_media_matrix_with_topo (src/surf_grid.cpp:109-145, :184-190)
+ * -- the surface z [ny][nx] (km, ix fastest: the reference's column-major
+ * (nx, ny) topo.z) smoothed per period with gaussian_smooth_geo_2
+ * (include/utils.h:505-573) at sigma_deg[iper] degrees (<= 0: not smoothed),
+ * its geographic gradient fx, fy (gradient_2_geo, include/utils.h:595-649),
+ * then m11 = (1+fy^2)/D, m22 = (1+fx^2)/D, m12 = -fx*fy/D with
+ * D = 1+fx^2+fy^2, written [nper][ny][nx] (the batch's period-major planes).
+ * zs_out (may be NULL): the smoothed surfaces [nper][ny][nx].  The reference's
+ * sigma is v(T)*T*wavelen_factor/(2 pi) km / (R_EARTH*DEG2RAD) (:115-118).
+ * Axes uniform and ascending (dlon = lon[1]-lon[0], dlat likewise, as the
+ * reference reads them); 3 <= nx <= 2048, ny >= 3, nper <= 65535, and
+ * 2*int(3 sigma/dlon)+1 <= 4095.  Host pointers, synchronous; kernel_ms (may
+ * be NULL): the device time of the smoothing and metric kernels. */
+int sat_topo_metric(int nx, int ny, int nper, const double* lon_deg, const double* lat_deg, const double* z,
+                    const double* sigma_deg, double* m11, double* m12, double* m22, double* zs_out,
+                    double* kernel_ms);
+
 /* Library identity: device name, build arch; returns SAT_EDEVICE without a GPU. */
 int sat_device_info(char* buf, int buflen);
 
