@@ -212,6 +212,34 @@ int sat_topo_metric(int nx, int ny, int nper, const double* lon_deg, const doubl
                     const double* sigma_deg, double* m11, double* m12, double* m22, double* zs_out,
                     double* kernel_ms);
 
+/* Surface-wave dispersion of ncol layered columns: the reference's
+ * surfdisp96_c (src/surfker/surfdisp96_c_api.f90:5, src/surfker/surfdisp96.f:57)
+ * batched over columns.  thk, vp, vs, rho: [ncol][nlayer] REAL*4, the last
+ * layer the half-space; t: [kmax] periods (s); iflsph 0 flat / 1 spherical
+ * earth; iwave 1 Love / 2 Rayleigh; mode >= 1 (1 fundamental); igr 0 phase /
+ * > 0 group velocity.  cg: [ncol][kmax] velocities (float-rounded, as the
+ * reference returns them).  flags (may be NULL): [ncol], 1 where the
+ * fundamental mode has no root (the reference stops the program there; cg is
+ * 0 from that period on).  2 <= nlayer <= 512, 1 <= kmax <= 1024.  Host
+ * pointers, synchronous; kernel_ms (may be NULL): the device time. */
+int sat_surfdisp96(int ncol, int nlayer, const float* thk, const float* vp, const float* vs, const float* rho,
+                   int iflsph, int iwave, int mode, int igr, int kmax, const double* t, double* cg, int* flags,
+                   double* kernel_ms);
+
+/* Phase- or group-velocity maps of a 3-D model: SurfGrid::fwdsurf
+ * (src/surf_grid.cpp:194-245) -- per column (ix, iy) the model at the depth
+ * nodes zgrids[nz] (vs3d, and vp3d/rho3d or, when both are NULL, Brocher's
+ * vp(vs) and rho(vp), include/utils.h:185-187,223-225; all [nx][ny][nz] FP64,
+ * the reference's RowMajor Tensor3r), layered as surfker::build_disp_req does
+ * (src/surfdisp.cpp:60-111: nz + 1 layers, the last node's layer repeated and
+ * a half-space), solved by surfdisp96 at nper periods.  svel:
+ * [nper][ny][nx] (the batch's period-major planes).  flags as for
+ * sat_surfdisp96, [ny][nx].  nz >= 2.  The reference calls it with iflsph 1,
+ * mode 1 (include/config.h:79-80). */
+int sat_fwdsurf(int nx, int ny, int nz, const double* zgrids, const double* vs3d, const double* vp3d,
+                const double* rho3d, int nper, const double* periods, int iflsph, int iwave, int mode, int igr,
+                double* svel, int* flags, double* kernel_ms);
+
 /* Library identity: device name, build arch; returns SAT_EDEVICE without a GPU. */
 int sat_device_info(char* buf, int buflen);
 

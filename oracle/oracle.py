@@ -8,6 +8,9 @@ bench.py's cpu_baseline leg, never by the product package surfattpp_amd.
 * ``_ref/libtwin_r8.so`` / ``_ref/libtwin_r4.so`` -- the reference's Fortran
   twin tests/eikonal_test/eikon_solver_2d.f90 compiled from its own source
   (promoted to 8-byte default REAL, and as shipped).
+* ``_ref/libsurfdisp96.so`` -- the reference's surfdisp96 (src/surfker/surfdisp96.f
+  with its C binding src/surfker/surfdisp96_c_api.f90) compiled from its own
+  source: the checker of the GPU dispersion solver.
 
 All 2-D fields are Fortran-ordered numpy arrays of shape (nx, ny), i.e. the
 reference's column-major (ix, iy) storage.
@@ -247,6 +250,31 @@ def twin_adjoint(xx_deg, yy_deg, spha, sphb, sphc, T, xrec_deg, yrec_deg, source
     _twin(kind).fsm_o1_jse_lonlat_2d_(_p(xx), _p(yy), C.byref(nx), C.byref(ny), _p(A), _p(B), _p(Cc),
                                       _p(TT), _p(Ta), _p(xr), _p(yr), _p(sa), C.byref(nr))
     return Ta
+
+
+_SURFDISP = None
+
+
+def surfdisp_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "libsurfdisp96.so"))
+
+
+def surfdisp96(thk, vp, vs, rho, periods, iflsph=1, iwave=2, mode=1, igr=0):
+    """The reference's surfdisp96_c (src/surfker/surfdisp96_c_api.f90:5) on one
+    layered column: REAL*4 thickness/vp/vs/rho (the last layer the half-space),
+    FP64 periods; returns the phase (igr=0) or group velocities per period."""
+    global _SURFDISP
+    if _SURFDISP is None:
+        L = C.CDLL(os.path.join(_HERE, "_ref", "libsurfdisp96.so"))
+        L.surfdisp96_c.restype = None
+        L.surfdisp96_c.argtypes = [C.c_void_p] * 4 + [C.c_int] * 6 + [C.c_void_p] * 2
+        _SURFDISP = L
+    a = [np.ascontiguousarray(x, dtype=np.float32) for x in (thk, vp, vs, rho)]
+    t = np.ascontiguousarray(periods, dtype=np.float64)
+    cg = np.zeros(t.size)
+    _SURFDISP.surfdisp96_c(*(x.ctypes.data for x in a), a[0].size, iflsph, iwave, mode, igr, t.size,
+                           t.ctypes.data, cg.ctypes.data)
+    return cg
 
 
 def combine_kernels(adj_s, sen_vs, sen_vp, sen_rho, vs3d, vp3d, mode=0):

@@ -1,0 +1,127 @@
+"""The GPU dispersion solver (sat_surfdisp96 / sat_fwdsurf, surfattpp_amd.surfdisp)
+against the reference's surfdisp96: the committed golden vectors it produced,
+the reference's unit-test values and the closed forms the checker is pinned
+by; sat_fwdsurf's on-device layering against the host's build_disp_req.
+
+Bar: |c - c_ref| <= 2e-6 c_ref (the solver's own stopping tolerance is 1e-6
+relative; the result is a float); the fraction of bit-identical values is
+printed.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from surfattpp_amd import surfdisp as SD
+from surfattpp_amd import synth
+from tests import test_surfdisp_oracle as TO
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "surfdisp_vectors.npz")
+CASES = ("rayleigh_phase_sph", "rayleigh_phase_flat", "rayleigh_group_sph", "love_phase_sph")
+
+
+def _cols(g):
+    n = int(g["ncol"])
+    nl = max(g[f"col{i}_thk"].size for i in range(n))
+    return n, nl
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_golden_columns(case):
+    g = np.load(GOLD)
+    n = int(g["ncol"])
+    sph, iw, igr = (int(x) for x in g[case + "_flags"])
+    # columns of equal layer count go in one call
+    by_len = {}
+    for i in range(n):
+        by_len.setdefault(g[f"col{i}_thk"].size, []).append(i)
+    worst, same, total = 0.0, 0, 0
+    for nl, ids in by_len.items():
+        arr = [np.stack([g[f"col{i}_{k}"] for i in ids]) for k in ("thk", "vp", "vs", "rho")]
+        cg = SD.surfdisp96(*arr, g["periods"], iflsph=sph, iwave=iw, igr=igr)
+        ref = g[case][ids]
+        worst = max(worst, float(np.abs(cg / ref - 1.0).max()))
+        same += int((cg == ref).sum())
+        total += ref.size
+    print(f"{case}: max rel err {worst:.2e}, bit-identical {same}/{total}", flush=True)
+    assert worst <= 2e-6
+
+
+def test_reference_unit_test_and_closed_forms():
+    thk, vp, vs, rho = TO.unit_model()
+    c = SD.surfdisp96(thk, vp, vs, rho, TO.PERIODS, iflsph=0, iwave=2, igr=0)
+    assert abs(c[1] - 2.6747) <= 1e-4 and abs(c[3] - 3.8387) <= 1e-4
+    for v in (2.0, 3.5, 4.4):
+        t, a, b, r = SD.build_disp_req(np.linspace(0.0, 30.0, 7), np.full(7, v))
+        c = SD.surfdisp96(t, a, b, r, TO.PERIODS, iflsph=0, iwave=2, igr=0)
+        assert np.abs(c / TO.rayleigh_halfspace(float(a[0]), float(b[0])) - 1.0).max() <= 2e-6
+    t = np.array([10.0, 0.0], np.float32)
+    for T in (3.0, 10.0, 30.0):
+        c = SD.surfdisp96(t, np.float32([5.2, 6.9]), np.float32([3.0, 4.0]), np.float32([2.6, 3.0]),
+                          np.array([T]), iflsph=0, iwave=1, igr=0)[0]
+        ref = TO.love_layer(10.0, 3.0, 4.0, float(np.float32(2.6)), float(np.float32(3.0)), T)
+        assert abs(c / ref - 1.0) <= 2e-6
+
+
+def _model(nx, ny, seed=2):
+    """The 121x141 workload's layered model: 30 nodes over 0-60 km, Vs 2.8 -> 4.6
+    km/s, times (1 + a seeded +-5 % lateral GRF of 30 km correlation at 2 km)."""
+    rng = synth.Rng(seed)
+    dep = np.linspace(0.0, 60.0, 30)
+    d = synth.grf(rng, nx, ny, 2.0, 30.0, 0.05)  # (ny, nx)
+    vs = np.linspace(2.8, 4.6, 30)[None, None, :] * (1.0 + d.T[:, :, None])
+    return dep, np.ascontiguousarray(vs)
+
+
+def test_fwdsurf_layers_match_host_request():
+    dep, vs3d = _model(13, 11)
+    periods = synth.periods_log(20)
+    svel = SD.fwdsurf(dep, vs3d, periods)
+    nx, ny, _ = vs3d.shape
+    thk, vp, vs, rho = SD.build_disp_req(dep, vs3d.reshape(nx * ny, -1))
+    cg = SD.surfdisp96(thk, vp, vs, rho, periods)            # columns ix*ny + iy
+    assert np.array_equal(svel, cg.reshape(nx, ny, -1).transpose(2, 1, 0))
+    if O.surfdisp_available():
+        for ix, iy in ((0, 0), (6, 5), (12, 10)):
+            i = ix * ny + iy
+            ref = O.surfdisp96(thk[i], vp[i], vs[i], rho[i], periods)
+            assert np.abs(svel[:, iy, ix] / ref - 1.0).max() <= 2e-6
+
+
+def test_fwdsurf_workload_timing():
+    """The 121x141 dispersion workload's maps (20 periods 4-40 s) and a 201x201
+    model at 30 periods: device time beside the reference's solver on the host
+    (a sample of columns on one core)."""
+    for nx, ny, nper in ((121, 141, 20), (201, 201, 30)):
+        dep, vs3d = _model(nx, ny)
+        periods = synth.periods_log(nper)
+        tm = {}
+        svel = SD.fwdsurf(dep, vs3d, periods, timing=tm)
+        msg = f"fwdsurf {nx}x{ny} x {nper} periods: GPU {tm['kernel_ms']:.1f} ms"
+        if O.surfdisp_available():
+            thk, vp, vs, rho = SD.build_disp_req(dep, vs3d.reshape(nx * ny, -1))
+            sample = np.arange(0, nx * ny, max(1, nx * ny // 40))
+            t0 = time.time()
+            for i in sample:
+                ref = O.surfdisp96(thk[i], vp[i], vs[i], rho[i], periods)
+                ix, iy = divmod(int(i), ny)
+                assert np.abs(svel[:, iy, ix] / ref - 1.0).max() <= 2e-6
+            per_col = (time.time() - t0) / sample.size
+            msg += f"; reference on one host core {per_col * 1e3:.2f} ms/column -> {per_col * nx * ny:.1f} s"
+        print(msg, flush=True)
+        assert np.all((svel > 2.0) & (svel < 5.0))
+
+
+def test_flags_and_invalid_arguments():
+    t = np.float32([[5.0, 5.0, 0.0]])
+    cg, fl = SD.surfdisp96(t, np.float32([[3.0, 3.0, 3.0]]), np.float32([[2.5, 2.5, 2.5]]),
+                           np.float32([[2.5, 2.5, 2.5]]), np.array([5.0, 10.0]), iflsph=0, return_flags=True)
+    assert fl.tolist() == [0] and np.all(cg > 0)
+    from surfattpp_amd import _lib
+    with pytest.raises(_lib.SatError):
+        SD.surfdisp96(t, t, t, t, np.array([-1.0]))        # a period must be positive
+    with pytest.raises(_lib.SatError):
+        SD.surfdisp96(t[:, :1], t[:, :1], t[:, :1], t[:, :1], np.array([5.0]))  # a half-space needs a layer above
