@@ -247,10 +247,51 @@ def extra_workloads():
     out = {"topo-uneven-101..321-1.5km-12per": dict(
         _time_pair(topo), grids=[[g["lon"].size, g["lat"].size, len(g["ev_iper"])] for g in topo])}
     del topo
-    out["disp-121x141-2km-25src-20per"] = _time_pair(synth.config_dispersion_shape())
+    disp = synth.config_dispersion()
+    out["disp-121x141-2km-25src-20per"] = dict(_time_pair(disp), maps="surfdisp96 on the GPU (sat_fwdsurf)")
+    del disp
     out["large-401x481-1km-400src-30per"] = _time_pair(synth.config_large())
     out["combine-201x201-30z-30per"] = combine_workload()
     out["topo-metric-321x321-1.5km-12per"] = topo_metric_workload()
+    out["fwdsurf-201x201-30z-30per"] = fwdsurf_workload()
+    return out
+
+
+def fwdsurf_workload(nx=201, ny=201, nper=30):
+    """Rayleigh phase-velocity maps of a 201x201-column layered model (30 depth
+    nodes, 0-60 km) at 30 periods 4-40 s by surfdisp96 on the GPU
+    (SurfGrid::fwdsurf, src/surf_grid.cpp:194-245), beside the reference's
+    surfdisp96 (oracle/_ref, built from its Fortran source) timed on a sample of
+    columns on one host core and scaled to all columns over the host's cores."""
+    from surfattpp_amd import surfdisp as SD
+    from surfattpp_amd import synth
+    dep, vs3d, vp3d, rho3d = synth.layered_model(nx, ny, synth.Rng(2))
+    periods = synth.periods_log(nper)
+    ms = []
+    for _ in range(3):
+        t = {}
+        svel = SD.fwdsurf(dep, vs3d, periods, vp3d=vp3d, rho3d=rho3d, timing=t)
+        ms.append(t["kernel_ms"])
+    best = min(ms)
+    out = {"kernel_ms": round(best, 2), "columns": nx * ny, "periods": nper,
+           "columns_per_s": round(nx * ny / best * 1e3, 1)}
+    from oracle import oracle as O
+    if O.surfdisp_available():
+        thk, vp, vs, rho = SD.build_disp_req(dep, vs3d.reshape(nx * ny, -1), vp3d.reshape(nx * ny, -1),
+                                             rho3d.reshape(nx * ny, -1))
+        sample = np.arange(0, nx * ny, (nx * ny) // 200)
+        t0 = time.time()
+        worst = 0.0
+        for i in sample:
+            ref = O.surfdisp96(thk[i], vp[i], vs[i], rho[i], periods)
+            ix, iy = divmod(int(i), ny)
+            worst = max(worst, float(np.abs(svel[:, iy, ix] / ref - 1.0).max()))
+        per_col = (time.time() - t0) / sample.size
+        cores = min(16, os.cpu_count() or 1)
+        out["cpu_reference"] = {"s_per_column_one_core": round(per_col, 6), "cores": cores,
+                                "columns_per_s": round(cores / per_col, 1),
+                                "sample": f"{sample.size} columns, reference surfdisp96 (amdflang -O2)"}
+        out["max_rel_err_vs_reference"] = worst
     return out
 
 

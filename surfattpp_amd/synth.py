@@ -370,14 +370,61 @@ def config_large(seed: int = 5, nx: int = 401, ny: int = 481, nsta: int = 400, n
     return b
 
 
+def layered_model(nx: int, ny: int, rng: Rng, spacing_km: float = 2.0, amp: float = 0.05, corr_km: float = 30.0,
+                  nz: int = 30, z_max: float = 60.0, vs_top: float = 2.8, vs_bottom: float = 4.6):
+    """The 121x141 workload's 3-D model: nz depth nodes over 0..z_max km, Vs
+    linear vs_top -> vs_bottom km/s times (1 + a seeded +-amp lateral GRF of
+    corr_km correlation), Vp = 1.73 Vs, Brocher (2005) density of Vp
+    (include/utils.h:223-225).  Returns (zgrids, vs3d, vp3d, rho3d), the 3-D
+    arrays (nx, ny, nz) like the reference's RowMajor model tensors."""
+    from .surfdisp import vp2rho
+    dep = np.linspace(0.0, z_max, nz)
+    d = grf(rng, nx, ny, spacing_km, corr_km, amp)          # (ny, nx)
+    vs = np.linspace(vs_top, vs_bottom, nz)[None, None, :] * (1.0 + d.T[:, :, None])
+    vp = 1.73 * vs
+    return dep, np.ascontiguousarray(vs), np.ascontiguousarray(vp), np.ascontiguousarray(vp2rho(vp))
+
+
+def config_dispersion(seed: int = 2, nx: int = 121, ny: int = 141, spacing_km: float = 2.0, nperiod: int = 20,
+                      amp: float = 0.05, corr_km: float = 30.0):
+    """121x141 nodes at 2 km, the 25 stations of config_checkerboard, 20 periods
+    log-spaced 4-40 s (500 forward+adjoint pairs), with per-period Rayleigh
+    phase-velocity maps computed on the GPU by surfdisp96 from the layered
+    model of layered_model (spherical earth, fundamental mode, as
+    SurfGrid::fwdsurf calls it, src/surf_grid.cpp:194-245).  Observed times:
+    great-circle times at each period's mean map velocity."""
+    from .surfdisp import fwdsurf
+    rng = Rng(seed)
+    d = km_to_deg(spacing_km)
+    lon, lat = centred_axis(nx, d), centred_axis(ny, d)
+    cb = config_checkerboard()
+    sta_lon, sta_lat = cb["sta_lon"], cb["sta_lat"]
+    periods = periods_log(nperiod)
+    dep, vs3d, vp3d, rho3d = layered_model(nx, ny, rng, spacing_km, amp, corr_km)
+    svel = fwdsurf(dep, vs3d, periods, vp3d=vp3d, rho3d=rho3d, iflsph=1, iwave=2, mode=1, igr=0)
+    vbar = svel.reshape(nperiod, -1).mean(axis=1)
+    ones = np.ones((nperiod, ny, nx))
+    ev_src, ev_iper, rec_idx, off, nrec = all_pairs_events(sta_lon, sta_lat, nperiod)
+    tt_obs = great_circle_time(sta_lon[rec_idx], sta_lat[rec_idx], sta_lon[ev_src].repeat(nrec),
+                               sta_lat[ev_src].repeat(nrec), vbar[ev_iper].repeat(nrec))
+    weight = np.ones(rec_idx.size)
+    meta = dict(name=f"disp-{nx}x{ny}", spacing_km=spacing_km, nsta=len(sta_lon), nperiod=nperiod,
+                periods=periods.tolist(), vbar=vbar.tolist(), amp=amp, corr_km=corr_km, seed=seed,
+                model="30 nodes 0-60 km, Vs 2.8-4.6 km/s, Vp 1.73 Vs, Brocher density, +-5 % GRF 30 km")
+    b = _assemble(lon, lat, svel, ones, 0 * ones, ones, sta_lon, sta_lat, ev_src, ev_iper, rec_idx, off,
+                  nrec, tt_obs, weight, meta)
+    b["model"] = (dep, vs3d, vp3d, rho3d)
+    return b
+
+
 def config_dispersion_shape(seed: int = 2, nx: int = 121, ny: int = 141, spacing_km: float = 2.0,
                             nperiod: int = 20, amp: float = 0.05, corr_km: float = 30.0):
     """121x141 nodes at 2 km with the 25 stations of config_checkerboard (same seed,
     same degrees) and 20 periods log-spaced 4-40 s: 500 forward+adjoint pairs.
     The per-period phase-velocity maps are the mean dispersion curve
     (mean_phase_velocity) times (1 + a seeded +-5 % GRF of 30 km correlation),
-    one realisation shared by all periods -- a stand-in for maps from a layered
-    Vs model through a dispersion solver, which this package does not build.
+    one realisation shared by all periods: a host-only stand-in for
+    config_dispersion's GPU-computed maps, for runs without a GPU.
     Observed times: great-circle times at the period's mean velocity."""
     rng = Rng(seed)
     d = km_to_deg(spacing_km)

@@ -125,3 +125,36 @@ def test_flags_and_invalid_arguments():
         SD.surfdisp96(t, t, t, t, np.array([-1.0]))        # a period must be positive
     with pytest.raises(_lib.SatError):
         SD.surfdisp96(t[:, :1], t[:, :1], t[:, :1], t[:, :1], np.array([5.0]))  # a half-space needs a layer above
+
+
+def test_dispersion_workload_500_pairs():
+    """121x141 at 2 km, 25 stations x 20 periods, maps from surfdisp96 on the GPU:
+    the eikonal batch on those maps equals the oracle (FP64, a subset of pairs)
+    and FP32 stays within 1e-5 of FP64 on every period's kernel."""
+    from surfattpp_amd import PREC_F32
+    from surfattpp_amd import preproc as PP
+    from tests import refcases as RC
+    b = synth.config_dispersion()
+    dep, vs3d, vp3d, rho3d = b["model"]
+    periods = np.array(b["meta"]["periods"])
+    if O.surfdisp_available():  # a few map columns against the reference solver
+        thk, vp, vs, rho = SD.build_disp_req(dep, vs3d[60, 70], vp3d[60, 70], rho3d[60, 70])
+        ref = O.surfdisp96(thk, vp, vs, rho, periods)
+        assert np.abs(b["svel"][:, 70, 60] / ref - 1.0).max() <= 2e-6
+    sub = synth.subset_events(b, [0, 137, 311, 499])
+    with PP.EikonalBatch(sub) as bt:
+        bt.run()
+        r = bt.results()
+    ro = O.forward_for_event(sub)
+    assert RC.rel_inf(r["syn"], ro["syn"]) <= 1e-12
+    np.testing.assert_array_equal(r["iters_fwd"], ro["iters_fwd"])
+    for p in np.unique(sub["ev_iper"]):
+        assert RC.rel_inf(r["adj_s"][p], ro["adj_s"][p]) <= 1e-12
+    with PP.EikonalBatch(b) as b64, PP.EikonalBatch(b, precision=PREC_F32) as b32:
+        b64.run()
+        b32.run()
+        r64, r32 = b64.results(), b32.results()
+    errs = [RC.rel_inf(r32["adj_s"][p], r64["adj_s"][p]) for p in range(20)]
+    print(f"500 pairs on surfdisp96 maps: K err max {max(errs):.3e}", flush=True)
+    np.testing.assert_array_equal(r32["iters_fwd"], r64["iters_fwd"])
+    assert max(errs) <= 1e-5
