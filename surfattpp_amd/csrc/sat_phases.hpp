@@ -112,10 +112,22 @@ __device__ __forceinline__ void latency_critical_priority() {
 
 // Rows in flight per lane of the multi-wave sweeps: the register ring of
 // kRingDepth rows where it fits, two rows for the FP32 forward at C >= 3
-// (its per-row body is ten registers per column).
+// (its per-row body is ten registers per column), and kRingDepthC1 rows for
+// the FP64 sweeps at one column per lane -- the re-solves, whose steps are
+// short enough that four rows in flight leave memory latency exposed (at 8
+// rows the forward kernel holds 256 VGPRs without spilling; two columns per
+// lane would spill, so they keep kRingDepth).
+#ifndef SAT_RING_DEPTH_C1
+#define SAT_RING_DEPTH_C1 8
+#endif
+constexpr int kRingDepthC1 = SAT_RING_DEPTH_C1;
 template <int PREC, int C>
 constexpr int mw_fwd_depth() {
-  return (PREC == SAT_PREC_F32 && C >= 3) ? 2 : kRingDepth;
+  return (PREC == SAT_PREC_F32 && C >= 3) ? 2 : (PREC == SAT_PREC_F64 && C == 1) ? kRingDepthC1 : kRingDepth;
+}
+template <int PREC, int C>
+constexpr int mw_adj_depth() {
+  return (PREC == SAT_PREC_F64 && C == 1) ? kRingDepthC1 : kRingDepth;
 }
 
 template <int PREC> struct PhaseT;
@@ -529,7 +541,7 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
       cx.dy = g.dy;
     }
     IterOut io;
-    const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? kRingDepth : kAdjDepth1>(uniform(cx), Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,
+    const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? mw_adj_depth<PREC, C>() : kAdjDepth1>(uniform(cx), Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,
                                                                            io, exchange_ptr<Pol, W>(xch), red);
     if (lane == 0) {
       pd.iters_a[e] = io.iters;

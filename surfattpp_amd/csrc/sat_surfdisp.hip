@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 
 #include "../../include/surfatt_eik.h"
 
@@ -364,14 +365,25 @@ __device__ double nevill(double t, double c1, double c2, double del1, double del
   return c3;
 }
 
-// getsol (surfdisp96.f:399-491).  del1st is the period-equation value saved at
-// the first period of a mode (a SAVEd common in the reference).
-// The bracketing walk ends in the reference when c1 leaves [cm, betmx + dc) or
-// a sign change is found; max_steps (twice the walk across that range, set by
-// the caller) only stops a walk that inputs outside the solver's domain would
-// make endless, reporting no root.
+// getsol (surfdisp96.f:399-491), for G lanes that solve one column together.
+// del1st is the period-equation value saved at the first period of a mode (a
+// SAVEd common in the reference).
+//
+// The bracketing walk steps c by dc until the period equation changes sign;
+// the points it visits depend only on c, dc, clow and the direction, not on
+// the equation's values.  So the G lanes evaluate the next G points at once
+// (lane g the g-th), exchange the values, and replay the walk over them in
+// order: the first sign change, or the first bound left, ends it exactly where
+// the one-point-at-a-time walk would, with the same points and values; values
+// past that point are discarded.  G = 1 is the reference's walk.  Neville /
+// halving refinement is sequential and all G lanes run it redundantly.
+//
+// max_steps (twice the walk across [cm, betmx + dc], set by the caller) only
+// stops a walk that inputs outside the solver's domain would make endless,
+// reporting no root.
+template <int G>
 __device__ int getsol(double t1, double& c1, double clow, double dc, double cm, float betmx, int ifunc, int ifirst,
-                      double& del1st, const Col& m, int llw, int max_steps) {
+                      double& del1st, const Col& m, int llw, int max_steps, int g) {
   const double twopi = 2.0 * 3.141592653589793;
   double omega = twopi / t1;
   double wvno = omega / c1;
@@ -385,28 +397,68 @@ __device__ int getsol(double t1, double& c1, double clow, double dc, double cm, 
     idir = +1;
   else
     idir = -1;
-  double c2;
-  for (int step = 0;; step++) {
-    if (step > max_steps) return -1;
-    c2 = (idir > 0) ? c1 + dc : c1 - dc;
-    if (c2 <= clow) {
-      idir = +1;
-      c1 = clow;
-      continue;
+  int steps = 0;
+  for (;;) {
+    // the next G points of the walk from (c1, idir), with the clow resets
+    double pts[G];
+    {
+      double c = c1;
+      int dir = idir;
+#pragma unroll
+      for (int i = 0; i < G; i++) {
+        double c2 = dir > 0 ? c + dc : c - dc;
+        if (c2 <= clow) {
+          dir = +1;
+          c = clow;
+          c2 = c + dc;  // clow + dc > clow: the walk goes on upward
+        }
+        pts[i] = c2;
+        c = c2;
+      }
     }
+    double mine = pts[0];
+#pragma unroll
+    for (int i = 1; i < G; i++)
+      if (g == i) mine = pts[i];
     omega = twopi / t1;
-    wvno = omega / c2;
-    const double del2 = dltar(wvno, omega, ifunc, m, llw);
-    if (dsign1(del1) != dsign1(del2)) {
-      const double cn = nevill(t1, c1, c2, del1, del2, ifunc, m, llw, twopi);
-      c1 = cn;
-      if (c1 > (double)betmx) return -1;
-      return 1;
+    wvno = omega / mine;
+    const double dmine = dltar(wvno, omega, ifunc, m, llw);
+    double dels[G];
+#pragma unroll
+    for (int i = 0; i < G; i++) dels[i] = (G == 1) ? dmine : __shfl(dmine, i, G);
+    // replay the walk over the G points
+    double c = c1;
+    int dir = idir;
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+      if (++steps > max_steps) return -1;
+      double c2 = dir > 0 ? c + dc : c - dc;
+      if (c2 <= clow) {
+        dir = +1;
+        c = clow;
+        if (++steps > max_steps) return -1;
+        c2 = c + dc;
+      }
+      const double del2 = dels[i];
+      if (dsign1(del1) != dsign1(del2)) {
+        const double cn = nevill(t1, c, c2, del1, del2, ifunc, m, llw, twopi);
+        c1 = cn;
+        if (c1 > (double)betmx) return -1;
+        return 1;
+      }
+      c = c2;
+      del1 = del2;
+      if (c < cm) {
+        c1 = c;
+        return -1;
+      }
+      if (c >= ((double)betmx + dc)) {
+        c1 = c;
+        return -1;
+      }
     }
-    c1 = c2;
-    del1 = del2;
-    if (c1 < cm) return -1;
-    if (c1 >= ((double)betmx + dc)) return -1;
+    c1 = c;
+    idir = dir;
   }
 }
 
@@ -439,30 +491,30 @@ struct DispDev {
   double* cg;        // [ncol][kmax]
   int* flags;        // [ncol]
   float* minvs;      // [ncol] minval of the input vs
+  double* cstart;    // [ncol] starting phase velocity cc (surfdisp96.f:217-236)
+  float* betmx;      // [ncol] largest shear velocity after flattening
+  int* llw;          // [ncol] 1, or 2 with a water layer on top
 };
 
-// surfdisp96 (surfdisp96.f:57-374) for one column.
-__global__ __launch_bounds__(64) void surfdisp_kernel(DispDev g) {
+__device__ inline Col column(const DispDev& g, int col) {
+  const size_t s = (size_t)g.ncol, lm = (size_t)g.mmax * s;
+  return Col{g.lay + col, g.lay + lm + col, g.lay + 2 * lm + col, g.lay + 3 * lm + col, g.lay + 4 * lm + col,
+             g.lay + 5 * lm + col, g.lay + 6 * lm + col, s, g.mmax};
+}
+
+// surfdisp96.f:98-236 for one column: earth flattening (sphere with iflag 0,
+// then the density mapping of the requested wave type), the extremal shear
+// velocities and the starting value.  Writes the transformed layers back and
+// the column's cc, betmx and llw.
+__global__ void prep_kernel(DispDev g) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= g.ncol) return;
-  const size_t s = (size_t)g.ncol, lm = (size_t)g.mmax * s;
-  Col m{g.lay + col, g.lay + lm + col, g.lay + 2 * lm + col, g.lay + 3 * lm + col, g.lay + 4 * lm + col,
-        g.lay + 5 * lm + col, g.lay + 6 * lm + col, s, g.mmax};
-  const int mmax = g.mmax, kmax = g.kmax;
-  double* cgc = g.cg + (size_t)col * kmax;
-  auto C = [&](int k) -> double& { return g.c[(size_t)k * s + col]; };   // 0-based period
-  auto CB = [&](int k) -> double& { return g.cb[(size_t)k * s + col]; };
-  int flag = 0;
-  const float sone0 = 1.500f;
-  const float vsmin = g.minvs[col];
-  const float ddc0 = vsmin < 1.0f ? 0.001f : (vsmin < 2.0f ? 0.002f : 0.005f);
-  const float h0 = 0.005f;
+  const Col m = column(g, col);
+  const int mmax = g.mmax;
   int llw = 1;
   if (m.B(0) <= 0.0f) llw = 2;
-  const double one = 1.0e-2;
-  // sphere(0, 0): earth flattening of thickness and velocities
   float dhalf = 0.0f;
-  if (g.iflsph == 1) {
+  if (g.iflsph == 1) {  // sphere(0, 0)
     const double ar = 6370.0;
     double dr = 0.0, r0 = ar;
     m.D(mmax - 1) = 1.0f;
@@ -513,11 +565,6 @@ __global__ __launch_bounds__(64) void surfdisp_kernel(DispDev g) {
     }
     m.D(mmax - 1) = 0.0f;
   }
-  const float ddc = ddc0;
-  float sone = sone0;
-  const float h = h0;
-  if (sone < 0.01f) sone = 2.0f;
-  const double onea = (double)sone;
   float cc1;
   if (jsol == 0)
     cc1 = betmn;
@@ -525,7 +572,39 @@ __global__ __launch_bounds__(64) void surfdisp_kernel(DispDev g) {
     cc1 = gtsolh(m.A(jmn), m.B(jmn));
   cc1 = 0.95f * cc1;
   cc1 = 0.90f * cc1;
-  const double cc = (double)cc1;
+  g.cstart[col] = (double)cc1;
+  g.betmx[col] = betmx;
+  g.llw[col] = llw;
+}
+
+// surfdisp96.f:237-374 for one column, G lanes per column (getsol).  The lanes
+// of a column hold the same state and write the same values.
+template <int G>
+__global__ __launch_bounds__(64) void surfdisp_kernel(DispDev g) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = tid / G, lane = tid % G;
+  if (col >= g.ncol) return;  // whole groups: G divides the block size
+  const size_t s = (size_t)g.ncol;
+  const Col m = column(g, col);
+  const int kmax = g.kmax;
+  double* cgc = g.cg + (size_t)col * kmax;
+  auto C = [&](int k) -> double& { return g.c[(size_t)k * s + col]; };   // 0-based period
+  auto CB = [&](int k) -> double& { return g.cb[(size_t)k * s + col]; };
+  int flag = 0;
+  const float sone0 = 1.500f;
+  const float vsmin = g.minvs[col];
+  const float ddc0 = vsmin < 1.0f ? 0.001f : (vsmin < 2.0f ? 0.002f : 0.005f);
+  const float h0 = 0.005f;
+  const int llw = g.llw[col];
+  const float betmx = g.betmx[col];
+  const double one = 1.0e-2;
+  const int ifunc = g.iwave;
+  const float ddc = ddc0;
+  float sone = sone0;
+  const float h = h0;
+  if (sone < 0.01f) sone = 2.0f;
+  const double onea = (double)sone;
+  const double cc = g.cstart[col];
   double dc = (double)ddc;
   dc = fabs(dc);
   double c1 = cc;
@@ -575,7 +654,7 @@ __global__ __launch_bounds__(64) void surfdisp_kernel(DispDev g) {
         c1 = C(k - 1) - onea * dc;
         clow = cm;
       }
-      int iret = getsol(t1, c1, clow, dc, cm, betmx, ifunc, ifirst, del1st, m, llw, max_steps);
+      int iret = getsol<G>(t1, c1, clow, dc, cm, betmx, ifunc, ifirst, del1st, m, llw, max_steps, lane);
       if (iret == -1) {
         kfail = k;
         break;
@@ -586,7 +665,7 @@ __global__ __launch_bounds__(64) void surfdisp_kernel(DispDev g) {
         ifirst = 0;
         clow = CB(k) + one * dc;
         c1 = c1 - onea * dc;
-        iret = getsol(t1, c1, clow, dc, cm, betmx, ifunc, ifirst, del1st, m, llw, max_steps);
+        iret = getsol<G>(t1, c1, clow, dc, cm, betmx, ifunc, ifirst, del1st, m, llw, max_steps, lane);
         if (iret == -1) c1 = C(k);
         CB(k) = c1;
       } else {
@@ -609,6 +688,42 @@ __global__ __launch_bounds__(64) void surfdisp_kernel(DispDev g) {
     }
   }
   if (g.flags) g.flags[col] = flag;
+}
+
+// Lanes per column: the most (up to 8) that keep columns x lanes within one
+// wave per SIMD.  More lanes shorten a column's walk but add redundant
+// refinement work, so they pay only while each wave has its SIMD to itself
+// (measured: 3,233 columns run twice as fast with 8 lanes as with 1; 40,401
+// columns -- 0.6 waves per SIMD with 1 lane -- 16 % slower with 2 and 26 %
+// with 4).
+int lanes_per_column(int ncol) {
+  const long target = 1024L * 64;  // 256 CUs x 4 SIMDs x one wave of 64 lanes
+  if ((long)ncol * 8 <= target) return 8;
+  if ((long)ncol * 4 <= target) return 4;
+  if ((long)ncol * 2 <= target) return 2;
+  return 1;
+}
+
+// SURFATT_SURFDISP_LANES=1|2|4|8 fixes the lanes per column (the results do
+// not depend on it; tests compare the settings).
+int solve_lanes(int ncol) {
+  if (const char* e = std::getenv("SURFATT_SURFDISP_LANES")) {
+    const int v = std::atoi(e);
+    if (v == 1 || v == 2 || v == 4 || v == 8) return v;
+  }
+  return lanes_per_column(ncol);
+}
+
+hipError_t launch_solve(const DispDev& g, int G) {
+  const long threads = (long)g.ncol * G;
+  const dim3 grid((unsigned)((threads + 63) / 64)), block(64);
+  switch (G) {
+    case 8: hipLaunchKernelGGL(surfdisp_kernel<8>, grid, block, 0, nullptr, g); break;
+    case 4: hipLaunchKernelGGL(surfdisp_kernel<4>, grid, block, 0, nullptr, g); break;
+    case 2: hipLaunchKernelGGL(surfdisp_kernel<2>, grid, block, 0, nullptr, g); break;
+    default: hipLaunchKernelGGL(surfdisp_kernel<1>, grid, block, 0, nullptr, g); break;
+  }
+  return hipGetLastError();
 }
 
 // refine_request (src/surfdisp.cpp:60-111) on a column of the 3-D model: nz
@@ -739,19 +854,22 @@ extern "C" int sat_surfdisp96(int ncol, int nlayer, const float* thk, const floa
   hipEvent_t e0 = nullptr, e1 = nullptr;
   DispDev g{};
   // one allocation: t | c | cb | cg | flags | lay | minvs | the four input arrays
-  const size_t bytes = sizeof(double) * ((size_t)kmax + 3 * (size_t)kmax * ncol) + sizeof(int) * ncol +
-                       sizeof(float) * (7 * lm + ncol + 4 * nin);
+  const size_t bytes = sizeof(double) * ((size_t)kmax + 3 * (size_t)kmax * ncol + ncol) + sizeof(int) * 2 * ncol +
+                       sizeof(float) * (7 * lm + 2 * (size_t)ncol + 4 * nin);
   D_CHECK(hipMalloc(&buf, bytes));
   {
     double* dt = (double*)buf;
     double* dc = dt + kmax;
     double* dcb = dc + (size_t)kmax * ncol;
     double* dcg = dcb + (size_t)kmax * ncol;
-    int* dfl = (int*)(dcg + (size_t)kmax * ncol);
-    float* lay = (float*)(dfl + ncol);
+    double* cst = dcg + (size_t)kmax * ncol;
+    int* dfl = (int*)(cst + ncol);
+    int* llw = dfl + ncol;
+    float* lay = (float*)(llw + ncol);
     float* minvs = lay + 7 * lm;
-    float* in = minvs + ncol;
-    g = DispDev{ncol, nlayer, kmax, iflsph, iwave, mode, igr, dt, lay, dc, dcb, dcg, dfl, minvs};
+    float* bmx = minvs + ncol;
+    float* in = bmx + ncol;
+    g = DispDev{ncol, nlayer, kmax, iflsph, iwave, mode, igr, dt, lay, dc, dcb, dcg, dfl, minvs, cst, bmx, llw};
     D_CHECK(hipMemcpy(dt, t, sizeof(double) * kmax, hipMemcpyHostToDevice));
     D_CHECK(hipMemcpy(in, thk, sizeof(float) * nin, hipMemcpyHostToDevice));
     D_CHECK(hipMemcpy(in + nin, vp, sizeof(float) * nin, hipMemcpyHostToDevice));
@@ -765,8 +883,9 @@ extern "C" int sat_surfdisp96(int ncol, int nlayer, const float* thk, const floa
   D_CHECK(hipEventCreate(&e0));
   D_CHECK(hipEventCreate(&e1));
   D_CHECK(hipEventRecord(e0, nullptr));
-  hipLaunchKernelGGL(surfdisp_kernel, dim3((ncol + 63) / 64), dim3(64), 0, nullptr, g);
+  hipLaunchKernelGGL(prep_kernel, dim3((ncol + 63) / 64), dim3(64), 0, nullptr, g);
   D_CHECK(hipGetLastError());
+  D_CHECK(launch_solve(g, solve_lanes(ncol)));
   D_CHECK(hipEventRecord(e1, nullptr));
   D_CHECK(hipEventSynchronize(e1));
   if (kernel_ms) {
@@ -807,22 +926,25 @@ extern "C" int sat_fwdsurf(int nx, int ny, int nz, const double* zgrids, const d
   DispDev g{};
   LayerDev L{};
   const int nmod = vp3d ? 3 : 1;
-  const size_t bytes = sizeof(double) * ((size_t)nper + 3 * (size_t)nper * ncol + nz + nmod * n3) +
-                       sizeof(int) * ncol + sizeof(float) * (7 * lm + ncol);
+  const size_t bytes = sizeof(double) * ((size_t)nper + 3 * (size_t)nper * ncol + ncol + nz + nmod * n3) +
+                       sizeof(int) * 2 * ncol + sizeof(float) * (7 * lm + 2 * (size_t)ncol);
   D_CHECK(hipMalloc(&buf, bytes));
   {
     double* dt = (double*)buf;
     double* dc = dt + nper;
     double* dcb = dc + (size_t)nper * ncol;
     double* dcg = dcb + (size_t)nper * ncol;
-    double* dz = dcg + (size_t)nper * ncol;
+    double* cst = dcg + (size_t)nper * ncol;
+    double* dz = cst + ncol;
     double* dvs = dz + nz;
     double* dvp = vp3d ? dvs + n3 : nullptr;
     double* drho = vp3d ? dvs + 2 * n3 : nullptr;
     int* dfl = (int*)(dvs + nmod * n3);
-    float* lay = (float*)(dfl + ncol);
+    int* llw = dfl + ncol;
+    float* lay = (float*)(llw + ncol);
     float* minvs = lay + 7 * lm;
-    g = DispDev{ncol, mmax, nper, iflsph, iwave, mode, igr, dt, lay, dc, dcb, dcg, dfl, minvs};
+    float* bmx = minvs + ncol;
+    g = DispDev{ncol, mmax, nper, iflsph, iwave, mode, igr, dt, lay, dc, dcb, dcg, dfl, minvs, cst, bmx, llw};
     L = LayerDev{nx, ny, nz, mmax, dz, dvs, dvp, drho, lay, minvs, 1};
     D_CHECK(hipMemcpy(dt, periods, sizeof(double) * nper, hipMemcpyHostToDevice));
     D_CHECK(hipMemcpy(dz, zgrids, sizeof(double) * nz, hipMemcpyHostToDevice));
@@ -838,8 +960,9 @@ extern "C" int sat_fwdsurf(int nx, int ny, int nz, const double* zgrids, const d
   D_CHECK(hipEventRecord(e0, nullptr));
   hipLaunchKernelGGL(layers_kernel, dim3((ncol + 255) / 256), dim3(256), 0, nullptr, L);
   D_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(surfdisp_kernel, dim3((ncol + 63) / 64), dim3(64), 0, nullptr, g);
+  hipLaunchKernelGGL(prep_kernel, dim3((ncol + 63) / 64), dim3(64), 0, nullptr, g);
   D_CHECK(hipGetLastError());
+  D_CHECK(launch_solve(g, solve_lanes(ncol)));
   D_CHECK(hipEventRecord(e1, nullptr));
   D_CHECK(hipEventSynchronize(e1));
   if (kernel_ms) {

@@ -158,3 +158,23 @@ def test_dispersion_workload_500_pairs():
     print(f"500 pairs on surfdisp96 maps: K err max {max(errs):.3e}", flush=True)
     np.testing.assert_array_equal(r32["iters_fwd"], r64["iters_fwd"])
     assert max(errs) <= 1e-5
+
+
+def test_lanes_per_column_do_not_change_results(monkeypatch):
+    """The speculative bracketing (G lanes per column evaluating the walk's next G
+    points at once) visits the same points as the one-lane walk: equal bits
+    for every G, on the golden columns and on a model's maps."""
+    g = np.load(GOLD)
+    ids = [i for i in range(int(g["ncol"])) if g[f"col{i}_thk"].size == 31]
+    arr = [np.stack([g[f"col{i}_{k}"] for i in ids]) for k in ("thk", "vp", "vs", "rho")]
+    dep, vs3d = _model(61, 53)
+    periods = synth.periods_log(20)
+    out = {}
+    for lanes in (1, 2, 4, 8):
+        monkeypatch.setenv("SURFATT_SURFDISP_LANES", str(lanes))
+        tm = {}
+        out[lanes] = (SD.surfdisp96(*arr, g["periods"], igr=1), SD.fwdsurf(dep, vs3d, periods, timing=tm))
+        print(f"lanes {lanes}: fwdsurf 61x53 x 20 periods {tm['kernel_ms']:.2f} ms", flush=True)
+    for lanes in (2, 4, 8):
+        assert np.array_equal(out[lanes][0], out[1][0]) and np.array_equal(out[lanes][1], out[1][1])
+    assert np.array_equal(out[1][0], g["rayleigh_group_sph"][ids])
