@@ -1340,7 +1340,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
               return SAT_OK;
             });
             if (rc) return rc;
-            SAT_CHECK(launch_delta<SAT_PREC_F64>(dim3((cn + 63) / 64), s2, q));
+            SAT_CHECK(launch_delta<SAT_PREC_F64>(dim3((unsigned)cn), s2, q));  // one wave per event
             rc = fan_out2([&](int gi, hipStream_t st) -> int {
               const int ub = std::min(b->group_count[gi], b->pool_size);
               PhaseDev a = q;
@@ -1429,7 +1429,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
       if (b->calc_adj && cn) {
         SAT_CHECK(f32 ? launch_adj_setup<SAT_PREC_F32>(node_grid, b->stream, pc)
                       : launch_adj_setup<SAT_PREC_F64>(node_grid, b->stream, pc));
-        const dim3 ev_grid((unsigned)((cn + 63) / 64));
+        const dim3 ev_grid((unsigned)cn);  // one wave per event
         SAT_CHECK(f32 ? launch_delta<SAT_PREC_F32>(ev_grid, b->stream, pc) : launch_delta<SAT_PREC_F64>(ev_grid, b->stream, pc));
       }
       SAT_CHECK(mark(3, b->stream));

@@ -439,50 +439,65 @@ __global__ void adj_setup_kernel(PhaseDev pd) {
   }
 }
 
-// src/eikonal_solver.cpp:356-375 and src/preproc.cpp:142, one thread per event,
-// receivers in order.  FP64: delta accumulated in the orientation-0 plane, then
-// copied.  FP32: accumulated in FP64 in the event's T0d plane (no longer needed
-// once T is formed), then rounded once into both planes.
+// src/eikonal_solver.cpp:356-375 and src/preproc.cpp:142, one wave per event.
+// The reference adds each receiver's four bilinear shares into delta in
+// receiver order (delta starts at 0: adj_setup_kernel); a node shared by
+// several receivers gets ((0 + v1) + v2) + ... in that order.  Here the wave
+// lists the event's 4*nrec shares (node, value) in that order in LDS, and the
+// lane holding a node's first share sums all of that node's shares in order
+// and stores the sum: the same bits, no dependent global read-modify-write
+// chain.  FP64: stored in both orientation planes.  FP32: summed in FP64 and
+// rounded once into both planes.  Events with more than kDeltaShares shares
+// take the receiver loop on one lane (the same sums, accumulated in place).
+// The chi term is summed by lane 0 in receiver order.
+constexpr int kDeltaShares = 1024;
+
 template <int PREC>
-__global__ void delta_kernel(PhaseDev pd) {
+__global__ __launch_bounds__(kWave) void delta_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
-  const int e = pd.ev_base + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  __shared__ int s_node[kDeltaShares];
+  __shared__ double s_val[kDeltaShares];
+  const int e = pd.ev_base + (int)blockIdx.x;
   if (e >= pd.ev_base + pd.nevent || pd.skip[e] != pd.want_skip) return;
+  const int lane = threadIdx.x;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
-  double s = 0.0;
-  for (int r = 0; r < ev.nrec; r++) {
-    const int k = ev.rec_off + r;
-    const double d = pd.syn[k] - pd.tt_obs[k];
-    s += pd.weight[k] * (d * d);
+  if (lane == 0) {
+    double s = 0.0;
+    for (int r = 0; r < ev.nrec; r++) {
+      const int k = ev.rec_off + r;
+      const double d = pd.syn[k] - pd.tt_obs[k];
+      s += pd.weight[k] * (d * d);
+    }
+    pd.chi_ev[e] = 0.5 * s;
   }
-  pd.chi_ev[e] = 0.5 * s;
-  const int k0 = ev.rec_off, k1 = ev.rec_off + ev.nrec;
+  const int k0 = ev.rec_off, ns = 4 * ev.nrec;
   V* dl0 = static_cast<V*>(pd.dl[0]) + ev.w_off;
   V* dl1 = static_cast<V*>(pd.dl[1]) + ev.w_off;
-  double* acc = pd.T0d + ev.pl_off;  // FP32 only
-  for (int pass = 0; pass < ((PREC == SAT_PREC_F64) ? 2 : 3); pass++) {
-    for (int k = k0; k < k1; k++) {
-      const int2 cc = pd.rcell[k];
-      const double w = pd.rwt[3 * (size_t)k], r1 = pd.rwt[3 * (size_t)k + 1], r2 = pd.rwt[3 * (size_t)k + 2];
-      for (int m = 0; m < 4; m++) {  // order of :371-374
-        const int i = cc.x + (m >> 1), j = cc.y + (m & 1);
+  auto share = [&](int c, int& i, int& j) -> double {  // share m of receiver k, order of :371-374
+    const int k = k0 + (c >> 2), m = c & 3;
+    const int2 cc = pd.rcell[k];
+    const double w = pd.rwt[3 * (size_t)k], r1 = pd.rwt[3 * (size_t)k + 1], r2 = pd.rwt[3 * (size_t)k + 2];
+    i = cc.x + (m >> 1);
+    j = cc.y + (m & 1);
+    return (m == 0) ? w * (1 - r1) * (1 - r2) : (m == 1) ? w * (1 - r1) * r2 : (m == 2) ? w * r1 * (1 - r2) : w * r1 * r2;
+  };
+  if (ns > kDeltaShares) {
+    if (lane != 0) return;
+    double* acc = pd.T0d + ev.pl_off;  // FP32: an FP64 accumulator (T0d is no longer needed once T is formed)
+    for (int pass = 0; pass < ((PREC == SAT_PREC_F64) ? 2 : 3); pass++) {
+      for (int c = 0; c < ns; c++) {
+        int i, j;
+        const double v = share(c, i, j);
         const size_t w0 = wave_index(g, 0, i, j), w1 = wave_index(g, 1, i, j);
         const size_t pp = i + (size_t)j * g.nx;
         if constexpr (PREC == SAT_PREC_F64) {
-          if (pass == 0) {
-            const double v = (m == 0) ? w * (1 - r1) * (1 - r2) : (m == 1) ? w * (1 - r1) * r2
-                           : (m == 2) ? w * r1 * (1 - r2) : w * r1 * r2;
-            dl0[w0] += v;
-          } else {
-            dl1[w1] = dl0[w0];
-          }
+          if (pass == 0) dl0[w0] += v;
+          else dl1[w1] = dl0[w0];
         } else {
           if (pass == 0) {
             acc[pp] = 0.0;
           } else if (pass == 1) {
-            const double v = (m == 0) ? w * (1 - r1) * (1 - r2) : (m == 1) ? w * (1 - r1) * r2
-                           : (m == 2) ? w * r1 * (1 - r2) : w * r1 * r2;
             acc[pp] += v;
           } else {
             const float f = (float)acc[pp];
@@ -491,6 +506,36 @@ __global__ void delta_kernel(PhaseDev pd) {
           }
         }
       }
+    }
+    return;
+  }
+  for (int c = lane; c < ns; c += kWave) {
+    int i, j;
+    s_val[c] = share(c, i, j);
+    s_node[c] = i + j * g.nx;
+  }
+  __syncthreads();
+  for (int c = lane; c < ns; c += kWave) {
+    const int n = s_node[c];
+    bool first = true;
+    for (int q = 0; q < c; q++)
+      if (s_node[q] == n) {
+        first = false;
+        break;
+      }
+    if (!first) continue;
+    double sum = 0.0;
+    for (int q = c; q < ns; q++)
+      if (s_node[q] == n) sum += s_val[q];
+    const int j = n / g.nx, i = n - j * g.nx;
+    const size_t w0 = wave_index(g, 0, i, j), w1 = wave_index(g, 1, i, j);
+    if constexpr (PREC == SAT_PREC_F64) {
+      dl0[w0] = sum;
+      dl1[w1] = sum;
+    } else {
+      const float f = (float)sum;
+      dl0[w0] = f;
+      dl1[w1] = f;
     }
   }
 }
