@@ -91,3 +91,47 @@ def test_shim_forward_for_event_matches_oracle(tmp_path):
     assert RC.rel_inf(syn, ro["syn"]) <= 1e-12
     assert RC.rel_inf(K, ro["adj_s"]) <= 1e-12
     assert abs(chi - ro["chi"]) <= 1e-12 * abs(ro["chi"])
+
+
+def test_surfker_shim_compiles(tmp_path):
+    if not os.path.exists(os.path.join(LIBDIR, "libsurfatt_eik.so")):
+        pytest.skip("library not built")
+    _build(tmp_path, "surfker_driver")
+
+
+@pytest.mark.gpu
+def test_surfker_shim_matches_python_and_reference(tmp_path):
+    """include/surfatt_surfker.hpp on the GPU: surfker::surfdisp equals the
+    reference's solver on the layers it built (bit for bit where the reference
+    build is available), fwdsurf and topo_media equal the Python paths."""
+    from surfattpp_amd import surfdisp as SD
+    from surfattpp_amd import topo as TG
+    exe = _build(tmp_path, "surfker_driver")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = {}
+    for line in r.stdout.splitlines():
+        k, *v = line.split()
+        rows.setdefault(k, []).append(np.array([float(x) for x in v]))
+    per = np.array([5.0, 10.0, 20.0, 30.0, 40.0])
+    for i, (iw, igr, sph) in enumerate(((2, 0, 0), (2, 1, 0), (1, 0, 0))):
+        lay = [rows[k][i] for k in ("layers_thk", "layers_vp", "layers_vs", "layers_rho")]
+        c = rows["c"][i]
+        g = SD.surfdisp96(*lay, per, iflsph=sph, iwave=iw, igr=igr)
+        assert np.array_equal(c, g)
+        if O.surfdisp_available():
+            assert np.array_equal(c, O.surfdisp96(*lay, per, iflsph=sph, iwave=iw, igr=igr))
+    assert abs(rows["c"][0][1] - 2.6747) <= 1e-4 and abs(rows["c"][0][3] - 3.8387) <= 1e-4
+    nx, ny, nz = 3, 2, 6
+    z = 10.0 * np.arange(nz)
+    ix, iy, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    vs3 = 2.9 + 0.3 * k + 0.05 * ix - 0.04 * iy
+    maps = SD.fwdsurf(z, vs3, per)                                  # (nper, ny, nx)
+    assert np.array_equal(rows["fwdsurf"][0], maps.transpose(2, 1, 0).ravel())  # [ix][iy][iper]
+    xg, yg = 120.0 + 0.02 * np.arange(9), 30.0 + 0.02 * np.arange(7)
+    jj, ii = np.meshgrid(np.arange(7), np.arange(9), indexing="ij")
+    zz = 1.5 * np.exp(-((ii - 4) ** 2 + (jj - 3) ** 2) / 6.0)        # (ny, nx)
+    sig = TG.sigma_for_periods([3.0, 3.4], [4.0, 8.0], 2.5)
+    m11, m12, m22 = TG.topo_metric(zz, xg, yg, sig)
+    for name, m in (("m11", m11), ("m12", m12), ("m22", m22)):
+        np.testing.assert_allclose(rows[name][0], m.transpose(2, 1, 0).ravel(), rtol=0, atol=1e-15)
