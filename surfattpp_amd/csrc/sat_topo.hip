@@ -124,11 +124,14 @@ __global__ __launch_bounds__(kThreads) void smooth_kernel(TopoDev d) {
         const int klo = max(0, hx - i), khi = min(K - 1, nx - 1 - i + hx);
         double sn = num[c], sd = den[c];
         for (int r = 0; r < rows; r++) {
-          const double* zr = z + (size_t)(r0 + r) * nx;
+          const double* __restrict__ zr = z + (size_t)(r0 + r) * nx + (i - hx);  // window column i - hx + k
           const double* wr = W + r * K;
-          for (int k = klo; k <= khi; k++) {  // window column i - hx + k in [0, nx)
+          // unrolled so that the loads of several window nodes are in flight
+          // at once; the sums keep the node order
+#pragma unroll 8
+          for (int k = klo; k <= khi; k++) {
             const double w = wr[k];
-            sn += w * zr[i - hx + k];
+            sn += w * zr[k];
             sd += w;
           }
         }
