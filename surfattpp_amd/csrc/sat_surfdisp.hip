@@ -146,10 +146,20 @@ __device__ double dltar4(double wvno, double omga, const Col& m, int llw) {
   e[2] = rho1 * (gamm1 - gammk * ra * rb);
   e[3] = rho1 * rb;
   e[4] = wvno2 - ra * rb;
+  // the next layer's values are loaded one layer ahead, so their loads are in
+  // flight while the current layer's matrix is formed
+  float nA = m.A(mm - 1), nB = m.B(mm - 1), nD = m.D(mm - 1), nR = m.R(mm - 1);
   for (int l = mm - 1; l >= llw - 1; l--) {
-    xka = omega / (double)m.A(l);
-    xkb = omega / (double)m.B(l);
-    t = (double)m.B(l) / omega;
+    const float lA = nA, lB = nB, lD = nD, lR = nR;
+    if (l >= 1) {
+      nA = m.A(l - 1);
+      nB = m.B(l - 1);
+      nD = m.D(l - 1);
+      nR = m.R(l - 1);
+    }
+    xka = omega / (double)lA;
+    xkb = omega / (double)lB;
+    t = (double)lB / omega;
     gammk = 2.0 * t * t;
     gam = gammk * wvno2;
     wvnop = wvno + xka;
@@ -158,8 +168,8 @@ __device__ double dltar4(double wvno, double omga, const Col& m, int llw) {
     wvnop = wvno + xkb;
     wvnom = fabs(wvno - xkb);
     rb = sqrt(wvnop * wvnom);
-    const double dpth = (double)m.D(l);
-    rho1 = (double)m.R(l);
+    const double dpth = (double)lD;
+    rho1 = (double)lR;
     const double p = ra * dpth, q = rb * dpth;
     const VarOut v = var(p, q, ra, rb, wvno, xka, xkb, dpth);
     // dnka (surfdisp96.f:1047-1094)
@@ -241,15 +251,22 @@ __device__ double dltar1(double wvno, double omega, const Col& m, int llw) {
   double rb = sqrt(wvnop * wvnom);
   double e1 = rho1 * rb;
   double e2 = 1.0 / (beta1 * beta1);
+  float nB = m.B(mm - 1), nD = m.D(mm - 1), nR = m.R(mm - 1);  // one layer ahead
   for (int l = mm - 1; l >= llw - 1; l--) {
-    beta1 = (double)m.B(l);
-    rho1 = (double)m.R(l);
+    const float lB = nB, lD = nD, lR = nR;
+    if (l >= 1) {
+      nB = m.B(l - 1);
+      nD = m.D(l - 1);
+      nR = m.R(l - 1);
+    }
+    beta1 = (double)lB;
+    rho1 = (double)lR;
     const double xmu = rho1 * beta1 * beta1;
     xkb = omega / beta1;
     wvnop = wvno + xkb;
     wvnom = fabs(wvno - xkb);
     rb = sqrt(wvnop * wvnom);
-    const double q = (double)m.D(l) * rb;
+    const double q = (double)lD * rb;
     double cosq, y, z, sinq;
     if (wvno < xkb) {
       sinq = sin(q);
@@ -258,7 +275,7 @@ __device__ double dltar1(double wvno, double omega, const Col& m, int llw) {
       cosq = cos(q);
     } else if (wvno == xkb) {
       cosq = 1.0;
-      y = (double)m.D(l);
+      y = (double)lD;
       z = 0.0;
     } else {
       double fac = 0.0;
