@@ -182,3 +182,35 @@ def test_batch_update_media_and_obs():
         rg = bt.results()
     assert RC.rel_inf(rg["adj_s"], ro["adj_s"]) <= TOL64
     assert abs(rg["chi"] - ro["chi"]) <= TOL64 * abs(ro["chi"])
+
+
+def test_receiver_spread_shared_cells_and_many_receivers_f64():
+    """The receiver spread in receiver order (src/eikonal_solver.cpp:356-375):
+    receivers that share delta cells (repeated positions, clusters within a
+    cell or two) and an event with more receivers (300) than the one-wave LDS
+    spread takes, in FP64 against the oracle and in per-pair mode."""
+    b = synth.config_checkerboard()
+    lon, lat = b["lon"], b["lat"]
+    rng = synth.Rng(21)
+    dx, dy = lon[1] - lon[0], lat[1] - lat[0]
+    # event 0: 300 receivers anywhere inside; event 1: 60 receivers in a 2x2-cell
+    # cluster, every fifth one repeated; event 2: 8 copies of one position
+    u = rng.uniform(600)
+    r0 = (lon[3] + u[:300] * (lon[-4] - lon[3]), lat[3] + u[300:] * (lat[-4] - lat[3]))
+    v = rng.uniform(120)
+    cx, cy = lon[30] + 0.3 * dx, lat[35] + 0.6 * dy
+    c_lon, c_lat = cx + 2 * dx * v[:60], cy + 2 * dy * v[60:]
+    c_lon[::5], c_lat[::5] = c_lon[0], c_lat[0]
+    r2 = (np.full(8, lon[20] + 0.25 * dx), np.full(8, lat[40] + 0.75 * dy))
+    rec_lon = np.concatenate([r0[0], c_lon, r2[0]])
+    rec_lat = np.concatenate([r0[1], c_lat, r2[1]])
+    nrec = np.array([300, 60, 8], np.int32)
+    off = np.array([0, 300, 360], np.int32)
+    src = [(lon[10], lat[12]), (lon[45], lat[50]), (lon[33], lat[20])]
+    tt = synth.great_circle_time(rec_lon, rec_lat, np.repeat([s[0] for s in src], nrec),
+                                 np.repeat([s[1] for s in src], nrec), 3.09)
+    bb = dict(b, ev_iper=np.zeros(3, np.int32), ev_lon=np.array([s[0] for s in src]),
+              ev_lat=np.array([s[1] for s in src]), ev_rec_off=off, ev_nrec=nrec, rec_lon=rec_lon,
+              rec_lat=rec_lat, tt_obs=tt, weight=1.0 + 0.5 * rng.uniform(368))
+    rg, ro = _batch_vs_oracle(bb)
+    print("chi", rg["chi"], "oracle", ro["chi"], flush=True)
