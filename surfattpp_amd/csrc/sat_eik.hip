@@ -252,7 +252,7 @@ struct GridHost {
   float4* d_rowtab = nullptr;
   char* d_wmed = nullptr;     // batched sweeps: wavefront-major media, per orientation
   char* d_wmedmw = nullptr;   // FP32 batches: FP64 wavefront-major media of the re-solve geometry
-  GridDev gmw{};              // FP32 batches: FP64 media, kMultiWave waves per grid (re-solves)
+  GridDev gmw{};              // FP32 batches: FP64 media, resolve_waves() waves per grid (re-solves)
   GridDev gseg{};             // g with one wave per pair (per-pair kernels; equals g when W == 1)
   int W = 1;                  // waves per pair of the batched sweeps
 
@@ -335,10 +335,12 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
     for (int i = 0; i < nmed; i++)
       for (int o = 0; o < 2; o++) g.wmed[i][o] = G.d_wmed + (size_t)(2 * i + o) * pb;
     if (prec == SAT_PREC_F32) {
-      // re-solve geometry: kMultiWave waves per grid, C = ceil(nx / (64 kMultiWave))
+      // re-solve geometry: Wr = resolve_waves(ceil(nx / 64)) waves per grid,
+      // C = ceil(nx / (64 Wr))
       GridDev& m = G.gmw;
       m = G.g64;
-      m.C = (nx + kWave * kMultiWave - 1) / (kWave * kMultiWave);
+      const int wr = resolve_waves((nx + kWave - 1) / kWave);
+      m.C = (nx + kWave * wr - 1) / (kWave * wr);
       m.La = (nx + m.C - 1) / m.C;
       m.P = m.La * m.C;
       m.wplane = (size_t)(ny + m.La - 1) * m.La * m.C;
@@ -1202,6 +1204,12 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   pd.conv = b->d_conv;
   pd.skip = b->d_skip;
   pd.want_skip = 0;
+  {
+    // the receiver spread's LDS: the largest event's shares, at most kDeltaShares
+    int max_nrec = 0;
+    for (const EventX& x : b->hx) max_nrec = std::max(max_nrec, x.nrec);
+    pd.delta_cap = std::min(kDeltaShares, std::max(4 * max_nrec, kScan));
+  }
   pd.ov_list = b->d_ov;
   pd.ov_count = ov_count;
   pd.pool_counter = pool_counters;
@@ -1226,8 +1234,10 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   for (const GridHost& G : b->grids) init_n = std::max({init_n, G.g.wplane, G.gmw.wplane});
   const unsigned init_bx = (unsigned)((init_n + 255) / 256);
   hipEvent_t* mk = b->mk;
-  // columns per lane of the re-solve geometry: ceil(nx / (64 kMultiWave)) = ceil(C / kMultiWave)
-  auto cmw = [&](size_t gi) { return (b->group_C[gi] + kMultiWave - 1) / kMultiWave; };
+  // waves per pair and columns per lane of the re-solve geometry:
+  // Wr = resolve_waves(C), ceil(nx / (64 Wr)) = ceil(C / Wr)
+  auto wmw = [&](size_t gi) { return resolve_waves(b->group_C[gi]); };
+  auto cmw = [&](size_t gi) { return (b->group_C[gi] + wmw(gi) - 1) / wmw(gi); };
 
   auto mark = [&](int i, hipStream_t st) -> hipError_t {
     return capturing ? hipSuccess : hipEventRecord(mk[i], st);
@@ -1325,7 +1335,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
             SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(init_bx, ub, 2), st, a));
             a.counter = resf_heads + gi;
             const dim3 grid = res_slots(gi, a, ub);
-            SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(0, cmw(gi), kMultiWave, grid, st, a, 0, 0, gi));
+            SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(0, cmw(gi), wmw(gi), grid, st, a, 0, 0, gi));
             return SAT_OK;
           });
           if (rc) return rc;
@@ -1348,7 +1358,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
               a.list_count = fb_count + gi;
               a.counter = resa_heads + gi;
               const dim3 grid = res_slots(gi, a, ub);
-              SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(cmw(gi), kMultiWave, grid, st, a, 0, 0));
+              SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(cmw(gi), wmw(gi), grid, st, a, 0, 0));
               return SAT_OK;
             });
             if (rc) return rc;

@@ -19,9 +19,17 @@ hipError_t launch_trig(dim3 grid, hipStream_t s, const PhaseDev& pd);
 template <int PREC>
 hipError_t launch_init(dim3 grid, hipStream_t s, const PhaseDev& pd);
 // Sweep kernels: W = 1 (one wave per event, C in [1, kMaxC]) or a work-group
-// of W waves per event: FP64 W = kMultiWave, C in [1, 2] (the re-solves of FP32
-// batches); FP32 W = 2 (C in [1, 4]) or 4 (C in [1, 2]).
+// of W waves per event: FP64 W = kMultiWave or kMultiWaveWide, C in [1, 2]
+// (the re-solves of FP32 batches); FP32 W = 2 (C in [1, 4]) or 4 (C in [1, 2]).
 constexpr int kMultiWave = 4;
+constexpr int kMultiWaveWide = 8;
+// Waves per pair of the FP64 re-solves of a grid whose one-wave sweeps take C1
+// columns per lane (C1 = ceil(nx / 64)): four up to 256 columns; eight for
+// 257..384 columns, which then sweep one column per lane instead of two.  The
+// re-solves are latency-bound; one column per lane halves a step but makes a
+// sweep longer by a third and its barrier span eight waves: measured, the
+// 321-column grid's re-solves gain 10 %, the 401-column grid's lose 11 %.
+inline int resolve_waves(int C1) { return (C1 > kMultiWave && C1 <= 6) ? kMultiWaveWide : kMultiWave; }
 template <int PREC>
 hipError_t launch_fwd_sweep(int iso, int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first,
                             int count, int group);

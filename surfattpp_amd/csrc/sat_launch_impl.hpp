@@ -53,8 +53,8 @@ hipError_t launch_fwd_sweep(int iso, int C, int W, dim3 grid, hipStream_t s, con
   });
 }
 
-// Multi-wave sweeps (a work-group of W waves per event): FP64 W = kMultiWave,
-// C <= 2, general metric (the re-solves of FP32 batches); FP32 W = 2 (C <= 4)
+// Multi-wave sweeps (a work-group of W waves per event): FP64 W = kMultiWave
+// or kMultiWaveWide, C <= 2, general metric (the re-solves of FP32 batches); FP32 W = 2 (C <= 4)
 // and W = 4 (C <= 2), both metrics.
 template <int PREC, int W, int C, bool ISO>
 static void launch_fwd_mw1(dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count, int group) {
@@ -65,9 +65,11 @@ template <int PREC>
 hipError_t launch_fwd_sweep_mw(int iso, int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first,
                                int count, int group) {
   if constexpr (PREC == SAT_PREC_F64) {
-    if (W != kMultiWave || iso) return hipErrorInvalidValue;
-    if (C == 1) launch_fwd_mw1<PREC, kMultiWave, 1, false>(grid, s, pd, first, count, group);
-    else if (C == 2) launch_fwd_mw1<PREC, kMultiWave, 2, false>(grid, s, pd, first, count, group);
+    if (iso) return hipErrorInvalidValue;
+    if (W == kMultiWave && C == 1) launch_fwd_mw1<PREC, kMultiWave, 1, false>(grid, s, pd, first, count, group);
+    else if (W == kMultiWave && C == 2) launch_fwd_mw1<PREC, kMultiWave, 2, false>(grid, s, pd, first, count, group);
+    else if (W == kMultiWaveWide && C == 1)  // grids of 257..384 columns (resolve_waves)
+      launch_fwd_mw1<PREC, kMultiWaveWide, 1, false>(grid, s, pd, first, count, group);
     else return hipErrorInvalidValue;
   } else {
     auto go = [&]<int WW, int CC>() {
@@ -93,7 +95,7 @@ hipError_t launch_adj_setup(dim3 grid, hipStream_t s, const PhaseDev& pd) {
 
 template <int PREC>
 hipError_t launch_delta(dim3 grid, hipStream_t s, const PhaseDev& pd) {
-  hipLaunchKernelGGL((delta_kernel<PREC>), grid, dim3(kWave), 0, s, pd);
+  hipLaunchKernelGGL((delta_kernel<PREC>), grid, dim3(kWave), delta_lds_bytes(pd.delta_cap), s, pd);
   return hipGetLastError();
 }
 
@@ -111,9 +113,9 @@ hipError_t launch_adj_sweep_mw(int C, int W, dim3 grid, hipStream_t s, const Pha
     hipLaunchKernelGGL((adj_sweep_kernel<PREC, CC, WW>), grid, dim3(kWave * WW), 0, s, pd, first, count);
   };
   if constexpr (PREC == SAT_PREC_F64) {
-    if (W != kMultiWave) return hipErrorInvalidValue;
-    if (C == 1) go.template operator()<kMultiWave, 1>();
-    else if (C == 2) go.template operator()<kMultiWave, 2>();
+    if (W == kMultiWave && C == 1) go.template operator()<kMultiWave, 1>();
+    else if (W == kMultiWave && C == 2) go.template operator()<kMultiWave, 2>();
+    else if (W == kMultiWaveWide && C == 1) go.template operator()<kMultiWaveWide, 1>();
     else return hipErrorInvalidValue;
   } else {
     if (W == 2 && C == 1) go.template operator()<2, 1>();

@@ -71,6 +71,7 @@ struct PhaseDev {
   unsigned char* skip;  // FP32 batches: 0 FP32 result, 1 FP64 re-solve by these kernels,
                         // 2 FP64 re-solve by the per-pair kernel (pool full)
   int want_skip;        // the skip value of the events this pass processes
+  int delta_cap;        // shares per event the receiver spread holds in LDS (launch's dynamic LDS)
   int* fb_list;         // FP32 forward: pool re-solves, per launch group
   int* fb_count;
   int* ov_list;         // FP32 forward: re-solves beyond the pool, per launch group
@@ -113,21 +114,22 @@ __device__ __forceinline__ void latency_critical_priority() {
 // Rows in flight per lane of the multi-wave sweeps: the register ring of
 // kRingDepth rows where it fits, two rows for the FP32 forward at C >= 3
 // (its per-row body is ten registers per column), and kRingDepthC1 rows for
-// the FP64 sweeps at one column per lane -- the re-solves, whose steps are
-// short enough that four rows in flight leave memory latency exposed (at 8
-// rows the forward kernel holds 256 VGPRs without spilling; two columns per
-// lane would spill, so they keep kRingDepth).
+// the FP64 sweeps at one column per lane and four waves -- the re-solves,
+// whose steps are short enough that four rows in flight leave memory latency
+// exposed (at 8 rows the forward kernel holds 256 VGPRs without spilling; two
+// columns per lane, or eight waves with their 256-VGPR limit, would spill, so
+// they keep kRingDepth).
 #ifndef SAT_RING_DEPTH_C1
 #define SAT_RING_DEPTH_C1 8
 #endif
 constexpr int kRingDepthC1 = SAT_RING_DEPTH_C1;
-template <int PREC, int C>
+template <int PREC, int C, int W>
 constexpr int mw_fwd_depth() {
-  return (PREC == SAT_PREC_F32 && C >= 3) ? 2 : (PREC == SAT_PREC_F64 && C == 1) ? kRingDepthC1 : kRingDepth;
+  return (PREC == SAT_PREC_F32 && C >= 3) ? 2 : (PREC == SAT_PREC_F64 && C == 1 && W <= 4) ? kRingDepthC1 : kRingDepth;
 }
-template <int PREC, int C>
+template <int PREC, int C, int W>
 constexpr int mw_adj_depth() {
-  return (PREC == SAT_PREC_F64 && C == 1) ? kRingDepthC1 : kRingDepth;
+  return (PREC == SAT_PREC_F64 && C == 1 && W <= 4) ? kRingDepthC1 : kRingDepth;
 }
 
 template <int PREC> struct PhaseT;
@@ -354,7 +356,7 @@ void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
     __syncthreads();
     const auto cx = uniform(phase_fwd_ctx<PREC, ISO, Pol>(pd, g, ev));
     IterOut io;
-    const U* u = iterate<Pol, WaveLayout, C, W, (W > 1) ? mw_fwd_depth<PREC, C>() : kFwdDepth1>(cx, Q0, Q1, Q2, pd.fwd_maxiter, kFwdTol,
+    const U* u = iterate<Pol, WaveLayout, C, W, (W > 1) ? mw_fwd_depth<PREC, C, W>() : kFwdDepth1>(cx, Q0, Q1, Q2, pd.fwd_maxiter, kFwdTol,
                                                                           io, exchange_ptr<Pol, W>(xch), red);
     if (lane == 0) {
       pd.conv[2 * (size_t)e] = io.linf;
@@ -446,17 +448,25 @@ __global__ void adj_setup_kernel(PhaseDev pd) {
 // lists the event's 4*nrec shares (node, value) in that order in LDS, and the
 // lane holding a node's first share sums all of that node's shares in order
 // and stores the sum: the same bits, no dependent global read-modify-write
-// chain.  FP64: stored in both orientation planes.  FP32: summed in FP64 and
-// rounded once into both planes.  Events with more than kDeltaShares shares
-// take the receiver loop on one lane (the same sums, accumulated in place).
-// The chi term is summed by lane 0 in receiver order.
-constexpr int kDeltaShares = 1024;
+// chain.  The scans read kScan shares per batch so their loads overlap.
+// FP64: stored in both orientation planes.  FP32: summed in FP64 and rounded
+// once into both planes.  The launch sizes the LDS for its largest event
+// (delta_cap shares, at most kDeltaShares); an event with more shares takes
+// the receiver loop on one lane (the same sums, accumulated in place).  The
+// chi term is summed by lane 0 in receiver order.
+constexpr int kDeltaShares = 2048;
+constexpr int kScan = 16;
+__host__ __device__ inline size_t delta_lds_bytes(int cap) {
+  return (size_t)(cap + kScan) * (sizeof(double) + sizeof(int));
+}
 
 template <int PREC>
 __global__ __launch_bounds__(kWave) void delta_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
-  __shared__ int s_node[kDeltaShares];
-  __shared__ double s_val[kDeltaShares];
+  extern __shared__ double dsh[];
+  const int cap = pd.delta_cap;
+  double* s_val = dsh;                                    // [cap + kScan]
+  int* s_node = reinterpret_cast<int*>(dsh + cap + kScan);  // [cap + kScan]
   const int e = pd.ev_base + (int)blockIdx.x;
   if (e >= pd.ev_base + pd.nevent || pd.skip[e] != pd.want_skip) return;
   const int lane = threadIdx.x;
@@ -482,7 +492,7 @@ __global__ __launch_bounds__(kWave) void delta_kernel(PhaseDev pd) {
     j = cc.y + (m & 1);
     return (m == 0) ? w * (1 - r1) * (1 - r2) : (m == 1) ? w * (1 - r1) * r2 : (m == 2) ? w * r1 * (1 - r2) : w * r1 * r2;
   };
-  if (ns > kDeltaShares) {
+  if (ns > cap) {
     if (lane != 0) return;
     double* acc = pd.T0d + ev.pl_off;  // FP32: an FP64 accumulator (T0d is no longer needed once T is formed)
     for (int pass = 0; pass < ((PREC == SAT_PREC_F64) ? 2 : 3); pass++) {
@@ -509,24 +519,40 @@ __global__ __launch_bounds__(kWave) void delta_kernel(PhaseDev pd) {
     }
     return;
   }
-  for (int c = lane; c < ns; c += kWave) {
-    int i, j;
-    s_val[c] = share(c, i, j);
-    s_node[c] = i + j * g.nx;
+  for (int c = lane; c < ns + kScan; c += kWave) {
+    if (c < ns) {
+      int i, j;
+      s_val[c] = share(c, i, j);
+      s_node[c] = i + j * g.nx;
+    } else {
+      s_val[c] = 0.0;
+      s_node[c] = -1;  // scan padding: matches no node
+    }
   }
   __syncthreads();
   for (int c = lane; c < ns; c += kWave) {
     const int n = s_node[c];
     bool first = true;
-    for (int q = 0; q < c; q++)
-      if (s_node[q] == n) {
-        first = false;
-        break;
-      }
+    for (int q0 = 0; q0 < c && first; q0 += kScan) {
+      bool hit = false;
+#pragma unroll
+      for (int u = 0; u < kScan; u++) hit |= (q0 + u < c) & (s_node[q0 + u] == n);
+      first = !hit;
+    }
     if (!first) continue;
     double sum = 0.0;
-    for (int q = c; q < ns; q++)
-      if (s_node[q] == n) sum += s_val[q];
+    for (int q0 = c; q0 < ns; q0 += kScan) {  // reads up to ns - 1 + kScan - 1: the padding
+      int nd[kScan];
+      double vv[kScan];
+#pragma unroll
+      for (int u = 0; u < kScan; u++) {
+        nd[u] = s_node[q0 + u];
+        vv[u] = s_val[q0 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < kScan; u++)
+        if (nd[u] == n) sum += vv[u];  // in share order
+    }
     const int j = n / g.nx, i = n - j * g.nx;
     const size_t w0 = wave_index(g, 0, i, j), w1 = wave_index(g, 1, i, j);
     if constexpr (PREC == SAT_PREC_F64) {
@@ -586,7 +612,7 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
       cx.dy = g.dy;
     }
     IterOut io;
-    const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? mw_adj_depth<PREC, C>() : kAdjDepth1>(uniform(cx), Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,
+    const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? mw_adj_depth<PREC, C, W>() : kAdjDepth1>(uniform(cx), Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,
                                                                            io, exchange_ptr<Pol, W>(xch), red);
     if (lane == 0) {
       pd.iters_a[e] = io.iters;
