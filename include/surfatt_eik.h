@@ -194,6 +194,14 @@ int sat_combine_kernels(int nx, int ny, int nz, int nper, int mode, const double
                         const double* vs3d, const double* vp3d, double* ker_vs, double* ker_vp,
                         double* ker_rho, double* kernel_ms);
 
+/* sat_combine_kernels with the per-period kernels of a batch's last run, read
+ * in place on the device (no copy through the host): grid `grid` of the
+ * batch, its nx, ny, nperiod; the other arguments as sat_combine_kernels.
+ * Waits for the batch's last run.  SAT_EINVAL for a batch without adjoint. */
+int sat_batch_combine_kernels(sat_batch* b, int grid, int nz, int mode, const double* sen_vs, const double* sen_vp,
+                              const double* sen_rho, const double* vs3d, const double* vp3d, double* ker_vs,
+                              double* ker_vp, double* ker_rho, double* kernel_ms);
+
 /* The anisotropic metric of a topographic surface for every period:
  * SurfGrid::build_media_matrix_with_topo (src/surf_grid.cpp:109-145, :184-190)
  * -- the surface z [ny][nx] (km, ix fastest: the reference's column-major

@@ -108,13 +108,16 @@ __global__ __launch_bounds__(kThreads) void combine_kernel(CombineDev d) {
 
 }  // namespace
 
-extern "C" int sat_combine_kernels(int nx, int ny, int nz, int nper, int mode, const double* adj_s,
-                                   const double* sen_vs, const double* sen_vp, const double* sen_rho,
-                                   const double* vs3d, const double* vp3d, double* ker_vs, double* ker_vp,
-                                   double* ker_rho, double* kernel_ms) {
+namespace sat {
+
+// The combination with the per-period kernels either on the host (adj_s,
+// uploaded) or already on the device (adj_dev, [nper][ny][nx], read in place).
+int combine_impl(int nx, int ny, int nz, int nper, int mode, const double* adj_s, const double* adj_dev,
+                 const double* sen_vs, const double* sen_vp, const double* sen_rho, const double* vs3d,
+                 const double* vp3d, double* ker_vs, double* ker_vp, double* ker_rho, double* kernel_ms) {
   if (nx < 1 || ny < 1 || nz < 1 || nper < 1 || mode < 0 || mode > 2) return SAT_EINVAL;
   if (nper > kMaxPeriods) return SAT_EINVAL;  // the staged rows must fit in LDS
-  if (!adj_s || !sen_vs || !ker_vs) return SAT_EINVAL;
+  if ((!adj_s && !adj_dev) || !sen_vs || !ker_vs) return SAT_EINVAL;
   if (mode == 0 && (!sen_vp || !sen_rho || !vs3d || !vp3d)) return SAT_EINVAL;
   if (mode > 0 && (!sen_vp || !ker_vp || !ker_rho || (mode == 1 && !sen_rho))) return SAT_EINVAL;
   int ndev = 0;
@@ -125,19 +128,20 @@ extern "C" int sat_combine_kernels(int nx, int ny, int nz, int nper, int mode, c
   hipEvent_t e0 = nullptr, e1 = nullptr;
   CombineDev d{};
   // one allocation: adj | s_vs | s_vp | s_rho | vs3d | vp3d | k_vs | k_vp | k_rho
-  const size_t total = nper * n2 + 3 * n4 + 5 * n3;
+  const size_t total = (adj_dev ? 0 : nper * n2) + 3 * n4 + 5 * n3;
   C_CHECK(hipMalloc(&buf, sizeof(double) * total));
   {
     double* p = buf;
     auto take = [&](size_t n) { double* q = p; p += n; return q; };
-    double* adj = take(nper * n2);
+    double* adj = take(adj_dev ? 0 : nper * n2);
     double* s_vs = take(n4);
     double* s_vp = take(n4);
     double* s_rho = take(n4);
     double* v_s = take(n3);
     double* v_p = take(n3);
-    d = CombineDev{nx, ny, nz, nper, mode, adj, s_vs, s_vp, s_rho, v_s, v_p, take(n3), take(n3), take(n3)};
-    C_CHECK(hipMemcpy(adj, adj_s, sizeof(double) * nper * n2, hipMemcpyHostToDevice));
+    d = CombineDev{nx, ny, nz, nper, mode, adj_dev ? adj_dev : adj, s_vs, s_vp, s_rho, v_s, v_p, take(n3),
+                   take(n3), take(n3)};
+    if (!adj_dev) C_CHECK(hipMemcpy(adj, adj_s, sizeof(double) * nper * n2, hipMemcpyHostToDevice));
     C_CHECK(hipMemcpy(s_vs, sen_vs, sizeof(double) * n4, hipMemcpyHostToDevice));
     if (sen_vp) C_CHECK(hipMemcpy(s_vp, sen_vp, sizeof(double) * n4, hipMemcpyHostToDevice));
     if (sen_rho && mode != 2) C_CHECK(hipMemcpy(s_rho, sen_rho, sizeof(double) * n4, hipMemcpyHostToDevice));
@@ -172,4 +176,15 @@ done:
   if (e1) (void)hipEventDestroy(e1);
   if (buf) (void)hipFree(buf);
   return rc;
+}
+
+}  // namespace sat
+
+extern "C" int sat_combine_kernels(int nx, int ny, int nz, int nper, int mode, const double* adj_s,
+                                   const double* sen_vs, const double* sen_vp, const double* sen_rho,
+                                   const double* vs3d, const double* vp3d, double* ker_vs, double* ker_vp,
+                                   double* ker_rho, double* kernel_ms) {
+  if (!adj_s) return SAT_EINVAL;
+  return sat::combine_impl(nx, ny, nz, nper, mode, adj_s, nullptr, sen_vs, sen_vp, sen_rho, vs3d, vp3d, ker_vs,
+                           ker_vp, ker_rho, kernel_ms);
 }

@@ -765,6 +765,12 @@ void event_boxes(const GridHost& G, const std::vector<double>& xx, const std::ve
 
 }  // namespace
 
+namespace sat {
+int combine_impl(int nx, int ny, int nz, int nper, int mode, const double* adj_s, const double* adj_dev,
+                 const double* sen_vs, const double* sen_vp, const double* sen_rho, const double* vs3d,
+                 const double* vp3d, double* ker_vs, double* ker_vp, double* ker_rho, double* kernel_ms);
+}  // namespace sat
+
 extern "C" {
 
 int sat_eik_forward(int precision, int nx, int ny, const double* lon_deg, const double* lat_deg,
@@ -1529,6 +1535,18 @@ int sat_batch_sync(sat_batch* b) {
   if (!b) return SAT_EINVAL;
   SAT_CHECK(hipStreamSynchronize(b->stream));
   return SAT_OK;
+}
+
+int sat_batch_combine_kernels(sat_batch* b, int grid, int nz, int mode, const double* sen_vs, const double* sen_vp,
+                              const double* sen_rho, const double* vs3d, const double* vp3d, double* ker_vs,
+                              double* ker_vp, double* ker_rho, double* kernel_ms) {
+  if (!b || !b->calc_adj || grid < 0 || grid >= (int)b->grids.size()) return SAT_EINVAL;
+  size_t off = 0;  // the grid's per-period kernels in the batch's kernel planes (grids in order)
+  for (int gi = 0; gi < grid; gi++) off += (size_t)b->grids[gi].g.nperiod * b->grids[gi].g.plane;
+  const GridDev& g = b->grids[grid].g;
+  SAT_CHECK(hipStreamSynchronize(b->stream));  // the last run's kernels are complete
+  return sat::combine_impl(g.nx, g.ny, nz, g.nperiod, mode, nullptr, b->d_K + off, sen_vs, sen_vp, sen_rho, vs3d,
+                           vp3d, ker_vs, ker_vp, ker_rho, kernel_ms);
 }
 
 int sat_batch_last_ms(sat_batch* b, double* total_ms, double* solve_ms) {

@@ -170,6 +170,37 @@ class EikonalBatch:
         _lib.check("sat_batch_conv", rc)
         return last[: self.nevent], prev[: self.nevent], n.value
 
+    def combine_kernels(self, sen_vs, sen_vp, sen_rho=None, vs3d=None, vp3d=None, mode: int = 0, grid: int = 0,
+                        timing: dict | None = None):
+        """combine_kernels (below) on this batch's per-period kernels of its last
+        run, read on the device in place (grid `grid`)."""
+        nper, ny, nx = self.shapes[grid]
+        s_vs = _c64(sen_vs)
+        nz = s_vs.shape[2]
+        shape4 = (nx, ny, nz, nper)
+        for a in (s_vs, sen_vp) + ((sen_rho,) if mode != COMBINE_VS_VP_RHO_SCALED else ()):
+            if a is None or np.shape(a) != shape4:
+                raise ValueError(f"sensitivities must be {shape4}")
+        s_vp = _c64(sen_vp)
+        s_rho = _c64(sen_rho) if sen_rho is not None else None
+        v_s = _c64(vs3d) if vs3d is not None else None
+        v_p = _c64(vp3d) if vp3d is not None else None
+        if mode == COMBINE_VS and (s_rho is None or v_s is None or v_p is None
+                                   or v_s.shape != (nx, ny, nz) or v_p.shape != (nx, ny, nz)):
+            raise ValueError("mode 0 needs sen_rho and vs3d, vp3d of shape (nx, ny, nz)")
+        k_vs = np.empty((nx, ny, nz))
+        k_vp = np.empty((nx, ny, nz)) if mode else None
+        k_rho = np.empty((nx, ny, nz)) if mode else None
+        ms = C.c_double(0.0)
+        ptr = lambda a: a.ctypes.data if a is not None else None  # noqa: E731
+        rc = _lib.load().sat_batch_combine_kernels(self._h, int(grid), nz, int(mode), ptr(s_vs), ptr(s_vp),
+                                                   ptr(s_rho), ptr(v_s), ptr(v_p), ptr(k_vs), ptr(k_vp),
+                                                   ptr(k_rho), C.byref(ms))
+        _lib.check("sat_batch_combine_kernels", rc)
+        if timing is not None:
+            timing["kernel_ms"] = ms.value
+        return k_vs, k_vp, k_rho
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().sat_batch_destroy(self._h)

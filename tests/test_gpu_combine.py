@@ -34,3 +34,27 @@ def test_combine_full_size_timed():
     np.testing.assert_array_equal(k0, want)
     nbytes = 8 * (3 * s[0].size + adj.size + 2 * vs.size + k0.size)
     print(f"combine 201x201x30x30: {t['kernel_ms']:.3f} ms, {nbytes / t['kernel_ms'] / 1e6:.0f} GB/s", flush=True)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_combine_reads_device_kernels(mode):
+    """sat_batch_combine_kernels on a batch's device kernels equals
+    sat_combine_kernels on the same kernels copied to the host, bit for bit,
+    for each grid of an uneven batch."""
+    from surfattpp_amd import synth
+    grids = [synth.config_grf(seed=7, nx=41, ny=37, nsta=5, nperiod=3),
+             synth.config_grf(seed=8, nx=33, ny=45, nsta=4, nperiod=2)]
+    with PP.EikonalBatch(grids) as bt:
+        bt.run()
+        res = bt.results()
+        for gi, g in enumerate(grids):
+            K = res["adj_s"][gi]
+            nper, ny, nx = K.shape
+            _, s, vs, vp = _inputs(nx, ny, 7, nper, 30 + gi + mode)
+            want = PP.combine_kernels(K, *s, vs, vp, mode=mode)
+            got = bt.combine_kernels(*s, vs, vp, mode=mode, grid=gi)
+            for a, b in zip(got, want):
+                if b is None:
+                    assert a is None
+                else:
+                    np.testing.assert_array_equal(a, b)
