@@ -188,11 +188,13 @@ def subset_events(batch: dict, ev_ids) -> dict:
     new_off = np.concatenate([[0], np.cumsum(nrecs)[:-1]]).astype(np.int32) if ev_ids.size else np.zeros(0, np.int32)
     out = dict(batch)
     for k in ("ev_iper", "ev_lon", "ev_lat", "ev_src"):
-        out[k] = batch[k][ev_ids].copy()
+        if k in batch:
+            out[k] = batch[k][ev_ids].copy()
     out["ev_rec_off"] = new_off
     out["ev_nrec"] = nrecs.astype(np.int32).copy()
     for k in ("rec_idx", "rec_lon", "rec_lat", "tt_obs", "weight"):
-        out[k] = batch[k][sel].copy()
+        if k in batch:
+            out[k] = batch[k][sel].copy()
     return out
 
 

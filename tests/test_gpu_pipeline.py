@@ -1,0 +1,61 @@
+"""One inversion iteration's data flow on the GPU, from the reference's example
+00 inputs (examples/00_checkerboard_iso/input_params.yml): the observation
+table (rows of src_rec_file_ph.csv, tests/golden) -> the station-box grid
+(0.02 deg, 5 cells of margin) -> the initial 1-D model (depth 0-15 km every
+0.5 km, Vs rising linearly 1.8 -> 4.2 km/s, init_model_type 0) -> Rayleigh
+phase-velocity maps by surfdisp96 at the table's periods (sat_fwdsurf) -> the
+forward + adjoint batch on those maps -> model kernels from the batch's device
+kernels (sat_batch_combine_kernels).  Checked: map columns against the
+reference's solver, the batch (FP64, a subset of events) against the oracle,
+the model kernels against the host-input combine of the same kernels."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from surfattpp_amd import preproc as PP
+from surfattpp_amd import src_rec as S
+from surfattpp_amd import surfdisp as SD
+from surfattpp_amd import synth
+from tests import refcases as RC
+from tests.test_combine_oracle import _inputs
+
+pytestmark = pytest.mark.gpu
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "src_rec_checkerboard.csv")
+
+
+def test_iteration_from_example_inputs():
+    sr = S.SrcRec().load(TABLE)
+    lon, lat = S.build_model_axes(S.build_stas(sr, None), [0.02, 0.02], 5)
+    nx, ny = lon.size, lat.size
+    z = np.arange(0.0, 15.0 + 1e-9, 0.5)
+    vs1 = 1.8 + (4.2 - 1.8) * z / z[-1]
+    vs3d = np.ascontiguousarray(np.broadcast_to(vs1, (nx, ny, z.size)))
+    periods = sr.periods_info.periods
+    svel = SD.fwdsurf(z, vs3d, periods)                        # (nper, ny, nx)
+    if O.surfdisp_available():
+        thk, vp, vs, rho = SD.build_disp_req(z, vs1)
+        np.testing.assert_array_equal(svel[:, 0, 0], O.surfdisp96(thk, vp, vs, rho, periods))
+    b = sr.to_batch(lon, lat, svel)
+    print(f"grid {nx}x{ny}, {len(b['ev_iper'])} events, periods {periods.min()}-{periods.max()} s, "
+          f"maps {svel.min():.3f}-{svel.max():.3f} km/s", flush=True)
+    sub = synth.subset_events(b, [0, 7, 19, 31])
+    ro = O.forward_for_event(sub)
+    with PP.EikonalBatch(sub) as bt:
+        bt.run()
+        rg = bt.results()
+    assert RC.rel_inf(rg["syn"], ro["syn"]) <= 1e-12
+    assert RC.rel_inf(rg["adj_s"], ro["adj_s"]) <= 1e-12
+    with PP.EikonalBatch(b) as bt:
+        bt.run()
+        res = bt.results()
+        nper = periods.size
+        _, sens, vsm, vpm = _inputs(nx, ny, z.size, nper, 77)
+        k_dev = bt.combine_kernels(*sens, vsm, vpm, mode=0)
+        k_host = PP.combine_kernels(res["adj_s"], *sens, vsm, vpm, mode=0)
+    np.testing.assert_array_equal(k_dev[0], k_host[0])
+    assert np.isfinite(res["chi"]) and np.isfinite(k_dev[0]).all()
+    sr.scatter_syn(res["syn"])
+    sr.gather_syn_tt()
+    assert np.all(sr.tt_fwd > 0)
