@@ -202,6 +202,21 @@ int sat_batch_combine_kernels(sat_batch* b, int grid, int nz, int mode, const do
                               const double* sen_rho, const double* vs3d, const double* vp3d, double* ker_vs,
                               double* ker_vp, double* ker_rho, double* kernel_ms);
 
+/* The anisotropic terms of preproc::accumulate_kernels (src/preproc.cpp:
+ * 37-65, taken when is_anisotropy is set) for grid `grid` of the batch's last
+ * run: per (period, node), over the period's events in batch order, with
+ * Tx, Ty = gradient_2_geo(T) (include/utils.h:595-649) of the event's FP64
+ * travel time and Ta its masked adjoint field,
+ *   adj_xi  += Ta * (Tx^2 (c^2 - a^2) + 2 TxTy (ac - bc) + Ty^2 (b^2 - c^2))
+ *   adj_eta += Ta * (2 ac Tx^2 - 2 (ab + c^2) TxTy + 2 bc Ty^2).
+ * a, b, c: SurfGrid's a, b, c (the topography metric: m11, m22 and -m12 of
+ * sat_topo_metric), [nperiod][ny][nx] host arrays; adj_xi, adj_eta the same
+ * shape.  Waits for the last run; synchronous.  SAT_EINVAL for a batch
+ * without adjoint or one whose runs are split into chunks (its fields are
+ * not all resident at the end of a run). */
+int sat_batch_aniso_kernels(sat_batch* b, int grid, const double* a, const double* bm, const double* c,
+                            double* adj_xi, double* adj_eta);
+
 /* The anisotropic metric of a topographic surface for every period:
  * SurfGrid::build_media_matrix_with_topo (src/surf_grid.cpp:109-145, :184-190)
  * -- the surface z [ny][nx] (km, ix fastest: the reference's column-major

@@ -31,7 +31,8 @@ EXPORTS = (
     "sat_batch_create", "sat_batch_update_media", "sat_batch_update_obs", "sat_batch_run",
     "sat_batch_sync", "sat_batch_results", "sat_batch_last_ms", "sat_batch_destroy",
     "sat_batch_set_f32_band", "sat_batch_conv", "sat_batch_phase_ms", "sat_batch_waves",
-    "sat_combine_kernels", "sat_batch_combine_kernels", "sat_topo_metric", "sat_surfdisp96", "sat_fwdsurf", "sat_device_info",
+    "sat_combine_kernels", "sat_batch_combine_kernels", "sat_batch_aniso_kernels",
+    "sat_topo_metric", "sat_surfdisp96", "sat_fwdsurf", "sat_device_info",
 )
 
 
@@ -98,6 +99,7 @@ def load(path: str = LIB_PATH):
     L.sat_batch_waves.argtypes = [vp, vp]
     L.sat_combine_kernels.argtypes = [C.c_int] * 5 + [vp] * 9 + [dp]
     L.sat_batch_combine_kernels.argtypes = [vp, C.c_int, C.c_int, C.c_int] + [vp] * 8 + [dp]
+    L.sat_batch_aniso_kernels.argtypes = [vp, C.c_int] + [vp] * 5
     L.sat_topo_metric.argtypes = [C.c_int] * 3 + [vp] * 8 + [dp]
     L.sat_surfdisp96.argtypes = [C.c_int, C.c_int] + [vp] * 4 + [C.c_int] * 5 + [vp] * 3 + [dp]
     L.sat_fwdsurf.argtypes = [C.c_int] * 3 + [vp] * 4 + [C.c_int, vp] + [C.c_int] * 4 + [vp, vp, dp]

@@ -161,6 +161,73 @@ __global__ void reduce_kernel(ReduceDev rd) {
   }
 }
 
+// Anisotropic kernel terms of accumulate_kernels (src/preproc.cpp:37-65) for
+// the (grid, period) planes of one grid: per node, over the plane's events in
+// batch order, the travel-time gradient (gradient_2_geo, include/utils.h:
+// 595-649: central differences inside, one-sided at the edges, dx = dlon_i *
+// R * DEG2RAD * cos(lat_j), dy = dlat_j * R * DEG2RAD) and
+//   adj_xi  += Ta * (Tx^2 (c^2 - a^2) + 2 TxTy (ac - bc) + Ty^2 (b^2 - c^2)),
+//   adj_eta += Ta * (2 ac Tx^2 - 2 (ab + c^2) TxTy + 2 bc Ty^2),
+// a, b, c the topography metric (SurfGrid a, b, c), in the reference's
+// operation order, FP64.
+struct AnisoDev {
+  GridDev g;
+  int nper, pl0;                // planes pl0 .. pl0 + nper - 1 of the reduction CSR
+  const int* ev_off;
+  const int* ev_ids;
+  const EventDev* events;
+  const double* T;              // per event, column-major, at events[e].ta_off
+  const void* ta;               // masked Ta, sweep precision, same offsets
+  const double* abc;            // a, b, c: [3][nper][ny][nx]
+  double* xi;                   // [nper][ny][nx]
+  double* eta;
+};
+
+template <typename V>
+__global__ void aniso_kernel(AnisoDev ad) {
+  const GridDev& g = ad.g;
+  const int nx = g.nx, ny = g.ny;
+  const size_t N = (size_t)nx * ny, total = N * ad.nper;
+  const double km = kR * kDEG2RAD;
+  const V* ta = static_cast<const V*>(ad.ta);
+  for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
+    const int p = (int)(id / N);
+    const size_t n = id - (size_t)p * N;
+    const int iy = (int)(n / nx), ix = (int)(n - (size_t)iy * nx);
+    const double cos_lat = g.trig_yy[iy].y;
+    auto dxk = [&](int i) { return ((g.lon[i] - g.lon[i - 1]) * km) * cos_lat; };
+    auto dyk = [&](int j) { return (g.lat[j] - g.lat[j - 1]) * km; };
+    const double a = ad.abc[id], b = ad.abc[total + id], c = ad.abc[2 * total + id];
+    const double a2 = a * a, b2 = b * b, c2 = c * c, ac = a * c, bc = b * c, ab = a * b;
+    double xi = 0.0, eta = 0.0;
+    const int pl = ad.pl0 + p;
+    for (int q = ad.ev_off[pl]; q < ad.ev_off[pl + 1]; q++) {
+      const EventDev& ev = ad.events[ad.ev_ids[q]];
+      const double* f = ad.T + ev.ta_off;
+      const double* row = f + (size_t)iy * nx;
+      double tx, ty;
+      if (ix == 0)
+        tx = (row[1] - row[0]) / dxk(1);
+      else if (ix == nx - 1)
+        tx = (row[nx - 1] - row[nx - 2]) / dxk(nx - 1);
+      else
+        tx = (row[ix + 1] - row[ix - 1]) / (2.0 * dxk(ix));
+      if (iy == 0)
+        ty = (f[nx + ix] - f[ix]) / dyk(1);
+      else if (iy == ny - 1)
+        ty = (f[(size_t)(ny - 1) * nx + ix] - f[(size_t)(ny - 2) * nx + ix]) / dyk(ny - 1);
+      else
+        ty = (f[(size_t)(iy + 1) * nx + ix] - f[(size_t)(iy - 1) * nx + ix]) / (2.0 * dyk(iy));
+      const double tx2 = tx * tx, ty2 = ty * ty, txty = tx * ty;
+      const double adj = (double)ta[ev.ta_off + n];
+      xi = xi + adj * (tx2 * (-a2 + c2) + 2.0 * txty * (ac - bc) + ty2 * (-c2 + b2));
+      eta = eta + adj * (2.0 * ac * tx2 - 2.0 * (ab + c2) * txty + 2.0 * bc * ty2);
+    }
+    ad.xi[id] = xi;
+    ad.eta[id] = eta;
+  }
+}
+
 // Start-of-run state: work-queue heads and list lengths, re-solve marks and
 // iteration counts zeroed, flags from the creation-time flags.  A kernel, not
 // memset/memcpy nodes, so that the run can be captured in a graph and replayed.
@@ -1547,6 +1614,42 @@ int sat_batch_combine_kernels(sat_batch* b, int grid, int nz, int mode, const do
   SAT_CHECK(hipStreamSynchronize(b->stream));  // the last run's kernels are complete
   return sat::combine_impl(g.nx, g.ny, nz, g.nperiod, mode, nullptr, b->d_K + off, sen_vs, sen_vp, sen_rho, vs3d,
                            vp3d, ker_vs, ker_vp, ker_rho, kernel_ms);
+}
+
+int sat_batch_aniso_kernels(sat_batch* b, int grid, const double* a, const double* bm, const double* c,
+                            double* adj_xi, double* adj_eta) {
+  if (!b || !b->calc_adj || grid < 0 || grid >= (int)b->grids.size() || !a || !bm || !c || !adj_xi || !adj_eta)
+    return SAT_EINVAL;
+  if (b->chunk_first.size() != 1) return SAT_EINVAL;  // every event's T and Ta must still be on the device
+  const GridDev& g = b->grids[grid].g;
+  const int nper = g.nperiod;
+  int pl0 = 0;
+  for (int gi = 0; gi < grid; gi++) pl0 += b->grids[gi].g.nperiod;
+  const size_t total = (size_t)nper * g.plane;
+  int rc = SAT_OK;
+  double* buf = nullptr;
+  SAT_CHECK(hipStreamSynchronize(b->stream));  // the last run's kernels are complete
+  SAT_CHECK(hipMalloc(&buf, sizeof(double) * 5 * total));
+  {
+    AnisoDev ad{g, nper, pl0, b->rd.ev_off, b->rd.ev_ids, b->d_events, b->d_T, b->d_ta, buf, buf + 3 * total,
+                buf + 4 * total};
+    hipError_t e = hipMemcpy(buf, a, sizeof(double) * total, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(buf + total, bm, sizeof(double) * total, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(buf + 2 * total, c, sizeof(double) * total, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 65535);
+      if (b->precision == SAT_PREC_F32)
+        hipLaunchKernelGGL(aniso_kernel<float>, dim3(blocks), dim3(256), 0, nullptr, ad);
+      else
+        hipLaunchKernelGGL(aniso_kernel<double>, dim3(blocks), dim3(256), 0, nullptr, ad);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(adj_xi, ad.xi, sizeof(double) * total, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(adj_eta, ad.eta, sizeof(double) * total, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = (e == hipErrorOutOfMemory) ? SAT_ENOMEM : SAT_EDEVICE;
+  }
+  (void)hipFree(buf);
+  return rc;
 }
 
 int sat_batch_last_ms(sat_batch* b, double* total_ms, double* solve_ms) {

@@ -201,6 +201,25 @@ class EikonalBatch:
             timing["kernel_ms"] = ms.value
         return k_vs, k_vp, k_rho
 
+    def aniso_kernels(self, m11, m12, m22, grid: int = 0):
+        """The anisotropic terms of accumulate_kernels (src/preproc.cpp:37-65)
+        for this batch's last run on grid `grid`: (adj_xi, adj_eta), each
+        (nperiod, ny, nx).  m11, m12, m22 (nperiod, ny, nx): the grid's
+        topography metric (SurfGrid a = m11, b = m22, c = -m12,
+        src/surf_grid.cpp:186-189)."""
+        if not 0 <= grid < len(self.shapes):
+            raise ValueError(f"grid {grid} is not in this batch")
+        shape = tuple(self.shapes[grid])
+        planes = [_c64(m) for m in (m11, m22, -np.asarray(m12, dtype=np.float64))]
+        for m in planes:
+            if m.shape != shape:
+                raise ValueError(f"metric planes must be {shape}")
+        xi, eta = np.empty(shape), np.empty(shape)
+        rc = _lib.load().sat_batch_aniso_kernels(self._h, int(grid), *(m.ctypes.data for m in planes),
+                                                 xi.ctypes.data, eta.ctypes.data)
+        _lib.check("sat_batch_aniso_kernels", rc)
+        return xi, eta
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().sat_batch_destroy(self._h)
