@@ -110,3 +110,14 @@ def test_aniso_isotropic_metric_and_errors():
         bt.run()
         with pytest.raises(_lib.SatError):
             bt.aniso_kernels(b["m11"], b["m12"], b["m22"])
+
+
+def test_aniso_refuses_chunked_batches():
+    """A run split into chunks reuses the workspace: the earlier chunks' fields
+    are gone at its end, so the terms are refused rather than built from them."""
+    g1 = S.config_grf(seed=31, nx=41, ny=33, nsta=6, nperiod=2, spacing_km=1.5)
+    g2 = S.config_grf(seed=32, nx=131, ny=57, nsta=5, nperiod=2, spacing_km=1.5)
+    with PP.EikonalBatch([g1, g2], workspace_cap=1 << 20) as bt:
+        bt.run()
+        with pytest.raises(_lib.SatError):
+            bt.aniso_kernels(g1["m11"], g1["m12"], g1["m22"])
