@@ -121,3 +121,20 @@ def test_aniso_refuses_chunked_batches():
         bt.run()
         with pytest.raises(_lib.SatError):
             bt.aniso_kernels(g1["m11"], g1["m12"], g1["m22"])
+
+
+def test_aniso_full_workload_timed():
+    """The 201x201 workload (100 stations x 30 periods, 3,000 pairs), FP32
+    batch: the terms over all 3,000 resident field pairs, twice (same bits)."""
+    import time
+    b = S.config_grf()
+    with PP.EikonalBatch(b, precision=_lib.PREC_F32) as bt:
+        bt.run()
+        xi, eta = bt.aniso_kernels(b["m11"], b["m12"], b["m22"])
+        t0 = time.perf_counter()
+        xi2, eta2 = bt.aniso_kernels(b["m11"], b["m12"], b["m22"])
+        dt = time.perf_counter() - t0
+    print(f"201x201 x 30 periods, 3000 events: {dt * 1e3:.2f} ms per call (host copies included)", flush=True)
+    assert np.isfinite(xi).all() and np.isfinite(eta).all() and np.abs(xi).max() > 0.0
+    np.testing.assert_array_equal(xi2, xi)
+    np.testing.assert_array_equal(eta2, eta)
