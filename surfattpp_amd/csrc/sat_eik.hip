@@ -199,29 +199,37 @@ __global__ void aniso_kernel(AnisoDev ad) {
     auto dyk = [&](int j) { return (g.lat[j] - g.lat[j - 1]) * km; };
     const double a = ad.abc[id], b = ad.abc[total + id], c = ad.abc[2 * total + id];
     const double a2 = a * a, b2 = b * b, c2 = c * c, ac = a * c, bc = b * c, ab = a * b;
+    // the stencil of gradient_2_geo at this node, the same for every event:
+    // central differences inside, one-sided at the edges
+    const size_t oxp = n + (ix < nx - 1 ? 1 : 0), oxm = n - (ix > 0 ? 1 : 0);
+    const size_t oyp = n + (iy < ny - 1 ? (size_t)nx : 0), oym = n - (iy > 0 ? (size_t)nx : 0);
+    const double dxd = (ix == 0) ? dxk(1) : (ix == nx - 1) ? dxk(nx - 1) : 2.0 * dxk(ix);
+    const double dyd = (iy == 0) ? dyk(1) : (iy == ny - 1) ? dyk(ny - 1) : 2.0 * dyk(iy);
     double xi = 0.0, eta = 0.0;
     const int pl = ad.pl0 + p;
-    for (int q = ad.ev_off[pl]; q < ad.ev_off[pl + 1]; q++) {
-      const EventDev& ev = ad.events[ad.ev_ids[q]];
-      const double* f = ad.T + ev.ta_off;
-      const double* row = f + (size_t)iy * nx;
-      double tx, ty;
-      if (ix == 0)
-        tx = (row[1] - row[0]) / dxk(1);
-      else if (ix == nx - 1)
-        tx = (row[nx - 1] - row[nx - 2]) / dxk(nx - 1);
-      else
-        tx = (row[ix + 1] - row[ix - 1]) / (2.0 * dxk(ix));
-      if (iy == 0)
-        ty = (f[nx + ix] - f[ix]) / dyk(1);
-      else if (iy == ny - 1)
-        ty = (f[(size_t)(ny - 1) * nx + ix] - f[(size_t)(ny - 2) * nx + ix]) / dyk(ny - 1);
-      else
-        ty = (f[(size_t)(iy + 1) * nx + ix] - f[(size_t)(iy - 1) * nx + ix]) / (2.0 * dyk(iy));
-      const double tx2 = tx * tx, ty2 = ty * ty, txty = tx * ty;
-      const double adj = (double)ta[ev.ta_off + n];
-      xi = xi + adj * (tx2 * (-a2 + c2) + 2.0 * txty * (ac - bc) + ty2 * (-c2 + b2));
-      eta = eta + adj * (2.0 * ac * tx2 - 2.0 * (ab + c2) * txty + 2.0 * bc * ty2);
+    const int qe = ad.ev_off[pl + 1];
+    // kEv events' loads in flight at once; the sums still run in batch order
+    constexpr int kEv = 4;
+    for (int q = ad.ev_off[pl]; q < qe; q += kEv) {
+      double tx[kEv], ty[kEv], adj[kEv];
+#pragma unroll
+      for (int k = 0; k < kEv; k++) {
+        if (q + k < qe) {
+          const EventDev& ev = ad.events[ad.ev_ids[q + k]];
+          const double* f = ad.T + ev.ta_off;
+          tx[k] = (f[oxp] - f[oxm]) / dxd;
+          ty[k] = (f[oyp] - f[oym]) / dyd;
+          adj[k] = (double)ta[ev.ta_off + n];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kEv; k++) {
+        if (q + k < qe) {
+          const double tx2 = tx[k] * tx[k], ty2 = ty[k] * ty[k], txty = tx[k] * ty[k];
+          xi = xi + adj[k] * (tx2 * (-a2 + c2) + 2.0 * txty * (ac - bc) + ty2 * (-c2 + b2));
+          eta = eta + adj[k] * (2.0 * ac * tx2 - 2.0 * (ab + c2) * txty + 2.0 * bc * ty2);
+        }
+      }
     }
     ad.xi[id] = xi;
     ad.eta[id] = eta;
