@@ -21,7 +21,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import multiprocessing as mp
 import os
 import sys
 import time
@@ -52,34 +51,64 @@ def fwd_sweep_bytes(nx, ny, iters_f):
     return float(FWD_WORD_BYTES * N * 4 * FWD_WORDS_PER_NODE_SWEEP * iters_f.astype(np.float64).sum())
 
 
-def _cpu_worker(args):
-    """One host core: its round-robin share of pairs (include/parallel.h:39-41),
-    stopping at the deadline."""
-    batch, ids, deadline = args
+def host_cpu_info() -> dict:
+    """What the host offers: the CPUs this process may run on, the online CPUs,
+    the share a job is given (OMP_NUM_THREADS, set per GPU on a shared box) and
+    the CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or affinity
+    return {"affinity_cpus": affinity, "nproc": os.cpu_count() or 1, "job_cpu_share": share, "model": model}
+
+
+def cpu_baseline(batch, seconds: float, threads: int, info: dict):
+    """The FP64 oracle (-O3 -mavx2 -mfma) on `threads` host threads of this
+    process: pairs dealt round-robin i % threads (include/parallel.h:39-41), one
+    pair at a time per thread, each thread stopping at the deadline.  ctypes
+    releases the GIL for the duration of each solve, so the threads run on as
+    many cores."""
+    from concurrent.futures import ThreadPoolExecutor
+
     from oracle import oracle as O
     from surfattpp_amd import synth
-    done = 0
-    for e in ids:
-        if time.time() > deadline:
-            break
-        O.forward_for_event(synth.subset_events(batch, [e]), lib=O.bench_lib())
-        done += 1
-    return done
-
-
-def cpu_baseline(batch, seconds: float, cores: int):
+    lib = O.bench_lib()
     ne = len(batch["ev_iper"])
+    subsets = [synth.subset_events(batch, [e]) for e in range(ne)]
     deadline = time.time() + seconds
-    shares = [(batch, list(range(r, ne, cores)), deadline) for r in range(cores)]
+
+    def work(rank):
+        done, busy = 0, 0.0
+        for e in range(rank, ne, threads):
+            if time.time() > deadline:
+                break
+            t = time.perf_counter()
+            O.forward_for_event(subsets[e], lib=lib)
+            busy += time.perf_counter() - t
+            done += 1
+        return done, busy
+
     t0 = time.time()
-    ctx = mp.get_context("fork")
-    with ctx.Pool(cores) as pool:
-        done = pool.map(_cpu_worker, shares)
+    with ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(work, range(threads)))
     dt = time.time() - t0
-    n = int(sum(done))
-    return dict(value=n / dt, unit="fwd+adjoint solves/s", cores=cores, kind="port",
+    n = int(sum(d for d, _ in res))
+    per_pair = sum(b for _, b in res) / max(n, 1)
+    return dict(value=n / dt, unit="fwd+adjoint solves/s", cores=threads, kind="port",
+                s_per_pair_one_core=round(per_pair, 4),
+                all_cpus_linear=round(info["affinity_cpus"] / per_pair, 1),
+                host=info,
                 sample=f"{n} of {ne} pairs of the same workload (FP64 oracle, -O3 -mavx2 -mfma), "
-                       f"round-robin over {cores} processes, {dt:.1f} s wall")
+                       f"round-robin over {threads} threads of one process ({info['job_cpu_share']} CPUs "
+                       f"is this job's share of the {info['affinity_cpus']}-CPU host), {dt:.1f} s wall; "
+                       f"all_cpus_linear: the measured per-pair time scaled to every CPU of the host")
 
 
 def main():
@@ -156,13 +185,27 @@ def main():
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    # The kernel is bound by VALU issue, not HBM: its wave64 VALU instruction
+    # count per launch (rocprofv3 SQ_INSTS_VALU, committed under profiles/) at 4
+    # cycles each against the device's SIMD-cycles over this run's kernel time
+    # at the peak engine clock; and the share of the lane-steps that update a
+    # node (the lane-skewed sweep's ramps and idle lanes issue too).
+    valu = None
+    vf = os.path.join(ROOT, "profiles", "fwd_sweep_f32_grf201_valu.json")
+    if os.path.exists(vf):
+        with open(vf) as f:
+            valu = json.load(f)
+    simds, clock_hz = 4 * cu_count(dev), 2.4e9
+    issue_frac = (round(valu["valu_insts_per_launch"] * 4 / (simds * clock_hz * kern_ms * 1e-3), 4)
+                  if valu else None)
+    busy = busy_lane_fraction(nx, ny, waves=1)
 
     extra = None if args.no_extra else extra_workloads()
 
     cpu = None
     if not args.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(b, args.cpu_seconds, cores)
+        info = host_cpu_info()
+        cpu = cpu_baseline(b, args.cpu_seconds, max(1, min(info["affinity_cpus"], info["job_cpu_share"])), info)
 
     line = {
         "metric": METRIC,
@@ -181,17 +224,27 @@ def main():
                    "receivers_per_pair": int(b["ev_nrec"][0]), "sweep_precision": "fp32 (sigma = tau - 1)",
                    "fp64_parts": "T0 init, T, adjoint coefficients, convergence norm, misfit, kernel accumulation",
                    "device": dev},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "valu", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "fwd_sweep_kernel<SAT_PREC_F32,4,iso>", "kernel_ms": round(kern_ms, 3),
-                     "algorithmic_bytes_per_launch": bytes_launch},
+                     "algorithmic_bytes_per_launch": bytes_launch,
+                     "hbm_frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "valu_issue_frac": issue_frac,
+                     "valu_insts_per_launch": valu["valu_insts_per_launch"] if valu else None,
+                     "busy_lane_frac": round(busy, 4),
+                     "note": "achieved/frac: algorithmic HBM bytes over the kernel time against 8 TB/s; the "
+                             "kernel is bound by VALU issue: valu_issue_frac = SQ_INSTS_VALU x 4 cycles / "
+                             f"({simds} SIMDs x 2.4 GHz x kernel time)"},
         "phases_ms": phase_mean,
         "cpu_baseline": cpu,
         "err_vs_fp64": {"kernel_max_per_period": err_k, "syn": err_syn, "chi": err_chi},
         "batched_f64": {"value": round(ne / (f64_ms[0] * 1e-3), 2), "ms_per_step": round(f64_ms[0], 3)},
         "plain_f64_per_solve": {"value": round(plain_rate, 3), "pairs_timed": args.plain_pairs},
-        "speedup": {"vs_plain_f64": round(value / plain_rate, 1),
-                    "vs_cpu": (round(value / cpu["value"], 1) if cpu else None)},
+        "speedup": {"vs_batched_f64": round(value / (ne / (f64_ms[0] * 1e-3)), 3),
+                    "vs_plain_f64_per_solve": round(value / plain_rate, 1),
+                    "vs_cpu": (round(value / cpu["value"], 1) if cpu else None),
+                    "note": "vs_batched_f64 is the precision speed-up (the same batched kernels in FP64); "
+                            "the per-solve path is one pair per launch, one wave on the device"},
         "iters": {"fwd_mean": float(iters_f.mean()), "adj_mean": float(iters_a.mean()),
                   "fwd_hist": np.bincount(iters_f).tolist(), "adj_hist": np.bincount(iters_a).tolist()},
         "workloads": extra,
@@ -199,6 +252,24 @@ def main():
     print(json.dumps(line), flush=True)
     if args.calib_copy_mib > 0:
         calib_copy(args.calib_copy_mib << 20)
+
+
+def cu_count(dev: str) -> int:
+    """Compute units from the library's device string ("... | CUs=256 | ...")."""
+    for part in dev.split("|"):
+        part = part.strip()
+        if part.startswith("CUs="):
+            return int(part[4:])
+    return 256
+
+
+def busy_lane_fraction(nx: int, ny: int, waves: int = 1) -> float:
+    """Share of the lane-steps of one forward sweep that update a node: the
+    sweep runs ny + La - 1 steps on 64*waves lanes of C columns each (C =
+    ceil(ceil(nx/64)/waves), La = ceil(nx/C))."""
+    C = -(-(-(-nx // 64)) // waves)
+    La = -(-nx // C)
+    return nx * ny / (64 * waves * C * (ny + La - 1))
 
 
 def _kernel_err(r32, r64, ngrid):
@@ -287,9 +358,11 @@ def fwdsurf_workload(nx=201, ny=201, nper=30):
             ix, iy = divmod(int(i), ny)
             worst = max(worst, float(np.abs(svel[:, iy, ix] / ref - 1.0).max()))
         per_col = (time.time() - t0) / sample.size
-        cores = min(16, os.cpu_count() or 1)
+        info = host_cpu_info()
+        cores = max(1, min(info["affinity_cpus"], info["job_cpu_share"]))
         out["cpu_reference"] = {"s_per_column_one_core": round(per_col, 6), "cores": cores,
                                 "columns_per_s": round(cores / per_col, 1),
+                                "all_cpus_linear": round(info["affinity_cpus"] / per_col, 1),
                                 "sample": f"{sample.size} columns, reference surfdisp96 (amdflang -O2)"}
         out["max_rel_err_vs_reference"] = worst
     return out

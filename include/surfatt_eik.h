@@ -211,9 +211,10 @@ int sat_batch_combine_kernels(sat_batch* b, int grid, int nz, int mode, const do
  *   adj_eta += Ta * (2 ac Tx^2 - 2 (ab + c^2) TxTy + 2 bc Ty^2).
  * a, b, c: SurfGrid's a, b, c (the topography metric: m11, m22 and -m12 of
  * sat_topo_metric), [nperiod][ny][nx] host arrays; adj_xi, adj_eta the same
- * shape.  Waits for the last run; synchronous.  SAT_EINVAL for a batch
- * without adjoint or one whose runs are split into chunks (its fields are
- * not all resident at the end of a run). */
+ * shape.  Waits for the last run (any mode; FP32 pairs re-solved in FP64 use
+ * their FP64 fields); synchronous.  SAT_EINVAL before the first run, for a
+ * batch without adjoint, or for one whose runs are split into chunks (its
+ * fields are not all resident at the end of a run). */
 int sat_batch_aniso_kernels(sat_batch* b, int grid, const double* a, const double* bm, const double* c,
                             double* adj_xi, double* adj_eta);
 

@@ -72,6 +72,21 @@ struct GridDev {
   const void* wmed[6][2];
 };
 
+// Geometry of a grid swept by W waves of 64 lanes: C = ceil(ceil(nx / 64) / W)
+// columns per lane, La = ceil(nx / C) active lanes, and the size of one
+// wavefront-major plane (below), (ny + La - 1) * La * C elements.
+struct WaveGeom {
+  int C, La;
+  size_t wplane;
+};
+__host__ __device__ inline WaveGeom wave_geom(int nx, int ny, int W) {
+  WaveGeom w;
+  w.C = ((nx + kWave - 1) / kWave + W - 1) / W;
+  w.La = (nx + w.C - 1) / w.C;
+  w.wplane = (size_t)(ny + w.La - 1) * w.La * w.C;
+  return w;
+}
+
 __device__ inline int seg_index(const GridDev& g, int ix, int iy) {
   return iy * g.P + ix;  // lane l, column k -> l*C + k == ix
 }

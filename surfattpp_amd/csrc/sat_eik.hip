@@ -398,9 +398,10 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
   G.gseg = g;  // one wave per pair: the segmented layout of the per-pair kernels
   G.W = W;
   if (wave && W > 1) {  // the batched sweeps' wave geometry (the segmented pitch P stays)
-    g.C = (L.C + W - 1) / W;
-    g.La = (nx + g.C - 1) / g.C;
-    g.wplane = (size_t)(ny + g.La - 1) * g.La * g.C;
+    const WaveGeom wg = wave_geom(nx, ny, W);
+    g.C = wg.C;
+    g.La = wg.La;
+    g.wplane = wg.wplane;
   }
   if (wave) {  // padding entries stay zero
     const int nmed = prec == SAT_PREC_F64 ? 4 : 6;
@@ -414,11 +415,11 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
       // C = ceil(nx / (64 Wr))
       GridDev& m = G.gmw;
       m = G.g64;
-      const int wr = resolve_waves((nx + kWave - 1) / kWave);
-      m.C = (nx + kWave * wr - 1) / (kWave * wr);
-      m.La = (nx + m.C - 1) / m.C;
+      const WaveGeom wg = wave_geom(nx, ny, resolve_waves((nx + kWave - 1) / kWave));
+      m.C = wg.C;
+      m.La = wg.La;
       m.P = m.La * m.C;
-      m.wplane = (size_t)(ny + m.La - 1) * m.La * m.C;
+      m.wplane = wg.wplane;
       for (int i = 0; i < 6; i++) m.seg[i] = nullptr;
       const size_t pbm = (size_t)nper * m.wplane * sizeof(double);
       SAT_CHECK(dalloc(&G.d_wmedmw, (size_t)4 * 2 * pbm));
@@ -664,6 +665,7 @@ struct sat_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool last_graph = false;
+  bool ran = false;                // a run has been queued: every event's T and Ta are on the device
 
   void release() {
     for (auto& G : grids) G.release();
@@ -1241,6 +1243,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   bd.iters_a = b->d_iters + ne1;
   bd.flags = b->d_iters + 2 * ne1;
   bd.ta_out = b->d_ta;
+  bd.T_out = b->d_T;
   bd.calc_adj = b->calc_adj;
   bd.fwd_maxiter = kFwdMaxIter;
   bd.adj_maxiter = kAdjMaxIter;
@@ -1581,6 +1584,7 @@ int sat_batch_run(sat_batch* b, int mode) {
     const int rc = enqueue_run(b, mode, false);
     if (rc) return rc;
     SAT_CHECK(hipEventRecord(b->mk[7], b->stream));
+    b->ran = true;
     return SAT_OK;
   }
   if (!b->graph_exec) {
@@ -1603,6 +1607,7 @@ int sat_batch_run(sat_batch* b, int mode) {
   SAT_CHECK(hipEventRecord(b->mk[0], b->stream));
   SAT_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
   SAT_CHECK(hipEventRecord(b->mk[7], b->stream));
+  b->ran = true;
   return SAT_OK;
 }
 
@@ -1628,7 +1633,8 @@ int sat_batch_aniso_kernels(sat_batch* b, int grid, const double* a, const doubl
                             double* adj_xi, double* adj_eta) {
   if (!b || !b->calc_adj || grid < 0 || grid >= (int)b->grids.size() || !a || !bm || !c || !adj_xi || !adj_eta)
     return SAT_EINVAL;
-  if (b->chunk_first.size() != 1) return SAT_EINVAL;  // every event's T and Ta must still be on the device
+  // every event's T and Ta must be on the device: after a run, and not split into chunks
+  if (!b->ran || b->chunk_first.size() != 1) return SAT_EINVAL;
   const GridDev& g = b->grids[grid].g;
   const int nper = g.nperiod;
   int pl0 = 0;

@@ -73,9 +73,11 @@ struct SegLayout {
     return SweepGeom{g.nx, g.ny, g.La, g.P, (int)g.seg_plane};
   }
   template <int C, int O>
-  __device__ static __forceinline__ int base(const SweepGeom& s, int j, int lane) { return j * s.P + lane * C; }
+  __host__ __device__ static __forceinline__ int base(const SweepGeom& s, int j, int lane) {
+    return j * s.P + lane * C;
+  }
   template <int C>
-  __device__ static __forceinline__ int trash(const SweepGeom& s, int lane) { return s.trash0 + lane * C; }
+  __host__ __device__ static __forceinline__ int trash(const SweepGeom& s, int lane) { return s.trash0 + lane * C; }
 };
 struct WaveLayout {
   static constexpr bool kStepRun = true;
@@ -83,12 +85,46 @@ struct WaveLayout {
     return SweepGeom{g.nx, g.ny, g.La, g.P, (int)g.wplane};
   }
   template <int C, int O>
-  __device__ static __forceinline__ int base(const SweepGeom& s, int j, int lane) {
+  __host__ __device__ static __forceinline__ int base(const SweepGeom& s, int j, int lane) {
     const int q = (O == 0) ? j + lane : j - lane + s.La - 1;
     return (q * s.La + lane) * C;
   }
   template <int C>
-  __device__ static __forceinline__ int trash(const SweepGeom& s, int lane) { return s.trash0 + lane * C; }
+  __host__ __device__ static __forceinline__ int trash(const SweepGeom& s, int lane) { return s.trash0 + lane * C; }
+};
+
+// Addressing of a sweep with directions (XD, YD), shared by the sweep loops
+// below and by the host-side check of every address they issue
+// (tests/cpp/layout_check.cpp).  Lane l handles sweep-row r at step r + lag(l).
+template <int XD, int YD>
+struct SweepAddr {
+  int lane, La, ny, lag, nsteps, lane_c, lag_c;
+  __host__ __device__ __forceinline__ SweepAddr(const SweepGeom& g, int lane_) {
+    lane = lane_;
+    La = g.La;
+    ny = g.ny;
+    lag = (XD > 0) ? lane : (La - 1 - lane);
+    nsteps = ny + La - 1;
+    // loads are issued for every lane and step; rows and lanes outside the grid
+    // read an in-bounds location whose values are not used (see kStepRun)
+    lane_c = imin(lane, La - 1);
+    lag_c = (XD > 0) ? lane_c : (La - 1 - lane_c);
+  }
+  __host__ __device__ __forceinline__ int rowj(int r) const { return (YD > 0) ? r : (ny - 1 - r); }
+  __host__ __device__ __forceinline__ bool in_range(int r) const { return lane < La && r >= 0 && r < ny; }
+  // the clamped grid row of sweep-row r (row tables, neighbour presence)
+  __host__ __device__ __forceinline__ int row_clamped(int r) const { return rowj(imin(imax(r, 0), ny - 1)); }
+  // element offset of the C values the lane loads for sweep-row r
+  template <class Lay, int C>
+  __host__ __device__ __forceinline__ int load_base(const SweepGeom& g, int r) const {
+    constexpr int O = (XD * YD > 0) ? 0 : 1;
+    if constexpr (Lay::kStepRun) {
+      const int ta = imin(r + lag, nsteps - 1);  // the step this row belongs to
+      return Lay::template base<C, O>(g, rowj(ta - lag_c), lane_c);
+    } else {
+      return Lay::template base<C, O>(g, row_clamped(r), lane_c);
+    }
+  }
 };
 
 // Policy P provides:
@@ -138,18 +174,26 @@ __device__ __forceinline__ Exchange<Pol, W>* exchange_ptr(ExchangeLds<Pol, W>& x
 // dropped; under SegLayout they go to the plane's trash row.  Either way the
 // store instruction is unconditional (a branch around it would make the
 // compiler wait for the prefetches issued in the same step).
+// store_offset: the first store's byte offset in the plane's raw buffer
+// (WaveLayout; kDropOffset, past the plane's num_records, for a lane with no
+// live row) or element offset (SegLayout; the trash row for no live row); the
+// lane's C values go to consecutive elements from there.
+template <class Lay, int C, int OOUT, class V>
+__host__ __device__ __forceinline__ int store_offset(const SweepGeom& g, bool live, int j, int lane) {
+  if constexpr (Lay::kStepRun) return live ? Lay::template base<C, OOUT>(g, j, lane) * (int)sizeof(V) : kDropOffset;
+  else return live ? Lay::template base<C, OOUT>(g, j, lane) : Lay::template trash<C>(g, lane);
+}
 template <class Lay, int C, int OOUT, class V>
 __device__ __forceinline__ void store_row(const SweepGeom& g, V* __restrict__ tout, bool live, int j, int lane,
                                           const V (&newv)[C]) {
+  const int off = store_offset<Lay, C, OOUT, V>(g, live, j, lane);
   if constexpr (Lay::kStepRun) {
     const __amdgpu_buffer_rsrc_t rs = plane_rsrc(tout, g.trash0 * (int)sizeof(V));
-    const int off = live ? Lay::template base<C, OOUT>(g, j, lane) * (int)sizeof(V) : kDropOffset;
 #pragma unroll
     for (int k = 0; k < C; k++) bst(rs, off + k * (int)sizeof(V), newv[k]);
   } else {
-    const int base = live ? Lay::template base<C, OOUT>(g, j, lane) : Lay::template trash<C>(g, lane);
 #pragma unroll
-    for (int k = 0; k < C; k++) gst(tout + base + k, newv[k]);
+    for (int k = 0; k < C; k++) gst(tout + off + k, newv[k]);
   }
 }
 
@@ -165,34 +209,14 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
   constexpr int O = (XD * YD > 0) ? 0 : 1;
   const SweepGeom g = uniform(Lay::geom(*cx.g));
   const int lane = threadIdx.x;
-  const int La = g.La, nx = g.nx, ny = g.ny;
-  const bool lane_ok = lane < La;
-  const int lag = (XD > 0) ? lane : (La - 1 - lane);
-  const int nsteps = ny + La - 1;
-  auto rowj = [&](int r) { return (YD > 0) ? r : (ny - 1 - r); };
-  auto in_range = [&](int r) { return lane_ok && r >= 0 && r < ny; };
-  // loads are issued for every lane and step; rows and lanes outside the grid
-  // read an in-bounds location whose values are not used (see kStepRun)
-  const int lane_c = imin(lane, La - 1);
-  const int lag_c = (XD > 0) ? lane_c : (La - 1 - lane_c);
-  auto row_base = [&](int r, int& j) {  // j: clamped row (row tables); returns the address
-    j = rowj(imin(imax(r, 0), ny - 1));
-    if constexpr (Lay::kStepRun) {
-      const int ta = imin(r + lag, nsteps - 1);  // the step this row belongs to
-      return Lay::template base<C, O>(g, rowj(ta - lag_c), lane_c);
-    } else {
-      return Lay::template base<C, O>(g, j, lane_c);
-    }
-  };
+  const int nx = g.nx;
+  const SweepAddr<XD, YD> sa(g, lane);
+  const int lag = sa.lag, nsteps = sa.nsteps;
   auto load_head = [&](Head& H, int r) {
-    int j;
-    const int base = row_base(r, j);
-    Pol::template load_head<C, O>(cx, H, base, j, tin);
+    Pol::template load_head<C, O>(cx, H, sa.template load_base<Lay, C>(g, r), sa.row_clamped(r), tin);
   };
   auto load_body = [&](Body& B, int r) {
-    int j;
-    const int base = row_base(r, j);
-    Pol::template load_body<C, O, NORM>(cx, B, base, j, tst);
+    Pol::template load_body<C, O, NORM>(cx, B, sa.template load_base<Lay, C>(g, r), sa.row_clamped(r), tst);
   };
 
   Head cur_h, nxt_h, nn_h;
@@ -241,9 +265,9 @@ __device__ __forceinline__ void sweep(const typename Pol::Ctx& cx, const typenam
     {
       // every lane evaluates and stores at every step: rows outside the grid
       // compute on clamped inputs and store to the plane's trash row
-      const bool live = in_range(r);
-      const int j = rowj(imin(imax(r, 0), ny - 1));
-      const bool hS = j > 0, hN = j < ny - 1;
+      const bool live = sa.in_range(r);
+      const int j = sa.row_clamped(r);
+      const bool hS = j > 0, hN = j < g.ny - 1;
       V newv[C];
 #pragma unroll
       for (int kk = 0; kk < C; kk++) {
@@ -304,23 +328,12 @@ __device__ __forceinline__ void sweep_ring(const typename Pol::Ctx& cx, const ty
   constexpr int R = D + 1;
   const SweepGeom g = uniform(Lay::geom(*cx.g));
   const int lane = threadIdx.x;
-  const int La = g.La, nx = g.nx, ny = g.ny;
-  const bool lane_ok = lane < La;
-  const int lag = (XD > 0) ? lane : (La - 1 - lane);
-  const int nsteps = ny + La - 1;
-  auto rowj = [&](int r) { return (YD > 0) ? r : (ny - 1 - r); };
-  auto in_range = [&](int r) { return lane_ok && r >= 0 && r < ny; };
-  const int lane_c = imin(lane, La - 1);
-  const int lag_c = (XD > 0) ? lane_c : (La - 1 - lane_c);
+  const int nx = g.nx;
+  const SweepAddr<XD, YD> sa(g, lane);
+  const int lag = sa.lag, nsteps = sa.nsteps;
   auto load = [&](Head& H, Body& B, int r) {  // unconditional, in bounds (see kTrashElems, kStepRun)
-    const int j = rowj(imin(imax(r, 0), ny - 1));
-    int base;
-    if constexpr (Lay::kStepRun) {
-      const int ta = imin(r + lag, nsteps - 1);
-      base = Lay::template base<C, O>(g, rowj(ta - lag_c), lane_c);
-    } else {
-      base = Lay::template base<C, O>(g, j, lane_c);
-    }
+    const int j = sa.row_clamped(r);
+    const int base = sa.template load_base<Lay, C>(g, r);
     Pol::template load_head<C, O>(cx, H, base, j, tin);
     Pol::template load_body<C, O, NORM>(cx, B, base, j, tst);
   };
@@ -377,9 +390,9 @@ __device__ __forceinline__ void sweep_ring(const typename Pol::Ctx& cx, const ty
         }
       }
       {
-        const bool live = in_range(r);
-        const int j = rowj(imin(imax(r, 0), ny - 1));
-        const bool hS = j > 0, hN = j < ny - 1;
+        const bool live = sa.in_range(r);
+        const int j = sa.row_clamped(r);
+        const bool hS = j > 0, hN = j < g.ny - 1;
         V newv[C];
 #pragma unroll
         for (int kk = 0; kk < C; kk++) {

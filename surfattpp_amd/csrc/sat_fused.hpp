@@ -59,6 +59,8 @@ struct BatchDev {
   int* iters_a;
   int* flags;
   void* ta_out;          // masked Ta planes (the batch's sweep precision)
+  double* T_out;         // FP64 travel time per event, column-major, at the event's ta_off
+                         // (read by the anisotropic kernel terms); may be null
   double* conv;          // per event: Linf of the last two forward rounds
   int* fb_list;          // FP32 batches: events re-solved in FP64 (decision near the tolerance)
   int* fb_count;         // one counter per launch group
@@ -196,6 +198,10 @@ __device__ void run_pair(const BatchDev& bd, int e, const Slot& ws, int group) {
   const GridDev& g = bd.grids[ev.grid];
   int flags = 0;
   const IterOut fo = solve_forward<PREC, C>(g, ev.iper, ev.lon, ev.lat, ws, bd.fwd_maxiter, flags);
+  if (bd.T_out) {  // the event's T where the batched kernels leave it (an FP64 re-solve overwrites it)
+    double* To = bd.T_out + ev.ta_off;
+    for (size_t p = lane; p < g.plane; p += kWave) To[p] = ws.pl[0][p];
+  }
   if (lane == 0) {
     bd.conv[2 * (size_t)e] = fo.linf;
     bd.conv[2 * (size_t)e + 1] = fo.linf_prev;

@@ -20,6 +20,7 @@
 #pragma once
 #include <type_traits>
 
+#include "../../include/surfatt_eik.h"
 #include "sat_crmath.hpp"
 #include "sat_engine.hpp"
 #include "sat_policies.hpp"
@@ -176,7 +177,7 @@ __device__ inline int block_event(const PhaseDev& pd) {
 
 // Decode element q of an orientation-0 wavefront-major plane: column ix, row j;
 // false for padding.
-__device__ inline bool wave_decode0(const GridDev& g, int q, int& ix, int& j) {
+__host__ __device__ inline bool wave_decode0(const GridDev& g, int q, int& ix, int& j) {
   const int s = q / g.C, k = q - s * g.C;
   const int dg = s / g.La, l = s - dg * g.La;
   j = dg - l;
@@ -216,7 +217,7 @@ __global__ void trig_kernel(PhaseDev pd) {
 
 // Decode element q of an orientation-o wavefront-major plane: column ix, row j;
 // false for padding.
-__device__ inline bool wave_decode(const GridDev& g, int o, int q, int& ix, int& j) {
+__host__ __device__ inline bool wave_decode(const GridDev& g, int o, int q, int& ix, int& j) {
   const int s = q / g.C, k = q - s * g.C;
   const int dg = s / g.La, l = s - dg * g.La;
   j = (o == 0) ? dg - l : dg + l - (g.La - 1);

@@ -138,3 +138,50 @@ def test_aniso_full_workload_timed():
     assert np.isfinite(xi).all() and np.isfinite(eta).all() and np.abs(xi).max() > 0.0
     np.testing.assert_array_equal(xi2, xi)
     np.testing.assert_array_equal(eta2, eta)
+
+
+def test_aniso_after_per_solve_run_and_before_any_run():
+    """The per-pair kernel (SAT_RUN_PER_SOLVE) keeps T in its own scratch; it
+    must also leave each event's FP64 T where the terms read it.  Refused before
+    the first run (no field is on the device yet)."""
+    b = _topo_batch()
+    rx, re = _oracle_aniso(b)
+    with PP.EikonalBatch(b, precision=_lib.PREC_F64) as bt:
+        with pytest.raises(_lib.SatError):
+            bt.aniso_kernels(b["m11"], b["m12"], b["m22"])
+        bt.run()
+        xb, eb = bt.aniso_kernels(b["m11"], b["m12"], b["m22"])
+        bt.run(_lib.RUN_PER_SOLVE)
+        xs, es = bt.aniso_kernels(b["m11"], b["m12"], b["m22"])
+    print(f"per-solve vs batched: xi equal {np.array_equal(xs, xb)}, eta equal {np.array_equal(es, eb)}; "
+          f"vs oracle {_rel(xs, rx):.2e} {_rel(es, re):.2e}", flush=True)
+    assert _rel(xs, rx) <= 1e-9 and _rel(es, re) <= 1e-9
+    assert _rel(xs, xb) <= 1e-9 and _rel(es, eb) <= 1e-9
+
+
+def test_aniso_after_resolve_pool_overflow():
+    """FP32 batch whose band sends every pair to an FP64 re-solve: beyond the
+    re-solve pool (64 entries) the per-pair kernel re-solves them, and its T
+    must reach the terms as the pool's does."""
+    b = S.config_grf(seed=5, nx=31, ny=29, nsta=36, nperiod=2, spacing_km=2.0)
+    lon, lat = b["lon"], b["lat"]
+    km = S.R_EARTH * S.DEG2RAD
+    x = (lon[None, :] - lon[0]) * km
+    y = (lat[:, None] - lat[0]) * km
+    z = 2.0 * np.sin(x / 11.0) * np.cos(y / 7.0)
+    m11, m12, m22 = S.topo_metric(z, lon, lat, None)
+    for k, m in (("m11", m11), ("m12", m12), ("m22", m22)):
+        b[k] = np.repeat(m[None], 2, axis=0)
+    ne = len(b["ev_iper"])
+    assert ne > 64
+    rx, re = _oracle_aniso(b)
+    with PP.EikonalBatch(b, precision=_lib.PREC_F32) as bt:
+        bt.set_f32_band(0.999)
+        for mode in (_lib.RUN_BATCHED, _lib.RUN_PER_SOLVE):
+            bt.run(mode)
+            _, _, nres = bt.conv()
+            xi, eta = bt.aniso_kernels(b["m11"], b["m12"], b["m22"])
+            print(f"mode {mode}: {nres} of {ne} pairs re-solved in FP64; "
+                  f"xi {_rel(xi, rx):.2e} eta {_rel(eta, re):.2e}", flush=True)
+            assert nres > 64
+            assert _rel(xi, rx) <= 1e-5 and _rel(eta, re) <= 1e-5

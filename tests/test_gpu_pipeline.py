@@ -34,9 +34,6 @@ def test_iteration_from_example_inputs():
     vs3d = np.ascontiguousarray(np.broadcast_to(vs1, (nx, ny, z.size)))
     periods = sr.periods_info.periods
     svel = SD.fwdsurf(z, vs3d, periods)                        # (nper, ny, nx)
-    if O.surfdisp_available():
-        thk, vp, vs, rho = SD.build_disp_req(z, vs1)
-        np.testing.assert_array_equal(svel[:, 0, 0], O.surfdisp96(thk, vp, vs, rho, periods))
     b = sr.to_batch(lon, lat, svel)
     print(f"grid {nx}x{ny}, {len(b['ev_iper'])} events, periods {periods.min()}-{periods.max()} s, "
           f"maps {svel.min():.3f}-{svel.max():.3f} km/s", flush=True)
@@ -59,3 +56,18 @@ def test_iteration_from_example_inputs():
     sr.scatter_syn(res["syn"])
     sr.gather_syn_tt()
     assert np.all(sr.tt_fwd > 0)
+
+
+@pytest.mark.skipif(not O.surfdisp_available(),
+                    reason="oracle/_ref/libsurfdisp96.so (the reference's solver) is not built")
+def test_example_maps_match_reference_solver():
+    """The example's 1-D model: every map column equals the reference's solver."""
+    sr = S.SrcRec().load(TABLE)
+    lon, lat = S.build_model_axes(S.build_stas(sr, None), [0.02, 0.02], 5)
+    z = np.arange(0.0, 15.0 + 1e-9, 0.5)
+    vs1 = 1.8 + (4.2 - 1.8) * z / z[-1]
+    vs3d = np.ascontiguousarray(np.broadcast_to(vs1, (lon.size, lat.size, z.size)))
+    periods = sr.periods_info.periods
+    svel = SD.fwdsurf(z, vs3d, periods)
+    thk, vp, vs, rho = SD.build_disp_req(z, vs1)
+    np.testing.assert_array_equal(svel[:, 0, 0], O.surfdisp96(thk, vp, vs, rho, periods))

@@ -165,6 +165,47 @@ def test_interp2d_plane_exact_and_nan_outside():
     assert np.isnan(out).all()
 
 
+def _interp_brute(xg, yg, z, qx, qy, slip=0):
+    """Bilinear interpolation in the cell found by a linear search: the cell
+    [xg[i], xg[i+1]] x [yg[j], yg[j+1]] that holds the query (the last cell for
+    the upper edge), weights from the grid values.  slip = -1: the cell to the
+    left instead (extrapolated), the mistake a wrong cell search makes."""
+    if not (xg[0] <= qx <= xg[-1] and yg[0] <= qy <= yg[-1]):
+        return np.nan
+    i = max(k for k in range(xg.size - 1) if xg[k] <= qx)
+    i = min(max(i + slip, 0), xg.size - 2)
+    j = max(k for k in range(yg.size - 1) if yg[k] <= qy)
+    tx = (qx - xg[i]) / (xg[i + 1] - xg[i])
+    ty = (qy - yg[j]) / (yg[j + 1] - yg[j])
+    return ((1 - tx) * (1 - ty) * z[i, j] + tx * (1 - ty) * z[i + 1, j]
+            + (1 - tx) * ty * z[i, j + 1] + tx * ty * z[i + 1, j + 1])
+
+
+def test_interp2d_cell_choice_on_curved_fields():
+    """interp2d (include/utils.h:358-409) on fields that are not planes, where
+    only the right cell gives the right value: random values and x^2 + xy,
+    queries inside cells, on interior grid lines and nodes, and on the edges;
+    against scipy's RegularGridInterpolator (linear) and a brute-force cell
+    search (the reference tests' own intent, tests/eikonal_test/test_interp2d_dump.cpp)."""
+    from scipy.interpolate import RegularGridInterpolator
+    rng = np.random.default_rng(17)
+    xg, yg = np.linspace(-1.0, 2.0, 13), np.linspace(0.5, 1.5, 9)
+    fields = (rng.standard_normal((xg.size, yg.size)),
+              xg[:, None] ** 2 + xg[:, None] * yg[None, :] + 0 * yg[None, :])
+    xq = np.concatenate([rng.uniform(xg[0], xg[-1], 300), xg, xg[[3, 3, 7]], [xg[0], xg[-1], xg[0], xg[-1]]])
+    yq = np.concatenate([rng.uniform(yg[0], yg[-1], 300), rng.uniform(yg[0], yg[-1], xg.size),
+                         yg[[2, 5, 8]], [yg[0], yg[0], yg[-1], yg[-1]]])
+    for z in fields:
+        got = O.interp2d(xg, yg, z, xq, yq)
+        ref = RegularGridInterpolator((xg, yg), z, method="linear")(np.stack([xq, yq], axis=1))
+        brute = np.array([_interp_brute(xg, yg, z, a, b) for a, b in zip(xq, yq)])
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * np.abs(z).max())
+        np.testing.assert_allclose(got, brute, rtol=0, atol=1e-14 * np.abs(z).max())
+        # a one-cell slip (the left neighbour's cell, extrapolated) is visible here
+        slip = np.array([_interp_brute(xg, yg, z, a, b, slip=-1) for a, b in zip(xq[:300], yq[:300])])
+        assert np.abs(slip - got[:300]).max() > 1e-2 * np.abs(z).max()
+
+
 def test_mask_index_set_closed_form():
     rng = np.random.default_rng(11)
     for _ in range(50):

@@ -107,6 +107,7 @@ def test_surfker_shim_matches_python_and_reference(tmp_path):
     from surfattpp_amd import surfdisp as SD
     from surfattpp_amd import topo as TG
     exe = _build(tmp_path, "surfker_driver")
+    have_ref = O.surfdisp_available()
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     rows = {}
@@ -119,7 +120,7 @@ def test_surfker_shim_matches_python_and_reference(tmp_path):
         c = rows["c"][i]
         g = SD.surfdisp96(*lay, per, iflsph=sph, iwave=iw, igr=igr)
         assert np.array_equal(c, g)
-        if O.surfdisp_available():
+        if have_ref:
             assert np.array_equal(c, O.surfdisp96(*lay, per, iflsph=sph, iwave=iw, igr=igr))
     assert abs(rows["c"][0][1] - 2.6747) <= 1e-4 and abs(rows["c"][0][3] - 3.8387) <= 1e-4
     nx, ny, nz = 3, 2, 6
@@ -135,3 +136,6 @@ def test_surfker_shim_matches_python_and_reference(tmp_path):
     m11, m12, m22 = TG.topo_metric(zz, xg, yg, sig)
     for name, m in (("m11", m11), ("m12", m12), ("m22", m22)):
         np.testing.assert_allclose(rows[name][0], m.transpose(2, 1, 0).ravel(), rtol=0, atol=1e-15)
+    if not have_ref:
+        pytest.skip("other checks passed; the comparison with the reference's solver needs "
+                    "oracle/_ref/libsurfdisp96.so, which is not built")
