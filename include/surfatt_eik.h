@@ -61,6 +61,21 @@ int sat_eik_forward(int precision, int nx, int ny, const double* lon_deg, const 
                     const double* spha, const double* sphb, const double* sphc, const double* fun,
                     double src_lon_deg, double src_lat_deg, double* T_out, sat_info* info);
 
+/* sat_eik_forward for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments and results as
+ * sat_eik_forward, but lon_deg, lat_deg, spha, sphb, sphc, fun and T_out are
+ * DEVICE pointers (the caller allocates them); info stays a host pointer.
+ * Calls no hipMalloc/hipFree: the planes of the solve live in one static
+ * device array of 24 MiB per GPU, held until the process ends (enough for
+ * about 130,000 nodes in FP64, e.g. 360x360; a larger grid returns SAT_EINVAL
+ * with a message on stderr -- use sat_eik_forward).  Synchronous on the null
+ * stream.  Every call on one GPU shares that array: calls that overlap on one
+ * GPU (several host threads) race and must be serialised by the caller. */
+int sat_eik_forward_noalloc(int precision, int nx, int ny, const double* lon_deg, const double* lat_deg,
+                            const double* spha, const double* sphb, const double* sphc, const double* fun,
+                            double src_lon_deg, double src_lat_deg, double* T_out, sat_info* info);
+
 /* Replaces eikonal::FSM_O1_JSE_lonlat_2d (include/eikonal_solver.h:74-83,
  * src/eikonal_solver.cpp:323-512): the adjoint-state field for receiver
  * adjoint sources adj[nr] at (rec_lon_deg, rec_lat_deg), given the forward

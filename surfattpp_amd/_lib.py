@@ -27,7 +27,7 @@ FLAG_REC_OUTSIDE = 8
 
 # Every symbol declared in include/surfatt_eik.h.
 EXPORTS = (
-    "sat_eik_forward", "sat_eik_backward", "sat_eik_mask",
+    "sat_eik_forward", "sat_eik_forward_noalloc", "sat_eik_backward", "sat_eik_mask",
     "sat_batch_create", "sat_batch_update_media", "sat_batch_update_obs", "sat_batch_run",
     "sat_batch_sync", "sat_batch_results", "sat_batch_last_ms", "sat_batch_destroy",
     "sat_batch_set_f32_band", "sat_batch_conv", "sat_batch_phase_ms", "sat_batch_waves",
@@ -82,6 +82,8 @@ def load(path: str = LIB_PATH):
     vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
     L.sat_eik_forward.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp,
                                   C.c_double, C.c_double, vp, C.POINTER(sat_info)]
+    L.sat_eik_forward_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp,
+                                          C.c_double, C.c_double, vp, C.POINTER(sat_info)]
     L.sat_eik_backward.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp,
                                    C.c_int, vp, vp, vp, vp, C.POINTER(sat_info)]
     L.sat_eik_mask.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_double, C.c_double]

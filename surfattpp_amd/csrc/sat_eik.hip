@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <type_traits>
@@ -571,6 +572,167 @@ int solo_run(bool forward, int precision, int nx, int ny, const double* lon, con
 #undef SOLO_CHECK
 }
 
+// ----------------------------------------------------------------------------
+// Allocation-free per-solve forward (sat_eik_forward_noalloc): the same
+// kernels as sat_eik_forward, with every plane they need carved out of one
+// file-scope device array instead of allocated per call.  Each GPU holds its
+// own copy of the array for the life of the process; every call on a GPU
+// shares it, so calls that overlap on one GPU race.
+// ----------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(24) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// The array's device address on the current device: hipGetSymbolAddress once
+// per device (the host-side name of the array is not a device address).
+int scratch_address(void** out) {
+  static std::mutex mu;
+  static void* addr[kNoAllocMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices) return SAT_EDEVICE;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      std::fprintf(stderr, "[surfatt_eik] hipGetSymbolAddress(scratch): %s\n", hipGetErrorString(e));
+      return SAT_EDEVICE;
+    }
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return SAT_OK;
+}
+
+}  // namespace
+
+namespace sat {
+// Axes in radians (src/eikonal_solver.cpp:41-47) and the allocating entry's
+// uniform-axis test (uniform_axis) on device-resident axes: meta[0] = 1 if
+// both axes are uniform and ascending, meta[1] = dx, meta[2] = dy.
+__global__ void noalloc_axes_kernel(int nx, int ny, const double* lon, const double* lat, double* xx, double* yy,
+                                    double* meta) {
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  const double dlon = lon[1] - lon[0], dlat = lat[1] - lat[0];
+  if (threadIdx.x == 0 && (!(dlon > 0.0) || !isfinite(dlon) || !(dlat > 0.0) || !isfinite(dlat))) bad = 1;
+  for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+    xx[i] = lon[i] * kDEG2RAD;
+    if (i > 0 && !(fabs((lon[i] - lon[i - 1]) - dlon) <= 1e-6 * fabs(dlon))) atomicOr(&bad, 1);
+  }
+  for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+    yy[j] = lat[j] * kDEG2RAD;
+    if (j > 0 && !(fabs((lat[j] - lat[j - 1]) - dlat) <= 1e-6 * fabs(dlat))) atomicOr(&bad, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    meta[0] = bad ? 0.0 : 1.0;
+    meta[1] = lon[1] * kDEG2RAD - lon[0] * kDEG2RAD;  // xx[1] - xx[0]
+    meta[2] = lat[1] * kDEG2RAD - lat[0] * kDEG2RAD;
+  }
+}
+
+// Zero n 8-byte words (a kernel, not a memset node).
+__global__ void fill_zero_kernel(unsigned long long* p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+}  // namespace sat
+
+namespace {
+
+int forward_noalloc(int precision, int nx, int ny, const double* lon, const double* lat, const double* spha,
+                    const double* sphb, const double* sphc, const double* fun, double x0, double y0, double* T_out,
+                    sat_info* info) {
+  if (precision != SAT_PREC_F64 && precision != SAT_PREC_F32) return SAT_EINVAL;
+  if (!lon || !lat || !spha || !sphb || !sphc || !fun || !T_out) return SAT_EINVAL;
+  if (nx < 3 || ny < 3 || !std::isfinite(x0) || !std::isfinite(y0)) return SAT_EINVAL;
+  const Layout L = layout_for(nx);
+  if (L.C > kMaxC) return SAT_EINVAL;
+  int rc = device_check();
+  if (rc) return rc;
+  // the scratch array's layout for this grid
+  const size_t N = (size_t)nx * ny, S = (size_t)L.P * ny, es = elem_size(precision);
+  const SlotGeom sg = slot_geom(precision, S, N);
+  size_t off = 0;
+  auto take = [&](size_t bytes) { const size_t o = off; off += round_up(bytes, 256); return o; };
+  const size_t o_xx = take(nx * sizeof(double)), o_yy = take(ny * sizeof(double));
+  const size_t o_meta = take(4 * sizeof(double)), o_info = take(4 * sizeof(double));
+  const size_t o_trig = take(ny * sizeof(double2)), o_row = take(ny * sizeof(float4));
+  size_t o_pl[4], o_seg[6];
+  for (int i = 0; i < 4; i++) o_pl[i] = take(N * sizeof(double));
+  for (int i = 0; i < 6; i++) o_seg[i] = take(S * es);
+  const size_t o_slot = take(sg.slot);
+  if (off > kNoAllocScratchBytes) {
+    std::fprintf(stderr,
+                 "[surfatt_eik] sat_eik_forward_noalloc: a %dx%d grid needs %zu bytes of scratch; the static "
+                 "scratch holds %zu (use sat_eik_forward)\n",
+                 nx, ny, off, kNoAllocScratchBytes);
+    return SAT_EINVAL;
+  }
+  void* base = nullptr;
+  rc = scratch_address(&base);
+  if (rc) return rc;
+  char* sp = static_cast<char*>(base);
+  auto at = [&](size_t o) { return sp + o; };
+  double* meta = reinterpret_cast<double*>(at(o_meta));
+  hipStream_t stream = nullptr;
+  hipLaunchKernelGGL(noalloc_axes_kernel, dim3(1), dim3(256), 0, stream, nx, ny, lon, lat,
+                     reinterpret_cast<double*>(at(o_xx)), reinterpret_cast<double*>(at(o_yy)), meta);
+  SAT_CHECK(hipGetLastError());
+  double hm[3];
+  SAT_CHECK(hipMemcpy(hm, meta, sizeof(hm), hipMemcpyDeviceToHost));
+  if (hm[0] != 1.0) return SAT_EINVAL;  // non-uniform or descending axis
+  GridDev g{};
+  g.nx = nx; g.ny = ny; g.nperiod = 1;
+  g.C = L.C; g.La = L.La; g.P = L.P;
+  g.seg_plane = S; g.plane = N;
+  g.dx = hm[1]; g.dy = hm[2];
+  g.lon = lon; g.lat = lat;
+  g.xx = reinterpret_cast<const double*>(at(o_xx));
+  g.yy = reinterpret_cast<const double*>(at(o_yy));
+  double* pl[4];
+  for (int i = 0; i < 4; i++) pl[i] = reinterpret_cast<double*>(at(o_pl[i]));
+  g.f_plain = pl[0]; g.aA_plain = pl[1]; g.b_plain = pl[2]; g.c_plain = pl[3]; g.vinv3 = nullptr;
+  for (int i = 0; i < 6; i++) g.seg[i] = at(o_seg[i]);
+  g.trig_yy = reinterpret_cast<const double2*>(at(o_trig));
+  g.rowtab = reinterpret_cast<const float4*>(at(o_row));
+  g.iso = 0;
+  g.wplane = (size_t)(ny + L.La - 1) * L.La * L.C;
+  for (int i = 0; i < 6; i++) g.wmed[i][0] = g.wmed[i][1] = nullptr;
+  hipLaunchKernelGGL(row_kernel, dim3((ny + 63) / 64), dim3(64), 0, stream, g, reinterpret_cast<double2*>(at(o_trig)),
+                     reinterpret_cast<float4*>(at(o_row)));
+  SAT_CHECK(hipGetLastError());
+  {  // media mode 1: (spha, sphb, sphc, fun) as FSM_UW_PS_lonlat_2d takes them
+    const int blocks = (int)std::min<size_t>((S + 255) / 256, 65535);
+    hipLaunchKernelGGL(media_kernel, dim3(blocks), dim3(256), 0, stream, g, 1, precision, spha, sphb, sphc, fun, pl[0],
+                       pl[1], pl[2], pl[3], nullptr, at(o_seg[0]), at(o_seg[1]), at(o_seg[2]), at(o_seg[3]),
+                       at(o_seg[4]), at(o_seg[5]));
+    SAT_CHECK(hipGetLastError());
+  }
+  {  // the slot starts at zero, as the allocating entry's does (the previous call's planes are stale)
+    const size_t words = sg.slot / 8;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<size_t>((words + 255) / 256, 4096)), dim3(256), 0,
+                       stream, reinterpret_cast<unsigned long long*>(at(o_slot)), words);
+    SAT_CHECK(hipGetLastError());
+  }
+  SoloDev sd{};
+  sd.g = g;
+  sd.src_lon = x0; sd.src_lat = y0;
+  sd.out = T_out; sd.ws = at(o_slot); sd.ws_seg = sg.seg; sd.ws_plain = sg.plain;
+  sd.info = reinterpret_cast<double*>(at(o_info));
+  sd.maxiter = kFwdMaxIter;
+  SAT_CHECK(precision == SAT_PREC_F64 ? launch_solo<SAT_PREC_F64>(1, g.C, stream, sd)
+                                      : launch_solo<SAT_PREC_F32>(1, g.C, stream, sd));
+  double hinfo[4];
+  SAT_CHECK(hipMemcpy(hinfo, sd.info, sizeof(hinfo), hipMemcpyDeviceToHost));  // waits for the solve
+  if (info) {
+    info->iters = (int)hinfo[0]; info->flags = (int)hinfo[1]; info->l1 = hinfo[2]; info->linf = hinfo[3];
+  }
+  return SAT_OK;
+}
+
 }  // namespace
 
 struct sat_batch {
@@ -855,6 +1017,13 @@ int sat_eik_forward(int precision, int nx, int ny, const double* lon_deg, const 
                     double src_lon_deg, double src_lat_deg, double* T_out, sat_info* info) {
   return solo_run(true, precision, nx, ny, lon_deg, lat_deg, spha, sphb, sphc, fun, src_lon_deg, src_lat_deg, 0,
                   nullptr, nullptr, nullptr, T_out, info);
+}
+
+int sat_eik_forward_noalloc(int precision, int nx, int ny, const double* lon_deg, const double* lat_deg,
+                            const double* spha, const double* sphb, const double* sphc, const double* fun,
+                            double src_lon_deg, double src_lat_deg, double* T_out, sat_info* info) {
+  return forward_noalloc(precision, nx, ny, lon_deg, lat_deg, spha, sphb, sphc, fun, src_lon_deg, src_lat_deg, T_out,
+                         info);
 }
 
 int sat_eik_backward(int precision, int nx, int ny, const double* lon_deg, const double* lat_deg,
