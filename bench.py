@@ -129,7 +129,7 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
-    from surfattpp_amd import PREC_F32, PREC_F64, RUN_PER_SOLVE, _lib, synth
+    from surfattpp_amd import PREC_F32, PREC_F32P, PREC_F64, RUN_PER_SOLVE, _lib, synth
     from surfattpp_amd import preproc as PP
 
     dev = _lib.device_info()
@@ -170,6 +170,26 @@ def main():
     bt.close()
     ms_per_step = wall / args.steps * 1e3
     value = ne * args.steps / wall
+
+    # the same workload with the FP64-polished update (SAT_PREC_F32P)
+    bp = PP.EikonalBatch(b, precision=PREC_F32P)
+    for _ in range(args.warmup):
+        bp.run()
+    bp.sync()
+    t0 = time.perf_counter()
+    php = []
+    for _ in range(args.steps):
+        bp.run(sync=False)
+        bp.sync()
+        php.append(bp.phase_ms())
+    wall_p = time.perf_counter() - t0
+    rp = bp.results()
+    bp.close()
+    f32p = {"value": round(ne * args.steps / wall_p, 2), "ms_per_step": round(wall_p / args.steps * 1e3, 3),
+            "fwd_sweeps_ms": round(float(np.mean([p["fwd_sweeps"] for p in php])), 3),
+            "err_vs_fp64": {"kernel_max_per_period": _kernel_err(rp, r64, 1),
+                            "syn": float(np.abs(rp["syn"] - r64["syn"]).max() / np.abs(r64["syn"]).max())},
+            "iters_fwd_equal": bool(np.array_equal(rp["iters_fwd"], r64["iters_fwd"]))}
 
     err_k = max(float(np.abs(r32["adj_s"][p] - r64["adj_s"][p]).max() / np.abs(r64["adj_s"][p]).max())
                 for p in range(r64["adj_s"].shape[0]))
@@ -239,6 +259,7 @@ def main():
         "cpu_baseline": cpu,
         "err_vs_fp64": {"kernel_max_per_period": err_k, "syn": err_syn, "chi": err_chi},
         "batched_f64": {"value": round(ne / (f64_ms[0] * 1e-3), 2), "ms_per_step": round(f64_ms[0], 3)},
+        "batched_f32p": f32p,
         "plain_f64_per_solve": {"value": round(plain_rate, 3), "pairs_timed": args.plain_pairs},
         "speedup": {"vs_batched_f64": round(value / (ne / (f64_ms[0] * 1e-3)), 3),
                     "vs_plain_f64_per_solve": round(value / plain_rate, 1),
@@ -280,9 +301,10 @@ def _kernel_err(r32, r64, ngrid):
 
 
 def _time_pair(grids, runs=2):
-    """FP64 (one run after a warm-up) and FP32 (`runs` runs after a warm-up) of one batch:
-    pairs/s of each, kernel and syn error of FP32 against FP64, FP32 phases."""
-    from surfattpp_amd import PREC_F32, PREC_F64
+    """FP64 (one run after a warm-up), FP32 and FP32 with the polished update
+    (`runs` runs after a warm-up each) of one batch: pairs/s of each, kernel and
+    syn error against FP64, FP32 phases."""
+    from surfattpp_amd import PREC_F32, PREC_F32P, PREC_F64
     from surfattpp_amd import preproc as PP
     ngrid = len(grids) if isinstance(grids, list) else 1
     with PP.EikonalBatch(grids, precision=PREC_F64) as b64:
@@ -290,23 +312,30 @@ def _time_pair(grids, runs=2):
         b64.run()
         ms64 = b64.last_ms()[0]
         r64 = b64.results()
-    with PP.EikonalBatch(grids, precision=PREC_F32) as b32:
-        b32.run()
-        t = []
-        for _ in range(runs):
-            b32.run()
-            t.append(b32.last_ms()[0])
-        ph = b32.phase_ms()
-        r32 = b32.results()
-        nres = b32.conv()[2]
+    def timed(prec):
+        with PP.EikonalBatch(grids, precision=prec) as bt:
+            bt.run()
+            t = []
+            for _ in range(runs):
+                bt.run()
+                t.append(bt.last_ms()[0])
+            return float(np.mean(t)), bt.phase_ms(), bt.results(), bt.conv()[2]
+
+    ms32, ph, r32, nres = timed(PREC_F32)
+    msp, php, rp, _ = timed(PREC_F32P)
     ne = len(r64["iters_fwd"])
-    ms32 = float(np.mean(t))
+
+    def err(r):
+        return {"kernel_max_per_period": _kernel_err(r, r64, ngrid),
+                "syn": float(np.abs(r["syn"] - r64["syn"]).max() / np.abs(r64["syn"]).max())}
     return {"pairs": ne, "value_f32": round(ne / (ms32 * 1e-3), 2), "ms_f32": round(ms32, 3),
             "value_f64": round(ne / (ms64 * 1e-3), 2), "ms_f64": round(ms64, 3),
-            "err_vs_fp64": {"kernel_max_per_period": _kernel_err(r32, r64, ngrid),
-                            "syn": float(np.abs(r32["syn"] - r64["syn"]).max() / np.abs(r64["syn"]).max())},
+            "err_vs_fp64": err(r32),
             "iters_fwd_equal": bool(np.array_equal(r32["iters_fwd"], r64["iters_fwd"])),
-            "fp64_resolves": int(nres), "phases_ms_f32": {k: round(v, 3) for k, v in ph.items()}}
+            "fp64_resolves": int(nres), "phases_ms_f32": {k: round(v, 3) for k, v in ph.items()},
+            "f32p": {"value": round(ne / (msp * 1e-3), 2), "ms": round(msp, 3), "err_vs_fp64": err(rp),
+                     "iters_fwd_equal": bool(np.array_equal(rp["iters_fwd"], r64["iters_fwd"])),
+                     "fwd_sweeps_ms": round(php["fwd_sweeps"], 3)}}
 
 
 def extra_workloads():

@@ -34,9 +34,16 @@ extern "C" {
 /* Arithmetic of the sweeps.  SAT_PREC_F64 reproduces the reference's FP64
  * operation order (kernels within 1e-12 of the FP64 oracle).  SAT_PREC_F32
  * sweeps the factored perturbation in FP32 with FP64 source initialisation,
- * convergence norm, misfit and kernel accumulation. */
+ * convergence norm, misfit and kernel accumulation.  SAT_PREC_F32P (batched
+ * runs only) is SAT_PREC_F32 where the forward node update of isotropic grids
+ * finds its candidate in FP32 and then takes that candidate's root in FP64
+ * from FP64 inputs (one Newton step): travel times ~30x closer to FP64 and
+ * kernels ~4x, for a slower forward sweep.  Grids with a topography metric,
+ * per-solve runs (SAT_RUN_PER_SOLVE) and sat_eik_forward/backward treat it as
+ * SAT_PREC_F32; its batched sweeps use one wave per pair. */
 #define SAT_PREC_F64 0
 #define SAT_PREC_F32 1
+#define SAT_PREC_F32P 2
 
 #define SAT_FLAG_NOT_CONVERGED 1   /* forward hit MAXITER (eikonal_solver.cpp:304) */
 #define SAT_FLAG_SRC_ON_BOUNDARY 2 /* a fixed near-source node is on the edge (:123) */

@@ -9,7 +9,7 @@ include/surfatt_eik.h (library: libsurfatt_eik.so, built in-tree):
   surfattpp_amd.synth                          (seeded workloads, topography metric)
 """
 from . import _lib  # noqa: F401
-from ._lib import PREC_F32, PREC_F64, RUN_BATCHED, RUN_GRAPH, RUN_PER_SOLVE  # noqa: F401
+from ._lib import PREC_F32, PREC_F32P, PREC_F64, RUN_BATCHED, RUN_GRAPH, RUN_PER_SOLVE  # noqa: F401
 
-__all__ = ["eikonal", "preproc", "synth", "tune", "PREC_F32", "PREC_F64", "RUN_BATCHED", "RUN_GRAPH",
-           "RUN_PER_SOLVE"]
+__all__ = ["eikonal", "preproc", "synth", "tune", "PREC_F32", "PREC_F32P", "PREC_F64", "RUN_BATCHED",
+           "RUN_GRAPH", "RUN_PER_SOLVE"]

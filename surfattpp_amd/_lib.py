@@ -17,6 +17,7 @@ SAT_ENOMEM = 12
 SAT_EDEVICE = 5
 PREC_F64 = 0
 PREC_F32 = 1
+PREC_F32P = 2  # FP32 batched sweeps with the FP64-polished isotropic update
 RUN_BATCHED = 0
 RUN_PER_SOLVE = 1
 RUN_GRAPH = 2

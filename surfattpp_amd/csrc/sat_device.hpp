@@ -66,10 +66,16 @@ struct GridDev {
   const double2* trig_yy;  // per row: sin(yy), cos(yy), correctly rounded
   int iso;                 // spha = sphb = 1, sphc = 0 on every node of every period
   const float4* rowtab;    // iso grids, per row: a, b, 1/sqrt(a), 1/sqrt(b) (FP32 sweeps)
+  // Isotropic FP32 batched sweeps solve in scaled coordinates X = sqrt(a) Tx,
+  // Y = sqrt(b) Ty (a = 1/(R cos(lat))^2, b = 1/R^2): per row
+  // rowha = sqrt(a)/dx = 1/(R cos(lat) dx), kb = sqrt(b)/dy = 1/(R dy), FP64.
+  const double* rowha;
+  double kb;
   // the forward media again in wavefront-major order (batched sweeps), one copy
   // per orientation, [nperiod] planes of wplane elements each
   size_t wplane;           // (ny + La - 1) * La * C
   const void* wmed[6][2];
+  const double* wf64[2];   // FP32 batches: the slowness again in FP64, wavefront-major per orientation
 };
 
 // Geometry of a grid swept by W waves of 64 lanes: C = ceil(ceil(nx / 64) / W)

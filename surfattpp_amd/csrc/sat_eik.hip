@@ -81,6 +81,8 @@ __global__ void media_kernel(GridDev g, int mode, int prec, const double* __rest
     size_t wi[2] = {0, 0};
     if (wave)
       for (int o = 0; o < 2; o++) wi[o] = (size_t)ip * g.wplane + wave_index(g, o, q, iy);
+    if (wave && g.wf64[0])
+      for (int o = 0; o < 2; o++) const_cast<double*>(g.wf64[o])[wi[o]] = vf;
     if (prec == SAT_PREC_F64) {
       const double v[4] = {va, vb, vc, vf};
       for (int i = 0; i < 4; i++) {
@@ -106,12 +108,13 @@ __global__ void media_kernel(GridDev g, int mode, int prec, const double* __rest
 // Per row: correctly rounded sin/cos of the latitude; isotropic grids also the
 // FP32 row metric {a, b, 1/sqrt(a), 1/sqrt(b)} (src/eikonal_solver.cpp:55-65 with
 // spha = sphb = 1, sphc = 0).
-__global__ void row_kernel(GridDev g, double2* trig_yy, float4* rowtab) {
+__global__ void row_kernel(GridDev g, double2* trig_yy, float4* rowtab, double* rowha) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
   if (iy >= g.ny) return;
   double s, c;
   cr::sincos_cr(g.yy[iy], s, c);
   trig_yy[iy] = make_double2(s, c);
+  if (rowha) rowha[iy] = 1.0 / (kR * c * g.dx);  // sqrt(a)/dx, a = 1/(R cos)^2
   if (rowtab) {
     const double R2 = kR * kR;
     const double a = 1.0 / (R2 * (c * c)), b = 1.0 / R2;
@@ -326,6 +329,8 @@ struct GridHost {
   GridDev g64{};              // FP32 batches: g with the FP64 media
   double2* d_trig_yy = nullptr;
   float4* d_rowtab = nullptr;
+  double* d_rowha = nullptr;
+  double* d_wf64 = nullptr;   // FP32 batches: FP64 slowness, wavefront-major, per orientation
   char* d_wmed = nullptr;     // batched sweeps: wavefront-major media, per orientation
   char* d_wmedmw = nullptr;   // FP32 batches: FP64 wavefront-major media of the re-solve geometry
   GridDev gmw{};              // FP32 batches: FP64 media, resolve_waves() waves per grid (re-solves)
@@ -334,7 +339,7 @@ struct GridHost {
 
   void release() {
     void* ps[] = {d_lon, d_lat, d_xx, d_yy, d_model, d_fp, d_aA, d_b, d_c, d_v3, d_seg, d_seg64, d_trig_yy, d_rowtab,
-                  d_wmed, d_wmedmw};
+                  d_rowha, d_wf64, d_wmed, d_wmedmw};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = GridHost();
@@ -346,7 +351,7 @@ static size_t elem_size(int prec) { return prec == SAT_PREC_F64 ? sizeof(double)
 // wave: also the wavefront-major media of the batched sweeps, laid out for W
 // waves per pair (W * 64 lanes, ceil(ceil(nx / 64) / W) columns per lane).
 static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const double* lon, const double* lat,
-                      bool wave, int W = 1) {
+                      bool wave, int W = 1, bool polish = false) {
   if (nx < 3 || ny < 3 || nper < 1 || !lon || !lat) return SAT_EINVAL;
   if (!uniform_axis(lon, nx) || !uniform_axis(lat, ny)) return SAT_EINVAL;
   const Layout L = layout_for(nx);
@@ -385,10 +390,13 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
   for (int i = 0; i < 6; i++) g.seg[i] = G.d_seg + (size_t)i * seg_bytes;
   SAT_CHECK(dalloc(&G.d_trig_yy, ny));
   SAT_CHECK(dalloc(&G.d_rowtab, ny));
+  SAT_CHECK(dalloc(&G.d_rowha, ny));
   g.trig_yy = G.d_trig_yy;
   g.rowtab = G.d_rowtab;
+  g.rowha = G.d_rowha;
+  g.kb = 1.0 / (kR * g.dy);  // sqrt(b)/dy, b = 1/R^2
   g.iso = 0;
-  hipLaunchKernelGGL(row_kernel, dim3((ny + 63) / 64), dim3(64), 0, nullptr, g, G.d_trig_yy, G.d_rowtab);
+  hipLaunchKernelGGL(row_kernel, dim3((ny + 63) / 64), dim3(64), 0, nullptr, g, G.d_trig_yy, G.d_rowtab, G.d_rowha);
   SAT_CHECK(hipGetLastError());
   SAT_CHECK(hipDeviceSynchronize());
   g.wplane = (size_t)(ny + L.La - 1) * L.La * L.C;
@@ -411,6 +419,15 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
     SAT_CHECK(hipMemset(G.d_wmed, 0, (size_t)nmed * 2 * pb));
     for (int i = 0; i < nmed; i++)
       for (int o = 0; o < 2; o++) g.wmed[i][o] = G.d_wmed + (size_t)(2 * i + o) * pb;
+    if (prec == SAT_PREC_F32 && polish) {
+      // the FP64 slowness of the polished isotropic sweeps (g only: the FP64
+      // copies g64 and gmw were taken above and keep wf64 null)
+      const size_t pf = (size_t)nper * g.wplane;
+      SAT_CHECK(dalloc(&G.d_wf64, 2 * pf));
+      SAT_CHECK(hipMemset(G.d_wf64, 0, 2 * pf * sizeof(double)));
+      g.wf64[0] = G.d_wf64;
+      g.wf64[1] = G.d_wf64 + pf;
+    }
     if (prec == SAT_PREC_F32) {
       // re-solve geometry: Wr = resolve_waves(ceil(nx / 64)) waves per grid,
       // C = ceil(nx / (64 Wr))
@@ -499,6 +516,7 @@ int solo_run(bool forward, int precision, int nx, int ny, const double* lon, con
              const double* spha, const double* sphb, const double* sphc, const double* fun_or_T, double x0,
              double y0, int nr, const double* rlon, const double* rlat, const double* adj, double* out,
              sat_info* info) {
+  if (precision == SAT_PREC_F32P) precision = SAT_PREC_F32;  // the per-solve kernels have no polished update
   if (precision != SAT_PREC_F64 && precision != SAT_PREC_F32) return SAT_EINVAL;
   if (!spha || !sphb || !sphc || !fun_or_T || !out) return SAT_EINVAL;
   if (!forward && nr > 0 && (!rlon || !rlat || !adj)) return SAT_EINVAL;
@@ -645,6 +663,7 @@ namespace {
 int forward_noalloc(int precision, int nx, int ny, const double* lon, const double* lat, const double* spha,
                     const double* sphb, const double* sphc, const double* fun, double x0, double y0, double* T_out,
                     sat_info* info) {
+  if (precision == SAT_PREC_F32P) precision = SAT_PREC_F32;  // the per-solve kernels have no polished update
   if (precision != SAT_PREC_F64 && precision != SAT_PREC_F32) return SAT_EINVAL;
   if (!lon || !lat || !spha || !sphb || !sphc || !fun || !T_out) return SAT_EINVAL;
   if (nx < 3 || ny < 3 || !std::isfinite(x0) || !std::isfinite(y0)) return SAT_EINVAL;
@@ -702,7 +721,7 @@ int forward_noalloc(int precision, int nx, int ny, const double* lon, const doub
   g.wplane = (size_t)(ny + L.La - 1) * L.La * L.C;
   for (int i = 0; i < 6; i++) g.wmed[i][0] = g.wmed[i][1] = nullptr;
   hipLaunchKernelGGL(row_kernel, dim3((ny + 63) / 64), dim3(64), 0, stream, g, reinterpret_cast<double2*>(at(o_trig)),
-                     reinterpret_cast<float4*>(at(o_row)));
+                     reinterpret_cast<float4*>(at(o_row)), nullptr);
   SAT_CHECK(hipGetLastError());
   {  // media mode 1: (spha, sphb, sphc, fun) as FSM_UW_PS_lonlat_2d takes them
     const int blocks = (int)std::min<size_t>((S + 255) / 256, 65535);
@@ -736,7 +755,8 @@ int forward_noalloc(int precision, int nx, int ny, const double* lon, const doub
 }  // namespace
 
 struct sat_batch {
-  int precision = SAT_PREC_F64;
+  int precision = SAT_PREC_F64;    // SAT_PREC_F64 or SAT_PREC_F32 (also for SAT_PREC_F32P batches)
+  bool polish = false;             // SAT_PREC_F32P: isotropic grids sweep with the FP64-polished update
   int calc_adj = 1;
   int nevent = 0, nrec = 0;
   std::vector<GridHost> grids;
@@ -860,9 +880,9 @@ int build_groups(sat_batch* b) {
   const int ne = b->nevent;
   std::vector<int> order(ne);
   std::iota(order.begin(), order.end(), 0);
-  auto key = [&](int e) {
+  auto key = [&](int e) {  // metric mode: kMetricGeneral, kMetricIso, kMetricIsoPolish
     const GridHost& G = b->grids[b->he[e].grid];
-    return std::make_tuple(G.W, G.gseg.C, G.g.iso);
+    return std::make_tuple(G.W, G.gseg.C, G.g.iso ? (b->polish ? kMetricIsoPolish : kMetricIso) : kMetricGeneral);
   };
   b->group_W.clear();
   b->group_wC.clear();
@@ -944,7 +964,7 @@ int build_groups(sat_batch* b) {
 // of the grid whose pairs take the most sweep steps
 // ((ny + La - 1) * C per sweep), so that batches too small to fill the device
 // split their longest pairs over more lanes.
-int choose_waves(const sat_batch_desc& d, bool f32, int cus, std::vector<int>& w) {
+int choose_waves(const sat_batch_desc& d, bool f32, bool polish, int cus, std::vector<int>& w) {
   const int ng = d.ngrid;
   w.assign(ng, 1);
   std::vector<long> nev(ng, 0);
@@ -953,8 +973,8 @@ int choose_waves(const sat_batch_desc& d, bool f32, int cus, std::vector<int>& w
   for (int gi = 0; gi < ng; gi++) {
     const int v = d.grids[gi].waves;
     if (v != 0 && v != 1 && v != 2 && v != 4) return SAT_EINVAL;
-    if (v > 1 && !f32) return SAT_EINVAL;
-    if (v == 0) autow[gi] = f32;
+    if (v > 1 && (!f32 || polish)) return SAT_EINVAL;  // FP64 and polished sweeps: one wave per pair
+    if (v == 0) autow[gi] = f32 && !polish;
     else w[gi] = v;
   }
   auto steps = [&](int gi, int W) {  // sweep steps x columns of one pair
@@ -1054,7 +1074,8 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   if (!d || !out) return SAT_EINVAL;
   *out = nullptr;
   if (d->ngrid < 1 || !d->grids || d->nevent < 0 || (d->nevent > 0 && !d->events)) return SAT_EINVAL;
-  if (d->precision != SAT_PREC_F64 && d->precision != SAT_PREC_F32) return SAT_EINVAL;
+  if (d->precision != SAT_PREC_F64 && d->precision != SAT_PREC_F32 && d->precision != SAT_PREC_F32P)
+    return SAT_EINVAL;
   if (d->nrec_total < 0 || (d->nrec_total > 0 && (!d->rec_lon_deg || !d->rec_lat_deg || !d->tt_obs || !d->weight)))
     return SAT_EINVAL;
   std::vector<int> rec_ev((size_t)std::max(d->nrec_total, 1), -1);
@@ -1082,7 +1103,8 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
       return fail(e_ == hipErrorOutOfMemory ? SAT_ENOMEM : SAT_EDEVICE);      \
     }                                                                         \
   } while (0)
-  b->precision = d->precision;
+  b->polish = d->precision == SAT_PREC_F32P;
+  b->precision = b->polish ? SAT_PREC_F32 : d->precision;
   const bool f32 = b->precision == SAT_PREC_F32;
   b->calc_adj = d->calc_adj ? 1 : 0;
   b->nevent = d->nevent;
@@ -1103,14 +1125,15 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   {
     hipDeviceProp_t prop;
     B_CHECK(hipGetDeviceProperties(&prop, 0));
-    rc = choose_waves(*d, f32, prop.multiProcessorCount, waves);
+    rc = choose_waves(*d, f32, b->polish, prop.multiProcessorCount, waves);
     if (rc) return fail(rc);
   }
   size_t max_seg = 0, max_plane = 0, max_wplane = 0;
   int max_nx = 0;
   for (int gi = 0; gi < d->ngrid; gi++) {
     const sat_grid_desc& gd = d->grids[gi];
-    rc = grid_setup(b->grids[gi], b->precision, gd.nx, gd.ny, gd.nperiod, gd.lon_deg, gd.lat_deg, true, waves[gi]);
+    rc = grid_setup(b->grids[gi], b->precision, gd.nx, gd.ny, gd.nperiod, gd.lon_deg, gd.lat_deg, true, waves[gi],
+                    b->polish);
     if (rc) return fail(rc);
     rc = grid_media(b->grids[gi], 0, gd.svel, gd.m11, gd.m12, gd.m22, b->stream);
     if (rc) return fail(rc);
@@ -1135,11 +1158,13 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   b->base_flags.assign(std::max(ne, 1), 0);
   // Per-event workspace, in bytes: wavefront-major work planes, FP64 T (and
   // the FP64 T0 of FP32 sweeps), the masked Ta, the column trig table.
+  // (the work planes hold 8-byte elements in polished batches: their
+  // isotropic sweeps read the T0 gradient in FP64)
   const size_t es = elem_size(b->precision);
+  const size_t wes = b->polish ? sizeof(double) : es;
   auto ev_bytes = [&](const GridDev& g) {
     const size_t w = round_up(g.wplane, 64), pl = round_up(g.plane, 64), col = round_up(g.nx, 64);
-    return 6 * w * es + (f32 ? 2 : 1) * pl * sizeof(double) + (b->calc_adj ? pl * es : 0) +
-           col * sizeof(double2);
+    return 6 * w * wes + (f32 ? 2 : 1) * pl * sizeof(double) + (b->calc_adj ? pl * es : 0) + col * sizeof(double2);
   };
   size_t ws_all = 0;
   for (int e = 0; e < ne; e++) {
@@ -1244,8 +1269,8 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   // per-event planes of the phase pipeline (sized for the largest chunk)
   B_CHECK(dalloc(&b->d_evs, 3 * (size_t)std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_coltab, std::max<size_t>(col_total, 1)));
-  // 6 forward planes of es bytes, or 2 face-flux planes of 2*es and 2 delta planes
-  const size_t work_bytes = 6 * std::max<size_t>(w_total, 1) * es;
+  // 6 forward planes of wes bytes, or 2 face-flux planes of 2*es and 2 delta planes
+  const size_t work_bytes = 6 * std::max<size_t>(w_total, 1) * wes;
   B_CHECK(dalloc(&b->d_work, work_bytes));
   B_CHECK(hipMemset(b->d_work, 0, work_bytes));
   if (f32) B_CHECK(dalloc(&b->d_T0d, std::max<size_t>(pl_total, 1)));
@@ -1429,8 +1454,8 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   pd.rec_lo = 0;
   pd.evs = b->d_evs;
   pd.coltab = b->d_coltab;
-  const size_t es = elem_size(b->precision);
-  const size_t wb = b->w_total * es;  // bytes of one sweep-precision plane set
+  // bytes of one work plane (8-byte elements in polished batches)
+  const size_t wb = b->w_total * (b->polish ? sizeof(double) : elem_size(b->precision));
   for (int o = 0; o < 2; o++) {
     pd.t0[o] = b->d_work + (0 + o) * wb;
     pd.t0x[o] = b->d_work + (2 + o) * wb;
@@ -1481,6 +1506,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   pd.order = b->d_order;
   pd.fwd_maxiter = kFwdMaxIter;
   pd.adj_maxiter = kAdjMaxIter;
+  pd.polish = b->polish ? 1 : 0;
   const unsigned node_bx = (unsigned)((b->max_plane + 255) / 256);
   // init_kernel: one thread per wavefront-major element (either geometry) or node
   size_t init_n = b->max_plane;

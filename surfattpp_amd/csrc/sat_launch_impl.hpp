@@ -42,23 +42,32 @@ template <int PREC>
 hipError_t launch_fwd_sweep(int iso, int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first,
                             int count, int group) {
   if (W != 1) return launch_fwd_sweep_mw<PREC>(iso, C, W, grid, s, pd, first, count, group);
+  // iso: kMetricGeneral, kMetricIso or kMetricIsoPolish (FP32 only)
+  if (iso == kMetricIsoPolish && PREC != SAT_PREC_F32) return hipErrorInvalidValue;
   return with_C(C, [&]<int CC>() {
     if constexpr (PREC == SAT_PREC_F32) {
-      if (iso) {
-        hipLaunchKernelGGL((fwd_sweep_kernel<PREC, CC, true, 1>), grid, dim3(kWave), 0, s, pd, first, count, group);
+      if (iso == kMetricIso) {
+        hipLaunchKernelGGL((fwd_sweep_kernel<PREC, CC, kMetricIso, 1>), grid, dim3(kWave), 0, s, pd, first, count,
+                           group);
+        return;
+      }
+      if (iso == kMetricIsoPolish) {
+        hipLaunchKernelGGL((fwd_sweep_kernel<PREC, CC, kMetricIsoPolish, 1>), grid, dim3(kWave), 0, s, pd, first,
+                           count, group);
         return;
       }
     }
-    hipLaunchKernelGGL((fwd_sweep_kernel<PREC, CC, false, 1>), grid, dim3(kWave), 0, s, pd, first, count, group);
+    hipLaunchKernelGGL((fwd_sweep_kernel<PREC, CC, kMetricGeneral, 1>), grid, dim3(kWave), 0, s, pd, first, count,
+                       group);
   });
 }
 
 // Multi-wave sweeps (a work-group of W waves per event): FP64 W = kMultiWave
 // or kMultiWaveWide, C <= 2, general metric (the re-solves of FP32 batches); FP32 W = 2 (C <= 4)
 // and W = 4 (C <= 2), both metrics.
-template <int PREC, int W, int C, bool ISO>
+template <int PREC, int W, int C, int MODE>
 static void launch_fwd_mw1(dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count, int group) {
-  hipLaunchKernelGGL((fwd_sweep_kernel<PREC, C, ISO, W>), grid, dim3(kWave * W), 0, s, pd, first, count, group);
+  hipLaunchKernelGGL((fwd_sweep_kernel<PREC, C, MODE, W>), grid, dim3(kWave * W), 0, s, pd, first, count, group);
 }
 
 template <int PREC>
@@ -66,15 +75,17 @@ hipError_t launch_fwd_sweep_mw(int iso, int C, int W, dim3 grid, hipStream_t s, 
                                int count, int group) {
   if constexpr (PREC == SAT_PREC_F64) {
     if (iso) return hipErrorInvalidValue;
-    if (W == kMultiWave && C == 1) launch_fwd_mw1<PREC, kMultiWave, 1, false>(grid, s, pd, first, count, group);
-    else if (W == kMultiWave && C == 2) launch_fwd_mw1<PREC, kMultiWave, 2, false>(grid, s, pd, first, count, group);
+    if (W == kMultiWave && C == 1) launch_fwd_mw1<PREC, kMultiWave, 1, kMetricGeneral>(grid, s, pd, first, count, group);
+    else if (W == kMultiWave && C == 2)
+      launch_fwd_mw1<PREC, kMultiWave, 2, kMetricGeneral>(grid, s, pd, first, count, group);
     else if (W == kMultiWaveWide && C == 1)  // grids of 257..384 columns (resolve_waves)
-      launch_fwd_mw1<PREC, kMultiWaveWide, 1, false>(grid, s, pd, first, count, group);
+      launch_fwd_mw1<PREC, kMultiWaveWide, 1, kMetricGeneral>(grid, s, pd, first, count, group);
     else return hipErrorInvalidValue;
   } else {
+    if (iso == kMetricIsoPolish) return hipErrorInvalidValue;  // polished sweeps: one wave per pair
     auto go = [&]<int WW, int CC>() {
-      if (iso) launch_fwd_mw1<PREC, WW, CC, true>(grid, s, pd, first, count, group);
-      else launch_fwd_mw1<PREC, WW, CC, false>(grid, s, pd, first, count, group);
+      if (iso) launch_fwd_mw1<PREC, WW, CC, kMetricIso>(grid, s, pd, first, count, group);
+      else launch_fwd_mw1<PREC, WW, CC, kMetricGeneral>(grid, s, pd, first, count, group);
     };
     if (W == 2 && C == 1) go.template operator()<2, 1>();
     else if (W == 2 && C == 2) go.template operator()<2, 2>();

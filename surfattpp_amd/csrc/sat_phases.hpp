@@ -90,6 +90,7 @@ struct PhaseDev {
   const int* order;
   int* counter;
   int fwd_maxiter, adj_maxiter;
+  int polish;           // SAT_PREC_F32P batch: isotropic grids' T0 gradient planes are FP64, scaled
 };
 
 // Rows in flight per lane (register ring depth, sat_engine.hpp): multi-wave
@@ -140,6 +141,7 @@ template <> struct PhaseT<SAT_PREC_F64> {
   using F = double2;    // face fluxes
   using Fwd = FwdF64;
   using FwdIso = FwdF64;
+  using FwdIsoP = FwdF64;
   using Adj = AdjF64Face;
 };
 template <> struct PhaseT<SAT_PREC_F32> {
@@ -148,6 +150,7 @@ template <> struct PhaseT<SAT_PREC_F32> {
   using F = float2;
   using Fwd = FwdF32;
   using FwdIso = FwdF32Iso;
+  using FwdIsoP = FwdF32IsoP;
   using Adj = AdjF32Face;
 };
 
@@ -264,27 +267,43 @@ __global__ void init_kernel(PhaseDev pd) {
   if (pass < 2) {
     const size_t w = ev.w_off + idx;
     static_cast<V*>(pd.t0[pass])[w] = (V)T0;
-    static_cast<V*>(pd.t0x[pass])[w] = (V)T0x;
-    static_cast<V*>(pd.t0y[pass])[w] = (V)T0y;
+    if (PREC == SAT_PREC_F32 && pd.polish && g.iso) {
+      // the polished isotropic FP32 sweeps' scaled gradient, FP64: sqrt(a) T0x, sqrt(b) T0y
+      static_cast<double*>(pd.t0x[pass])[w] = T0x / (kR * cyy);
+      static_cast<double*>(pd.t0y[pass])[w] = T0y / kR;
+    } else {
+      static_cast<V*>(pd.t0x[pass])[w] = (V)T0x;
+      static_cast<V*>(pd.t0y[pass])[w] = (V)T0y;
+    }
   } else {
     pd.T0d[ev.pl_off + idx] = T0;
   }
 }
 
-template <int PREC, bool ISO, class Pol>
+// Metric modes of the forward sweeps: the general metric, the isotropic metric
+// (spha = sphb = 1, sphc = 0), and the isotropic metric with the FP64-polished
+// FP32 update (SAT_PREC_F32P batches).
+constexpr int kMetricGeneral = 0, kMetricIso = 1, kMetricIsoPolish = 2;
+
+template <int PREC, int MODE, class Pol>
 __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const EventX& ev) {
+  constexpr bool ISO = MODE != kMetricGeneral;
+  constexpr bool POL = PREC == SAT_PREC_F32 && MODE == kMetricIsoPolish;
   using PT = PhaseT<PREC>;
   using V = typename PT::V;
   typename Pol::Ctx cx;
   const size_t po = (size_t)ev.iper * g.wplane;
   cx.g = &g;
   cx.fb = FixedBox{ev.fb[0], ev.fb[1], ev.fb[2], ev.fb[3]};
+  // the polished isotropic FP32 sweeps: FP64 scaled gradient and slowness (FwdF32IsoP)
+  using G = std::conditional_t<POL, double, V>;
 #pragma unroll
   for (int o = 0; o < 2; o++) {
     cx.t0[o] = static_cast<const V*>(pd.t0[o]) + ev.w_off;
-    cx.t0x[o] = static_cast<const V*>(pd.t0x[o]) + ev.w_off;
-    cx.t0y[o] = static_cast<const V*>(pd.t0y[o]) + ev.w_off;
-    cx.f[o] = static_cast<const V*>(g.wmed[3][o]) + po;
+    cx.t0x[o] = static_cast<const G*>(pd.t0x[o]) + ev.w_off;
+    cx.t0y[o] = static_cast<const G*>(pd.t0y[o]) + ev.w_off;
+    if constexpr (POL) cx.f[o] = g.wf64[o] + po;
+    else cx.f[o] = static_cast<const V*>(g.wmed[3][o]) + po;
     if constexpr (!(PREC == SAT_PREC_F32 && ISO)) {
       cx.a[o] = static_cast<const V*>(g.wmed[0][o]) + po;
       cx.b[o] = static_cast<const V*>(g.wmed[1][o]) + po;
@@ -303,9 +322,14 @@ __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const
       cx.idy = 1.0 / g.dy;
     }
   } else {
-    if constexpr (ISO) cx.rowtab = g.rowtab;
-    cx.idx = (float)(1.0 / g.dx);
-    cx.idy = (float)(1.0 / g.dy);
+    if constexpr (POL) {
+      cx.rowha = g.rowha;
+      cx.kb = g.kb;
+    } else {
+      if constexpr (ISO) cx.rowtab = g.rowtab;
+      cx.idx = (float)(1.0 / g.dx);
+      cx.idy = (float)(1.0 / g.dy);
+    }
   }
   return cx;
 }
@@ -315,20 +339,26 @@ __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const
 // with 3 x 1024 resident waves a 3,000-pair batch is resident at once and the
 // VALU-bound sweeps of three pairs interleave on each SIMD -- otherwise the
 // compiler's own choice.
-template <int PREC, int C, bool ISO, int W>
+// The polished update holds the node's FP64 gradient and slowness in the row
+// pipeline: two waves per SIMD (no spills) where the plain update fits three.
+template <int PREC, int C, int MODE, int W>
 constexpr int fwd_waves_per_eu() {
-  return (PREC == SAT_PREC_F32 && W == 1 && (ISO ? C <= 4 : C <= 2)) ? 3 : 1;
+  if (PREC != SAT_PREC_F32 || W != 1) return 1;
+  if (MODE == kMetricIsoPolish) return C <= 4 ? 2 : 1;
+  return (MODE == kMetricIso ? C <= 4 : C <= 2) ? 3 : 1;
 }
 
 // Persistent forward sweeps, one work-group of W waves per event.
-template <int PREC, int C, bool ISO, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fwd_waves_per_eu<PREC, C, ISO, W>())))
+template <int PREC, int C, int MODE, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(fwd_waves_per_eu<PREC, C, MODE, W>())))
 void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
   using PT = PhaseT<PREC>;
   using U = typename PT::U;
   // multi-wave FP64 launches are the re-solves of FP32 batches (FwdF64R)
-  using Pol = std::conditional_t<(PREC == SAT_PREC_F64 && W > 1), FwdF64R,
-                                 std::conditional_t<ISO, typename PT::FwdIso, typename PT::Fwd>>;
+  using Pol = std::conditional_t<
+      (PREC == SAT_PREC_F64 && W > 1), FwdF64R,
+      std::conditional_t<MODE == kMetricIsoPolish, typename PT::FwdIsoP,
+                         std::conditional_t<MODE == kMetricIso, typename PT::FwdIso, typename PT::Fwd>>>;
   const int lane = threadIdx.x;
   const int nthr = kWave * W;
   __shared__ ExchangeLds<Pol, W> xch;
@@ -355,7 +385,7 @@ void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
       Q0[q] = !real ? U(0) : kSigma ? (fixed ? U(0) : U(9)) : (fixed ? U(1) : U(10));
     }
     __syncthreads();
-    const auto cx = uniform(phase_fwd_ctx<PREC, ISO, Pol>(pd, g, ev));
+    const auto cx = uniform(phase_fwd_ctx<PREC, MODE, Pol>(pd, g, ev));
     IterOut io;
     const U* u = iterate<Pol, WaveLayout, C, W, (W > 1) ? mw_fwd_depth<PREC, C, W>() : kFwdDepth1>(cx, Q0, Q1, Q2, pd.fwd_maxiter, kFwdTol,
                                                                           io, exchange_ptr<Pol, W>(xch), red);

@@ -513,6 +513,175 @@ struct FwdF32Iso {
   }
 };
 
+// Isotropic metric (spha = sphb = 1, sphc = 0 on every node), FP32 candidate
+// search with an FP64 polish of the winner.
+//
+// Coordinates.  With X = sqrt(a) Tx, Y = sqrt(b) Ty the node equation
+// a Tx^2 + b Ty^2 = f^2 is X^2 + Y^2 = f^2, and every candidate and
+// acceptance test of fwd_update_f32<true> is the same after scaling: the T0
+// gradient planes hold sqrt(a) T0x and sqrt(b) T0y, h = t0 sqrt(a)/dx (per row)
+// and k = t0 sqrt(b)/dy, the one-point distances become f, and the causality
+// signs are unchanged (sqrt(a), sqrt(b) > 0).
+//
+// Polish.  Rounding the per-node inputs (T0 gradient, slowness) to FP32
+// perturbs each node's equation by ~1e-7 of f^2 and moves its root by ~1e-7
+// of a cell's travel time; along rays these add up to T noise whose
+// differences, taken by the adjoint's face fluxes, reached 1e-5 of the
+// kernels on the 401x481 workload.  So the search finds the winning candidate
+// and its increment w in FP32, and then the winner's residual is evaluated in
+// FP64 from the FP64 inputs and neighbour values, and one Newton step
+// w' = w - r(w)/r'(w) (the correction itself in FP32) gives the FP64 root to
+// ~1e-14 relative.  The new value is taken when it is below the old one in
+// FP64 (also where the FP32 search saw no decrease: those tiny decreases
+// move the convergence norm).  The node's own T0 enters through h only and
+// stays FP32.
+// A candidate tagged with its group in the low 3 bits of its FP32 encoding:
+// the running minimum then carries the winner's group (a change of at most 7
+// ulp in the compared value; the polish recomputes the value anyway).
+__device__ __forceinline__ float tag_group(float c, unsigned g) {
+  return __builtin_bit_cast(float, (__builtin_bit_cast(unsigned, c) & ~7u) | g);
+}
+
+__device__ __forceinline__ double fwd_update_f32p(double u_self, float t0, double t0xd, double t0yd, double fd,
+                                                  double had, double kbd, double uW, float T0W, double uE, float T0E,
+                                                  double uS, float T0S, double uN, float T0N) {
+#pragma clang fp contract(fast)
+  const float kInf = __FLT_MAX__;  // "rejected" (see tag_group)
+  const float kNaNf = __builtin_nanf("");
+  const double u_ref = fmin(fmin(uW, uE), fmin(uS, uN));
+  const float ts = (float)(1.0 + u_ref);
+  const float dW = (float)(uW - u_ref), dE = (float)(uE - u_ref);
+  const float dS = (float)(uS - u_ref), dN = (float)(uN - u_ref);
+  const float t0x = (float)t0xd, t0y = (float)t0yd, f = (float)fd;
+  const float h = t0 * (float)had, kk = t0 * (float)kbd;
+  const float f2 = f * f;
+  const float pxm = t0x + h, pxp = t0x - h, pym = t0y + kk, pyp = t0y - kk;
+  const float Axm = pxm * pxm, Axp = pxp * pxp, Bym = pym * pym, Byp = pyp * pyp;
+  // groups: 0..3 two-point quadrants (bit 1: x neighbour E, bit 0: y neighbour N),
+  // 4..7 one-point W, E, S, N.  A NaN candidate (an absent neighbour, a
+  // negative discriminant) stays a quiet NaN when tagged; the minimum ignores it.
+  float best = kInf;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const bool xm = q < 2, ym = !(q & 1);
+    const float sxh = xm ? -h : h, syk = ym ? -kk : kk;
+    const float dX = xm ? dW : dE, dY = ym ? dS : dN;
+    const float px = xm ? pxm : pxp, py = ym ? pym : pyp;
+    const float dr = fminf(dX, dY);
+    const float tr = ts + dr;
+    const float X0 = t0x * tr + sxh * (dX - dr);
+    const float Y0 = t0y * tr + syk * (dY - dr);
+    const float Aq = (xm ? Axm : Axp) + (ym ? Bym : Byp);
+    const float Bh = px * X0 + py * Y0;
+    const float Cq = X0 * X0 + Y0 * Y0 - f2;
+    float D = Bh * Bh - Aq * Cq;
+    D = (fabsf(Aq) >= 1e-30f) ? D : kNaNf;
+    const float sq = __builtin_amdgcn_sqrtf(D);
+    const float sv = -(Bh + __builtin_copysignf(sq, Bh));
+    const float w0 = sv * __builtin_amdgcn_rcpf(Aq);
+    const float w1 = Cq * __builtin_amdgcn_rcpf(sv);
+    float cw[2];
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+      const float w = g ? w1 : w0;
+      const float X = X0 + px * w;
+      const float Y = Y0 + py * w;
+      const float m = fmaxf(xm ? -X : X, ym ? -Y : Y);
+      cw[g] = (m <= 0.0f) ? w : kInf;
+    }
+    best = fminf(best, tag_group(dr + fminf(cw[0], cw[1]), (unsigned)q));
+  }
+  const float rt0 = __builtin_amdgcn_rcpf(t0);
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const float dq = (d == 0) ? dW : (d == 1) ? dE : (d == 2) ? dS : dN;
+    const float T0n = (d == 0) ? T0W : (d == 1) ? T0E : (d == 2) ? T0S : T0N;
+    const float td = (d < 2) ? t0x : t0y;
+    const float p = (d == 0) ? pxm : (d == 1) ? pxp : (d == 2) ? pym : pyp;
+    const float rp = __builtin_amdgcn_rcpf(p);
+    const float tnd = ts + dq;
+    const float Td0 = td * tnd;
+    const float wp = (f - Td0) * rp;
+    const float wm = (-f - Td0) * rp;
+    const float thw = tnd * fmaxf(T0n * rt0 - 1.0f, -0.5f);
+    const float cp = (wp >= thw) ? wp : kInf;
+    const float cm = (wm >= thw) ? wm : kInf;
+    best = fminf(best, tag_group(dq + fminf(cp, cm), 4u + (unsigned)d));
+  }
+  // FP64 polish of the winner, one residual form for every group: the x and
+  // y terms of the two-point quadratic X^2 + Y^2 = f^2, the one-point
+  // equations being the same with the other term dropped (X = +-f).  When no
+  // candidate was accepted best is ~FLT_MAX and so is v: the old value stays.
+  const unsigned g = __builtin_bit_cast(unsigned, best) & 7u;
+  const bool two = g < 4u;
+  const bool useW = two ? (g < 2u) : (g == 4u);
+  const bool useS = two ? !(g & 1u) : (g == 6u);
+  const bool hasX = two || g < 6u;
+  const bool hasY = two || g >= 6u;
+  const double t0d = (double)t0;
+  const double h64 = t0d * had, k64 = t0d * kbd;
+  const double ux = useW ? uW : uE, uy = useS ? uS : uN;
+  const double dX = ux - u_ref, dY = uy - u_ref;
+  const double dr = two ? fmin(dX, dY) : (hasX ? dX : dY);
+  const double tr = 1.0 + u_ref + dr;
+  const double sxh = useW ? -h64 : h64, syk = useS ? -k64 : k64;
+  const double px = t0xd - sxh, py = t0yd - syk;
+  const double w = (double)best - dr;
+  const double X = hasX ? t0xd * tr + sxh * (dX - dr) + px * w : 0.0;
+  const double Y = hasY ? t0yd * tr + syk * (dY - dr) + py * w : 0.0;
+  const double r = X * X + Y * Y - fd * fd;
+  const double rd = 2.0 * (X * px + Y * py);
+  const double v = u_ref + dr + (w - (double)((float)r * __builtin_amdgcn_rcpf((float)rd)));
+  return (v < u_self) ? v : u_self;
+}
+
+struct FwdF32IsoP {
+  using V = double;  // sigma storage
+  using A = float;   // T0 of a neighbour
+  static constexpr bool kAux = true;
+  struct Ctx {
+    const GridDev* g;
+    const float* t0[2];
+    const double *t0x[2], *t0y[2], *f[2];  // sqrt(a) T0x, sqrt(b) T0y, slowness (FP64)
+    const double* rowha;                    // sqrt(a)/dx per row
+    double kb;                              // sqrt(b)/dy
+    FixedBox fb;
+  };
+  template <int C> struct Head { double u[C]; float aux[C]; };
+  template <int C> struct Body { double st[C], t0x[C], t0y[C], f[C]; double ha; };
+  template <int C, int O>
+  __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
+#pragma unroll
+    for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.t0[O], base + k); }
+  }
+  template <int C, int O, bool NORM>
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int j, const double* tst) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      B.t0x[k] = gld(cx.t0x[O], base + k);
+      B.t0y[k] = gld(cx.t0y[O], base + k);
+      B.f[k] = gld(cx.f[O], base + k);
+      B.st[k] = NORM ? gld(tst, base + k) : 0.0;
+    }
+    B.ha = gld(cx.rowha, j);
+  }
+  __device__ static bool fixed(const Ctx& cx, int ix, int j) {
+    return j >= cx.fb.y0 && j <= cx.fb.y1 && ix >= cx.fb.x0 && ix <= cx.fb.x1;
+  }
+  template <int C>
+  __device__ static double update(const Ctx& cx, const Head<C>& hd, const Body<C>& r, int k, int, double uW,
+                                  float aW, bool hW, double uE, float aE, bool hE, double uS, float aS, bool hS,
+                                  double uN, float aN, bool hN) {
+    const double kNaN = __builtin_nan("");
+    return fwd_update_f32p(hd.u[k], hd.aux[k], r.t0x[k], r.t0y[k], r.f[k], r.ha, cx.kb, hW ? uW : kNaN, aW,
+                           hE ? uE : kNaN, aE, hS ? uS : kNaN, aS, hN ? uN : kNaN, aN);
+  }
+  template <int C>
+  __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, double v) {
+    return fabs(v - r.st[k]) * (double)h.aux[k];
+  }
+};
+
 // Per-solve FP32 adjoint: coefficients divided through by d (one segmented plane each).
 struct AdjF32 {
   using V = float;

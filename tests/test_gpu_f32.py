@@ -124,3 +124,48 @@ def test_batch_f32_resolve_pool_and_overflow():
     assert RC.rel_inf(r32["syn"], r64["syn"]) <= TOL_T
     for p in range(r64["adj_s"].shape[0]):
         assert RC.rel_inf(r32["adj_s"][p], r64["adj_s"][p]) <= TOL_K
+
+
+def test_batch_f32p_polished_update_full_workload():
+    """SAT_PREC_F32P on the 201x201 workload: the FP32 search with the FP64
+    polish of the winning candidate.  Travel times (syn) within 1e-9 of the
+    FP64 path (FP32 alone: ~1e-8), every period's kernel within 2.5e-6 (FP32
+    alone: ~5e-6; what remains is the FP32 adjoint), iteration counts equal."""
+    from surfattpp_amd import PREC_F32P
+    b = synth.config_grf()
+    with PP.EikonalBatch(b) as b64:
+        b64.run()
+        r64 = b64.results()
+    with PP.EikonalBatch(b, precision=PREC_F32P) as bp:
+        bp.run()
+        rp = bp.results()
+        assert bp.waves() == [1]
+    errs = [RC.rel_inf(rp["adj_s"][p], r64["adj_s"][p]) for p in range(r64["adj_s"].shape[0])]
+    es = RC.rel_inf(rp["syn"], r64["syn"])
+    print(f"F32P: per-period K err max {max(errs):.3e}, syn err {es:.3e}", flush=True)
+    np.testing.assert_array_equal(rp["iters_fwd"], r64["iters_fwd"])
+    assert max(errs) <= 2.5e-6
+    assert es <= 1e-9
+
+
+def test_batch_f32p_small_vs_oracle_and_topography_fallback():
+    """F32P against the oracle on a small two-period batch, and on a grid with
+    a topography metric (no polished update there: the same results as
+    SAT_PREC_F32)."""
+    from surfattpp_amd import PREC_F32P
+    b = synth.config_grf(seed=11, nx=61, ny=57, nsta=8, nperiod=2, spacing_km=1.5)
+    ro = O.forward_for_event(b)
+    with PP.EikonalBatch(b, precision=PREC_F32P) as bp:
+        bp.run()
+        rp = bp.results()
+    np.testing.assert_array_equal(rp["iters_fwd"], ro["iters_fwd"])
+    assert RC.rel_inf(rp["syn"], ro["syn"]) <= 1e-9
+    for p in range(2):
+        assert RC.rel_inf(rp["adj_s"][p], ro["adj_s"][p]) <= 2.5e-6
+    t = synth.config_topo_grid(33, 61, nsta=6, nperiod=2)
+    with PP.EikonalBatch(t, precision=PREC_F32, waves=1) as b32, PP.EikonalBatch(t, precision=PREC_F32P) as bp:
+        b32.run()
+        bp.run()
+        r32, rp = b32.results(), bp.results()
+    np.testing.assert_array_equal(rp["syn"], r32["syn"])
+    np.testing.assert_array_equal(rp["adj_s"], r32["adj_s"])
