@@ -150,7 +150,13 @@ typedef struct sat_batch_desc {
 
 typedef struct sat_batch sat_batch; /* opaque: device inputs, workspace, results */
 
-/* Upload inputs and allocate all device memory; no allocation afterwards. */
+/* Upload inputs and allocate all device memory; no allocation afterwards.
+ * Ownership: the per-period planes (svel, m11, m12, m22), the receiver arrays
+ * and the observations may lie in host or device memory, here and in
+ * sat_batch_update_media / sat_batch_update_obs (the runtime tells them
+ * apart); so may the destinations of sat_batch_results and the metric and
+ * outputs of sat_batch_aniso_kernels.  The axes lon_deg / lat_deg are read on
+ * the host (their uniformity is checked there). */
 int sat_batch_create(const sat_batch_desc* desc, sat_batch** out);
 /* Replace one grid's per-period planes (next model of an inversion). */
 int sat_batch_update_media(sat_batch* b, int grid, const double* svel, const double* m11,
@@ -210,11 +216,37 @@ int sat_batch_destroy(sat_batch* b);
  * ker_vp, ker_rho (and sen_rho in mode 2, vs3d / vp3d outside mode 0) may be
  * NULL where unused.  Host pointers, synchronous; kernel_ms (may be NULL):
  * the device kernel's time.  nper <= 96.  The anisotropic (Gc, Gs) and density
- * terms are not built (off in every configuration). */
+ * kernels: sat_combine_kernels_ex. */
 int sat_combine_kernels(int nx, int ny, int nz, int nper, int mode, const double* adj_s,
                         const double* sen_vs, const double* sen_vp, const double* sen_rho,
                         const double* vs3d, const double* vp3d, double* ker_vs, double* ker_vp,
                         double* ker_rho, double* kernel_ms);
+
+/* preproc::combine_kernels with all of its branches (src/preproc.cpp:196-342):
+ * the isotropic kernels of sat_combine_kernels, and
+ *  - is_anisotropy (r1 != NULL): the vs/vp/rho terms of modes 1 and 2 scaled
+ *    by 1/sqrt(1 + r1^2 + r2^2) (:253-256) and the Gc, Gs kernels (:307-338)
+ *    from adj_s, adj_xi, adj_eta (sat_batch_aniso_kernels), the phase velocity
+ *    svel and the sensitivities sen_gc, sen_gs;
+ *  - is_kden (adj_den != NULL): the density kernel ker_den, the vs term's
+ *    expression with adj_den in place of adj_s (:264-267, :294-300).
+ * Layouts: adj_s, adj_xi, adj_eta, adj_den, svel [nper][ny][nx]; r1, r2
+ * [nx][ny][nper] (SurfGrid r1_loc, r2_loc from prepare_aniso_media,
+ * src/surf_grid.cpp:362-392); sen_* [nx][ny][nz][nper]; vs3d, vp3d and every
+ * ker_* [nx][ny][nz].  Every pointer may be host or device memory.  FP64, the
+ * reference's operations in its order.  nper <= 96. */
+typedef struct sat_combine_desc {
+  int nx, ny, nz, nper, mode;  /* mode as sat_combine_kernels */
+  const double* adj_s;
+  const double *sen_vs, *sen_vp, *sen_rho;
+  const double *vs3d, *vp3d;   /* mode 0 */
+  const double *r1, *r2;       /* NULL: isotropic */
+  const double *adj_xi, *adj_eta, *svel;
+  const double *sen_gc, *sen_gs;
+  const double* adj_den;       /* NULL: no density kernel */
+  double *ker_vs, *ker_vp, *ker_rho, *ker_gc, *ker_gs, *ker_den;
+} sat_combine_desc;
+int sat_combine_kernels_ex(const sat_combine_desc* d, double* kernel_ms);
 
 /* sat_combine_kernels with the per-period kernels of a batch's last run, read
  * in place on the device (no copy through the host): grid `grid` of the
@@ -239,6 +271,26 @@ int sat_batch_combine_kernels(sat_batch* b, int grid, int nz, int mode, const do
  * fields are not all resident at the end of a run). */
 int sat_batch_aniso_kernels(sat_batch* b, int grid, const double* a, const double* bm, const double* c,
                             double* adj_xi, double* adj_eta);
+
+/* The same terms formed inside every run, chunked runs included (each chunk
+ * adds its events before its workspace is reused): register grid `grid`'s
+ * metric a, bm, c ([nperiod][ny][nx], host or device) once; from the next
+ * run on, sat_batch_aniso_kernels(b, grid, NULL, NULL, NULL, adj_xi, adj_eta)
+ * returns what the last run formed.  All three NULL unregisters.  The
+ * counterpart of is_anisotropy in accumulate_kernels (src/preproc.cpp:37-65). */
+int sat_batch_set_aniso_metric(sat_batch* b, int grid, const double* a, const double* bm, const double* c);
+
+/* The density kernel of forward_for_event (is_kden, src/preproc.cpp:127-136,
+ * accumulate_kernels :32-34): with on != 0 every run also solves, for every
+ * pair, the adjoint of the receiver sources -weight on the pair's forward
+ * field, and accumulates kden_s[iper] += field * svel^-3 in FP64 in batch
+ * order.  The field is not masked: the reference masks adj_field a second
+ * time (:136), not kden_field.  The run's adjoint work doubles.
+ * sat_batch_kden_results copies the last run's kden_s (the layout of adj_s in
+ * sat_batch_results; host or device memory); SAT_EINVAL when that run did not
+ * form it. */
+int sat_batch_set_kden(sat_batch* b, int on);
+int sat_batch_kden_results(sat_batch* b, double* kden_s);
 
 /* The anisotropic metric of a topographic surface for every period:
  * SurfGrid::build_media_matrix_with_topo (src/surf_grid.cpp:109-145, :184-190)

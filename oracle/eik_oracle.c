@@ -545,6 +545,28 @@ int sat_oracle_forward_for_event(int nx, int ny, const double* xgrids, const dou
                                  const double* tt_obs, const double* weight,
                                  int is_calc_adj, double* syn, double* adj_s, double* chi_out,
                                  int* iters_fwd, int* iters_adj, int* flags) {
+    return sat_oracle_forward_for_event_kden(nx, ny, xgrids, ygrids, nperiod, svel, m11, m12, m22, nevent,
+                                             ev_iper, ev_lon, ev_lat, ev_rec_off, ev_nrec, rec_lon, rec_lat,
+                                             tt_obs, weight, is_calc_adj, syn, adj_s, chi_out, iters_fwd,
+                                             iters_adj, flags, NULL);
+}
+
+/*
+ * As sat_oracle_forward_for_event, and with kden_s != NULL the density kernel
+ * of is_kden (src/preproc.cpp:127-136): per event a second adjoint solve on the
+ * same T from the receiver sources -weight (:128-130), NOT masked (the
+ * reference masks adj_field again at :136, not kden_field), accumulated as
+ * kden_s[iper] += kden_field .* svel^-3 (:32-34).
+ */
+int sat_oracle_forward_for_event_kden(int nx, int ny, const double* xgrids, const double* ygrids,
+                                      int nperiod, const double* svel, const double* m11,
+                                      const double* m12, const double* m22,
+                                      int nevent, const int* ev_iper, const double* ev_lon,
+                                      const double* ev_lat, const int* ev_rec_off, const int* ev_nrec,
+                                      const double* rec_lon, const double* rec_lat,
+                                      const double* tt_obs, const double* weight,
+                                      int is_calc_adj, double* syn, double* adj_s, double* chi_out,
+                                      int* iters_fwd, int* iters_adj, int* flags, double* kden_s) {
     (void)nperiod;
     const size_t N = (size_t)nx * (size_t)ny;
     double* fun = (double*)malloc(sizeof(double) * N);
@@ -589,6 +611,14 @@ int sat_oracle_forward_for_event(int nx, int ny, const double* xgrids, const dou
             sat_oracle_mask(nx, ny, xgrids, ygrids, Ta, ev_lon[e], ev_lat[e]); /* :125 */
             double* K = adj_s + (size_t)iper * N;
             for (size_t k = 0; k < N; k++) K[k] += Ta[k] * f_pow_m3(sv[k]); /* :29-31 */
+            if (kden_s) {
+                for (int r = 0; r < nr; r++) adj[r] = -weight[off + r] * 1.0; /* :128-130 */
+                rc = sat_oracle_adjoint(nx, ny, xgrids, ygrids, a11, a22, msphc, T, nr, rec_lon + off,
+                                        rec_lat + off, adj, 0, Ta, &ita);
+                if (rc) break;
+                double* D = kden_s + (size_t)iper * N;
+                for (size_t k = 0; k < N; k++) D[k] += Ta[k] * f_pow_m3(sv[k]); /* :32-34 */
+            }
             double s = 0.0;
             for (int r = 0; r < nr; r++) {
                 const double d = syn[off + r] - tt_obs[off + r];
@@ -651,6 +681,93 @@ int sat_oracle_combine_kernels(int nx, int ny, int nz, int nper, int mode, const
     }
     if (mode == 2)  /* :276-278, after every period */
       for (size_t i = 0; i < n3; i++) ker_rho[i] = ker_vs[i] * 0.33;
+  }
+  return SAT_ORACLE_OK;
+}
+
+/* ------------------------------------------------------------------------
+ * preproc::combine_kernels with the anisotropic and density branches
+ * (src/preproc.cpp:196-342): as sat_oracle_combine_kernels, plus
+ *  - is_anisotropy (r1 != NULL): scale = 1/sqrt(1 + r1^2 + r2^2) on the vs, vp,
+ *    rho (and density) terms of the use_alpha_beta_rho branch (:253-256), and
+ *    the Gc, Gs kernels (:307-338) from adj_tt, adj_xi, adj_eta, the phase
+ *    velocity svel and sen_gc / sen_gs;
+ *  - is_kden (adj_den != NULL): the density kernel, the vs term's expression
+ *    with adj_den for adj_tt (:264-267, :294-300).
+ * r1, r2: [nx][ny][nper] (SurfGrid r1_loc, r2_loc: RowMajor Tensor3r, from
+ * prepare_aniso_media, src/surf_grid.cpp:362-392); adj_xi, adj_eta, adj_den,
+ * svel: [nper][ny][nx]; sen_gc, sen_gs: [nx][ny][nz][nper]; kernels [nx][ny][nz].
+ * The statements run per period in the reference's order: the isotropic loop
+ * (:246-301), the rho scaling (:276-278), then the Gc/Gs loop (:309-338).
+ * ------------------------------------------------------------------------ */
+int sat_oracle_combine_kernels_ex(int nx, int ny, int nz, int nper, int mode, const double* adj_tt,
+                                  const double* sen_vs, const double* sen_vp, const double* sen_rho,
+                                  const double* vs3d, const double* vp3d, const double* r1p, const double* r2p,
+                                  const double* adj_xi, const double* adj_eta, const double* svel,
+                                  const double* sen_gc, const double* sen_gs, const double* adj_den,
+                                  double* ker_vs, double* ker_vp, double* ker_rho, double* ker_gc,
+                                  double* ker_gs, double* ker_den) {
+  if (nx < 1 || ny < 1 || nz < 1 || nper < 1 || mode < 0 || mode > 2) return SAT_ORACLE_EINVAL;
+  const int aniso = r1p != NULL, kden = adj_den != NULL;
+  const size_t n2 = (size_t)nx * ny, n3 = n2 * nz;
+  for (size_t i = 0; i < n3; i++) {
+    ker_vs[i] = 0.0;
+    if (mode > 0) { ker_vp[i] = 0.0; ker_rho[i] = 0.0; }
+    if (aniso) { ker_gc[i] = 0.0; ker_gs[i] = 0.0; }
+    if (kden) ker_den[i] = 0.0;
+  }
+  for (int iper = 0; iper < nper; iper++) {
+    for (int ix = 0; ix < nx; ix++) {
+      for (int iy = 0; iy < ny; iy++) {
+        const size_t o2 = (size_t)iper * n2 + ix + (size_t)iy * nx;
+        const double att = adj_tt[o2];
+        const double aden = kden ? adj_den[o2] : 0.0;
+        const size_t o_r = ((size_t)ix * ny + iy) * nper + iper;
+        const double r1 = aniso ? r1p[o_r] : 0.0;
+        const double r2 = aniso ? r2p[o_r] : 0.0;
+        const double scale = 1.0 / sqrt(1.0 + r1 * r1 + r2 * r2);
+        for (int k = 0; k < nz; k++) {
+          const size_t o3 = ((size_t)ix * ny + iy) * nz + k;
+          const size_t o4 = o3 * nper + iper;
+          if (mode == 0) {
+            const double dab = o_dalpha_dbeta(vs3d[o3]);
+            const double dra = o_drho_dalpha(vp3d[o3]);
+            ker_vs[o3] -= att * (sen_vs[o4] + sen_vp[o4] * dab + sen_rho[o4] * dra * dab);
+            if (kden) ker_den[o3] -= aden * (sen_vs[o4] + sen_vp[o4] * dab + sen_rho[o4] * dra * dab);
+          } else {
+            ker_vs[o3] -= att * scale * sen_vs[o4];
+            ker_vp[o3] -= att * scale * sen_vp[o4];
+            if (kden) ker_den[o3] -= aden * scale * sen_vs[o4];
+            if (mode == 1) ker_rho[o3] -= att * scale * sen_rho[o4];
+          }
+        }
+      }
+    }
+    if (mode == 2)
+      for (size_t i = 0; i < n3; i++) ker_rho[i] = ker_vs[i] * 0.33;
+    if (aniso) {
+      for (int ix = 0; ix < nx; ix++) {
+        for (int iy = 0; iy < ny; iy++) {
+          const size_t o2 = (size_t)iper * n2 + ix + (size_t)iy * nx;
+          const size_t o_r = ((size_t)ix * ny + iy) * nper + iper;
+          const double r1 = r1p[o_r], r2 = r2p[o_r];
+          const double D = 1.0 + r1 * r1 + r2 * r2;
+          const double sqrtD = sqrt(D);
+          const double D15 = D * sqrtD;
+          const double D2 = D * D;
+          const double sv = svel[o2];
+          const double att = adj_tt[o2], axi = adj_xi[o2], aeta = adj_eta[o2];
+          for (int k = 0; k < nz; k++) {
+            const size_t o3 = ((size_t)ix * ny + iy) * nz + k;
+            const size_t o4 = o3 * nper + iper;
+            ker_gc[o3] += (-att * r1 / D15 + axi * (1.0 - r1 * r1 + r2 * r2) / (sv * D2) +
+                           aeta * (-2.0 * r1 * r2) / (sv * D2)) * sen_gc[o4];
+            ker_gs[o3] += (-att * r2 / D15 + axi * (-2.0 * r1 * r2) / (sv * D2) +
+                           aeta * (1.0 + r1 * r1 - r2 * r2) / (sv * D2)) * sen_gs[o4];
+          }
+        }
+      }
+    }
   }
   return SAT_ORACLE_OK;
 }

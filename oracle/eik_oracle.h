@@ -40,10 +40,28 @@ int sat_oracle_forward_for_event(int nx, int ny, const double* xgrids, const dou
                                  int is_calc_adj, double* syn, double* adj_s, double* chi_out,
                                  int* iters_fwd, int* iters_adj, int* flags);
 
+int sat_oracle_forward_for_event_kden(int nx, int ny, const double* xgrids, const double* ygrids,
+                                      int nperiod, const double* svel, const double* m11,
+                                      const double* m12, const double* m22,
+                                      int nevent, const int* ev_iper, const double* ev_lon,
+                                      const double* ev_lat, const int* ev_rec_off, const int* ev_nrec,
+                                      const double* rec_lon, const double* rec_lat,
+                                      const double* tt_obs, const double* weight,
+                                      int is_calc_adj, double* syn, double* adj_s, double* chi_out,
+                                      int* iters_fwd, int* iters_adj, int* flags, double* kden_s);
+
 int sat_oracle_combine_kernels(int nx, int ny, int nz, int nper, int mode, const double* adj_tt,
                                const double* sen_vs, const double* sen_vp, const double* sen_rho,
                                const double* vs3d, const double* vp3d, double* ker_vs, double* ker_vp,
                                double* ker_rho);
+
+int sat_oracle_combine_kernels_ex(int nx, int ny, int nz, int nper, int mode, const double* adj_tt,
+                                  const double* sen_vs, const double* sen_vp, const double* sen_rho,
+                                  const double* vs3d, const double* vp3d, const double* r1, const double* r2,
+                                  const double* adj_xi, const double* adj_eta, const double* svel,
+                                  const double* sen_gc, const double* sen_gs, const double* adj_den,
+                                  double* ker_vs, double* ker_vp, double* ker_rho, double* ker_gc,
+                                  double* ker_gs, double* ker_den);
 
 #ifdef __cplusplus
 }

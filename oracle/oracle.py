@@ -71,11 +71,15 @@ def _bind(L):
     L.sat_oracle_interp2d.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp]
     L.sat_oracle_combine_kernels.restype = C.c_int
     L.sat_oracle_combine_kernels.argtypes = [C.c_int] * 5 + [_dp] * 9
+    L.sat_oracle_combine_kernels_ex.restype = C.c_int
+    L.sat_oracle_combine_kernels_ex.argtypes = [C.c_int] * 5 + [_dp] * 20
     L.sat_oracle_forward_for_event.restype = C.c_int
     L.sat_oracle_forward_for_event.argtypes = [
         C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp, _dp,
         C.c_int, _ip, _dp, _dp, _ip, _ip, _dp, _dp, _dp, _dp,
         C.c_int, _dp, _dp, _dp, _ip, _ip, _ip]
+    L.sat_oracle_forward_for_event_kden.restype = C.c_int
+    L.sat_oracle_forward_for_event_kden.argtypes = L.sat_oracle_forward_for_event.argtypes + [_dp]
     return L
 
 
@@ -166,13 +170,15 @@ def interp2d(xgrid, ygrid, z, xq, yq):
     return out
 
 
-def forward_for_event(batch, is_calc_adj=True, lib=None):
+def forward_for_event(batch, is_calc_adj=True, lib=None, kden=False):
     """preproc::forward_for_event (src/preproc.cpp:71) over one grid's events.
 
     ``batch`` is a surfattpp_amd-style dict (see tests/inputs) with keys
     lon, lat, svel/m11/m12/m22 (nperiod, ny, nx), ev_iper, ev_lon, ev_lat,
     ev_rec_off, ev_nrec, rec_lon, rec_lat, tt_obs, weight.
-    Returns dict(syn, adj_s (nperiod, ny, nx), chi, iters_fwd, iters_adj, flags).
+    Returns dict(syn, adj_s (nperiod, ny, nx), chi, iters_fwd, iters_adj, flags),
+    with kden also kden_s (nperiod, ny, nx): is_kden's density kernel
+    (src/preproc.cpp:127-136).
     """
     lon, lat = _v64(batch["lon"]), _v64(batch["lat"])
     nx, ny = lon.size, lat.size
@@ -195,14 +201,18 @@ def forward_for_event(batch, is_calc_adj=True, lib=None):
     ita = np.zeros(ne, dtype=np.int32)
     fl = np.zeros(ne, dtype=np.int32)
     L = lib if lib is not None else globals()["lib"]()
-    rc = L.sat_oracle_forward_for_event(
+    kden_s = np.zeros((nper, ny, nx)) if kden else None
+    rc = L.sat_oracle_forward_for_event_kden(
         nx, ny, _p(lon), _p(lat), nper, _p(planes["svel"]), _p(planes["m11"]), _p(planes["m12"]),
         _p(planes["m22"]), ne, _ip_(ev_iper), _p(ev_lon), _p(ev_lat), _ip_(off), _ip_(nrec),
         _p(rl), _p(rb), _p(tt), _p(w), int(bool(is_calc_adj)), _p(syn), _p(adj_s), _p(chi),
-        _ip_(itf), _ip_(ita), _ip_(fl))
+        _ip_(itf), _ip_(ita), _ip_(fl), _p(kden_s) if kden else None)
     if rc:
         raise RuntimeError(f"oracle forward_for_event failed rc={rc}")
-    return dict(syn=syn[: rl.size], adj_s=adj_s, chi=float(chi[0]), iters_fwd=itf, iters_adj=ita, flags=fl)
+    out = dict(syn=syn[: rl.size], adj_s=adj_s, chi=float(chi[0]), iters_fwd=itf, iters_adj=ita, flags=fl)
+    if kden:
+        out["kden_s"] = kden_s
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -291,3 +301,36 @@ def combine_kernels(adj_s, sen_vs, sen_vp, sen_rho, vs3d, vp3d, mode=0):
     if rc:
         raise RuntimeError(f"oracle combine_kernels failed rc={rc}")
     return tuple(out)
+
+
+def combine_kernels_ex(adj_s, sen_vs, sen_vp, sen_rho=None, vs3d=None, vp3d=None, mode=0, r1=None, r2=None,
+                       adj_xi=None, adj_eta=None, svel=None, sen_gc=None, sen_gs=None, adj_den=None):
+    """preproc::combine_kernels with the anisotropic (Gc, Gs) and density
+    branches (src/preproc.cpp:196-342).  r1, r2: (nx, ny, nper) or None
+    (isotropic); adj_xi, adj_eta, svel, adj_den: (nper, ny, nx); sen_*:
+    (nx, ny, nz, nper).  Returns dict of the kernels (nx, ny, nz) that the
+    options form: vs, vp, rho, gc, gs, den."""
+    c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    adj = c(adj_s)
+    nper, ny, nx = adj.shape
+    s_vs, s_vp, s_rho = c(sen_vs), c(sen_vp), c(sen_rho)
+    nz = s_vs.shape[2]
+    aniso, kden = r1 is not None, adj_den is not None
+    out = {k: np.zeros((nx, ny, nz)) for k in ("vs", "vp", "rho", "gc", "gs", "den")}
+    args = [adj, s_vs, s_vp, s_rho, c(vs3d), c(vp3d), c(r1), c(r2), c(adj_xi), c(adj_eta), c(svel), c(sen_gc),
+            c(sen_gs), c(adj_den)]
+    keep = [a for a in args if a is not None]
+    ptr = lambda a: None if a is None else _p(a)  # noqa: E731
+    rc = lib().sat_oracle_combine_kernels_ex(nx, ny, nz, nper, int(mode), *(ptr(a) for a in args),
+                                             *(_p(out[k]) for k in ("vs", "vp", "rho", "gc", "gs", "den")))
+    del keep
+    if rc:
+        raise RuntimeError(f"oracle combine_kernels_ex failed rc={rc}")
+    res = {"vs": out["vs"]}
+    if mode:
+        res.update(vp=out["vp"], rho=out["rho"])
+    if aniso:
+        res.update(gc=out["gc"], gs=out["gs"])
+    if kden:
+        res["den"] = out["den"]
+    return res

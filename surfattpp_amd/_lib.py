@@ -32,7 +32,8 @@ EXPORTS = (
     "sat_batch_create", "sat_batch_update_media", "sat_batch_update_obs", "sat_batch_run",
     "sat_batch_sync", "sat_batch_results", "sat_batch_last_ms", "sat_batch_destroy",
     "sat_batch_set_f32_band", "sat_batch_conv", "sat_batch_phase_ms", "sat_batch_waves",
-    "sat_combine_kernels", "sat_batch_combine_kernels", "sat_batch_aniso_kernels",
+    "sat_combine_kernels", "sat_combine_kernels_ex", "sat_batch_combine_kernels", "sat_batch_aniso_kernels", "sat_batch_set_aniso_metric",
+    "sat_batch_set_kden", "sat_batch_kden_results",
     "sat_topo_metric", "sat_surfdisp96", "sat_fwdsurf", "sat_device_info",
 )
 
@@ -58,6 +59,13 @@ class sat_grid_desc(C.Structure):
 class sat_event_desc(C.Structure):
     _fields_ = [("grid", C.c_int), ("iper", C.c_int), ("src_lon_deg", C.c_double),
                 ("src_lat_deg", C.c_double), ("rec_off", C.c_int), ("nrec", C.c_int)]
+
+
+class sat_combine_desc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nper", C.c_int), ("mode", C.c_int)] + \
+        [(n, C.c_void_p) for n in ("adj_s", "sen_vs", "sen_vp", "sen_rho", "vs3d", "vp3d", "r1", "r2", "adj_xi",
+                                   "adj_eta", "svel", "sen_gc", "sen_gs", "adj_den", "ker_vs", "ker_vp", "ker_rho",
+                                   "ker_gc", "ker_gs", "ker_den")]
 
 
 class sat_batch_desc(C.Structure):
@@ -101,8 +109,12 @@ def load(path: str = LIB_PATH):
     L.sat_batch_phase_ms.argtypes = [vp, dp, C.c_int]
     L.sat_batch_waves.argtypes = [vp, vp]
     L.sat_combine_kernels.argtypes = [C.c_int] * 5 + [vp] * 9 + [dp]
+    L.sat_combine_kernels_ex.argtypes = [C.POINTER(sat_combine_desc), dp]
     L.sat_batch_combine_kernels.argtypes = [vp, C.c_int, C.c_int, C.c_int] + [vp] * 8 + [dp]
     L.sat_batch_aniso_kernels.argtypes = [vp, C.c_int] + [vp] * 5
+    L.sat_batch_set_aniso_metric.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.sat_batch_set_kden.argtypes = [vp, C.c_int]
+    L.sat_batch_kden_results.argtypes = [vp, vp]
     L.sat_topo_metric.argtypes = [C.c_int] * 3 + [vp] * 8 + [dp]
     L.sat_surfdisp96.argtypes = [C.c_int, C.c_int] + [vp] * 4 + [C.c_int] * 5 + [vp] * 3 + [dp]
     L.sat_fwdsurf.argtypes = [C.c_int] * 3 + [vp] * 4 + [C.c_int, vp] + [C.c_int] * 4 + [vp, vp, dp]

@@ -7,19 +7,25 @@ SAT_INSTANTIATE_PHASE(SAT_PREC_F64)
 namespace sat {
 
 // src/preproc.cpp:105-115 per receiver, and the receiver's delta cell and
-// weight (src/eikonal_solver.cpp:357-369).
+// weight (src/eikonal_solver.cpp:357-369); in the density-kernel pass only the
+// weight of the source -w.
 __global__ void recv_kernel(PhaseDev pd, int nrec) {
   for (int k = pd.rec_lo + (int)(blockIdx.x * blockDim.x + threadIdx.x); k < pd.rec_lo + nrec;
        k += gridDim.x * blockDim.x) {
     const int e = pd.rec_ev[k];
-    if (e < pd.ev_base || e >= pd.ev_base + pd.nevent || pd.skip[e] != pd.want_skip) continue;
+    if (e < pd.ev_base || e >= pd.ev_base + pd.nevent || !pass_event(pd, e)) continue;
     const EventX& ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];
-    const double s = interp2d_point(g.lon, g.nx, g.lat, g.ny, pd.T + ev.pl_off, pd.rec_lon[k], pd.rec_lat[k]);
-    pd.syn[k] = s;
-    const double adj = pd.weight[k] * (s - pd.tt_obs[k]);
-    pd.adjsrc[k] = adj;
-    if (s != s) atomicOr(pd.flags + e, FLAG_REC_OUTSIDE);
+    double adj;
+    if (pd.kden_pass) {  // the density kernel's source: -w (src/preproc.cpp:129-130)
+      adj = -pd.weight[k];
+    } else {
+      const double s = interp2d_point(g.lon, g.nx, g.lat, g.ny, pd.T + ev.pl_off, pd.rec_lon[k], pd.rec_lat[k]);
+      pd.syn[k] = s;
+      adj = pd.weight[k] * (s - pd.tt_obs[k]);
+      pd.adjsrc[k] = adj;
+      if (s != s) atomicOr(pd.flags + e, FLAG_REC_OUTSIDE);
+    }
     const double xrec = pd.rec_lon[k] * kDEG2RAD;
     const double yrec = pd.rec_lat[k] * kDEG2RAD;
     int idx0 = (int)floor((xrec - g.xx[0]) / g.dx);

@@ -105,6 +105,16 @@ __global__ void media_kernel(GridDev g, int mode, int prec, const double* __rest
   }
 }
 
+// Isotropic metric test on the device: *flag |= 1 where a node has
+// spha != 1, sphb != 1 or sphc != 0 (the planes may have come from host or
+// device memory).
+__global__ void iso_check_kernel(const double* ma, const double* mb, const double* mc, size_t n, int* flag) {
+  int bad = 0;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    bad |= !(ma[k] == 1.0 && mb[k] == 1.0 && mc[k] == 0.0);
+  if (__any(bad) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag, 1);
+}
+
 // Per row: correctly rounded sin/cos of the latitude; isotropic grids also the
 // FP32 row metric {a, b, 1/sqrt(a), 1/sqrt(b)} (src/eikonal_solver.cpp:55-65 with
 // spha = sphb = 1, sphc = 0).
@@ -185,6 +195,8 @@ struct AnisoDev {
   const double* abc;            // a, b, c: [3][nper][ny][nx]
   double* xi;                   // [nper][ny][nx]
   double* eta;
+  int ev_lo, ev_hi;             // the events of this pass (a chunk of the batch order)
+  int accumulate;               // add to xi, eta (a later chunk) instead of overwriting them
 };
 
 template <typename V>
@@ -209,16 +221,21 @@ __global__ void aniso_kernel(AnisoDev ad) {
     const size_t oyp = n + (iy < ny - 1 ? (size_t)nx : 0), oym = n - (iy > 0 ? (size_t)nx : 0);
     const double dxd = (ix == 0) ? dxk(1) : (ix == nx - 1) ? dxk(nx - 1) : 2.0 * dxk(ix);
     const double dyd = (iy == 0) ? dyk(1) : (iy == ny - 1) ? dyk(ny - 1) : 2.0 * dyk(iy);
-    double xi = 0.0, eta = 0.0;
+    // a later chunk continues the previous chunks' sums: the batch order is kept
+    double xi = ad.accumulate ? ad.xi[id] : 0.0, eta = ad.accumulate ? ad.eta[id] : 0.0;
     const int pl = ad.pl0 + p;
     const int qe = ad.ev_off[pl + 1];
     // kEv events' loads in flight at once; the sums still run in batch order
     constexpr int kEv = 4;
-    for (int q = ad.ev_off[pl]; q < qe; q += kEv) {
+    int q0 = ad.ev_off[pl];
+    int q1 = qe;
+    while (q0 < q1 && ad.ev_ids[q0] < ad.ev_lo) q0++;  // the plane's events in batch order: the chunk's run
+    while (q1 > q0 && ad.ev_ids[q1 - 1] >= ad.ev_hi) q1--;
+    for (int q = q0; q < q1; q += kEv) {
       double tx[kEv], ty[kEv], adj[kEv];
 #pragma unroll
       for (int k = 0; k < kEv; k++) {
-        if (q + k < qe) {
+        if (q + k < q1) {
           const EventDev& ev = ad.events[ad.ev_ids[q + k]];
           const double* f = ad.T + ev.ta_off;
           tx[k] = (f[oxp] - f[oxm]) / dxd;
@@ -228,7 +245,7 @@ __global__ void aniso_kernel(AnisoDev ad) {
       }
 #pragma unroll
       for (int k = 0; k < kEv; k++) {
-        if (q + k < qe) {
+        if (q + k < q1) {
           const double tx2 = tx[k] * tx[k], ty2 = ty[k] * ty[k], txty = tx[k] * ty[k];
           xi = xi + adj[k] * (tx2 * (-a2 + c2) + 2.0 * txty * (ac - bc) + ty2 * (-c2 + b2));
           eta = eta + adj[k] * (2.0 * ac * tx2 - 2.0 * (ab + c2) * txty + 2.0 * bc * ty2);
@@ -257,6 +274,10 @@ __global__ void reset_kernel(int* counters, size_t n_counters, unsigned char* sk
       flags[e] = base_flags[e];
     }
   }
+}
+
+__global__ void zero_ints_kernel(int* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 __global__ void chi_kernel(const double* chi_ev, int n, double* chi) {
@@ -330,6 +351,7 @@ struct GridHost {
   double2* d_trig_yy = nullptr;
   float4* d_rowtab = nullptr;
   double* d_rowha = nullptr;
+  int* d_flag = nullptr;      // iso_check_kernel's result
   double* d_wf64 = nullptr;   // FP32 batches: FP64 slowness, wavefront-major, per orientation
   char* d_wmed = nullptr;     // batched sweeps: wavefront-major media, per orientation
   char* d_wmedmw = nullptr;   // FP32 batches: FP64 wavefront-major media of the re-solve geometry
@@ -339,7 +361,7 @@ struct GridHost {
 
   void release() {
     void* ps[] = {d_lon, d_lat, d_xx, d_yy, d_model, d_fp, d_aA, d_b, d_c, d_v3, d_seg, d_seg64, d_trig_yy, d_rowtab,
-                  d_rowha, d_wf64, d_wmed, d_wmedmw};
+                  d_rowha, d_wf64, d_wmed, d_wmedmw, d_flag};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = GridHost();
@@ -391,6 +413,7 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
   SAT_CHECK(dalloc(&G.d_trig_yy, ny));
   SAT_CHECK(dalloc(&G.d_rowtab, ny));
   SAT_CHECK(dalloc(&G.d_rowha, ny));
+  SAT_CHECK(dalloc(&G.d_flag, 1));
   g.trig_yy = G.d_trig_yy;
   g.rowtab = G.d_rowtab;
   g.rowha = G.d_rowha;
@@ -456,19 +479,25 @@ static int grid_media(GridHost& G, int mode, const double* p0, const double* p1,
   const double* src[4] = {p0, p1, p2, p3};
   for (int i = 0; i < 4; i++)
     if (!src[i]) return SAT_EINVAL;
+  // the planes come from host or device memory (hipMemcpyDefault tells them apart)
+  for (int i = 0; i < 4; i++) {
+    SAT_CHECK(hipMemcpyAsync(G.d_model + i * n, src[i], sizeof(double) * n, hipMemcpyDefault, stream));
+  }
   {  // isotropic metric: spha = sphb = 1 and sphc = 0 everywhere
-    const double* ma = (mode == 0) ? p1 : p0;   // m11 | spha
-    const double* mb = (mode == 0) ? p3 : p1;   // m22 | sphb
-    const double* mc = p2;                      // m12 | sphc
-    bool iso = true;
-    for (size_t k = 0; k < n && iso; k++) iso = ma[k] == 1.0 && mb[k] == 1.0 && mc[k] == 0.0;
-    G.g.iso = iso ? 1 : 0;
+    const double* ma = G.d_model + (size_t)((mode == 0) ? 1 : 0) * n;  // m11 | spha
+    const double* mb = G.d_model + (size_t)((mode == 0) ? 3 : 1) * n;  // m22 | sphb
+    const double* mc = G.d_model + 2 * n;                             // m12 | sphc
+    SAT_CHECK(hipMemsetAsync(G.d_flag, 0, sizeof(int), stream));
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(iso_check_kernel, dim3(blocks), dim3(256), 0, stream, ma, mb, mc, n, G.d_flag);
+    SAT_CHECK(hipGetLastError());
+    int flag = 0;
+    SAT_CHECK(hipMemcpyAsync(&flag, G.d_flag, sizeof(int), hipMemcpyDeviceToHost, stream));
+    SAT_CHECK(hipStreamSynchronize(stream));
+    G.g.iso = flag ? 0 : 1;
     G.g64.iso = G.g.iso;
     G.gmw.iso = G.g.iso;
     G.gseg.iso = G.g.iso;
-  }
-  for (int i = 0; i < 4; i++) {
-    SAT_CHECK(hipMemcpyAsync(G.d_model + i * n, src[i], sizeof(double) * n, hipMemcpyHostToDevice, stream));
   }
   const size_t total = (size_t)G.nper * G.ny * G.g.P;
   const int threads = 256;
@@ -848,9 +877,19 @@ struct sat_batch {
   hipGraphExec_t graph_exec = nullptr;
   bool last_graph = false;
   bool ran = false;                // a run has been queued: every event's T and Ta are on the device
+  // accumulate_kernels' optional terms, formed inside every run (chunks included):
+  // the anisotropic terms of the grids with a registered metric and the density kernel
+  std::vector<double*> d_aniso;    // per grid: a, b, c, then xi, eta ([nper][ny][nx] each), or null
+  std::vector<char> aniso_ran;     // per grid: the last run formed its terms
+  bool kden = false, kden_ran = false;
+  double* d_Kden = nullptr;        // [K_total] density accumulators (kden_s_local)
 
   void release() {
     for (auto& G : grids) G.release();
+    for (double* p : d_aniso)
+      if (p) (void)hipFree(p);
+    d_aniso.clear();
+    if (d_Kden) (void)hipFree(d_Kden);
     void* ps[] = {d_grids, d_grids_seg, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
                   d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_ov, d_evx64, d_work64, d_slots64, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
                   d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3};
@@ -1121,6 +1160,8 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
     B_CHECK(hipEventCreateWithFlags(&b->join2_ev[i], hipEventDisableTiming));
   }
   b->grids.resize(d->ngrid);
+  b->d_aniso.assign(d->ngrid, nullptr);
+  b->aniso_ran.assign(d->ngrid, 0);
   std::vector<int> waves;
   {
     hipDeviceProp_t prop;
@@ -1246,10 +1287,10 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(dalloc(&b->d_rec, 6 * std::max<size_t>(nr, 1)));
   B_CHECK(hipMemset(b->d_rec, 0, sizeof(double) * 6 * std::max<size_t>(nr, 1)));  // receivers of no event: syn 0
   if (nr) {
-    B_CHECK(hipMemcpy(b->d_rec, d->rec_lon_deg, sizeof(double) * nr, hipMemcpyHostToDevice));
-    B_CHECK(hipMemcpy(b->d_rec + nr, d->rec_lat_deg, sizeof(double) * nr, hipMemcpyHostToDevice));
-    B_CHECK(hipMemcpy(b->d_rec + 2 * nr, d->tt_obs, sizeof(double) * nr, hipMemcpyHostToDevice));
-    B_CHECK(hipMemcpy(b->d_rec + 3 * nr, d->weight, sizeof(double) * nr, hipMemcpyHostToDevice));
+    B_CHECK(hipMemcpy(b->d_rec, d->rec_lon_deg, sizeof(double) * nr, hipMemcpyDefault));
+    B_CHECK(hipMemcpy(b->d_rec + nr, d->rec_lat_deg, sizeof(double) * nr, hipMemcpyDefault));
+    B_CHECK(hipMemcpy(b->d_rec + 2 * nr, d->tt_obs, sizeof(double) * nr, hipMemcpyDefault));
+    B_CHECK(hipMemcpy(b->d_rec + 3 * nr, d->weight, sizeof(double) * nr, hipMemcpyDefault));
   }
   B_CHECK(dalloc(&b->d_rec_ev, std::max<size_t>(nr, 1)));
   B_CHECK(hipMemcpy(b->d_rec_ev, rec_ev.data(), sizeof(int) * rec_ev.size(), hipMemcpyHostToDevice));
@@ -1392,8 +1433,8 @@ int sat_batch_update_obs(sat_batch* b, const double* tt_obs, const double* weigh
   if (!b || (b->nrec > 0 && (!tt_obs || !weight))) return SAT_EINVAL;
   const size_t nr = (size_t)b->nrec;
   if (nr) {
-    SAT_CHECK(hipMemcpyAsync(b->d_rec + 2 * nr, tt_obs, sizeof(double) * nr, hipMemcpyHostToDevice, b->stream));
-    SAT_CHECK(hipMemcpyAsync(b->d_rec + 3 * nr, weight, sizeof(double) * nr, hipMemcpyHostToDevice, b->stream));
+    SAT_CHECK(hipMemcpyAsync(b->d_rec + 2 * nr, tt_obs, sizeof(double) * nr, hipMemcpyDefault, b->stream));
+    SAT_CHECK(hipMemcpyAsync(b->d_rec + 3 * nr, weight, sizeof(double) * nr, hipMemcpyDefault, b->stream));
     SAT_CHECK(hipStreamSynchronize(b->stream));
   }
   return SAT_OK;
@@ -1750,11 +1791,70 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
       if (!f32) hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks), dim3(threads), 0, b->stream, rd);
       else hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks), dim3(threads), 0, b->stream, rd);
       SAT_CHECK(hipGetLastError());
+      // The anisotropic terms of accumulate_kernels (src/preproc.cpp:37-65) of the
+      // grids with a registered metric, from the chunk's T and masked Ta.
+      for (size_t gi = 0; gi < b->grids.size(); gi++) {
+        if (!b->d_aniso[gi]) continue;
+        const GridDev& g = b->grids[gi].g;
+        int pl0 = 0;
+        for (size_t k = 0; k < gi; k++) pl0 += b->grids[k].g.nperiod;
+        const size_t tot = (size_t)g.nperiod * g.plane;
+        AnisoDev ad{g, g.nperiod, pl0, b->rd.ev_off, b->rd.ev_ids, b->d_events, b->d_T, b->d_ta, b->d_aniso[gi],
+                    b->d_aniso[gi] + 3 * tot, b->d_aniso[gi] + 4 * tot, rd.ev_lo, rd.ev_hi, c > 0};
+        const unsigned ab = (unsigned)std::min<size_t>((tot + 255) / 256, 65535);
+        if (f32) hipLaunchKernelGGL(aniso_kernel<float>, dim3(ab), dim3(256), 0, b->stream, ad);
+        else hipLaunchKernelGGL(aniso_kernel<double>, dim3(ab), dim3(256), 0, b->stream, ad);
+        SAT_CHECK(hipGetLastError());
+      }
+      // The density kernel (is_kden, src/preproc.cpp:127-136): a second adjoint
+      // solve per pair from the receiver sources -w, on the same face fluxes,
+      // not masked, accumulated as kden_s += field * svel^-3.  Every pair of
+      // the chunk, in the batch's precision (also the FP64 re-solved ones, on
+      // their FP64 T); Ta's planes are free again once K and the anisotropic
+      // terms have read them.
+      if (b->kden && cn) {
+        PhaseDev pk = pc;
+        pk.want_skip = -1;
+        pk.kden_pass = 1;
+        const int ng_c = (int)(g_hi - g_lo);
+        hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(256), 0, b->stream, adj_heads + g_lo, ng_c);
+        SAT_CHECK(hipGetLastError());
+        if (nrec_c) SAT_CHECK(launch_recv(dim3(rec_blocks), b->stream, pk, nrec_c));
+        SAT_CHECK(f32 ? launch_adj_setup<SAT_PREC_F32>(node_grid, b->stream, pk)
+                      : launch_adj_setup<SAT_PREC_F64>(node_grid, b->stream, pk));
+        SAT_CHECK(f32 ? launch_delta<SAT_PREC_F32>(dim3((unsigned)cn), b->stream, pk)
+                      : launch_delta<SAT_PREC_F64>(dim3((unsigned)cn), b->stream, pk));
+        const int rc = fan_out([&](int gi, hipStream_t st) -> int {
+          PhaseDev a = pk;
+          a.counter = adj_heads + gi;
+          a.slots = b->d_slots + (size_t)b->group_slot0[gi] * b->slot_bytes;
+          const int C = b->group_wC[gi], W = b->group_W[gi];
+          SAT_CHECK(f32 ? launch_adj_sweep<SAT_PREC_F32>(C, W, dim3(b->group_nblk[gi]), st, a, b->group_first[gi],
+                                                         b->group_count[gi])
+                        : launch_adj_sweep<SAT_PREC_F64>(C, W, dim3(b->group_nblk[gi]), st, a, b->group_first[gi],
+                                                         b->group_count[gi]));
+          return SAT_OK;
+        });
+        if (rc) return rc;
+        ReduceDev rk = rd;
+        rk.K = b->d_Kden;
+        if (!f32) hipLaunchKernelGGL(reduce_kernel<double>, dim3(blocks), dim3(threads), 0, b->stream, rk);
+        else hipLaunchKernelGGL(reduce_kernel<float>, dim3(blocks), dim3(threads), 0, b->stream, rk);
+        SAT_CHECK(hipGetLastError());
+      }
     }
   }
   hipLaunchKernelGGL(chi_kernel, dim3(1), dim3(1), 0, b->stream, b->d_chi_ev, b->calc_adj ? ne : 0, b->d_chi);
   SAT_CHECK(hipGetLastError());
   return SAT_OK;
+}
+
+// After a run: every event's fields are on the device, and the optional terms
+// that were enabled are formed.
+void mark_ran(sat_batch* b) {
+  b->ran = true;
+  for (size_t gi = 0; gi < b->d_aniso.size(); gi++) b->aniso_ran[gi] = b->d_aniso[gi] != nullptr && b->calc_adj;
+  b->kden_ran = b->kden && b->calc_adj;
 }
 
 void drop_graph(sat_batch* b) {
@@ -1779,7 +1879,7 @@ int sat_batch_run(sat_batch* b, int mode) {
     const int rc = enqueue_run(b, mode, false);
     if (rc) return rc;
     SAT_CHECK(hipEventRecord(b->mk[7], b->stream));
-    b->ran = true;
+    mark_ran(b);
     return SAT_OK;
   }
   if (!b->graph_exec) {
@@ -1802,7 +1902,7 @@ int sat_batch_run(sat_batch* b, int mode) {
   SAT_CHECK(hipEventRecord(b->mk[0], b->stream));
   SAT_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
   SAT_CHECK(hipEventRecord(b->mk[7], b->stream));
-  b->ran = true;
+  mark_ran(b);
   return SAT_OK;
 }
 
@@ -1824,10 +1924,64 @@ int sat_batch_combine_kernels(sat_batch* b, int grid, int nz, int mode, const do
                            vp3d, ker_vs, ker_vp, ker_rho, kernel_ms);
 }
 
+int sat_batch_set_aniso_metric(sat_batch* b, int grid, const double* a, const double* bm, const double* c) {
+  if (!b || !b->calc_adj || grid < 0 || grid >= (int)b->grids.size()) return SAT_EINVAL;
+  const GridDev& g = b->grids[grid].g;
+  const size_t tot = (size_t)g.nperiod * g.plane;
+  if (!a && !bm && !c) {  // unregister
+    if (b->d_aniso[grid]) {
+      SAT_CHECK(hipStreamSynchronize(b->stream));
+      SAT_CHECK(hipFree(b->d_aniso[grid]));
+      b->d_aniso[grid] = nullptr;
+      b->aniso_ran[grid] = 0;
+      drop_graph(b);
+    }
+    return SAT_OK;
+  }
+  if (!a || !bm || !c) return SAT_EINVAL;
+  SAT_CHECK(hipStreamSynchronize(b->stream));  // no run is using the planes
+  if (!b->d_aniso[grid]) {
+    SAT_CHECK(dalloc(&b->d_aniso[grid], 5 * tot));
+    drop_graph(b);  // a captured run does not form these terms
+  }
+  SAT_CHECK(hipMemcpy(b->d_aniso[grid], a, sizeof(double) * tot, hipMemcpyDefault));
+  SAT_CHECK(hipMemcpy(b->d_aniso[grid] + tot, bm, sizeof(double) * tot, hipMemcpyDefault));
+  SAT_CHECK(hipMemcpy(b->d_aniso[grid] + 2 * tot, c, sizeof(double) * tot, hipMemcpyDefault));
+  b->aniso_ran[grid] = 0;
+  return SAT_OK;
+}
+
+int sat_batch_set_kden(sat_batch* b, int on) {
+  if (!b || (on && !b->calc_adj)) return SAT_EINVAL;
+  if ((on != 0) == b->kden) return SAT_OK;
+  SAT_CHECK(hipStreamSynchronize(b->stream));
+  if (on && !b->d_Kden) SAT_CHECK(dalloc(&b->d_Kden, std::max<size_t>(b->K_total, 1)));
+  b->kden = on != 0;
+  b->kden_ran = false;
+  drop_graph(b);
+  return SAT_OK;
+}
+
+int sat_batch_kden_results(sat_batch* b, double* kden_s) {
+  if (!b || !kden_s || !b->kden_ran) return SAT_EINVAL;
+  SAT_CHECK(hipStreamSynchronize(b->stream));
+  SAT_CHECK(hipMemcpy(kden_s, b->d_Kden, sizeof(double) * b->K_total, hipMemcpyDefault));
+  return SAT_OK;
+}
+
 int sat_batch_aniso_kernels(sat_batch* b, int grid, const double* a, const double* bm, const double* c,
                             double* adj_xi, double* adj_eta) {
-  if (!b || !b->calc_adj || grid < 0 || grid >= (int)b->grids.size() || !a || !bm || !c || !adj_xi || !adj_eta)
-    return SAT_EINVAL;
+  if (!b || !b->calc_adj || grid < 0 || grid >= (int)b->grids.size() || !adj_xi || !adj_eta) return SAT_EINVAL;
+  if (!a && !bm && !c) {  // the terms the last run formed (sat_batch_set_aniso_metric)
+    if (!b->aniso_ran[grid]) return SAT_EINVAL;
+    const GridDev& g = b->grids[grid].g;
+    const size_t tot = (size_t)g.nperiod * g.plane;
+    SAT_CHECK(hipStreamSynchronize(b->stream));
+    SAT_CHECK(hipMemcpy(adj_xi, b->d_aniso[grid] + 3 * tot, sizeof(double) * tot, hipMemcpyDefault));
+    SAT_CHECK(hipMemcpy(adj_eta, b->d_aniso[grid] + 4 * tot, sizeof(double) * tot, hipMemcpyDefault));
+    return SAT_OK;
+  }
+  if (!a || !bm || !c) return SAT_EINVAL;
   // every event's T and Ta must be on the device: after a run, and not split into chunks
   if (!b->ran || b->chunk_first.size() != 1) return SAT_EINVAL;
   const GridDev& g = b->grids[grid].g;
@@ -1841,10 +1995,10 @@ int sat_batch_aniso_kernels(sat_batch* b, int grid, const double* a, const doubl
   SAT_CHECK(hipMalloc(&buf, sizeof(double) * 5 * total));
   {
     AnisoDev ad{g, nper, pl0, b->rd.ev_off, b->rd.ev_ids, b->d_events, b->d_T, b->d_ta, buf, buf + 3 * total,
-                buf + 4 * total};
-    hipError_t e = hipMemcpy(buf, a, sizeof(double) * total, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(buf + total, bm, sizeof(double) * total, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(buf + 2 * total, c, sizeof(double) * total, hipMemcpyHostToDevice);
+                buf + 4 * total, 0, b->nevent, 0};
+    hipError_t e = hipMemcpy(buf, a, sizeof(double) * total, hipMemcpyDefault);
+    if (e == hipSuccess) e = hipMemcpy(buf + total, bm, sizeof(double) * total, hipMemcpyDefault);
+    if (e == hipSuccess) e = hipMemcpy(buf + 2 * total, c, sizeof(double) * total, hipMemcpyDefault);
     if (e == hipSuccess) {
       const unsigned blocks = (unsigned)std::min<size_t>((total + 255) / 256, 65535);
       if (b->precision == SAT_PREC_F32)
@@ -1853,8 +2007,8 @@ int sat_batch_aniso_kernels(sat_batch* b, int grid, const double* a, const doubl
         hipLaunchKernelGGL(aniso_kernel<double>, dim3(blocks), dim3(256), 0, nullptr, ad);
       e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpy(adj_xi, ad.xi, sizeof(double) * total, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(adj_eta, ad.eta, sizeof(double) * total, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(adj_xi, ad.xi, sizeof(double) * total, hipMemcpyDefault);
+    if (e == hipSuccess) e = hipMemcpy(adj_eta, ad.eta, sizeof(double) * total, hipMemcpyDefault);
     if (e != hipSuccess) rc = (e == hipErrorOutOfMemory) ? SAT_ENOMEM : SAT_EDEVICE;
   }
   (void)hipFree(buf);
@@ -1894,15 +2048,16 @@ int sat_batch_results(sat_batch* b, double* syn, double* adj_s, double* chi, int
   SAT_CHECK(hipStreamSynchronize(b->stream));
   const size_t nr = (size_t)std::max(b->nrec, 1);
   const size_t ne = (size_t)std::max(b->nevent, 1);
-  if (syn && b->nrec) SAT_CHECK(hipMemcpy(syn, b->d_rec + 4 * nr, sizeof(double) * b->nrec, hipMemcpyDeviceToHost));
+  // destinations in host or device memory (hipMemcpyDefault tells them apart)
+  if (syn && b->nrec) SAT_CHECK(hipMemcpy(syn, b->d_rec + 4 * nr, sizeof(double) * b->nrec, hipMemcpyDefault));
   if (adj_s && b->K_total) {
-    if (b->calc_adj) SAT_CHECK(hipMemcpy(adj_s, b->d_K, sizeof(double) * b->K_total, hipMemcpyDeviceToHost));
-    else std::memset(adj_s, 0, sizeof(double) * b->K_total);
+    if (!b->calc_adj) SAT_CHECK(hipMemset(b->d_K, 0, sizeof(double) * b->K_total));  // no kernels: zeros
+    SAT_CHECK(hipMemcpy(adj_s, b->d_K, sizeof(double) * b->K_total, hipMemcpyDefault));
   }
-  if (chi) SAT_CHECK(hipMemcpy(chi, b->d_chi, sizeof(double), hipMemcpyDeviceToHost));
-  if (iters_fwd && b->nevent) SAT_CHECK(hipMemcpy(iters_fwd, b->d_iters, sizeof(int) * b->nevent, hipMemcpyDeviceToHost));
-  if (iters_adj && b->nevent) SAT_CHECK(hipMemcpy(iters_adj, b->d_iters + ne, sizeof(int) * b->nevent, hipMemcpyDeviceToHost));
-  if (flags && b->nevent) SAT_CHECK(hipMemcpy(flags, b->d_iters + 2 * ne, sizeof(int) * b->nevent, hipMemcpyDeviceToHost));
+  if (chi) SAT_CHECK(hipMemcpy(chi, b->d_chi, sizeof(double), hipMemcpyDefault));
+  if (iters_fwd && b->nevent) SAT_CHECK(hipMemcpy(iters_fwd, b->d_iters, sizeof(int) * b->nevent, hipMemcpyDefault));
+  if (iters_adj && b->nevent) SAT_CHECK(hipMemcpy(iters_adj, b->d_iters + ne, sizeof(int) * b->nevent, hipMemcpyDefault));
+  if (flags && b->nevent) SAT_CHECK(hipMemcpy(flags, b->d_iters + 2 * ne, sizeof(int) * b->nevent, hipMemcpyDefault));
   return SAT_OK;
 }
 

@@ -71,7 +71,9 @@ struct PhaseDev {
   double* conv;
   unsigned char* skip;  // FP32 batches: 0 FP32 result, 1 FP64 re-solve by these kernels,
                         // 2 FP64 re-solve by the per-pair kernel (pool full)
-  int want_skip;        // the skip value of the events this pass processes
+  int want_skip;        // the skip value of the events this pass processes (< 0: every event)
+  int kden_pass;        // the density-kernel adjoint (src/preproc.cpp:128-136): receiver sources -w,
+                        // the field not masked, syn, chi, iteration counts and flags left as they are
   int delta_cap;        // shares per event the receiver spread holds in LDS (launch's dynamic LDS)
   int* fb_list;         // FP32 forward: pool re-solves, per launch group
   int* fb_count;
@@ -170,6 +172,11 @@ __device__ inline int pop_work(int* counter, int* s_idx) {
     __syncthreads();
     return __builtin_amdgcn_readfirstlane(*s_idx);
   }
+}
+
+// Whether this pass processes event e (its re-solve mark is the pass's).
+__device__ __forceinline__ bool pass_event(const PhaseDev& pd, int e) {
+  return pd.want_skip < 0 || pd.skip[e] == pd.want_skip;
 }
 
 // Event of a node-parallel block: blockIdx.y, or the list entry blockIdx.y; -1 if none.
@@ -447,7 +454,7 @@ __global__ void adj_setup_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
   using F = typename PhaseT<PREC>::F;
   const int e = block_event(pd);
-  if (e < 0 || pd.skip[e] != pd.want_skip) return;
+  if (e < 0 || !pass_event(pd, e)) return;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -499,11 +506,11 @@ __global__ __launch_bounds__(kWave) void delta_kernel(PhaseDev pd) {
   double* s_val = dsh;                                    // [cap + kScan]
   int* s_node = reinterpret_cast<int*>(dsh + cap + kScan);  // [cap + kScan]
   const int e = pd.ev_base + (int)blockIdx.x;
-  if (e >= pd.ev_base + pd.nevent || pd.skip[e] != pd.want_skip) return;
+  if (e >= pd.ev_base + pd.nevent || !pass_event(pd, e)) return;
   const int lane = threadIdx.x;
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
-  if (lane == 0) {
+  if (lane == 0 && !pd.kden_pass) {
     double s = 0.0;
     for (int r = 0; r < ev.nrec; r++) {
       const int k = ev.rec_off + r;
@@ -619,7 +626,7 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
     const int idx = pop_work<W>(pd.counter, &s_idx);
     if (idx >= n_work) break;
     const int e = pd.list ? pd.list[idx] : pd.order[first + idx];
-    if (pd.skip[e] != pd.want_skip) continue;
+    if (!pass_event(pd, e)) continue;
     const EventX ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];
     const int nx = g.nx, ny = g.ny, n = (int)g.wplane;
@@ -645,13 +652,16 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
     IterOut io;
     const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? mw_adj_depth<PREC, C, W>() : kAdjDepth1>(uniform(cx), Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,
                                                                            io, exchange_ptr<Pol, W>(xch), red);
-    if (lane == 0) {
+    if (lane == 0 && !pd.kden_pass) {
       pd.iters_a[e] = io.iters;
       if (!io.converged) atomicOr(pd.flags + e, FLAG_ADJ_MAXITER);
     }
-    for (int iy = 0; iy < ny; iy++)  // :514-536 mask, column-major
+    // :514-536 mask, column-major (the density field is not masked: the
+    // reference masks adj_field a second time instead, src/preproc.cpp:136)
+    const bool masked = !pd.kden_pass;
+    for (int iy = 0; iy < ny; iy++)
       for (int ix = lane; ix < nx; ix += nthr) {
-        const bool m = ix >= ev.mb[0] && ix <= ev.mb[1] && iy >= ev.mb[2] && iy <= ev.mb[3];
+        const bool m = masked && ix >= ev.mb[0] && ix <= ev.mb[1] && iy >= ev.mb[2] && iy <= ev.mb[3];
         const V v = m ? V(0) : ta[wave_index(g, 0, ix, iy)];
         const size_t o = ev.pl_off + ix + (size_t)iy * nx;
         if (PREC == SAT_PREC_F64 && pd.ta_f32) static_cast<float*>(pd.ta_out)[o] = (float)v;

@@ -201,12 +201,7 @@ class EikonalBatch:
             timing["kernel_ms"] = ms.value
         return k_vs, k_vp, k_rho
 
-    def aniso_kernels(self, m11, m12, m22, grid: int = 0):
-        """The anisotropic terms of accumulate_kernels (src/preproc.cpp:37-65)
-        for this batch's last run on grid `grid`: (adj_xi, adj_eta), each
-        (nperiod, ny, nx).  m11, m12, m22 (nperiod, ny, nx): the grid's
-        topography metric (SurfGrid a = m11, b = m22, c = -m12,
-        src/surf_grid.cpp:186-189)."""
+    def _metric_planes(self, m11, m12, m22, grid):
         if not 0 <= grid < len(self.shapes):
             raise ValueError(f"grid {grid} is not in this batch")
         shape = tuple(self.shapes[grid])
@@ -214,11 +209,56 @@ class EikonalBatch:
         for m in planes:
             if m.shape != shape:
                 raise ValueError(f"metric planes must be {shape}")
+        return planes
+
+    def aniso_kernels(self, m11=None, m12=None, m22=None, grid: int = 0):
+        """The anisotropic terms of accumulate_kernels (src/preproc.cpp:37-65)
+        for this batch's last run on grid `grid`: (adj_xi, adj_eta), each
+        (nperiod, ny, nx).  m11, m12, m22 (nperiod, ny, nx): the grid's
+        topography metric (SurfGrid a = m11, b = m22, c = -m12,
+        src/surf_grid.cpp:186-189); all None: the terms the last run formed
+        for the metric set_aniso_metric registered (chunked runs included)."""
+        if not 0 <= grid < len(self.shapes):
+            raise ValueError(f"grid {grid} is not in this batch")
+        shape = tuple(self.shapes[grid])
         xi, eta = np.empty(shape), np.empty(shape)
-        rc = _lib.load().sat_batch_aniso_kernels(self._h, int(grid), *(m.ctypes.data for m in planes),
-                                                 xi.ctypes.data, eta.ctypes.data)
+        if m11 is None and m12 is None and m22 is None:
+            rc = _lib.load().sat_batch_aniso_kernels(self._h, int(grid), None, None, None,
+                                                     xi.ctypes.data, eta.ctypes.data)
+        else:
+            planes = self._metric_planes(m11, m12, m22, grid)
+            rc = _lib.load().sat_batch_aniso_kernels(self._h, int(grid), *(m.ctypes.data for m in planes),
+                                                     xi.ctypes.data, eta.ctypes.data)
         _lib.check("sat_batch_aniso_kernels", rc)
         return xi, eta
+
+    def set_aniso_metric(self, m11=None, m12=None, m22=None, grid: int = 0):
+        """Register grid `grid`'s topography metric: every later run forms the
+        anisotropic terms itself (aniso_kernels() with no metric returns them).
+        All None unregisters."""
+        if m11 is None and m12 is None and m22 is None:
+            rc = _lib.load().sat_batch_set_aniso_metric(self._h, int(grid), None, None, None)
+        else:
+            planes = self._metric_planes(m11, m12, m22, grid)
+            rc = _lib.load().sat_batch_set_aniso_metric(self._h, int(grid), *(m.ctypes.data for m in planes))
+        _lib.check("sat_batch_set_aniso_metric", rc)
+
+    def set_kden(self, on: bool = True):
+        """Form the density kernel (is_kden, src/preproc.cpp:127-136) in every
+        later run: a second adjoint per pair from the receiver sources -weight."""
+        _lib.check("sat_batch_set_kden", _lib.load().sat_batch_set_kden(self._h, int(bool(on))))
+
+    def kden_results(self):
+        """The last run's density kernel kden_s, per grid (nperiod, ny, nx), as
+        adj_s in results()."""
+        flat = np.zeros(max(sum(int(np.prod(s)) for s in self.shapes), 1))
+        _lib.check("sat_batch_kden_results", _lib.load().sat_batch_kden_results(self._h, flat.ctypes.data))
+        out, off = [], 0
+        for s in self.shapes:
+            n = int(np.prod(s))
+            out.append(flat[off:off + n].reshape(s))
+            off += n
+        return out if len(out) > 1 else out[0]
 
     def close(self):
         if getattr(self, "_h", None):
@@ -286,3 +326,51 @@ def combine_kernels(adj_s, sen_vs, sen_vp, sen_rho=None, vs3d=None, vp3d=None, m
     if timing is not None:
         timing["kernel_ms"] = ms.value
     return k_vs, k_vp, k_rho
+
+
+def combine_kernels_ex(adj_s, sen_vs, sen_vp, sen_rho=None, vs3d=None, vp3d=None, mode: int = COMBINE_VS,
+                       r1=None, r2=None, adj_xi=None, adj_eta=None, svel=None, sen_gc=None, sen_gs=None,
+                       adj_den=None, timing: dict | None = None) -> dict:
+    """preproc::combine_kernels with its anisotropic and density branches
+    (src/preproc.cpp:196-342) on the GPU (sat_combine_kernels_ex).
+
+    As combine_kernels, plus: r1, r2 (nx, ny, nper) -- SurfGrid r1_loc, r2_loc
+    (is_anisotropy; None: isotropic) with adj_xi, adj_eta (EikonalBatch.
+    aniso_kernels), svel (nper, ny, nx) and sen_gc, sen_gs (nx, ny, nz, nper);
+    adj_den (nper, ny, nx): the density accumulators (is_kden).  Returns a dict
+    of (nx, ny, nz) kernels: vs, and vp, rho (modes 1, 2), gc, gs, den as the
+    options form them."""
+    c = lambda a: None if a is None else _c64(a)  # noqa: E731
+    adj = _c64(adj_s)
+    nper, ny, nx = adj.shape
+    s_vs = _c64(sen_vs)
+    nz = s_vs.shape[2]
+    aniso, kden = r1 is not None, adj_den is not None
+    ins = dict(adj_s=adj, sen_vs=s_vs, sen_vp=c(sen_vp), sen_rho=c(sen_rho), vs3d=c(vs3d), vp3d=c(vp3d),
+               r1=c(r1), r2=c(r2), adj_xi=c(adj_xi), adj_eta=c(adj_eta), svel=c(svel), sen_gc=c(sen_gc),
+               sen_gs=c(sen_gs), adj_den=c(adj_den))
+    shapes = {"sen_vp": (nx, ny, nz, nper), "sen_rho": (nx, ny, nz, nper), "sen_gc": (nx, ny, nz, nper),
+              "sen_gs": (nx, ny, nz, nper), "vs3d": (nx, ny, nz), "vp3d": (nx, ny, nz), "r1": (nx, ny, nper),
+              "r2": (nx, ny, nper), "adj_xi": (nper, ny, nx), "adj_eta": (nper, ny, nx), "svel": (nper, ny, nx),
+              "adj_den": (nper, ny, nx)}
+    for k, shp in shapes.items():
+        if ins[k] is not None and ins[k].shape != shp:
+            raise ValueError(f"{k} must be {shp}")
+    out = {"vs": np.empty((nx, ny, nz))}
+    if mode:
+        out.update(vp=np.empty((nx, ny, nz)), rho=np.empty((nx, ny, nz)))
+    if aniso:
+        out.update(gc=np.empty((nx, ny, nz)), gs=np.empty((nx, ny, nz)))
+    if kden:
+        out["den"] = np.empty((nx, ny, nz))
+    d = _lib.sat_combine_desc()
+    d.nx, d.ny, d.nz, d.nper, d.mode = nx, ny, nz, nper, int(mode)
+    for k, a in ins.items():
+        setattr(d, k, a.ctypes.data if a is not None else None)
+    for k in ("vs", "vp", "rho", "gc", "gs", "den"):
+        setattr(d, "ker_" + k, out[k].ctypes.data if k in out else None)
+    ms = C.c_double(0.0)
+    _lib.check("sat_combine_kernels_ex", _lib.load().sat_combine_kernels_ex(C.byref(d), C.byref(ms)))
+    if timing is not None:
+        timing["kernel_ms"] = ms.value
+    return out

@@ -4,6 +4,8 @@ row, in file order):
 
   src_rec_checkerboard.csv  examples/00_checkerboard_iso/src_rec_file_ph.csv,
                             events MX00 and MX05 (16 periods, 23 receivers)
+  src_rec_checkerboard_gr.csv  examples/00_checkerboard_iso/src_rec_file_gr.csv
+                            (group-velocity times), the same two events
   src_rec_hawaii.csv        examples/04_hawaii_topo/src_rec_file_raw.csv,
                             its first 6 event names in byte order (15 periods,
                             uneven receiver counts; column order differs)
@@ -36,6 +38,8 @@ def main():
         sys.exit("reference examples not found")
     extract(os.path.join(REF, "00_checkerboard_iso", "src_rec_file_ph.csv"),
             os.path.join(HERE, "src_rec_checkerboard.csv"), lambda n: [b"MX00", b"MX05"])
+    extract(os.path.join(REF, "00_checkerboard_iso", "src_rec_file_gr.csv"),
+            os.path.join(HERE, "src_rec_checkerboard_gr.csv"), lambda n: [b"MX00", b"MX05"])
     extract(os.path.join(REF, "04_hawaii_topo", "src_rec_file_raw.csv"),
             os.path.join(HERE, "src_rec_hawaii.csv"), lambda n: n[:6])
 

@@ -29,7 +29,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     # sizes and field offsets of the ctypes mirrors against the C compiler's
     import subprocess
     structs = {"sat_info": _lib.sat_info, "sat_grid_desc": _lib.sat_grid_desc,
-               "sat_event_desc": _lib.sat_event_desc, "sat_batch_desc": _lib.sat_batch_desc}
+               "sat_event_desc": _lib.sat_event_desc, "sat_batch_desc": _lib.sat_batch_desc,
+               "sat_combine_desc": _lib.sat_combine_desc}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "surfatt_eik.h"', "int main(void) {"]
     for name, cls in structs.items():
         lines.append(f'  printf("{name} %zu\\n", sizeof({name}));')

@@ -58,3 +58,22 @@ def test_batch_combine_reads_device_kernels(mode):
                     assert a is None
                 else:
                     np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("aniso,kden", [(True, False), (False, True), (True, True)])
+def test_combine_ex_equals_oracle(mode, aniso, kden):
+    """sat_combine_kernels_ex (anisotropic Gc/Gs and density branches of
+    combine_kernels, src/preproc.cpp:246-338) equals the oracle's restatement
+    bit for bit in every mode."""
+    from tests.test_combine_oracle import _aniso_inputs
+    nx, ny, nz, nper = 9, 7, 6, 5
+    adj, s, vs, vp = _inputs(nx, ny, nz, nper, 30 + mode)
+    kw = _aniso_inputs(nx, ny, nz, nper, 40 + mode) if aniso else {}
+    if kden:
+        kw["adj_den"] = np.random.default_rng(50 + mode).normal(size=(nper, ny, nx))
+    ref = O.combine_kernels_ex(adj, *s, vs, vp, mode=mode, **kw)
+    got = PP.combine_kernels_ex(adj, *s, vs, vp, mode=mode, **kw)
+    assert set(got) == set(ref)
+    for k in ref:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)

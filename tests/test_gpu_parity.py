@@ -214,3 +214,66 @@ def test_receiver_spread_shared_cells_and_many_receivers_f64():
               rec_lat=rec_lat, tt_obs=tt, weight=1.0 + 0.5 * rng.uniform(368))
     rg, ro = _batch_vs_oracle(bb)
     print("chi", rg["chi"], "oracle", ro["chi"], flush=True)
+
+
+def test_batch_takes_device_inputs_and_outputs():
+    """Planes, receivers and observations given in device memory (the caller's
+    hipMalloc), results written to device memory: the same bits as host
+    pointers (the ownership contract of the replaced interface: host or
+    device pointers)."""
+    from surfattpp_amd import _lib
+    from tests.devmem import DevArray
+    import ctypes as C
+    b = synth.config_grf(seed=21, nx=47, ny=41, nsta=5, nperiod=2, spacing_km=1.5)
+    with PP.EikonalBatch(b) as bt:
+        bt.run()
+        ref = bt.results()
+    nper, ny, nx = b["svel"].shape
+    keep = {k: DevArray(np.ascontiguousarray(b[k]).ravel()) for k in ("svel", "m11", "m12", "m22", "rec_lon",
+                                                                    "rec_lat", "tt_obs", "weight")}
+    try:
+        lon, lat = np.ascontiguousarray(b["lon"]), np.ascontiguousarray(b["lat"])
+        g = _lib.sat_grid_desc(nx, ny, nper, lon.ctypes.data, lat.ctypes.data, keep["svel"].ptr, keep["m11"].ptr,
+                               keep["m12"].ptr, keep["m22"].ptr, 0)
+        gd = (_lib.sat_grid_desc * 1)(g)
+        ne = len(b["ev_iper"])
+        ed = (_lib.sat_event_desc * ne)()
+        for e in range(ne):
+            ed[e] = _lib.sat_event_desc(0, int(b["ev_iper"][e]), float(b["ev_lon"][e]), float(b["ev_lat"][e]),
+                                        int(b["ev_rec_off"][e]), int(b["ev_nrec"][e]))
+        d = _lib.sat_batch_desc()
+        d.ngrid, d.grids, d.nevent, d.events = 1, gd, ne, ed
+        d.nrec_total = len(b["rec_lon"])
+        d.rec_lon_deg, d.rec_lat_deg = keep["rec_lon"].ptr, keep["rec_lat"].ptr
+        d.tt_obs, d.weight = keep["tt_obs"].ptr, keep["weight"].ptr
+        d.precision, d.calc_adj, d.workspace_cap = _lib.PREC_F64, 1, 0
+        L = _lib.load()
+        h = C.c_void_p()
+        assert L.sat_batch_create(C.byref(d), C.byref(h)) == _lib.SAT_OK
+        try:
+            assert L.sat_batch_run(h, _lib.RUN_BATCHED) == _lib.SAT_OK
+            out_syn, out_k = DevArray(np.zeros(d.nrec_total)), DevArray(np.zeros(nper * ny * nx))
+            chi = DevArray(np.zeros(1))
+            assert L.sat_batch_results(h, out_syn.ptr, out_k.ptr, chi.ptr, None, None, None) == _lib.SAT_OK
+            np.testing.assert_array_equal(out_syn.get(d.nrec_total), ref["syn"])
+            np.testing.assert_array_equal(out_k.get(nper * ny * nx).reshape(nper, ny, nx), ref["adj_s"])
+            assert chi.get(1)[0] == ref["chi"]
+            # a new model given in device memory
+            sv2 = DevArray((np.ascontiguousarray(b["svel"]) * 1.02).ravel())
+            assert L.sat_batch_update_media(h, 0, sv2.ptr, keep["m11"].ptr, keep["m12"].ptr,
+                                            keep["m22"].ptr) == _lib.SAT_OK
+            assert L.sat_batch_run(h, _lib.RUN_BATCHED) == _lib.SAT_OK
+            syn2 = np.zeros(d.nrec_total)
+            assert L.sat_batch_results(h, syn2.ctypes.data, None, None, None, None, None) == _lib.SAT_OK
+            b2 = dict(b)
+            b2["svel"] = np.ascontiguousarray(b["svel"]) * 1.02
+            with PP.EikonalBatch(b2) as bt2:
+                bt2.run()
+                np.testing.assert_array_equal(syn2, bt2.results()["syn"])
+            for a in (out_syn, out_k, chi, sv2):
+                a.free()
+        finally:
+            L.sat_batch_destroy(h)
+    finally:
+        for a in keep.values():
+            a.free()

@@ -71,3 +71,54 @@ def test_example_maps_match_reference_solver():
     svel = SD.fwdsurf(z, vs3d, periods)
     thk, vp, vs, rho = SD.build_disp_req(z, vs1)
     np.testing.assert_array_equal(svel[:, 0, 0], O.surfdisp96(thk, vp, vs, rho, periods))
+
+
+TABLE_GR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "src_rec_checkerboard_gr.csv")
+
+
+def test_phase_then_group_velocity_data():
+    """Inversion::run_forward_adjoint (src/inversion.cpp:245-272) loops over the
+    data types, phase then group velocity: per type, maps from fwdsurf
+    (phase: igr 0, group: igr 1), forward_for_event on that type's table, and
+    misfit += chi * weight (the example's weights 0.5, 0.5).  Both tables of
+    example 00 (rows of src_rec_file_ph.csv / _gr.csv, tests/golden) on the
+    example's initial model: each type's batch equals the oracle on a subset
+    (FP64) and FP32 stays within 1e-5 of FP64; the group maps are slower than
+    the phase maps at every column (normal dispersion of this model)."""
+    from surfattpp_amd import PREC_F32
+    weights = {"ph": 0.5, "gr": 0.5}
+    misfit = {}
+    maps = {}
+    for kind, table, igr in (("ph", TABLE, 0), ("gr", TABLE_GR, 1)):
+        sr = S.SrcRec().load(table)
+        lon, lat = S.build_model_axes(S.build_stas(sr, None), [0.02, 0.02], 5)
+        nx, ny = lon.size, lat.size
+        z = np.arange(0.0, 15.0 + 1e-9, 0.5)
+        vs1 = 1.8 + (4.2 - 1.8) * z / z[-1]
+        vs3d = np.ascontiguousarray(np.broadcast_to(vs1, (nx, ny, z.size)))
+        periods = sr.periods_info.periods
+        svel = SD.fwdsurf(z, vs3d, periods, igr=igr)
+        maps[kind] = svel
+        assert np.all(svel > 0.5) and np.all(svel < 5.0)
+        b = sr.to_batch(lon, lat, svel)
+        sub = synth.subset_events(b, [0, 7, 19, 31])
+        ro = O.forward_for_event(sub)
+        with PP.EikonalBatch(sub) as bt:
+            bt.run()
+            rg = bt.results()
+        assert RC.rel_inf(rg["syn"], ro["syn"]) <= 1e-12
+        assert RC.rel_inf(rg["adj_s"], ro["adj_s"]) <= 1e-12
+        np.testing.assert_array_equal(rg["iters_fwd"], ro["iters_fwd"])
+        with PP.EikonalBatch(b) as b64, PP.EikonalBatch(b, precision=PREC_F32) as b32:
+            b64.run()
+            b32.run()
+            r64, r32 = b64.results(), b32.results()
+        errs = [RC.rel_inf(r32["adj_s"][p], r64["adj_s"][p]) for p in range(len(periods))]
+        print(f"{kind}: {len(b['ev_iper'])} events, chi {r64['chi']:.6f}, FP32 K err {max(errs):.2e}", flush=True)
+        assert max(errs) <= 1e-5
+        np.testing.assert_array_equal(r32["iters_fwd"], r64["iters_fwd"])
+        misfit[kind] = r64["chi"] * weights[kind]
+    assert np.all(maps["gr"] < maps["ph"])
+    total = misfit["ph"] + misfit["gr"]
+    print(f"misfit: ph {misfit['ph']:.6f} + gr {misfit['gr']:.6f} = {total:.6f}", flush=True)
+    assert np.isfinite(total) and total > 0
