@@ -33,7 +33,7 @@ int main(int argc, char** argv) {
     eikonal::mask_uniform_grid(xx, yy, Ta, src[0], src[1]);
     bool threw = false;
     try {
-      eikonal::FSM_UW_PS_lonlat_2d({0.0, 1.0}, yy, spha, sphb, sphc, fun, src[0], src[1]);
+      eikonal::FSM_UW_PS_lonlat_2d(std::vector<double>{0.0, 1.0}, yy, spha, sphb, sphc, fun, src[0], src[1]);
     } catch (const std::invalid_argument&) {
       threw = true;
     }

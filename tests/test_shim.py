@@ -93,6 +93,71 @@ def test_shim_forward_for_event_matches_oracle(tmp_path):
     assert abs(chi - ro["chi"]) <= 1e-12 * abs(ro["chi"])
 
 
+def test_shim_binds_eigen_shaped_types(tmp_path):
+    """The reference's call expressions (src/preproc.cpp:88-136) on Eigen-shaped
+    types and the reference's SrcRec / SurfGrid members compile against the
+    shim's templates (tests/cpp/shim_eigen_driver.cpp)."""
+    if not os.path.exists(os.path.join(LIBDIR, "libsurfatt_eik.so")):
+        pytest.skip("library not built")
+    _build(tmp_path, "shim_eigen_driver")
+
+
+def _write_batch(path, b):
+    nper, ny, nx = b["svel"].shape
+    ne, nrec = len(b["ev_iper"]), len(b["rec_lon"])
+    with open(path, "wb") as f:
+        f.write(np.array([nx, ny, nper, ne, nrec], dtype=np.int32).tobytes())
+        f.write(b["lon"].astype(np.float64).tobytes())
+        f.write(b["lat"].astype(np.float64).tobytes())
+        for k in ("svel", "m11", "m12", "m22"):
+            f.write(np.ascontiguousarray(np.transpose(b[k], (2, 1, 0))).tobytes())  # [ix][iy][iper]
+        for e in range(ne):
+            f.write(np.array([b["ev_iper"][e], b["ev_nrec"][e]], dtype=np.int32).tobytes())
+            f.write(np.array([b["ev_lon"][e], b["ev_lat"][e]], dtype=np.float64).tobytes())
+        for k in ("rec_lon", "rec_lat", "tt_obs", "weight"):
+            f.write(b[k].astype(np.float64).tobytes())
+
+
+@pytest.mark.gpu
+def test_shim_eigen_driver_matches_oracle(tmp_path):
+    """Both bindings of the Eigen-shaped driver against the oracle: (A) the
+    reference's per-event loop through eikonal:: and (B) surfatt::
+    forward_for_event over the reference's SrcRec / SurfGrid, with the density
+    kernel and the anisotropic terms, twice (the second call on a new velocity
+    model through the resident batch).  FP64: 1e-12 normalised."""
+    from tests.test_gpu_aniso import _oracle_aniso, _topo_batch
+    b = _topo_batch()
+    nper, ny, nx = b["svel"].shape
+    nrec, n2 = len(b["rec_lon"]), nper * ny * nx
+    inp, out = tmp_path / "batch.bin", tmp_path / "out.bin"
+    _write_batch(inp, b)
+    exe = _build(tmp_path, "shim_eigen_driver")
+    r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    res = np.fromfile(out, dtype=np.float64)
+    pos = 0
+
+    def take(n):
+        nonlocal pos
+        pos += n
+        return res[pos - n:pos]
+    b2 = dict(b, svel=b["svel"] * 1.01)
+    for label, batch, aniso in (("A", b, False), ("B", b, True), ("B2", b2, True)):
+        chi, syn = take(1)[0], take(nrec)
+        K, Kd = take(n2).reshape(nper, ny, nx), take(n2).reshape(nper, ny, nx)
+        ro = O.forward_for_event(batch, kden=True)
+        errs = [RC.rel_inf(syn, ro["syn"]), RC.rel_inf(K, ro["adj_s"]), RC.rel_inf(Kd, ro["kden_s"]),
+                abs(chi - ro["chi"]) / abs(ro["chi"])]
+        if aniso:
+            xi, eta = take(n2).reshape(nper, ny, nx), take(n2).reshape(nper, ny, nx)
+            rx, re = _oracle_aniso(batch)
+            errs += [RC.rel_inf(xi, rx), RC.rel_inf(eta, re)]
+        print(f"{label}: syn, adj_s, kden_s, chi" + (", adj_xi, adj_eta" if aniso else "") +
+              " errors " + " ".join(f"{e:.1e}" for e in errs), flush=True)
+        assert max(errs) <= 1e-12, label
+    assert pos == res.size
+
+
 def test_surfker_shim_compiles(tmp_path):
     if not os.path.exists(os.path.join(LIBDIR, "libsurfatt_eik.so")):
         pytest.skip("library not built")
