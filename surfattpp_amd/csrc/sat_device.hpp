@@ -85,6 +85,11 @@ struct WaveGeom {
   int C, La;
   size_t wplane;
 };
+// Two pairs per wave (FP32 isotropic forward sweeps): each half of the wave's
+// lanes sweeps one pair, at most kPair2MaxC columns per lane.
+constexpr int kHalfWave = 32;
+constexpr int kPair2MaxC = 7;
+
 __host__ __device__ inline WaveGeom wave_geom(int nx, int ny, int W) {
   WaveGeom w;
   w.C = ((nx + kWave - 1) / kWave + W - 1) / W;

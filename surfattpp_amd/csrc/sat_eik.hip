@@ -146,10 +146,29 @@ struct ReduceDev {
   double* K;
   size_t total_nodes;
   const size_t* node_prefix;   // prefix sum of plane_n (size nplanes+1)
+  const size_t* ev_taoff;      // parallel to ev_ids: the event's Ta plane offset
   int ev_lo, ev_hi;            // events of this pass (a chunk of the batch order)
   int accumulate;              // add to K (a later chunk) instead of overwriting it
 };
 
+// First position in ev_ids[q0, q1) (ascending) holding an event >= e.
+__device__ __forceinline__ int first_event_at_or_after(const int* ev_ids, int q0, int q1, int e) {
+  while (q0 < q1) {
+    const int m = (q0 + q1) >> 1;
+    if (ev_ids[m] < e) q0 = m + 1;
+    else q1 = m;
+  }
+  return q0;
+}
+
+// Per node of every (grid, period) plane: K += Ta * svel^-3 over the plane's
+// events of this pass, in batch order (src/preproc.cpp:28-31).  The events of
+// a plane are listed in ascending batch order, so a chunk's events are one run
+// of the list.  The Ta offsets come from a table parallel to the list, and the
+// loads of kReduceUnroll events are issued before they are summed (the sum
+// keeps the event order): one dependent load per event instead of a chain of
+// three, and several in flight per thread.
+constexpr int kReduceUnroll = 8;
 template <typename V>
 __global__ void reduce_kernel(ReduceDev rd) {
   const V* ta = static_cast<const V*>(rd.ta_out);
@@ -166,11 +185,21 @@ __global__ void reduce_kernel(ReduceDev rd) {
     // the chunks are consecutive ranges of the batch order: continuing from the
     // previous chunk's partial sum keeps the reference's summation order
     double acc = rd.accumulate ? rd.K[rd.plane_off[pl] + n] : 0.0;
-    for (int q = rd.ev_off[pl]; q < rd.ev_off[pl + 1]; q++) {
-      const int e = rd.ev_ids[q];
-      if (e < rd.ev_lo || e >= rd.ev_hi) continue;
-      acc += (double)ta[rd.events[e].ta_off + n] * v3;
+    int q0 = rd.ev_off[pl], q1 = rd.ev_off[pl + 1];
+    q0 = first_event_at_or_after(rd.ev_ids, q0, q1, rd.ev_lo);
+    q1 = first_event_at_or_after(rd.ev_ids, q0, q1, rd.ev_hi);
+    int q = q0;
+    for (; q + kReduceUnroll <= q1; q += kReduceUnroll) {
+      size_t off[kReduceUnroll];
+      V v[kReduceUnroll];
+#pragma unroll
+      for (int u = 0; u < kReduceUnroll; u++) off[u] = rd.ev_taoff[q + u];
+#pragma unroll
+      for (int u = 0; u < kReduceUnroll; u++) v[u] = ta[off[u] + n];
+#pragma unroll
+      for (int u = 0; u < kReduceUnroll; u++) acc += (double)v[u] * v3;
     }
+    for (; q < q1; q++) acc += (double)ta[rd.ev_taoff[q] + n] * v3;
     rd.K[rd.plane_off[pl] + n] = acc;
   }
 }
@@ -280,12 +309,24 @@ __global__ void zero_ints_kernel(int* p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
-__global__ void chi_kernel(const double* chi_ev, int n, double* chi) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double c = 0.0;
-    for (int e = 0; e < n; e++) c += chi_ev[e];
-    *chi = c;
+// One block of kChiThreads: each pass stages kChiThreads per-event terms in LDS
+// with one coalesced load, and thread 0 adds them in event order (the
+// reference's sequential sum; the loads of a pass are all in flight at once).
+constexpr int kChiThreads = 256;
+__global__ __launch_bounds__(kChiThreads) void chi_kernel(const double* chi_ev, int n, double* chi) {
+  __shared__ double v[kChiThreads];
+  double c = 0.0;
+  for (int e0 = 0; e0 < n; e0 += kChiThreads) {
+    const int e = e0 + (int)threadIdx.x;
+    v[threadIdx.x] = e < n ? chi_ev[e] : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int m = n - e0 < kChiThreads ? n - e0 : kChiThreads;
+      for (int i = 0; i < m; i++) c += v[i];
+    }
+    __syncthreads();
   }
+  if (threadIdx.x == 0) *chi = c;
 }
 
 // ----------------------------------------------------------------------------
@@ -358,10 +399,21 @@ struct GridHost {
   GridDev gmw{};              // FP32 batches: FP64 media, resolve_waves() waves per grid (re-solves)
   GridDev gseg{};             // g with one wave per pair (per-pair kernels; equals g when W == 1)
   int W = 1;                  // waves per pair of the batched sweeps
+  // FP32 batches, one wave per pair, at most 32 lanes of <= kPair2MaxC columns:
+  // the forward geometry of the two-pairs-per-wave sweeps (isotropic metric)
+  bool p2 = false;
+  GridDev gp2{};
+  char* d_wmedp2 = nullptr;   // its wavefront-major media
+
+  // wavefront-major elements of one event's work plane (either forward geometry)
+  size_t work_wplane() const { return p2 ? std::max(g.wplane, gp2.wplane) : g.wplane; }
+  bool p2_on = false;          // the batch's groups run this grid's pairs two per wave
+  // the forward geometry of the batched sweeps for the current grouping
+  const GridDev& gfwd() const { return p2_on ? gp2 : g; }
 
   void release() {
     void* ps[] = {d_lon, d_lat, d_xx, d_yy, d_model, d_fp, d_aA, d_b, d_c, d_v3, d_seg, d_seg64, d_trig_yy, d_rowtab,
-                  d_rowha, d_wf64, d_wmed, d_wmedmw, d_flag};
+                  d_rowha, d_wf64, d_wmed, d_wmedmw, d_flag, d_wmedp2};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = GridHost();
@@ -468,6 +520,23 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
       for (int i = 0; i < 4; i++)
         for (int o = 0; o < 2; o++) m.wmed[i][o] = G.d_wmedmw + (size_t)(2 * i + o) * pbm;
     }
+    const int c2 = (nx + kHalfWave - 1) / kHalfWave;
+    if (prec == SAT_PREC_F32 && !polish && W == 1 && c2 <= kPair2MaxC) {
+      // two pairs per wave: 32 lanes of c2 columns each
+      GridDev& q = G.gp2;
+      q = g;
+      q.C = c2;
+      q.La = (nx + c2 - 1) / c2;
+      q.wplane = (size_t)(ny + q.La - 1) * q.La * q.C;
+      for (int i = 0; i < 6; i++) q.seg[i] = nullptr;
+      q.wf64[0] = q.wf64[1] = nullptr;
+      const size_t pb2 = (size_t)nper * q.wplane * sizeof(float);
+      SAT_CHECK(dalloc(&G.d_wmedp2, (size_t)6 * 2 * pb2));
+      SAT_CHECK(hipMemset(G.d_wmedp2, 0, (size_t)6 * 2 * pb2));
+      for (int i = 0; i < 6; i++)
+        for (int o = 0; o < 2; o++) q.wmed[i][o] = G.d_wmedp2 + (size_t)(2 * i + o) * pb2;
+      G.p2 = true;
+    }
   }
   return SAT_OK;
 }
@@ -498,16 +567,25 @@ static int grid_media(GridHost& G, int mode, const double* p0, const double* p1,
     G.g64.iso = G.g.iso;
     G.gmw.iso = G.g.iso;
     G.gseg.iso = G.g.iso;
+    G.gp2.iso = G.g.iso;
   }
   const size_t total = (size_t)G.nper * G.ny * G.g.P;
   const int threads = 256;
   const int blocks = (int)std::min<size_t>((total + threads - 1) / threads, 65535);
-  const int npass = G.d_wmedmw ? 3 : G.d_seg64 ? 2 : 1;
+  // passes: the sweep-precision media (g), the FP64 media of FP32 batches
+  // (g64), their re-solve geometry (gmw), the two-pairs-per-wave geometry (gp2)
+  const GridDev* passes[4];
+  int npass = 0;
+  passes[npass++] = &G.g;
+  if (G.d_seg64) passes[npass++] = &G.g64;
+  if (G.d_wmedmw) passes[npass++] = &G.gmw;
+  if (G.p2) passes[npass++] = &G.gp2;
   for (int pass = 0; pass < npass; pass++) {
-    const GridDev& gd = (pass == 2) ? G.gmw : pass ? G.g64 : G.g;
+    const GridDev& gd = *passes[pass];
     void* sg[6];
     for (int i = 0; i < 6; i++) sg[i] = const_cast<void*>(gd.seg[i]);  // null: no segmented planes
-    hipLaunchKernelGGL(media_kernel, dim3(blocks), dim3(threads), 0, stream, gd, mode, pass ? SAT_PREC_F64 : G.prec,
+    const int mprec = (pass == 0 || &gd == &G.gp2) ? G.prec : SAT_PREC_F64;
+    hipLaunchKernelGGL(media_kernel, dim3(blocks), dim3(threads), 0, stream, gd, mode, mprec,
                        G.d_model, G.d_model + n, G.d_model + 2 * n, G.d_model + 3 * n, G.d_fp, G.d_aA, G.d_b,
                        G.d_c, (mode == 0) ? G.d_v3 : nullptr, sg[0], sg[1], sg[2], sg[3], sg[4], sg[5]);
     SAT_CHECK(hipGetLastError());
@@ -516,6 +594,9 @@ static int grid_media(GridHost& G, int mode, const double* p0, const double* p1,
 }
 
 static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Elements between the two halves' planes of a two-pairs-per-wave sweep slot.
+static size_t pair2_hstride(const GridDev& g) { return round_up(g.wplane, 64); }
 
 // Slot geometry for the largest grid of a precision.
 struct SlotGeom {
@@ -793,6 +874,7 @@ struct sat_batch {
   std::vector<EventX> hx;          // host copy of the phase events
   std::vector<int> base_flags;     // per event, set at creation (source on the boundary)
   GridDev* d_grids = nullptr;
+  GridDev* d_grids_fwd = nullptr;  // the forward geometry of each grid (GridHost::gfwd)
   GridDev* d_grids_seg = nullptr;  // one-wave geometry of the per-pair kernels
   GridDev* d_grids64 = nullptr;    // FP32 batches: grids with FP64 media (per-pair re-solves)
   GridDev* d_grids64mw = nullptr;  // FP32 batches: FP64 media, re-solve geometry (batched re-solves)
@@ -843,6 +925,8 @@ struct sat_batch {
   // group_C: columns per lane of one wave per pair (per-pair kernels, re-solves);
   // group_W, group_wC: waves per pair and columns per lane of the batched sweeps
   std::vector<int> group_C, group_iso, group_first, group_count, group_W, group_wC;
+  std::vector<int> group_p2;       // > 0: the two-pairs-per-wave forward sweeps' columns per lane (one grid per group)
+  std::vector<size_t> group_p2h;   // and their half-plane stride (pair2_hstride)
   // per group: sweep slots [group_slot0, group_slot0 + group_nblk); launch order
   // of the groups (most work first)
   std::vector<int> group_slot0, group_nblk, launch_order;
@@ -890,7 +974,7 @@ struct sat_batch {
       if (p) (void)hipFree(p);
     d_aniso.clear();
     if (d_Kden) (void)hipFree(d_Kden);
-    void* ps[] = {d_grids, d_grids_seg, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
+    void* ps[] = {d_grids, d_grids_fwd, d_grids_seg, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
                   d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_ov, d_evx64, d_work64, d_slots64, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
                   d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3};
     for (void* p : ps)
@@ -919,14 +1003,36 @@ int build_groups(sat_batch* b) {
   const int ne = b->nevent;
   std::vector<int> order(ne);
   std::iota(order.begin(), order.end(), 0);
+  // Two pairs per wave for the grids whose one-pair sweeps take one column per
+  // lane (nx <= 64): there the lane-skew ramp is La - 1 of ny + La - 1 steps
+  // and halving La pays (measured on the MI355X, 3,000 pairs: 61x71 forward
+  // sweeps 5.9 -> 5.1 ms); with more columns per lane the halves' longer
+  // per-step chains on fewer resident waves lose more than the ramp gains
+  // (101x101 10.3 -> 11.0 ms, 201x201 33.5 -> 41.1 ms).  SURFATT_PAIR2=0 / 1
+  // switches it off / on wherever a grid allows it (tests, tuning).
+  std::vector<char> use_p2(b->grids.size(), 0);
+  {
+    const char* env = std::getenv("SURFATT_PAIR2");
+    const int force = env ? std::atoi(env) : -1;
+    for (size_t gi = 0; gi < b->grids.size(); gi++) {
+      const GridHost& G = b->grids[gi];
+      if (!G.p2 || !G.g.iso) continue;
+      use_p2[gi] = force >= 0 ? (force > 0) : (G.g.C == 1);
+    }
+  }
   auto key = [&](int e) {  // metric mode: kMetricGeneral, kMetricIso, kMetricIsoPolish
     const GridHost& G = b->grids[b->he[e].grid];
-    return std::make_tuple(G.W, G.gseg.C, G.g.iso ? (b->polish ? kMetricIsoPolish : kMetricIso) : kMetricGeneral);
+    // two pairs per wave: a group holds one grid (the two halves share its geometry)
+    const int p2grid = use_p2[b->he[e].grid] ? b->he[e].grid : -1;
+    return std::make_tuple(G.W, G.gseg.C, G.g.iso ? (b->polish ? kMetricIsoPolish : kMetricIso) : kMetricGeneral,
+                           p2grid);
   };
   b->group_W.clear();
   b->group_wC.clear();
   b->group_C.clear();
   b->group_iso.clear();
+  b->group_p2.clear();
+  b->group_p2h.clear();
   b->group_first.clear();
   b->group_count.clear();
   b->group_slot0.clear();
@@ -946,10 +1052,13 @@ int build_groups(sat_batch* b) {
       const auto k = key(order[i]);
       int j = i;
       while (j < c0 + cn && key(order[j]) == k) j++;
+      const bool p2 = std::get<3>(k) >= 0;
       b->group_W.push_back(std::get<0>(k));
       b->group_C.push_back(std::get<1>(k));
       b->group_wC.push_back((std::get<1>(k) + std::get<0>(k) - 1) / std::get<0>(k));
       b->group_iso.push_back(std::get<2>(k));
+      b->group_p2.push_back(p2 ? b->grids[std::get<3>(k)].gp2.C : 0);
+      b->group_p2h.push_back(p2 ? pair2_hstride(b->grids[std::get<3>(k)].gp2) : 0);
       b->group_first.push_back(i);
       b->group_count.push_back(j - i);
       i = j;
@@ -963,6 +1072,8 @@ int build_groups(sat_batch* b) {
     int slot = 0;
     for (size_t gi = g_lo; gi < g_hi; gi++) {
       const long share = ((long)b->nslots_phase * b->group_count[gi] + cn - 1) / std::max(cn, 1);
+      // sweep slots (waves) of the group: one per pair in flight (a two-pairs
+      // group's forward sweeps use half of them, two pairs each)
       b->group_nblk.push_back((int)std::max(1L, std::min<long>(b->group_count[gi], share)));
       b->group_slot0.push_back(slot);
       slot += b->group_nblk.back();
@@ -979,14 +1090,18 @@ int build_groups(sat_batch* b) {
   }
   if (ne) SAT_CHECK(hipMemcpy(b->d_order, order.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
   std::vector<GridDev> hg(b->grids.size()), hg64(b->grids.size()), hgmw(b->grids.size()), hgs(b->grids.size());
+  std::vector<GridDev> hgf(b->grids.size());
   for (size_t gi = 0; gi < b->grids.size(); gi++) {
+    b->grids[gi].p2_on = use_p2[gi] != 0;
     hg[gi] = b->grids[gi].g;
+    hgf[gi] = b->grids[gi].gfwd();
     hg64[gi] = b->grids[gi].g64;
     hgmw[gi] = b->grids[gi].gmw;
     hgs[gi] = b->grids[gi].gseg;
     hgs[gi].iso = hg[gi].iso;
   }
   SAT_CHECK(hipMemcpy(b->d_grids, hg.data(), sizeof(GridDev) * hg.size(), hipMemcpyHostToDevice));
+  SAT_CHECK(hipMemcpy(b->d_grids_fwd, hgf.data(), sizeof(GridDev) * hgf.size(), hipMemcpyHostToDevice));
   SAT_CHECK(hipMemcpy(b->d_grids_seg, hgs.data(), sizeof(GridDev) * hgs.size(), hipMemcpyHostToDevice));
   if (b->d_grids64) SAT_CHECK(hipMemcpy(b->d_grids64, hg64.data(), sizeof(GridDev) * hg64.size(), hipMemcpyHostToDevice));
   if (b->d_grids64mw)
@@ -1180,7 +1295,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
     if (rc) return fail(rc);
     max_seg = std::max(max_seg, b->grids[gi].g.seg_plane);
     max_plane = std::max(max_plane, b->grids[gi].g.plane);
-    max_wplane = std::max(max_wplane, b->grids[gi].g.wplane);
+    max_wplane = std::max(max_wplane, b->grids[gi].work_wplane());
     max_nx = std::max(max_nx, gd.nx);
   }
   b->max_seg = max_seg;
@@ -1188,6 +1303,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   b->max_wplane = max_wplane;
   b->max_nx = max_nx;
   B_CHECK(dalloc(&b->d_grids, d->ngrid));
+  B_CHECK(dalloc(&b->d_grids_fwd, d->ngrid));
   B_CHECK(dalloc(&b->d_grids_seg, d->ngrid));
   if (f32) B_CHECK(dalloc(&b->d_grids64, d->ngrid));
   if (f32) B_CHECK(dalloc(&b->d_grids64mw, d->ngrid));
@@ -1203,13 +1319,14 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   // isotropic sweeps read the T0 gradient in FP64)
   const size_t es = elem_size(b->precision);
   const size_t wes = b->polish ? sizeof(double) : es;
-  auto ev_bytes = [&](const GridDev& g) {
-    const size_t w = round_up(g.wplane, 64), pl = round_up(g.plane, 64), col = round_up(g.nx, 64);
+  auto ev_bytes = [&](const GridHost& G) {
+    const GridDev& g = G.g;
+    const size_t w = round_up(G.work_wplane(), 64), pl = round_up(g.plane, 64), col = round_up(g.nx, 64);
     return 6 * w * wes + (f32 ? 2 : 1) * pl * sizeof(double) + (b->calc_adj ? pl * es : 0) + col * sizeof(double2);
   };
   size_t ws_all = 0;
   for (int e = 0; e < ne; e++) {
-    const size_t eb = ev_bytes(b->grids[d->events[e].grid].g);
+    const size_t eb = ev_bytes(b->grids[d->events[e].grid]);
     ws_all += eb;
     b->ws_event_max = std::max(b->ws_event_max, eb);
   }
@@ -1228,7 +1345,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
     const sat_event_desc& ev = d->events[e];
     const sat_grid_desc& gd = d->grids[ev.grid];
     const GridDev& g = b->grids[ev.grid].g;
-    const size_t eb = ev_bytes(g);
+    const size_t eb = ev_bytes(b->grids[ev.grid]);
     if (e == 0 || (b->ws_cap != 0 && bytes_c + eb > b->ws_cap)) {  // a new chunk
       b->chunk_first.push_back(e);
       b->chunk_count.push_back(0);
@@ -1247,7 +1364,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
     for (int j = 0; j < gd.ny; j++) yy[j] = gd.lat_deg[j] * kDEG2RAD;
     event_boxes(b->grids[ev.grid], xx, yy, gd.lon_deg, gd.lat_deg, ev.src_lon_deg, ev.src_lat_deg, x,
                 b->base_flags[e]);
-    w_c += round_up(g.wplane, 64);
+    w_c += round_up(b->grids[ev.grid].work_wplane(), 64);
     pl_c += round_up(g.plane, 64);
     col_c += round_up(g.nx, 64);
     w_total = std::max(w_total, w_c);
@@ -1323,10 +1440,20 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   // resident waves of the sweep kernels: 3 swept planes each (FP64 sigma/tau sized)
   hipDeviceProp_t prop;
   B_CHECK(hipGetDeviceProperties(&prop, 0));
-  const int waves_per_cu = 16;
+  // SURFATT_SWEEP_WAVES_PER_CU (1..64): fewer sweep slots, so that each slot
+  // takes several pairs from the work queue in turn (tuning and tests)
+  int waves_per_cu = 16;
+  if (const char* v = std::getenv("SURFATT_SWEEP_WAVES_PER_CU")) {
+    const int w = std::atoi(v);
+    if (w >= 1 && w <= 64) waves_per_cu = w;
+  }
   const int max_chunk = *std::max_element(b->chunk_count.begin(), b->chunk_count.end());
   b->nslots_phase = std::max(1, std::min(max_chunk, prop.multiProcessorCount * waves_per_cu));
-  b->slot_plane = round_up((max_wplane + kTrashElems) * sizeof(double), 256);  // + the sweeps' trash row
+  // + the sweeps' trash row; two-pairs-per-wave sweeps keep two halves per plane
+  size_t slot_elems = max_wplane + kTrashElems;
+  for (const GridHost& G : b->grids)
+    if (G.p2) slot_elems = std::max(slot_elems, 2 * pair2_hstride(G.gp2));
+  b->slot_plane = round_up(slot_elems * sizeof(double), 256);
   b->slot_bytes = 3 * b->slot_plane;
   // concurrent launch groups share the slots; each group's share is rounded up
   const size_t nslots_alloc = (size_t)b->nslots_phase + kMaxGroups;
@@ -1388,6 +1515,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   {
     std::vector<size_t> hs(plane_off);
     hs.insert(hs.end(), node_prefix.begin(), node_prefix.end());
+    for (int e : ev_ids) hs.push_back(b->he[e].ta_off);
     B_CHECK(dalloc(&b->d_red_sz, hs.size()));
     B_CHECK(hipMemcpy(b->d_red_sz, hs.data(), sizeof(size_t) * hs.size(), hipMemcpyHostToDevice));
     std::vector<int> hi(plane_n);
@@ -1402,6 +1530,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   rd.nplanes = b->nplanes;
   rd.plane_off = b->d_red_sz;
   rd.node_prefix = b->d_red_sz + plane_off.size();
+  rd.ev_taoff = b->d_red_sz + plane_off.size() + node_prefix.size();
   rd.plane_n = b->d_red_i;
   rd.ev_off = b->d_red_i + plane_n.size();
   rd.ev_ids = b->d_red_i + plane_n.size() + ev_off.size();
@@ -1731,17 +1860,26 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
       if (cn) {
         const dim3 col_grid((unsigned)((b->max_nx + 63) / 64), (unsigned)cn);
         SAT_CHECK(f32 ? launch_trig<SAT_PREC_F32>(col_grid, b->stream, pc) : launch_trig<SAT_PREC_F64>(col_grid, b->stream, pc));
-        SAT_CHECK(f32 ? launch_init<SAT_PREC_F32>(dim3(init_bx, cn, 3), b->stream, pc)
-                      : launch_init<SAT_PREC_F64>(dim3(init_bx, cn, 2), b->stream, pc));
+        PhaseDev pf = pc;  // the forward geometry (two pairs per wave where a grid runs them)
+        pf.grids = b->d_grids_fwd;
+        SAT_CHECK(f32 ? launch_init<SAT_PREC_F32>(dim3(init_bx, cn, 3), b->stream, pf)
+                      : launch_init<SAT_PREC_F64>(dim3(init_bx, cn, 2), b->stream, pf));
       }
       SAT_CHECK(mark(1, b->stream));
       {
         const int rc = fan_out([&](int gi, hipStream_t st) -> int {
           PhaseDev a = pc;
+          a.grids = b->d_grids_fwd;
           a.counter = fwd_heads + gi;
           a.fb_list = pc.fb_list + b->group_first[gi];
           a.ov_list = pc.ov_list + b->group_first[gi];
           a.slots = b->d_slots + (size_t)b->group_slot0[gi] * b->slot_bytes;
+          if (b->group_p2[gi] > 0) {
+            const int nw = (b->group_nblk[gi] + 1) / 2;
+            SAT_CHECK(launch_fwd_sweep_pair2(b->group_p2[gi], dim3(nw), st, a, b->group_first[gi], b->group_count[gi],
+                                             (int)gi, (int)b->group_p2h[gi]));
+            return SAT_OK;
+          }
           const int C = b->group_wC[gi], W = b->group_W[gi], iso = b->group_iso[gi];
           SAT_CHECK(f32 ? launch_fwd_sweep<SAT_PREC_F32>(iso, C, W, dim3(b->group_nblk[gi]), st, a,
                                                          b->group_first[gi], b->group_count[gi], gi)
@@ -1844,7 +1982,8 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
       }
     }
   }
-  hipLaunchKernelGGL(chi_kernel, dim3(1), dim3(1), 0, b->stream, b->d_chi_ev, b->calc_adj ? ne : 0, b->d_chi);
+  hipLaunchKernelGGL(chi_kernel, dim3(1), dim3(kChiThreads), 0, b->stream, b->d_chi_ev, b->calc_adj ? ne : 0,
+                     b->d_chi);
   SAT_CHECK(hipGetLastError());
   return SAT_OK;
 }

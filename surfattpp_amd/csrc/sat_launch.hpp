@@ -36,6 +36,9 @@ hipError_t launch_fwd_sweep(int iso, int C, int W, dim3 grid, hipStream_t s, con
 template <int PREC>
 hipError_t launch_fwd_sweep_mw(int iso, int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first,
                                int count, int group);
+// FP32 isotropic forward sweeps, two pairs per wave (C columns per lane, C <= kPair2MaxC)
+hipError_t launch_fwd_sweep_pair2(int C, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count,
+                                  int group, int hstride);
 hipError_t launch_recv(dim3 grid, hipStream_t s, const PhaseDev& pd, int nrec);
 template <int PREC>
 hipError_t launch_adj_setup(dim3 grid, hipStream_t s, const PhaseDev& pd);
