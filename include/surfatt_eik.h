@@ -59,6 +59,10 @@ typedef struct sat_info {
 
 /* ------------------------------------------------------------------------
  * Per-solve calls (host pointers, synchronous, one solve per launch).
+ *
+ * Grid sizes, here and in the batches: nx >= 3 and ny >= 3 as in the
+ * reference; nx <= 4096 (grids wider than 512 columns sweep with 2, 4 or 8
+ * waves per pair; wider than 4096: SAT_EINVAL).
  * ------------------------------------------------------------------------ */
 
 /* Replaces eikonal::FSM_UW_PS_lonlat_2d (include/eikonal_solver.h:32-40,

@@ -17,7 +17,8 @@ LIB = os.path.join(HERE, "libsurfatt_eik.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["sat_eik.hip", "k_pair64.hip", "k_pair32.hip", "k_phase64.hip", "k_phase32.hip", "k_mw64.hip",
-           "k_mw32.hip", "k_pair2.hip", "sat_combine.hip", "sat_topo.hip", "sat_surfdisp.hip"]
+           "k_mw32.hip", "k_pair2.hip", "k_wide64.hip", "k_wide32.hip", "k_widefused64.hip",
+           "k_widefused32.hip", "sat_combine.hip", "sat_topo.hip", "sat_surfdisp.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC",
     # the FP64 path reproduces the reference's operation order: no contraction

@@ -31,7 +31,24 @@ constexpr int kAdjMaxIter = 1000;                               // :339
 constexpr double kAdjTol = 1e-9;                                // :338
 constexpr double kAdjEps = 1e-14;                               // :337
 constexpr int kWave = 64;
-constexpr int kMaxC = 8;  // columns per lane: nx <= 512
+constexpr int kMaxC = 8;  // columns per lane of a one-wave sweep: nx <= 512
+
+// Wider grids (512 < nx <= 4096) sweep with kMaxC columns per lane on a
+// work-group of W = 2, 4 or 8 waves per pair (the lanes exchange their edge
+// values through LDS), in every path: per-solve, per-pair, batched FP64 and
+// FP32 and the FP64 re-solves.  wide_waves: that W, 1 for nx <= 512, 0 above
+// kMaxWideW * 512 columns (refused).
+constexpr int kMaxWideW = 8;
+__host__ __device__ constexpr int wide_waves(int nx) {
+  const int c1 = (nx + kWave - 1) / kWave;
+  if (c1 <= kMaxC) return 1;
+  for (int w = 2; w <= kMaxWideW; w *= 2)
+    if (c1 <= kMaxC * w) return w;
+  return 0;
+}
+// Elements past the end of a segmented swept plane: the stores of lanes with
+// no row at a step go there (one row of W waves' lanes, kMaxC columns each).
+__host__ __device__ constexpr int seg_trash_elems(int W) { return kWave * W * kMaxC; }
 
 constexpr int FLAG_NOT_CONVERGED = 1;
 constexpr int FLAG_SRC_ON_BOUNDARY = 2;
@@ -247,6 +264,24 @@ __device__ inline int wave_or_i(int v) {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
   return v;
 }
+
+// The same over a work-group of blockDim.x / 64 <= kMaxWideW waves (every
+// thread of the block calls them; one wave: no LDS, no barrier).
+__device__ inline int block_reduce_i(int v, int op) {  // op: 0 min, 1 max, 2 or
+  v = op == 0 ? wave_min_i(v) : op == 1 ? wave_max_i(v) : wave_or_i(v);
+  if (blockDim.x == kWave) return v;
+  __shared__ int buf[kMaxWideW];
+  __syncthreads();  // an earlier call's readers are done with buf
+  if ((threadIdx.x & (kWave - 1)) == 0) buf[threadIdx.x / kWave] = v;
+  __syncthreads();
+  int r = buf[0];
+  for (int w = 1; w < (int)(blockDim.x / kWave); w++)
+    r = op == 0 ? imin(r, buf[w]) : op == 1 ? imax(r, buf[w]) : (r | buf[w]);
+  return r;
+}
+__device__ inline int block_min_i(int v) { return block_reduce_i(v, 0); }
+__device__ inline int block_max_i(int v) { return block_reduce_i(v, 1); }
+__device__ inline int block_or_i(int v) { return block_reduce_i(v, 2); }
 
 // include/utils.h:28-36 bilinear at fractional index (idx0+r1, idy0+r2), column-major.
 __device__ inline double bilinear_cm(const double* M, int nx, int idx0, int idy0, double r1, double r2) {

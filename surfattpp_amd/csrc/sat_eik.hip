@@ -344,14 +344,18 @@ __global__ __launch_bounds__(kChiThreads) void chi_kernel(const double* chi_ev, 
 
 struct Layout {
   int C, La, P;
+  int W;  // waves per pair of the one-pair kernels (wide_waves; 0: too wide)
 };
 
 // Launch groups: one per (columns per lane C, isotropic metric).
 constexpr int kMaxGroups = 2 * kMaxC;
 
+// Up to 512 columns: one wave, C = ceil(nx / 64) columns per lane.  Wider:
+// kMaxC columns per lane on W = wide_waves(nx) waves.
 static Layout layout_for(int nx) {
   Layout L;
-  L.C = (nx + kWave - 1) / kWave;
+  L.W = wide_waves(nx);
+  L.C = L.W > 1 ? kMaxC : (nx + kWave - 1) / kWave;
   L.La = (nx + L.C - 1) / L.C;
   L.P = L.La * L.C;
   return L;
@@ -399,6 +403,7 @@ struct GridHost {
   GridDev gmw{};              // FP32 batches: FP64 media, resolve_waves() waves per grid (re-solves)
   GridDev gseg{};             // g with one wave per pair (per-pair kernels; equals g when W == 1)
   int W = 1;                  // waves per pair of the batched sweeps
+  int Wseg = 1;               // waves per pair of the one-pair kernels (> 1: a grid wider than 512 columns)
   // FP32 batches, one wave per pair, at most 32 lanes of <= kPair2MaxC columns:
   // the forward geometry of the two-pairs-per-wave sweeps (isotropic metric)
   bool p2 = false;
@@ -429,8 +434,9 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
   if (nx < 3 || ny < 3 || nper < 1 || !lon || !lat) return SAT_EINVAL;
   if (!uniform_axis(lon, nx) || !uniform_axis(lat, ny)) return SAT_EINVAL;
   const Layout L = layout_for(nx);
-  if (L.C > kMaxC) return SAT_EINVAL;
+  if (L.W == 0) return SAT_EINVAL;  // wider than kMaxWideW * 512 columns
   G.nx = nx; G.ny = ny; G.nper = nper; G.prec = prec;
+  G.Wseg = L.W;
   std::vector<double> xx(nx), yy(ny);
   for (int i = 0; i < nx; i++) xx[i] = lon[i] * kDEG2RAD;  // src/eikonal_solver.cpp:41
   for (int j = 0; j < ny; j++) yy[j] = lat[j] * kDEG2RAD;
@@ -481,7 +487,7 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
     for (int i = 0; i < 6; i++) G.g64.seg[i] = G.d_seg64 + (size_t)i * seg64_bytes;
   G.gseg = g;  // one wave per pair: the segmented layout of the per-pair kernels
   G.W = W;
-  if (wave && W > 1) {  // the batched sweeps' wave geometry (the segmented pitch P stays)
+  if (wave && W > 1 && G.Wseg == 1) {  // the batched sweeps' wave geometry (the segmented pitch P stays)
     const WaveGeom wg = wave_geom(nx, ny, W);
     g.C = wg.C;
     g.La = wg.La;
@@ -494,7 +500,7 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
     SAT_CHECK(hipMemset(G.d_wmed, 0, (size_t)nmed * 2 * pb));
     for (int i = 0; i < nmed; i++)
       for (int o = 0; o < 2; o++) g.wmed[i][o] = G.d_wmed + (size_t)(2 * i + o) * pb;
-    if (prec == SAT_PREC_F32 && polish) {
+    if (prec == SAT_PREC_F32 && polish && G.Wseg == 1) {  // (wider grids take the plain FP32 sweeps)
       // the FP64 slowness of the polished isotropic sweeps (g only: the FP64
       // copies g64 and gmw were taken above and keep wf64 null)
       const size_t pf = (size_t)nper * g.wplane;
@@ -513,6 +519,12 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
       m.La = wg.La;
       m.P = m.La * m.C;
       m.wplane = wg.wplane;
+      if (G.Wseg > 1) {  // wide grids: the batch's own geometry, Wseg waves of kMaxC columns
+        m.C = L.C;
+        m.La = L.La;
+        m.P = L.P;
+        m.wplane = (size_t)(ny + L.La - 1) * L.La * L.C;
+      }
       for (int i = 0; i < 6; i++) m.seg[i] = nullptr;
       const size_t pbm = (size_t)nper * m.wplane * sizeof(double);
       SAT_CHECK(dalloc(&G.d_wmedmw, (size_t)4 * 2 * pbm));
@@ -521,7 +533,7 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
         for (int o = 0; o < 2; o++) m.wmed[i][o] = G.d_wmedmw + (size_t)(2 * i + o) * pbm;
     }
     const int c2 = (nx + kHalfWave - 1) / kHalfWave;
-    if (prec == SAT_PREC_F32 && !polish && W == 1 && c2 <= kPair2MaxC) {
+    if (prec == SAT_PREC_F32 && !polish && W == 1 && G.Wseg == 1 && c2 <= kPair2MaxC) {
       // two pairs per wave: 32 lanes of c2 columns each
       GridDev& q = G.gp2;
       q = g;
@@ -602,12 +614,12 @@ static size_t pair2_hstride(const GridDev& g) { return round_up(g.wplane, 64); }
 struct SlotGeom {
   size_t seg, plain, slot;
 };
-static SlotGeom slot_geom(int prec, size_t max_seg_elems, size_t max_plane) {
+static SlotGeom slot_geom(int prec, size_t max_seg_elems, size_t max_plane, int W = 1) {
   SlotGeom s;
   const int nseg = prec == SAT_PREC_F64 ? Prec<SAT_PREC_F64>::kSeg : Prec<SAT_PREC_F32>::kSeg;
   const int npl = prec == SAT_PREC_F64 ? Prec<SAT_PREC_F64>::kPlain : Prec<SAT_PREC_F32>::kPlain;
   // FP32 path: sigma planes are FP64; every plane carries the sweeps' trash row
-  s.seg = round_up((max_seg_elems + kTrashElems) * sizeof(double), 256);
+  s.seg = round_up((max_seg_elems + seg_trash_elems(W)) * sizeof(double), 256);
   s.plain = round_up(max_plane * sizeof(double), 256);
   s.slot = nseg * s.seg + npl * s.plain;
   return s;
@@ -661,7 +673,7 @@ int solo_run(bool forward, int precision, int nx, int ny, const double* lon, con
   if (!forward) { ones.assign(N, 1.0); fin = ones.data(); }
   rc = grid_media(G, 1, spha, sphb, sphc, fin, stream);
   if (rc) { cleanup(); return rc; }
-  const SlotGeom sg = slot_geom(precision, G.g.seg_plane, N);
+  const SlotGeom sg = slot_geom(precision, G.g.seg_plane, N, G.Wseg);
   SOLO_CHECK(dalloc(&d_ws, sg.slot));
   SOLO_CHECK(hipMemset(d_ws, 0, sg.slot));
   SOLO_CHECK(dalloc(&d_out, N));
@@ -672,8 +684,8 @@ int solo_run(bool forward, int precision, int nx, int ny, const double* lon, con
   sd.out = d_out; sd.ws = d_ws; sd.ws_seg = sg.seg; sd.ws_plain = sg.plain; sd.info = d_info;
   if (forward) {
     sd.maxiter = kFwdMaxIter;
-    SOLO_CHECK(precision == SAT_PREC_F64 ? launch_solo<SAT_PREC_F64>(1, G.g.C, stream, sd)
-                                         : launch_solo<SAT_PREC_F32>(1, G.g.C, stream, sd));
+    SOLO_CHECK(precision == SAT_PREC_F64 ? launch_solo<SAT_PREC_F64>(1, G.g.C, G.Wseg, stream, sd)
+                                         : launch_solo<SAT_PREC_F32>(1, G.g.C, G.Wseg, stream, sd));
   } else {
     SOLO_CHECK(dalloc(&d_T, N));
     SOLO_CHECK(hipMemcpy(d_T, fun_or_T, sizeof(double) * N, hipMemcpyHostToDevice));
@@ -685,8 +697,8 @@ int solo_run(bool forward, int precision, int nx, int ny, const double* lon, con
     }
     sd.T_in = d_T; sd.nr = nr; sd.rlon = d_r; sd.rlat = d_r + nr; sd.adj = d_r + 2 * nr;
     sd.maxiter = kAdjMaxIter;
-    SOLO_CHECK(precision == SAT_PREC_F64 ? launch_solo<SAT_PREC_F64>(0, G.g.C, stream, sd)
-                                         : launch_solo<SAT_PREC_F32>(0, G.g.C, stream, sd));
+    SOLO_CHECK(precision == SAT_PREC_F64 ? launch_solo<SAT_PREC_F64>(0, G.g.C, G.Wseg, stream, sd)
+                                         : launch_solo<SAT_PREC_F32>(0, G.g.C, G.Wseg, stream, sd));
   }
   SOLO_CHECK(hipDeviceSynchronize());
   SOLO_CHECK(hipMemcpy(out, d_out, sizeof(double) * N, hipMemcpyDeviceToHost));
@@ -778,12 +790,12 @@ int forward_noalloc(int precision, int nx, int ny, const double* lon, const doub
   if (!lon || !lat || !spha || !sphb || !sphc || !fun || !T_out) return SAT_EINVAL;
   if (nx < 3 || ny < 3 || !std::isfinite(x0) || !std::isfinite(y0)) return SAT_EINVAL;
   const Layout L = layout_for(nx);
-  if (L.C > kMaxC) return SAT_EINVAL;
+  if (L.W == 0) return SAT_EINVAL;
   int rc = device_check();
   if (rc) return rc;
   // the scratch array's layout for this grid
   const size_t N = (size_t)nx * ny, S = (size_t)L.P * ny, es = elem_size(precision);
-  const SlotGeom sg = slot_geom(precision, S, N);
+  const SlotGeom sg = slot_geom(precision, S, N, L.W);
   size_t off = 0;
   auto take = [&](size_t bytes) { const size_t o = off; off += round_up(bytes, 256); return o; };
   const size_t o_xx = take(nx * sizeof(double)), o_yy = take(ny * sizeof(double));
@@ -852,8 +864,8 @@ int forward_noalloc(int precision, int nx, int ny, const double* lon, const doub
   sd.out = T_out; sd.ws = at(o_slot); sd.ws_seg = sg.seg; sd.ws_plain = sg.plain;
   sd.info = reinterpret_cast<double*>(at(o_info));
   sd.maxiter = kFwdMaxIter;
-  SAT_CHECK(precision == SAT_PREC_F64 ? launch_solo<SAT_PREC_F64>(1, g.C, stream, sd)
-                                      : launch_solo<SAT_PREC_F32>(1, g.C, stream, sd));
+  SAT_CHECK(precision == SAT_PREC_F64 ? launch_solo<SAT_PREC_F64>(1, g.C, L.W, stream, sd)
+                                      : launch_solo<SAT_PREC_F32>(1, g.C, L.W, stream, sd));
   double hinfo[4];
   SAT_CHECK(hipMemcpy(hinfo, sd.info, sizeof(hinfo), hipMemcpyDeviceToHost));  // waits for the solve
   if (info) {
@@ -927,6 +939,7 @@ struct sat_batch {
   std::vector<int> group_C, group_iso, group_first, group_count, group_W, group_wC;
   std::vector<int> group_p2;       // > 0: the two-pairs-per-wave forward sweeps' columns per lane (one grid per group)
   std::vector<size_t> group_p2h;   // and their half-plane stride (pair2_hstride)
+  std::vector<int> group_wide;     // 1: a grid wider than 512 columns (W waves of kMaxC columns, every kernel)
   // per group: sweep slots [group_slot0, group_slot0 + group_nblk); launch order
   // of the groups (most work first)
   std::vector<int> group_slot0, group_nblk, launch_order;
@@ -1024,8 +1037,10 @@ int build_groups(sat_batch* b) {
     const GridHost& G = b->grids[b->he[e].grid];
     // two pairs per wave: a group holds one grid (the two halves share its geometry)
     const int p2grid = use_p2[b->he[e].grid] ? b->he[e].grid : -1;
-    return std::make_tuple(G.W, G.gseg.C, G.g.iso ? (b->polish ? kMetricIsoPolish : kMetricIso) : kMetricGeneral,
-                           p2grid);
+    int mode;
+    if (b->precision == SAT_PREC_F64) mode = G.Wseg > 1 ? kMetricExact : (G.g.iso ? kMetricIso : kMetricGeneral);
+    else mode = G.g.iso ? ((b->polish && G.g.wf64[0]) ? kMetricIsoPolish : kMetricIso) : kMetricGeneral;
+    return std::make_tuple(G.W, G.gseg.C, mode, p2grid, G.Wseg > 1 ? 1 : 0);
   };
   b->group_W.clear();
   b->group_wC.clear();
@@ -1033,6 +1048,7 @@ int build_groups(sat_batch* b) {
   b->group_iso.clear();
   b->group_p2.clear();
   b->group_p2h.clear();
+  b->group_wide.clear();
   b->group_first.clear();
   b->group_count.clear();
   b->group_slot0.clear();
@@ -1055,7 +1071,10 @@ int build_groups(sat_batch* b) {
       const bool p2 = std::get<3>(k) >= 0;
       b->group_W.push_back(std::get<0>(k));
       b->group_C.push_back(std::get<1>(k));
-      b->group_wC.push_back((std::get<1>(k) + std::get<0>(k) - 1) / std::get<0>(k));
+      // (a wide grid: kMaxC columns per lane on every one of its waves)
+      const bool wide = std::get<4>(k) != 0;
+      b->group_wC.push_back(wide ? kMaxC : (std::get<1>(k) + std::get<0>(k) - 1) / std::get<0>(k));
+      b->group_wide.push_back(wide ? 1 : 0);
       b->group_iso.push_back(std::get<2>(k));
       b->group_p2.push_back(p2 ? b->grids[std::get<3>(k)].gp2.C : 0);
       b->group_p2h.push_back(p2 ? pair2_hstride(b->grids[std::get<3>(k)].gp2) : 0);
@@ -1126,6 +1145,13 @@ int choose_waves(const sat_batch_desc& d, bool f32, bool polish, int cus, std::v
   std::vector<bool> autow(ng, false);
   for (int gi = 0; gi < ng; gi++) {
     const int v = d.grids[gi].waves;
+    const int ww = wide_waves(d.grids[gi].nx);
+    if (ww == 0) return SAT_EINVAL;
+    if (ww > 1) {  // wider than 512 columns: wide_waves waves of kMaxC columns, in every precision
+      if (v != 0 && v != ww) return SAT_EINVAL;
+      w[gi] = ww;
+      continue;
+    }
     if (v != 0 && v != 1 && v != 2 && v != 4) return SAT_EINVAL;
     if (v > 1 && (!f32 || polish)) return SAT_EINVAL;  // FP64 and polished sweeps: one wave per pair
     if (v == 0) autow[gi] = f32 && !polish;
@@ -1481,12 +1507,14 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   }
 
   // fused path: one slot for per-solve runs; FP32 batches also FP64 slots for re-solves
-  b->sg = slot_geom(b->precision, max_seg, max_plane);
+  int max_wseg = 1;
+  for (const GridHost& G : b->grids) max_wseg = std::max(max_wseg, G.Wseg);
+  b->sg = slot_geom(b->precision, max_seg, max_plane, max_wseg);
   B_CHECK(dalloc(&b->d_ws, b->sg.slot));
   B_CHECK(hipMemset(b->d_ws, 0, b->sg.slot));  // also the all-zero FP32 delta plane
   if (f32) {
     b->nslots64 = std::max(1, std::min(ne, 64));
-    b->sg64 = slot_geom(SAT_PREC_F64, max_seg, max_plane);
+    b->sg64 = slot_geom(SAT_PREC_F64, max_seg, max_plane, max_wseg);
     B_CHECK(dalloc(&b->d_ws64, b->sg64.slot * (size_t)b->nslots64));
     B_CHECK(hipMemset(b->d_ws64, 0, b->sg64.slot * (size_t)b->nslots64));
   }
@@ -1685,8 +1713,11 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   hipEvent_t* mk = b->mk;
   // waves per pair and columns per lane of the re-solve geometry:
   // Wr = resolve_waves(C), ceil(nx / (64 Wr)) = ceil(C / Wr)
-  auto wmw = [&](size_t gi) { return resolve_waves(b->group_C[gi]); };
-  auto cmw = [&](size_t gi) { return (b->group_C[gi] + wmw(gi) - 1) / wmw(gi); };
+  // (wide grids: the batch's own W waves of kMaxC columns)
+  auto wmw = [&](size_t gi) { return b->group_wide[gi] ? b->group_W[gi] : resolve_waves(b->group_C[gi]); };
+  auto cmw = [&](size_t gi) { return b->group_wide[gi] ? kMaxC : (b->group_C[gi] + wmw(gi) - 1) / wmw(gi); };
+  // waves of the one-pair kernels (pair_kernel) of a group
+  auto wseg = [&](size_t gi) { return b->group_wide[gi] ? b->group_W[gi] : 1; };
 
   auto mark = [&](int i, hipStream_t st) -> hipError_t {
     return capturing ? hipSuccess : hipEventRecord(mk[i], st);
@@ -1825,7 +1856,8 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
           const int* list = (phase ? b->d_ov : b->d_fb) + b->group_first[gi];
           const int* cnt = phase ? ov_count + gi : fb_count + gi;
           const int nblk = std::min(b->group_count[gi], b->nslots64);
-          SAT_CHECK(launch_pair<SAT_PREC_F64>(1, b->group_C[gi], dim3(nblk), s2, a, 0, 0, (int)gi, list, cnt));
+          SAT_CHECK(launch_pair<SAT_PREC_F64>(1, b->group_C[gi], wseg(gi), dim3(nblk), s2, a, 0, 0, (int)gi, list,
+                                              cnt));
         }
       }
       SAT_CHECK(hipEventRecord(mk[6], b->stream2));
@@ -1845,10 +1877,12 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
           a.counter = solo_heads + first;
           a.fb_list = bd.fb_list + b->group_first[gi];
           if (!f32)
-            SAT_CHECK(launch_pair<SAT_PREC_F64>(0, b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi, nullptr,
+            SAT_CHECK(launch_pair<SAT_PREC_F64>(0, b->group_C[gi], wseg(gi), dim3(1), b->stream, a, first, 1, (int)gi,
+                                                nullptr,
                                                 nullptr));
           else
-            SAT_CHECK(launch_pair<SAT_PREC_F32>(1, b->group_C[gi], dim3(1), b->stream, a, first, 1, (int)gi, nullptr,
+            SAT_CHECK(launch_pair<SAT_PREC_F32>(1, b->group_C[gi], wseg(gi), dim3(1), b->stream, a, first, 1, (int)gi,
+                                                nullptr,
                                                 nullptr));
         }
       SAT_CHECK(mark(1, b->stream));

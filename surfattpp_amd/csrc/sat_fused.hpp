@@ -141,8 +141,24 @@ __device__ inline typename Prec<PREC>::Adj::Ctx adj_ctx(const GridDev& g, const 
   return cx;
 }
 
-// Forward solve (src/eikonal_solver.cpp:22-317) into the plain FP64 plane T = pl[0].
-template <int PREC, int C>
+// The LDS exchange of the multi-wave forward and adjoint sweeps of one
+// work-group: one buffer serves both (they run one after the other).
+template <int PREC, int W>
+__device__ inline char* fused_exchange_lds() {
+  using P = Prec<PREC>;
+  constexpr size_t nf = sizeof(ExchangeLds<typename P::Fwd, W>), na = sizeof(ExchangeLds<typename P::Adj, W>);
+  __shared__ __attribute__((aligned(16))) char buf[nf > na ? nf : na];
+  return buf;
+}
+template <int PREC, int W>
+__device__ inline double* fused_reduce_lds() {
+  __shared__ double red[W];
+  return red;
+}
+
+// Forward solve (src/eikonal_solver.cpp:22-317) into the plain FP64 plane T = pl[0],
+// by a work-group of W waves (W > 1: the lanes' edge values cross through LDS).
+template <int PREC, int C, int W = 1>
 __device__ inline IterOut solve_forward(const GridDev& g, int iper, double lon, double lat, const Slot& ws,
                                         int maxiter, int& flags) {
   using P = Prec<PREC>;
@@ -159,7 +175,10 @@ __device__ inline IterOut solve_forward(const GridDev& g, int iper, double lon, 
   __syncthreads();
   const auto cx = fwd_ctx<PREC>(g, iper, ws, fb);
   IterOut io;
-  const U* u = iterate<typename P::Fwd, SegLayout, C>(cx, Q[0], Q[1], Q[2], maxiter, kFwdTol, io);
+  auto& xch = *reinterpret_cast<ExchangeLds<typename P::Fwd, W>*>(fused_exchange_lds<PREC, W>());
+  const U* u = iterate<typename P::Fwd, SegLayout, C, W>(cx, Q[0], Q[1], Q[2], maxiter, kFwdTol, io,
+                                                          exchange_ptr<typename P::Fwd, W>(xch),
+                                                          fused_reduce_lds<PREC, W>());
   if (!io.converged) flags |= FLAG_NOT_CONVERGED;
   fwd_finish<V, U>(g, u, S[0], ws.pl[1], ws.pl[0]);
   __syncthreads();
@@ -167,7 +186,7 @@ __device__ inline IterOut solve_forward(const GridDev& g, int iper, double lon, 
 }
 
 // Adjoint solve (src/eikonal_solver.cpp:323-511) from T (plain FP64); returns Ta plane.
-template <int PREC, int C>
+template <int PREC, int C, int W = 1>
 __device__ inline const typename Prec<PREC>::V* solve_adjoint(const GridDev& g, int iper, const double* T, int nr,
                                                                const double* rlon, const double* rlat,
                                                                const double* adj, const Slot& ws, int maxiter,
@@ -181,8 +200,12 @@ __device__ inline const typename Prec<PREC>::V* solve_adjoint(const GridDev& g, 
   __syncthreads();
   const auto cx = adj_ctx<PREC>(g, ws);
   const int t0 = (PREC == SAT_PREC_F64) ? 6 : 5;
-  const V* ta = iterate<typename P::Adj, SegLayout, C>(cx, static_cast<V*>(ws.seg[t0]), static_cast<V*>(ws.seg[t0 + 1]),
-                                            static_cast<V*>(ws.seg[t0 + 2]), maxiter, kAdjTol, io);
+  auto& xch = *reinterpret_cast<ExchangeLds<typename P::Adj, W>*>(fused_exchange_lds<PREC, W>());
+  const V* ta = iterate<typename P::Adj, SegLayout, C, W>(cx, static_cast<V*>(ws.seg[t0]),
+                                                           static_cast<V*>(ws.seg[t0 + 1]),
+                                                           static_cast<V*>(ws.seg[t0 + 2]), maxiter, kAdjTol, io,
+                                                           exchange_ptr<typename P::Adj, W>(xch),
+                                                           fused_reduce_lds<PREC, W>());
   if (!io.converged) flags |= FLAG_ADJ_MAXITER;
   return ta;
 }
@@ -190,17 +213,18 @@ __device__ inline const typename Prec<PREC>::V* solve_adjoint(const GridDev& g, 
 // ----------------------------------------------------------------------------
 // One (source, period) pair: src/preproc.cpp:88-143.
 // ----------------------------------------------------------------------------
-template <int PREC, int C, typename TaT>
+template <int PREC, int C, typename TaT, int W = 1>
 __device__ void run_pair(const BatchDev& bd, int e, const Slot& ws, int group) {
   using V = typename Prec<PREC>::V;
   const int lane = threadIdx.x;
+  constexpr int NT = kWave * W;
   const EventDev ev = bd.events[e];
   const GridDev& g = bd.grids[ev.grid];
   int flags = 0;
-  const IterOut fo = solve_forward<PREC, C>(g, ev.iper, ev.lon, ev.lat, ws, bd.fwd_maxiter, flags);
+  const IterOut fo = solve_forward<PREC, C, W>(g, ev.iper, ev.lon, ev.lat, ws, bd.fwd_maxiter, flags);
   if (bd.T_out) {  // the event's T where the batched kernels leave it (an FP64 re-solve overwrites it)
     double* To = bd.T_out + ev.ta_off;
-    for (size_t p = lane; p < g.plane; p += kWave) To[p] = ws.pl[0][p];
+    for (size_t p = lane; p < g.plane; p += NT) To[p] = ws.pl[0][p];
   }
   if (lane == 0) {
     bd.conv[2 * (size_t)e] = fo.linf;
@@ -226,14 +250,14 @@ __device__ void run_pair(const BatchDev& bd, int e, const Slot& ws, int group) {
   const double* T = ws.pl[0];
   // :105-115 receivers, residuals, adjoint source
   int outside = 0;
-  for (int r = lane; r < ev.nrec; r += kWave) {
+  for (int r = lane; r < ev.nrec; r += NT) {
     const int k = ev.rec_off + r;
     const double s = interp2d_point(g.lon, g.nx, g.lat, g.ny, T, bd.rec_lon[k], bd.rec_lat[k]);
     bd.syn[k] = s;
     bd.adjsrc[k] = bd.weight[k] * (s - bd.tt_obs[k]);
     outside |= (s != s) ? 1 : 0;
   }
-  if (wave_or_i(outside)) flags |= FLAG_REC_OUTSIDE;
+  if (block_or_i(outside)) flags |= FLAG_REC_OUTSIDE;
   __syncthreads();
   int aiters = 0;
   if (bd.calc_adj) {
@@ -247,7 +271,7 @@ __device__ void run_pair(const BatchDev& bd, int e, const Slot& ws, int group) {
       bd.chi_ev[e] = 0.5 * s;
     }
     IterOut ao;
-    const V* ta = solve_adjoint<PREC, C>(g, ev.iper, T, ev.nrec, bd.rec_lon + ev.rec_off, bd.rec_lat + ev.rec_off,
+    const V* ta = solve_adjoint<PREC, C, W>(g, ev.iper, T, ev.nrec, bd.rec_lon + ev.rec_off, bd.rec_lat + ev.rec_off,
                                          bd.adjsrc + ev.rec_off, ws, bd.adj_maxiter, ao, flags);
     aiters = ao.iters;
     mask_store<V, TaT>(g, ev.lon, ev.lat, ta, static_cast<TaT*>(bd.ta_out) + ev.ta_off);  // :125
@@ -265,18 +289,26 @@ __device__ void run_pair(const BatchDev& bd, int e, const Slot& ws, int group) {
 // Persistent: each wave takes the next queue entry until the queue is empty.
 // The queue is order[first, first+count), or list[0, *dcount) when list != nullptr
 // (the FP64 re-solves of an FP32 batch, written by the preceding launch).
-template <int PREC, int C, typename TaT>
-__global__ __launch_bounds__(64) void pair_kernel(BatchDev bd, int first, int count, int group, const int* list,
-                                                  const int* dcount) {
+template <int PREC, int C, typename TaT, int W = 1>
+__global__ __launch_bounds__(64 * W) void pair_kernel(BatchDev bd, int first, int count, int group,
+                                                      const int* list, const int* dcount) {
   using P = Prec<PREC>;
   const Slot ws = make_slot(bd.ws + (size_t)blockIdx.x * bd.ws_slot, P::kSeg, P::kPlain, bd.ws_seg, bd.ws_plain);
   const int n = list ? *dcount : count;
+  __shared__ int s_idx;
   for (;;) {
     int idx = 0;
-    if (threadIdx.x == 0) idx = atomicAdd(bd.counter, 1);
-    idx = __shfl(idx, 0);
+    if constexpr (W == 1) {
+      if (threadIdx.x == 0) idx = atomicAdd(bd.counter, 1);
+      idx = __shfl(idx, 0);
+    } else {
+      if (threadIdx.x == 0) s_idx = atomicAdd(bd.counter, 1);
+      __syncthreads();
+      idx = s_idx;
+      __syncthreads();  // every thread has read it before thread 0 takes the next
+    }
     if (idx >= n) break;
-    run_pair<PREC, C, TaT>(bd, list ? list[idx] : bd.order[first + idx], ws, group);
+    run_pair<PREC, C, TaT, W>(bd, list ? list[idx] : bd.order[first + idx], ws, group);
   }
 }
 
@@ -298,28 +330,29 @@ struct SoloDev {
   double* info;            // iters, flags, l1, linf
 };
 
-template <int PREC, int C>
-__global__ __launch_bounds__(64) void solo_forward(SoloDev sd) {
+template <int PREC, int C, int W = 1>
+__global__ __launch_bounds__(64 * W) void solo_forward(SoloDev sd) {
   using P = Prec<PREC>;
   const Slot ws = make_slot(sd.ws, P::kSeg, P::kPlain, sd.ws_seg, sd.ws_plain);
   int flags = 0;
-  const IterOut io = solve_forward<PREC, C>(sd.g, 0, sd.src_lon, sd.src_lat, ws, sd.maxiter, flags);
-  for (size_t p = threadIdx.x; p < sd.g.plane; p += kWave) sd.out[p] = ws.pl[0][p];
+  const IterOut io = solve_forward<PREC, C, W>(sd.g, 0, sd.src_lon, sd.src_lat, ws, sd.maxiter, flags);
+  for (size_t p = threadIdx.x; p < sd.g.plane; p += kWave * W) sd.out[p] = ws.pl[0][p];
   if (threadIdx.x == 0) {
     sd.info[0] = io.iters; sd.info[1] = flags; sd.info[2] = io.l1; sd.info[3] = io.linf;
   }
 }
 
-template <int PREC, int C>
-__global__ __launch_bounds__(64) void solo_backward(SoloDev sd) {
+template <int PREC, int C, int W = 1>
+__global__ __launch_bounds__(64 * W) void solo_backward(SoloDev sd) {
   using P = Prec<PREC>;
   using V = typename P::V;
   const Slot ws = make_slot(sd.ws, P::kSeg, P::kPlain, sd.ws_seg, sd.ws_plain);
   int flags = 0;
   IterOut io;
-  const V* ta = solve_adjoint<PREC, C>(sd.g, 0, sd.T_in, sd.nr, sd.rlon, sd.rlat, sd.adj, ws, sd.maxiter, io, flags);
+  const V* ta = solve_adjoint<PREC, C, W>(sd.g, 0, sd.T_in, sd.nr, sd.rlon, sd.rlat, sd.adj, ws, sd.maxiter, io,
+                                          flags);
   for (int iy = 0; iy < sd.g.ny; iy++)
-    for (int ix = threadIdx.x; ix < sd.g.nx; ix += kWave)
+    for (int ix = threadIdx.x; ix < sd.g.nx; ix += kWave * W)
       sd.out[ix + (size_t)iy * sd.g.nx] = (double)ta[seg_index(sd.g, ix, iy)];
   if (threadIdx.x == 0) {
     sd.info[0] = io.iters; sd.info[1] = flags; sd.info[2] = io.l1; sd.info[3] = io.linf;

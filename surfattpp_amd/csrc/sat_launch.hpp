@@ -7,12 +7,28 @@
 
 namespace sat {
 
-// pair_kernel<PREC, C, TaT>; ta_f32: masked Ta stored as float (FP32 batches)
+// pair_kernel<PREC, C, TaT, W>; ta_f32: masked Ta stored as float (FP32 batches).
+// W = 1 (C in [1, kMaxC]) or, for grids wider than 512 columns, W = wide_waves(nx)
+// with C = kMaxC (the same for launch_solo).
 template <int PREC>
-hipError_t launch_pair(int ta_f32, int C, dim3 grid, hipStream_t s, const BatchDev& bd, int first, int count,
-                       int group, const int* list, const int* dcount);
+hipError_t launch_pair(int ta_f32, int C, int W, dim3 grid, hipStream_t s, const BatchDev& bd, int first,
+                       int count, int group, const int* list, const int* dcount);
 template <int PREC>
-hipError_t launch_solo(int forward, int C, hipStream_t s, const SoloDev& sd);
+hipError_t launch_solo(int forward, int C, int W, hipStream_t s, const SoloDev& sd);
+// Their wide-grid instantiations (k_wide64.hip, k_wide32.hip): W in {2, 4, 8}, C = kMaxC.
+template <int PREC>
+hipError_t launch_pair_wide(int ta_f32, int W, dim3 grid, hipStream_t s, const BatchDev& bd, int first, int count,
+                            int group, const int* list, const int* dcount);
+template <int PREC>
+hipError_t launch_solo_wide(int forward, int W, hipStream_t s, const SoloDev& sd);
+// The batched sweeps of wide grids: W in {2, 4, 8}, C = kMaxC; forward mode
+// kMetricExact (FP64 batches), 0 (the FP64 re-solves), kMetricGeneral or
+// kMetricIso (FP32 batches).
+template <int PREC>
+hipError_t launch_fwd_sweep_wide(int mode, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first,
+                                 int count, int group);
+template <int PREC>
+hipError_t launch_adj_sweep_wide(int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count);
 
 template <int PREC>
 hipError_t launch_trig(dim3 grid, hipStream_t s, const PhaseDev& pd);

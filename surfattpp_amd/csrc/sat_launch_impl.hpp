@@ -6,8 +6,12 @@
 namespace sat {
 
 template <int PREC>
-hipError_t launch_pair(int ta_f32, int C, dim3 grid, hipStream_t s, const BatchDev& bd, int first, int count,
-                       int group, const int* list, const int* dcount) {
+hipError_t launch_pair(int ta_f32, int C, int W, dim3 grid, hipStream_t s, const BatchDev& bd, int first,
+                       int count, int group, const int* list, const int* dcount) {
+  if (W != 1) {
+    if (C != kMaxC) return hipErrorInvalidValue;
+    return launch_pair_wide<PREC>(ta_f32, W, grid, s, bd, first, count, group, list, dcount);
+  }
   return with_C(C, [&]<int CC>() {
     if (ta_f32)
       hipLaunchKernelGGL((pair_kernel<PREC, CC, float>), grid, dim3(kWave), 0, s, bd, first, count, group, list,
@@ -19,7 +23,11 @@ hipError_t launch_pair(int ta_f32, int C, dim3 grid, hipStream_t s, const BatchD
 }
 
 template <int PREC>
-hipError_t launch_solo(int forward, int C, hipStream_t s, const SoloDev& sd) {
+hipError_t launch_solo(int forward, int C, int W, hipStream_t s, const SoloDev& sd) {
+  if (W != 1) {
+    if (C != kMaxC) return hipErrorInvalidValue;
+    return launch_solo_wide<PREC>(forward, W, s, sd);
+  }
   return with_C(C, [&]<int CC>() {
     if (forward) hipLaunchKernelGGL((solo_forward<PREC, CC>), dim3(1), dim3(kWave), 0, s, sd);
     else hipLaunchKernelGGL((solo_backward<PREC, CC>), dim3(1), dim3(kWave), 0, s, sd);
@@ -73,6 +81,7 @@ static void launch_fwd_mw1(dim3 grid, hipStream_t s, const PhaseDev& pd, int fir
 template <int PREC>
 hipError_t launch_fwd_sweep_mw(int iso, int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first,
                                int count, int group) {
+  if (C == kMaxC) return launch_fwd_sweep_wide<PREC>(iso, W, grid, s, pd, first, count, group);
   if constexpr (PREC == SAT_PREC_F64) {
     if (iso) return hipErrorInvalidValue;
     if (W == kMultiWave && C == 1) launch_fwd_mw1<PREC, kMultiWave, 1, kMetricGeneral>(grid, s, pd, first, count, group);
@@ -120,6 +129,7 @@ hipError_t launch_adj_sweep(int C, int W, dim3 grid, hipStream_t s, const PhaseD
 
 template <int PREC>
 hipError_t launch_adj_sweep_mw(int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count) {
+  if (C == kMaxC) return launch_adj_sweep_wide<PREC>(W, grid, s, pd, first, count);
   auto go = [&]<int WW, int CC>() {
     hipLaunchKernelGGL((adj_sweep_kernel<PREC, CC, WW>), grid, dim3(kWave * WW), 0, s, pd, first, count);
   };
@@ -143,9 +153,9 @@ hipError_t launch_adj_sweep_mw(int C, int W, dim3 grid, hipStream_t s, const Pha
 }  // namespace sat
 
 #define SAT_INSTANTIATE_PAIR(P)                                                                            \
-  template hipError_t sat::launch_pair<P>(int, int, dim3, hipStream_t, const BatchDev&, int, int, int,     \
+  template hipError_t sat::launch_pair<P>(int, int, int, dim3, hipStream_t, const BatchDev&, int, int, int, \
                                           const int*, const int*);                                         \
-  template hipError_t sat::launch_solo<P>(int, int, hipStream_t, const SoloDev&);
+  template hipError_t sat::launch_solo<P>(int, int, int, hipStream_t, const SoloDev&);
 
 // the multi-wave launchers are instantiated in their own translation units
 #define SAT_INSTANTIATE_PHASE(P)                                                                           \

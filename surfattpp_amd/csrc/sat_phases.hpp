@@ -129,10 +129,12 @@ __device__ __forceinline__ void latency_critical_priority() {
 constexpr int kRingDepthC1 = SAT_RING_DEPTH_C1;
 template <int PREC, int C, int W>
 constexpr int mw_fwd_depth() {
+  if (C >= 5) return 1;  // the wide grids' kMaxC columns per lane: the head/body pipeline
   return (PREC == SAT_PREC_F32 && C >= 3) ? 2 : (PREC == SAT_PREC_F64 && C == 1 && W <= 4) ? kRingDepthC1 : kRingDepth;
 }
 template <int PREC, int C, int W>
 constexpr int mw_adj_depth() {
+  if (C >= 5) return 1;
   return (PREC == SAT_PREC_F64 && C == 1 && W <= 4) ? kRingDepthC1 : kRingDepth;
 }
 
@@ -274,7 +276,7 @@ __global__ void init_kernel(PhaseDev pd) {
   if (pass < 2) {
     const size_t w = ev.w_off + idx;
     static_cast<V*>(pd.t0[pass])[w] = (V)T0;
-    if (PREC == SAT_PREC_F32 && pd.polish && g.iso) {
+    if (PREC == SAT_PREC_F32 && pd.polish && g.iso && g.wf64[0]) {
       // the polished isotropic FP32 sweeps' scaled gradient, FP64: sqrt(a) T0x, sqrt(b) T0y
       static_cast<double*>(pd.t0x[pass])[w] = T0x / (kR * cyy);
       static_cast<double*>(pd.t0y[pass])[w] = T0y / kR;
@@ -290,11 +292,14 @@ __global__ void init_kernel(PhaseDev pd) {
 // Metric modes of the forward sweeps: the general metric, the isotropic metric
 // (spha = sphb = 1, sphc = 0), and the isotropic metric with the FP64-polished
 // FP32 update (SAT_PREC_F32P batches).
-constexpr int kMetricGeneral = 0, kMetricIso = 1, kMetricIsoPolish = 2;
+// kMetricExact: FP64 multi-wave sweeps of FP64 batches over grids wider than
+// 512 columns, with the reference-order update (the other FP64 multi-wave
+// sweeps are the re-solves' reciprocal-based update).
+constexpr int kMetricGeneral = 0, kMetricIso = 1, kMetricIsoPolish = 2, kMetricExact = 3;
 
 template <int PREC, int MODE, class Pol>
 __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const EventX& ev) {
-  constexpr bool ISO = MODE != kMetricGeneral;
+  constexpr bool ISO = MODE == kMetricIso || MODE == kMetricIsoPolish;
   constexpr bool POL = PREC == SAT_PREC_F32 && MODE == kMetricIsoPolish;
   using PT = PhaseT<PREC>;
   using V = typename PT::V;
@@ -363,7 +368,7 @@ void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
   using U = typename PT::U;
   // multi-wave FP64 launches are the re-solves of FP32 batches (FwdF64R)
   using Pol = std::conditional_t<
-      (PREC == SAT_PREC_F64 && W > 1), FwdF64R,
+      (PREC == SAT_PREC_F64 && W > 1 && MODE != kMetricExact), FwdF64R,
       std::conditional_t<MODE == kMetricIsoPolish, typename PT::FwdIsoP,
                          std::conditional_t<MODE == kMetricIso, typename PT::FwdIso, typename PT::Fwd>>>;
   const int lane = threadIdx.x;

@@ -803,28 +803,30 @@ __device__ inline FixedBox fwd_init(const GridDev& g, int iper, double src_lon, 
 
   // The fixed set {|(xx-x0)/dx| <= 1} x {|(yy-y0)/dy| <= 1} is an index box
   // (the test is monotone in the coordinate).
+  const int nt = blockDim.x;  // 64 per wave of the work-group
   int bx0 = 1 << 30, bx1 = -1, by0 = 1 << 30, by1 = -1;
-  for (int ix = lane; ix < nx; ix += kWave)
+  for (int ix = lane; ix < nx; ix += nt)
     if (fabs((g.xx[ix] - x0) / dx) <= 1.0) { bx0 = imin(bx0, ix); bx1 = imax(bx1, ix); }
-  for (int iy = lane; iy < ny; iy += kWave)
+  for (int iy = lane; iy < ny; iy += nt)
     if (fabs((g.yy[iy] - y0) / dy) <= 1.0) { by0 = imin(by0, iy); by1 = imax(by1, iy); }
   FixedBox fb;
-  fb.x0 = wave_min_i(bx0);
-  fb.x1 = wave_max_i(bx1);
-  fb.y0 = wave_min_i(by0);
-  fb.y1 = wave_max_i(by1);
+  fb.x0 = block_min_i(bx0);
+  fb.x1 = block_max_i(bx1);
+  fb.y0 = block_min_i(by0);
+  fb.y1 = block_max_i(by1);
   if (fb.x0 <= fb.x1 && fb.y0 <= fb.y1 && (fb.x0 == 0 || fb.x1 == nx - 1 || fb.y0 == 0 || fb.y1 == ny - 1))
     flags |= FLAG_SRC_ON_BOUNDARY;
 
   constexpr bool kSigma = sizeof(V) == 4;
   // Correctly rounded sin/cos/acos (sat_crmath.hpp): the same bits as any
   // correctly rounded CPU evaluation.  sin/cos of the longitude offsets are
-  // evaluated once per column (a lane owns at most kMaxCols columns).
-  constexpr int kMaxCols = 8;
+  // evaluated once per column (a thread owns at most kMaxCols columns: the
+  // segmented pitch is at most kMaxC columns per lane of the work-group).
+  constexpr int kMaxCols = kMaxC;
   double sdl[kMaxCols], cdl[kMaxCols];
 #pragma unroll
   for (int m = 0; m < kMaxCols; m++) {
-    const int q = lane + kWave * m;
+    const int q = lane + nt * m;
     sdl[m] = 0.0;
     cdl[m] = 1.0;
     if (q < nx) cr::sincos_cr(g.xx[q] - x0, sdl[m], cdl[m]);
@@ -836,7 +838,7 @@ __device__ inline FixedBox fwd_init(const GridDev& g, int iper, double src_lon, 
     cr::sincos_cr(g.yy[iy], syy, cyy);
 #pragma unroll
     for (int m = 0; m < kMaxCols; m++) {
-      const int q = lane + kWave * m;
+      const int q = lane + nt * m;
       if (q >= P) continue;
       const size_t o = (size_t)iy * P + q;
       if (q >= nx) {
@@ -871,7 +873,7 @@ template <typename V, typename U>
 __device__ inline void fwd_finish(const GridDev& g, const U* u, const V* t0s, const double* T0d, double* T) {
   const int nx = g.nx, ny = g.ny;
   for (int iy = 0; iy < ny; iy++)
-    for (int ix = threadIdx.x; ix < nx; ix += kWave) {
+    for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
       const int o = seg_index(g, ix, iy);
       const size_t p = ix + (size_t)iy * nx;
       if (sizeof(V) == 8) T[p] = (double)u[o] * (double)t0s[o];
@@ -959,7 +961,7 @@ __device__ inline void adj_setup_f64(const GridDev& g, int iper, const double* _
 #pragma unroll
   for (int i = 0; i < 7; i++) S[i] = static_cast<double*>(ws.seg[i]);
   for (int iy = 0; iy < ny; iy++)
-    for (int q = threadIdx.x; q < P; q += kWave) {
+    for (int q = threadIdx.x; q < P; q += blockDim.x) {
       const size_t o = (size_t)iy * P + q;
       double v[7] = {0, 0, 0, 0, 0, 0, 0};
       if (q >= 1 && q < nx - 1 && iy >= 1 && iy < ny - 1) {
@@ -986,7 +988,7 @@ __device__ inline void adj_setup_f32(const GridDev& g, int iper, const double* _
   for (int i = 0; i < 6; i++) S[i] = static_cast<float*>(ws.seg[i]);
   double* dpl = ws.pl[2];
   for (int iy = 0; iy < ny; iy++)
-    for (int q = threadIdx.x; q < P; q += kWave) {
+    for (int q = threadIdx.x; q < P; q += blockDim.x) {
       const size_t o = (size_t)iy * P + q;
       float v[6] = {0, 0, 0, 0, 0, 0};
       if (q >= 1 && q < nx - 1 && iy >= 1 && iy < ny - 1) {
@@ -1067,7 +1069,7 @@ __device__ inline void mask_store(const GridDev& g, double src_lon, double src_l
   const int iy_lo = imax(0, (int)floor((src_lat - 1.5 * dyd - g.lat[0]) / dyd) + 1);
   const int iy_hi = imin(ny - 1, (int)ceil((src_lat + 1.5 * dyd - g.lat[0]) / dyd) - 1);
   for (int iy = 0; iy < ny; iy++)
-    for (int ix = threadIdx.x; ix < nx; ix += kWave) {
+    for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
       const bool m = ix >= ix_lo && ix <= ix_hi && iy >= iy_lo && iy <= iy_hi;
       out[ix + (size_t)iy * nx] = m ? O(0) : (O)ta[seg_index(g, ix, iy)];
     }

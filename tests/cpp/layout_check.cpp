@@ -77,8 +77,8 @@ static void check_loads(const SweepGeom& g, int nlanes, long plane) {
 }
 
 template <class Lay, int C, int XD, int YD, int OOUT, class V>
-static void check_stores(const SweepGeom& g, int nlanes, long plane) {
-  std::vector<unsigned char> hit((size_t)plane + kTrashElems, 0);
+static void check_stores(const SweepGeom& g, int nlanes, long plane, long trash = kTrashElems) {
+  std::vector<unsigned char> hit((size_t)plane + trash, 0);
   long live_n = 0;
   for (int lane = 0; lane < nlanes; lane++) {
     const SweepAddr<XD, YD> sa(g, lane);
@@ -112,7 +112,7 @@ static void check_stores(const SweepGeom& g, int nlanes, long plane) {
             hit[e] = 1;
             live_n++;
           } else {
-            REQUIRE(e >= plane && e < plane + kTrashElems, "trash store lane %d k %d at %ld", lane, k, e);
+            REQUIRE(e >= plane && e < plane + trash, "trash store lane %d k %d at %ld", lane, k, e);
           }
         }
       }
@@ -184,6 +184,33 @@ static void check_shape_C(int nx, int ny, int W) {
   }
 }
 
+// Grids wider than 512 columns: W = wide_waves(nx) waves of kMaxC columns per
+// lane, wavefront-major (batched sweeps) and segmented (per-pair and per-solve
+// kernels, trash row of seg_trash_elems(W) elements).
+static int check_wide(int nx, int ny) {
+  const int W = wide_waves(nx);
+  if (W <= 1) return 0;
+  constexpr int C = kMaxC;
+  const int La = (nx + C - 1) / C;
+  REQUIRE(La <= kWave * W && (long)(La - 1) * C < nx, "wide La %d W %d", La, W);
+  const int nlanes = kWave * W;
+  const long wp = (long)(ny + La - 1) * La * C - g_short;
+  SweepGeom gw{nx, ny, La, La * C, (int)wp};
+  check_loads<WaveLayout, C, 1, 1>(gw, nlanes, wp);
+  check_loads<WaveLayout, C, -1, -1>(gw, nlanes, wp);
+  check_stores<WaveLayout, C, 1, -1, 1, double>(gw, nlanes, wp);
+  check_stores<WaveLayout, C, -1, 1, 0, float>(gw, nlanes, wp);
+  const int P = La * C;
+  const long sp = (long)P * ny;
+  SweepGeom gs{nx, ny, La, P, (int)sp};
+  check_loads<SegLayout, C, 1, 1>(gs, nlanes, sp);
+  check_loads<SegLayout, C, -1, 1>(gs, nlanes, sp);
+  check_stores<SegLayout, C, 1, 1, 1, double>(gs, nlanes, sp, seg_trash_elems(W) - g_short);
+  check_stores<SegLayout, C, -1, -1, 0, double>(gs, nlanes, sp, seg_trash_elems(W) - g_short);
+  check_stores<SegLayout, C, 1, -1, 1, float>(gs, nlanes, sp, seg_trash_elems(W) - g_short);
+  return 1;
+}
+
 static int check_shape(int nx, int ny, int W) {
   const int C = wave_geom(nx, ny, W).C;
   switch (C) {
@@ -216,6 +243,9 @@ int main(int argc, char** argv) {
         if (W == 1 || nx > kWave * (W / 2)) shapes += check_shape(nx, ny, W);
   for (int s : {61, 71, 101, 161, 241, 321, 401, 481})
     for (int W : {1, 2, 4, 8}) shapes += check_shape(s, s + 20, W);
+  for (int nx : {513, 520, 600, 640, 1000, 1024, 1025, 1500, 2048, 2049, 3000, 4096})
+    for (int ny : {3, 7, 64}) shapes += check_wide(nx, ny);
+  REQUIRE(wide_waves(4097) == 0 && wide_waves(512) == 1 && wide_waves(513) == 2, "wide_waves");
   std::printf("ok %ld shapes, %ld checks\n", shapes, g_checks);
   return 0;
 }
