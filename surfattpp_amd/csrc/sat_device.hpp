@@ -206,6 +206,13 @@ __device__ __forceinline__ T gld(const T* base, unsigned i) {
   return gld(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off));
 }
 
+// Element i of a plane held in LDS (a generic pointer into the work-group's
+// shared memory).
+template <class T>
+__device__ __forceinline__ T lld(const T* p, unsigned i) {
+  return ((const __attribute__((address_space(3))) T*)p)[i];
+}
+
 // A wave-uniform copy of v (readfirstlane of every 32-bit word): for descriptors
 // read from device memory that the compiler cannot prove unchanged by the
 // kernel's own stores, so it loads them into vector registers.  Scalar copies

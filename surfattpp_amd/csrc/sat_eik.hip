@@ -378,6 +378,30 @@ static int device_check() {
   return SAT_OK;
 }
 
+// Dynamic LDS a one-wave per-pair adjoint may hold (the LDS tier, sat_fused.hpp):
+// the device's LDS per work-group (read once per process) less 4 KiB for the
+// kernels' static LDS.  SURFATT_ADJ_LDS=0 turns the tier off (read per call).
+static size_t adj_lds_budget() {
+  if (const char* v = std::getenv("SURFATT_ADJ_LDS"))
+    if (std::atoi(v) == 0) return 0;
+  static const size_t budget = [] {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return (size_t)0;
+    const size_t lds = (size_t)prop.sharedMemPerBlock;
+    return lds > 4096 ? lds - 4096 : (size_t)0;
+  }();
+  return budget;
+}
+// The dynamic LDS of a one-wave launch whose adjoints run on a grid of this
+// segmented plane size: the tier's need when it fits the budget, else 0.
+static size_t adj_lds_bytes(int prec, size_t seg_plane, int W) {
+  if (W != 1) return 0;
+  const size_t need = prec == SAT_PREC_F64 ? fused_adj_lds_bytes<SAT_PREC_F64>(seg_plane)
+                                           : fused_adj_lds_bytes<SAT_PREC_F32>(seg_plane);
+  return need <= adj_lds_budget() ? need : 0;
+}
+
 template <class T>
 static hipError_t dalloc(T** p, size_t n) {
   return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1));
@@ -697,6 +721,7 @@ int solo_run(bool forward, int precision, int nx, int ny, const double* lon, con
     }
     sd.T_in = d_T; sd.nr = nr; sd.rlon = d_r; sd.rlat = d_r + nr; sd.adj = d_r + 2 * nr;
     sd.maxiter = kAdjMaxIter;
+    sd.adj_lds_bytes = adj_lds_bytes(precision, G.g.seg_plane, G.Wseg);
     SOLO_CHECK(precision == SAT_PREC_F64 ? launch_solo<SAT_PREC_F64>(0, G.g.C, G.Wseg, stream, sd)
                                          : launch_solo<SAT_PREC_F32>(0, G.g.C, G.Wseg, stream, sd));
   }
@@ -943,6 +968,9 @@ struct sat_batch {
   // per group: sweep slots [group_slot0, group_slot0 + group_nblk); launch order
   // of the groups (most work first)
   std::vector<int> group_slot0, group_nblk, launch_order;
+  // per group: dynamic LDS of its one-wave per-pair launches (the adjoint's LDS
+  // tier; 0 where no grid of the group fits), batch precision and FP64
+  std::vector<size_t> group_lds, group_lds64;
   // Chunks: consecutive ranges of the batch order whose per-event device
   // workspace fits the cap, run one after the other (one chunk when uncapped):
   // events, receiver span, groups [chunk_g0[c], chunk_g0[c+1]).
@@ -1053,6 +1081,8 @@ int build_groups(sat_batch* b) {
   b->group_count.clear();
   b->group_slot0.clear();
   b->group_nblk.clear();
+  b->group_lds.clear();
+  b->group_lds64.clear();
   b->launch_order.clear();
   b->chunk_g0.assign(1, 0);
   const int nchunk = (int)b->chunk_first.size();
@@ -1096,10 +1126,17 @@ int build_groups(sat_batch* b) {
       b->group_nblk.push_back((int)std::max(1L, std::min<long>(b->group_count[gi], share)));
       b->group_slot0.push_back(slot);
       slot += b->group_nblk.back();
+      size_t lds = 0, lds64 = 0;
       for (int k = 0; k < b->group_count[gi]; k++) {
-        const GridDev& g = b->grids[b->he[order[b->group_first[gi] + k]].grid].g;
+        const GridHost& G = b->grids[b->he[order[b->group_first[gi] + k]].grid];
+        const GridDev& g = G.g;
         work[gi - g_lo] += (double)(g.ny + g.La - 1) * g.C;
+        const int w = b->group_wide[gi] ? b->group_W[gi] : 1;  // the per-pair kernels' waves
+        lds = std::max(lds, adj_lds_bytes(b->precision, std::max(G.gseg.seg_plane, g.seg_plane), w));
+        lds64 = std::max(lds64, adj_lds_bytes(SAT_PREC_F64, std::max(G.g64.seg_plane, g.seg_plane), w));
       }
+      b->group_lds.push_back(lds);
+      b->group_lds64.push_back(lds64);
     }
     std::vector<int> lo(g_hi - g_lo);
     std::iota(lo.begin(), lo.end(), (int)g_lo);
@@ -1853,6 +1890,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
           a.ws_plain = b->sg64.plain;
           a.counter = phase ? ov_heads + gi : resf_heads + gi;
           a.f32_band = 0.0;
+          a.adj_lds_bytes = b->group_lds64[gi];
           const int* list = (phase ? b->d_ov : b->d_fb) + b->group_first[gi];
           const int* cnt = phase ? ov_count + gi : fb_count + gi;
           const int nblk = std::min(b->group_count[gi], b->nslots64);
@@ -1876,6 +1914,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
           BatchDev a = bd;
           a.counter = solo_heads + first;
           a.fb_list = bd.fb_list + b->group_first[gi];
+          a.adj_lds_bytes = b->group_lds[gi];
           if (!f32)
             SAT_CHECK(launch_pair<SAT_PREC_F64>(0, b->group_C[gi], wseg(gi), dim3(1), b->stream, a, first, 1, (int)gi,
                                                 nullptr,
@@ -2290,8 +2329,8 @@ int sat_device_info(char* buf, int buflen) {
   hipDeviceProp_t prop;
   SAT_CHECK(hipGetDeviceProperties(&prop, 0));
   if (buf && buflen > 0)
-    std::snprintf(buf, buflen, "%s | %s | CUs=%d | build=gfx950", prop.name, prop.gcnArchName,
-                  prop.multiProcessorCount);
+    std::snprintf(buf, buflen, "%s | %s | CUs=%d | LDS/block=%zu | build=gfx950", prop.name, prop.gcnArchName,
+                  prop.multiProcessorCount, (size_t)prop.sharedMemPerBlock);
   return SAT_OK;
 }
 

@@ -69,6 +69,7 @@ struct SweepGeom {
 // instead sends each idle lane to its own distant line, every step.
 struct SegLayout {
   static constexpr bool kStepRun = false;
+  static constexpr bool kLds = false;
   __device__ static __forceinline__ SweepGeom geom(const GridDev& g) {
     return SweepGeom{g.nx, g.ny, g.La, g.P, (int)g.seg_plane};
   }
@@ -79,8 +80,14 @@ struct SegLayout {
   template <int C>
   __host__ __device__ static __forceinline__ int trash(const SweepGeom& s, int lane) { return s.trash0 + lane * C; }
 };
+// SegLayout planes held in LDS (the per-pair adjoint of grids that fit, sat_fused.hpp):
+// the same addressing; the swept planes' stores go to LDS.
+struct LdsSegLayout : SegLayout {
+  static constexpr bool kLds = true;
+};
 struct WaveLayout {
   static constexpr bool kStepRun = true;
+  static constexpr bool kLds = false;
   __device__ static __forceinline__ SweepGeom geom(const GridDev& g) {
     return SweepGeom{g.nx, g.ny, g.La, g.P, (int)g.wplane};
   }
@@ -187,7 +194,11 @@ template <class Lay, int C, int OOUT, class V>
 __device__ __forceinline__ void store_row(const SweepGeom& g, V* __restrict__ tout, bool live, int j, int lane,
                                           const V (&newv)[C]) {
   const int off = store_offset<Lay, C, OOUT, V>(g, live, j, lane);
-  if constexpr (Lay::kStepRun) {
+  if constexpr (Lay::kLds) {
+    auto* p = (__attribute__((address_space(3))) V*)tout;
+#pragma unroll
+    for (int k = 0; k < C; k++) p[off + k] = newv[k];
+  } else if constexpr (Lay::kStepRun) {
     const __amdgpu_buffer_rsrc_t rs = plane_rsrc(tout, g.trash0 * (int)sizeof(V));
 #pragma unroll
     for (int k = 0; k < C; k++) bst(rs, off + k * (int)sizeof(V), newv[k]);

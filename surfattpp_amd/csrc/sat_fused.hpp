@@ -21,6 +21,7 @@ template <int PREC> struct Prec;
 template <> struct Prec<SAT_PREC_F64> {
   using Fwd = FwdF64;
   using Adj = AdjF64;
+  using AdjLds = AdjF64Lds;
   using V = double;
   static constexpr int kSeg = 9;    // T0 T0x T0y u0 u1 u2 | a1p a2m b1p b2m dl d ta0 ta1 ta2
   static constexpr int kPlain = 1;  // T
@@ -28,6 +29,7 @@ template <> struct Prec<SAT_PREC_F64> {
 template <> struct Prec<SAT_PREC_F32> {
   using Fwd = FwdF32;
   using Adj = AdjF32;
+  using AdjLds = AdjF32Lds;
   using V = float;
   static constexpr int kSeg = 8;    // t0 t0x t0y u0 u1 u2 | cW cE cS cN cD ta0 ta1 ta2
   static constexpr int kPlain = 4;  // T, T0 (FP64), d, dl (kept all-zero between solves)
@@ -71,6 +73,7 @@ struct BatchDev {
   size_t ws_plain;       // bytes per plain FP64 plane
   int calc_adj;
   int fwd_maxiter, adj_maxiter;
+  size_t adj_lds_bytes;  // > 0: the launch's dynamic LDS; one-wave adjoints take the LDS tier when they fit
 };
 
 __device__ inline Slot make_slot(char* base, int nseg, int nplain, size_t seg_bytes, size_t plain_bytes) {
@@ -210,6 +213,61 @@ __device__ inline const typename Prec<PREC>::V* solve_adjoint(const GridDev& g, 
   return ta;
 }
 
+// The LDS tier of the per-pair adjoint: for a grid whose planes fit in the
+// work-group's LDS, the adjoint sweeps read and write LDS instead of the slot
+// in global memory -- FP32: the five coefficient planes and the three swept
+// planes; FP64: the three swept planes (the coefficients stay in the slot).
+// The set-up is that of solve_adjoint (into the slot), then copied; the same
+// updates in the same order give the same bits.  Plane strides are rounded to
+// 4 elements; each swept plane carries the trash row of the idle lanes' stores.
+__host__ __device__ inline size_t lds_round4(size_t n) { return (n + 3) / 4 * 4; }
+__device__ inline char* fused_dyn_lds() {
+  extern __shared__ __attribute__((aligned(16))) char sat_fused_dyn[];
+  return sat_fused_dyn;
+}
+template <int PREC>
+__host__ __device__ inline size_t fused_adj_lds_bytes(size_t seg_plane) {
+  const size_t sr = lds_round4(seg_plane), st = lds_round4(seg_plane + seg_trash_elems(1));
+  return PREC == SAT_PREC_F64 ? 3 * st * sizeof(double) : (5 * sr + 3 * st) * sizeof(float);
+}
+
+template <int PREC, int C>
+__device__ inline const typename Prec<PREC>::V* solve_adjoint_lds(const GridDev& g, int iper, const double* T,
+                                                                   int nr, const double* rlon, const double* rlat,
+                                                                   const double* adj, const Slot& ws, int maxiter,
+                                                                   IterOut& io, int& flags, char* lds) {
+  using P = Prec<PREC>;
+  using V = typename P::V;
+  if constexpr (PREC == SAT_PREC_F64) adj_setup_f64(g, iper, T, ws);
+  else adj_setup_f32(g, iper, T, ws);
+  __syncthreads();
+  adj_delta<V>(g, nr, rlon, rlat, adj, ws);
+  __syncthreads();
+  const size_t S = g.seg_plane, sr = lds_round4(S), st = lds_round4(S + seg_trash_elems(1));
+  V* L = reinterpret_cast<V*>(lds);
+  auto cx = adj_ctx<PREC>(g, ws);
+  constexpr int ncoef = (PREC == SAT_PREC_F64) ? 0 : 5;
+  V* ta = L + ncoef * sr;  // three swept planes of st elements
+  const int t0 = (PREC == SAT_PREC_F64) ? 6 : 5;
+  for (size_t q = threadIdx.x; q < S; q += blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < ncoef; i++) L[i * sr + q] = static_cast<const V*>(ws.seg[i])[q];
+    ta[q] = static_cast<const V*>(ws.seg[t0])[q];
+  }
+  if constexpr (PREC == SAT_PREC_F32) {
+    cx.cW = L;
+    cx.cE = L + sr;
+    cx.cS = L + 2 * sr;
+    cx.cN = L + 3 * sr;
+    cx.cD = L + 4 * sr;
+  }
+  __syncthreads();
+  const V* res = iterate<typename P::AdjLds, LdsSegLayout, C>(cx, ta, ta + st, ta + 2 * st, maxiter, kAdjTol, io);
+  if (!io.converged) flags |= FLAG_ADJ_MAXITER;
+  __syncthreads();
+  return res;
+}
+
 // ----------------------------------------------------------------------------
 // One (source, period) pair: src/preproc.cpp:88-143.
 // ----------------------------------------------------------------------------
@@ -271,7 +329,14 @@ __device__ void run_pair(const BatchDev& bd, int e, const Slot& ws, int group) {
       bd.chi_ev[e] = 0.5 * s;
     }
     IterOut ao;
-    const V* ta = solve_adjoint<PREC, C, W>(g, ev.iper, T, ev.nrec, bd.rec_lon + ev.rec_off, bd.rec_lat + ev.rec_off,
+    const V* ta = nullptr;
+    if constexpr (W == 1) {
+      if (bd.adj_lds_bytes > 0 && fused_adj_lds_bytes<PREC>(g.seg_plane) <= bd.adj_lds_bytes)
+        ta = solve_adjoint_lds<PREC, C>(g, ev.iper, T, ev.nrec, bd.rec_lon + ev.rec_off, bd.rec_lat + ev.rec_off,
+                                        bd.adjsrc + ev.rec_off, ws, bd.adj_maxiter, ao, flags, fused_dyn_lds());
+    }
+    if (!ta)
+      ta = solve_adjoint<PREC, C, W>(g, ev.iper, T, ev.nrec, bd.rec_lon + ev.rec_off, bd.rec_lat + ev.rec_off,
                                          bd.adjsrc + ev.rec_off, ws, bd.adj_maxiter, ao, flags);
     aiters = ao.iters;
     mask_store<V, TaT>(g, ev.lon, ev.lat, ta, static_cast<TaT*>(bd.ta_out) + ev.ta_off);  // :125
@@ -328,6 +393,7 @@ struct SoloDev {
   size_t ws_seg, ws_plain;
   int maxiter;
   double* info;            // iters, flags, l1, linf
+  size_t adj_lds_bytes;    // > 0: the launch's dynamic LDS; the adjoint takes the LDS tier when it fits
 };
 
 template <int PREC, int C, int W = 1>
@@ -349,8 +415,14 @@ __global__ __launch_bounds__(64 * W) void solo_backward(SoloDev sd) {
   const Slot ws = make_slot(sd.ws, P::kSeg, P::kPlain, sd.ws_seg, sd.ws_plain);
   int flags = 0;
   IterOut io;
-  const V* ta = solve_adjoint<PREC, C, W>(sd.g, 0, sd.T_in, sd.nr, sd.rlon, sd.rlat, sd.adj, ws, sd.maxiter, io,
-                                          flags);
+  const V* ta = nullptr;
+  if constexpr (W == 1) {
+    if (sd.adj_lds_bytes > 0 && fused_adj_lds_bytes<PREC>(sd.g.seg_plane) <= sd.adj_lds_bytes)
+      ta = solve_adjoint_lds<PREC, C>(sd.g, 0, sd.T_in, sd.nr, sd.rlon, sd.rlat, sd.adj, ws, sd.maxiter, io, flags,
+                                      fused_dyn_lds());
+  }
+  if (!ta)
+    ta = solve_adjoint<PREC, C, W>(sd.g, 0, sd.T_in, sd.nr, sd.rlon, sd.rlat, sd.adj, ws, sd.maxiter, io, flags);
   for (int iy = 0; iy < sd.g.ny; iy++)
     for (int ix = threadIdx.x; ix < sd.g.nx; ix += kWave * W)
       sd.out[ix + (size_t)iy * sd.g.nx] = (double)ta[seg_index(sd.g, ix, iy)];

@@ -67,6 +67,18 @@ hipError_t launch_adj_sweep_mw(int C, int W, dim3 grid, hipStream_t s, const Pha
 
 // Run f.template operator()<C>() for C in [1, kMaxC].
 template <class F>
+hipError_t with_C(int C, F&& f);
+
+// Lets a kernel take `bytes` of dynamic LDS beyond the 64 KiB a launch gets
+// without asking (the LDS tier of the per-pair adjoint).
+template <class K>
+hipError_t allow_dyn_lds(K* kernel, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes);
+}
+
+template <class F>
 hipError_t with_C(int C, F&& f) {
   switch (C) {
     case 1: f.template operator()<1>(); break;

@@ -12,14 +12,22 @@ hipError_t launch_pair(int ta_f32, int C, int W, dim3 grid, hipStream_t s, const
     if (C != kMaxC) return hipErrorInvalidValue;
     return launch_pair_wide<PREC>(ta_f32, W, grid, s, bd, first, count, group, list, dcount);
   }
-  return with_C(C, [&]<int CC>() {
-    if (ta_f32)
-      hipLaunchKernelGGL((pair_kernel<PREC, CC, float>), grid, dim3(kWave), 0, s, bd, first, count, group, list,
-                         dcount);
-    else if constexpr (PREC == SAT_PREC_F64)
-      hipLaunchKernelGGL((pair_kernel<PREC, CC, double>), grid, dim3(kWave), 0, s, bd, first, count, group, list,
-                         dcount);
+  const size_t lds = bd.adj_lds_bytes;  // the LDS tier of the adjoint (sat_fused.hpp)
+  hipError_t ea = hipSuccess;
+  const hipError_t e = with_C(C, [&]<int CC>() {
+    if (ta_f32) {
+      ea = allow_dyn_lds(pair_kernel<PREC, CC, float>, lds);
+      if (ea == hipSuccess)
+        hipLaunchKernelGGL((pair_kernel<PREC, CC, float>), grid, dim3(kWave), lds, s, bd, first, count, group, list,
+                           dcount);
+    } else if constexpr (PREC == SAT_PREC_F64) {
+      ea = allow_dyn_lds(pair_kernel<PREC, CC, double>, lds);
+      if (ea == hipSuccess)
+        hipLaunchKernelGGL((pair_kernel<PREC, CC, double>), grid, dim3(kWave), lds, s, bd, first, count, group,
+                           list, dcount);
+    }
   });
+  return ea != hipSuccess ? ea : e;
 }
 
 template <int PREC>
@@ -28,10 +36,17 @@ hipError_t launch_solo(int forward, int C, int W, hipStream_t s, const SoloDev& 
     if (C != kMaxC) return hipErrorInvalidValue;
     return launch_solo_wide<PREC>(forward, W, s, sd);
   }
-  return with_C(C, [&]<int CC>() {
-    if (forward) hipLaunchKernelGGL((solo_forward<PREC, CC>), dim3(1), dim3(kWave), 0, s, sd);
-    else hipLaunchKernelGGL((solo_backward<PREC, CC>), dim3(1), dim3(kWave), 0, s, sd);
+  hipError_t ea = hipSuccess;
+  const hipError_t e = with_C(C, [&]<int CC>() {
+    if (forward) {
+      hipLaunchKernelGGL((solo_forward<PREC, CC>), dim3(1), dim3(kWave), 0, s, sd);
+    } else {
+      ea = allow_dyn_lds(solo_backward<PREC, CC>, sd.adj_lds_bytes);
+      if (ea == hipSuccess)
+        hipLaunchKernelGGL((solo_backward<PREC, CC>), dim3(1), dim3(kWave), sd.adj_lds_bytes, s, sd);
+    }
   });
+  return ea != hipSuccess ? ea : e;
 }
 
 template <int PREC>

@@ -726,6 +726,49 @@ struct AdjF32 {
   }
 };
 
+// The per-pair adjoints with their swept planes in LDS (LdsSegLayout): FP32
+// with the coefficient planes in LDS too, FP64 with the coefficients read from
+// the slot in global memory.  The same updates; only the loads differ.
+struct AdjF32Lds : AdjF32 {
+  template <int C, int O>
+  __device__ static void load_head(const Ctx&, Head<C>& H, int base, int, const float* tin) {
+#pragma unroll
+    for (int k = 0; k < C; k++) { H.u[k] = lld(tin, base + k); H.aux[k] = 0.0f; }
+  }
+  template <int C, int O, bool NORM>
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const float* tst) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      B.cW[k] = lld(cx.cW, base + k);
+      B.cE[k] = lld(cx.cE, base + k);
+      B.cS[k] = lld(cx.cS, base + k);
+      B.cN[k] = lld(cx.cN, base + k);
+      B.cD[k] = lld(cx.cD, base + k);
+      B.st[k] = NORM ? lld(tst, base + k) : 0.0f;
+    }
+  }
+};
+struct AdjF64Lds : AdjF64 {
+  template <int C, int O>
+  __device__ static void load_head(const Ctx&, Head<C>& H, int base, int, const double* tin) {
+#pragma unroll
+    for (int k = 0; k < C; k++) { H.u[k] = lld(tin, base + k); H.aux[k] = 0.0; }
+  }
+  template <int C, int O, bool NORM>
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const double* tst) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      B.a1p[k] = gld(cx.a1p, base + k);
+      B.a2m[k] = gld(cx.a2m, base + k);
+      B.b1p[k] = gld(cx.b1p, base + k);
+      B.b2m[k] = gld(cx.b2m, base + k);
+      B.dl[k] = gld(cx.dl, base + k);
+      B.d[k] = gld(cx.d, base + k);
+      B.st[k] = NORM ? lld(tst, base + k) : 0.0;
+    }
+  }
+};
+
 // Batched FP32 adjoint, face form (see AdjF64Face).  The face fluxes are stored
 // pre-divided by dx and dy: g = (a1/dx, b1/dy), so d = max(gE.x,0) - min(g.x,0)
 // + max(gN.y,0) - min(g.y,0) and Ta = (delta + max(g.x,0)*Ta_W - min(gE.x,0)*Ta_E
