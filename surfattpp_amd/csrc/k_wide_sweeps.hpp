@@ -21,9 +21,11 @@ hipError_t launch_fwd_sweep_wide(int mode, int W, dim3 grid, hipStream_t s, cons
     return hipGetLastError();
   };
   if constexpr (PREC == SAT_PREC_F64) {
-    // FP64 batches: the reference-order update; the FP64 re-solves (mode 0): the reciprocal-based one
+    // FP64 batches: the reference-order update; the FP64 re-solves: the reciprocal-based
+    // one (general metric) or its isotropic form in scaled coordinates
     if (mode == kMetricExact) return by_w.template operator()<kMetricExact>();
     if (mode == kMetricGeneral) return by_w.template operator()<kMetricGeneral>();
+    if (mode == kMetricIso) return by_w.template operator()<kMetricIso>();
     return hipErrorInvalidValue;
   } else {
     if (mode == kMetricIso) return by_w.template operator()<kMetricIso>();

@@ -82,10 +82,11 @@ struct GridDev {
   const void* seg[6];
   const double2* trig_yy;  // per row: sin(yy), cos(yy), correctly rounded
   int iso;                 // spha = sphb = 1, sphc = 0 on every node of every period
-  const float4* rowtab;    // iso grids, per row: a, b, 1/sqrt(a), 1/sqrt(b) (FP32 sweeps)
   // Isotropic FP32 batched sweeps solve in scaled coordinates X = sqrt(a) Tx,
   // Y = sqrt(b) Ty (a = 1/(R cos(lat))^2, b = 1/R^2): per row
-  // rowha = sqrt(a)/dx = 1/(R cos(lat) dx), kb = sqrt(b)/dy = 1/(R dy), FP64.
+  // rowha = sqrt(a)/dx = 1/(R cos(lat) dx), kb = sqrt(b)/dy = 1/(R dy), FP64,
+  // and rowtab = {sqrt(a)/dx, sqrt(b)/dy} rounded to FP32.
+  const float2* rowtab;
   const double* rowha;
   double kb;
   // the forward media again in wavefront-major order (batched sweeps), one copy

@@ -116,20 +116,17 @@ __global__ void iso_check_kernel(const double* ma, const double* mb, const doubl
 }
 
 // Per row: correctly rounded sin/cos of the latitude; isotropic grids also the
-// FP32 row metric {a, b, 1/sqrt(a), 1/sqrt(b)} (src/eikonal_solver.cpp:55-65 with
-// spha = sphb = 1, sphc = 0).
-__global__ void row_kernel(GridDev g, double2* trig_yy, float4* rowtab, double* rowha) {
+// row metric's scale sqrt(a)/dx in FP64 and {sqrt(a)/dx, sqrt(b)/dy} in FP32
+// (src/eikonal_solver.cpp:55-65 with spha = sphb = 1, sphc = 0: a = 1/(R cos)^2,
+// b = 1/R^2).
+__global__ void row_kernel(GridDev g, double2* trig_yy, float2* rowtab, double* rowha) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
   if (iy >= g.ny) return;
   double s, c;
   cr::sincos_cr(g.yy[iy], s, c);
   trig_yy[iy] = make_double2(s, c);
   if (rowha) rowha[iy] = 1.0 / (kR * c * g.dx);  // sqrt(a)/dx, a = 1/(R cos)^2
-  if (rowtab) {
-    const double R2 = kR * kR;
-    const double a = 1.0 / (R2 * (c * c)), b = 1.0 / R2;
-    rowtab[iy] = make_float4((float)a, (float)b, (float)(1.0 / sqrt(a)), (float)(1.0 / sqrt(b)));
-  }
+  if (rowtab) rowtab[iy] = make_float2((float)(1.0 / (kR * c * g.dx)), (float)(1.0 / (kR * g.dy)));
 }
 
 // adj_s[plane][node] = sum over the plane's events (batch order) of Ta_e * svel^-3
@@ -309,6 +306,68 @@ __global__ void zero_ints_kernel(int* p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
+// Longest pairs first.  Block g reorders launch group g's queue,
+// order[rng[g].x, rng[g].x + rng[g].y), by the events' rounds in the previous
+// run (key[e]: the forward or the adjoint iteration count, 0 before the first
+// run), most rounds first, equal rounds keeping their previous order (a stable
+// counting sort).  The persistent sweep kernels pop their queue in this order:
+// the pairs that took the most rounds start first, one per SIMD, and the
+// shorter ones fill in behind them (longest-processing-time-first), so the
+// SIMDs' shares of sweep steps come out even.  No result depends on it: every
+// pair is solved from its own start values wherever it runs, and the kernel
+// and chi sums run in batch order.
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderBuckets = 16;  // 15 and more rounds share the first bucket
+__global__ __launch_bounds__(kOrderThreads) void order_kernel(int* order, int* tmp, const int* key,
+                                                              const int2* rng) {
+  constexpr int kWaves = kOrderThreads / kWave;
+  __shared__ int hist[kOrderBuckets], start[kOrderBuckets];
+  __shared__ int wcnt[kWaves][kOrderBuckets];
+  const int first = rng[blockIdx.x].x, n = rng[blockIdx.x].y;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  auto bucket = [&](int e) { return kOrderBuckets - 1 - imin(imax(key[e], 0), kOrderBuckets - 1); };
+  if (tid < kOrderBuckets) hist[tid] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += kOrderThreads) {
+    const int e = order[first + i];
+    tmp[first + i] = e;
+    atomicAdd(&hist[bucket(e)], 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int s = 0;
+    for (int c = 0; c < kOrderBuckets; c++) {
+      start[c] = s;
+      s += hist[c];
+    }
+  }
+  __syncthreads();
+  for (int base = 0; base < n; base += kOrderThreads) {
+    const int i = base + tid;
+    const int e = i < n ? tmp[first + i] : -1;
+    const int c = e >= 0 ? bucket(e) : kOrderBuckets;
+    int rank = 0;
+    for (int b = 0; b < kOrderBuckets; b++) {  // rank among this wave's lanes of bucket b
+      const unsigned long long m = __ballot(c == b);
+      if (lane == 0) wcnt[wv][b] = __popcll(m);
+      if (c == b) rank = __popcll(m & ((1ull << lane) - 1ull));
+    }
+    __syncthreads();
+    if (e >= 0) {
+      int pos = start[c] + rank;
+      for (int w = 0; w < wv; w++) pos += wcnt[w][c];
+      order[first + pos] = e;
+    }
+    __syncthreads();
+    if (tid < kOrderBuckets) {
+      int s = 0;
+      for (int w = 0; w < kWaves; w++) s += wcnt[w][tid];
+      start[tid] += s;
+    }
+    __syncthreads();
+  }
+}
+
 // One block of kChiThreads: each pass stages kChiThreads per-event terms in LDS
 // with one coalesced load, and thread 0 adds them in event order (the
 // reference's sequential sum; the loads of a pass are all in flight at once).
@@ -418,7 +477,7 @@ struct GridHost {
   char* d_seg64 = nullptr;    // FP32 batches: the FP64 media for the re-solves
   GridDev g64{};              // FP32 batches: g with the FP64 media
   double2* d_trig_yy = nullptr;
-  float4* d_rowtab = nullptr;
+  float2* d_rowtab = nullptr;
   double* d_rowha = nullptr;
   int* d_flag = nullptr;      // iso_check_kernel's result
   double* d_wf64 = nullptr;   // FP32 batches: FP64 slowness, wavefront-major, per orientation
@@ -825,7 +884,7 @@ int forward_noalloc(int precision, int nx, int ny, const double* lon, const doub
   auto take = [&](size_t bytes) { const size_t o = off; off += round_up(bytes, 256); return o; };
   const size_t o_xx = take(nx * sizeof(double)), o_yy = take(ny * sizeof(double));
   const size_t o_meta = take(4 * sizeof(double)), o_info = take(4 * sizeof(double));
-  const size_t o_trig = take(ny * sizeof(double2)), o_row = take(ny * sizeof(float4));
+  const size_t o_trig = take(ny * sizeof(double2)), o_row = take(ny * sizeof(float2));
   size_t o_pl[4], o_seg[6];
   for (int i = 0; i < 4; i++) o_pl[i] = take(N * sizeof(double));
   for (int i = 0; i < 6; i++) o_seg[i] = take(S * es);
@@ -863,12 +922,12 @@ int forward_noalloc(int precision, int nx, int ny, const double* lon, const doub
   g.f_plain = pl[0]; g.aA_plain = pl[1]; g.b_plain = pl[2]; g.c_plain = pl[3]; g.vinv3 = nullptr;
   for (int i = 0; i < 6; i++) g.seg[i] = at(o_seg[i]);
   g.trig_yy = reinterpret_cast<const double2*>(at(o_trig));
-  g.rowtab = reinterpret_cast<const float4*>(at(o_row));
+  g.rowtab = reinterpret_cast<const float2*>(at(o_row));
   g.iso = 0;
   g.wplane = (size_t)(ny + L.La - 1) * L.La * L.C;
   for (int i = 0; i < 6; i++) g.wmed[i][0] = g.wmed[i][1] = nullptr;
   hipLaunchKernelGGL(row_kernel, dim3((ny + 63) / 64), dim3(64), 0, stream, g, reinterpret_cast<double2*>(at(o_trig)),
-                     reinterpret_cast<float4*>(at(o_row)), nullptr);
+                     reinterpret_cast<float2*>(at(o_row)), nullptr);
   SAT_CHECK(hipGetLastError());
   {  // media mode 1: (spha, sphb, sphc, fun) as FSM_UW_PS_lonlat_2d takes them
     const int blocks = (int)std::min<size_t>((S + 255) / 256, 65535);
@@ -917,7 +976,11 @@ struct sat_batch {
   GridDev* d_grids64mw = nullptr;  // FP32 batches: FP64 media, re-solve geometry (batched re-solves)
   EventDev* d_events = nullptr;
   EventX* d_evx = nullptr;
-  int* d_order = nullptr;
+  int* d_order = nullptr;          // per launch group: the forward queue (event ids)
+  int* d_order_adj = nullptr;      // and the adjoint queue, each longest pairs first (order_kernel)
+  int* d_order_tmp = nullptr;
+  int2* d_group_rng = nullptr;     // per launch group: first, count (positions in the queues)
+  bool order_by_rounds = true;     // SURFATT_ORDER=0: queues in batch order (grouped), never reordered
   int* d_counter = nullptr;        // re-solve counts | fwd heads | adj heads | re-solve heads (one per group),
                                    // then one head per event for per-solve launches
   double* d_rec = nullptr;         // rec_lon | rec_lat | tt | w | syn | adjsrc
@@ -1015,7 +1078,7 @@ struct sat_batch {
       if (p) (void)hipFree(p);
     d_aniso.clear();
     if (d_Kden) (void)hipFree(d_Kden);
-    void* ps[] = {d_grids, d_grids_fwd, d_grids_seg, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
+    void* ps[] = {d_grids, d_grids_fwd, d_grids_seg, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_order_adj, d_order_tmp, d_group_rng, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
                   d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_ov, d_evx64, d_work64, d_slots64, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
                   d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3};
     for (void* p : ps)
@@ -1144,7 +1207,13 @@ int build_groups(sat_batch* b) {
     b->launch_order.insert(b->launch_order.end(), lo.begin(), lo.end());
     b->chunk_g0.push_back((int)g_hi);
   }
-  if (ne) SAT_CHECK(hipMemcpy(b->d_order, order.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+  if (ne) {
+    SAT_CHECK(hipMemcpy(b->d_order, order.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+    SAT_CHECK(hipMemcpy(b->d_order_adj, order.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+    std::vector<int2> rng(b->group_first.size());
+    for (size_t gi = 0; gi < rng.size(); gi++) rng[gi] = make_int2(b->group_first[gi], b->group_count[gi]);
+    if (!rng.empty()) SAT_CHECK(hipMemcpy(b->d_group_rng, rng.data(), sizeof(int2) * rng.size(), hipMemcpyHostToDevice));
+  }
   std::vector<GridDev> hg(b->grids.size()), hg64(b->grids.size()), hgmw(b->grids.size()), hgs(b->grids.size());
   std::vector<GridDev> hgf(b->grids.size());
   for (size_t gi = 0; gi < b->grids.size(); gi++) {
@@ -1461,6 +1530,10 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(dalloc(&b->d_base_flags, std::max(ne, 1)));
   B_CHECK(hipMemcpy(b->d_base_flags, b->base_flags.data(), sizeof(int) * std::max(ne, 1), hipMemcpyHostToDevice));
   B_CHECK(dalloc(&b->d_order, ne));
+  B_CHECK(dalloc(&b->d_order_adj, ne));
+  B_CHECK(dalloc(&b->d_order_tmp, ne));
+  B_CHECK(dalloc(&b->d_group_rng, std::max(ne, 1)));  // a group holds at least one event
+  if (const char* v = std::getenv("SURFATT_ORDER")) b->order_by_rounds = std::atoi(v) != 0;
 
   // receivers
   const size_t nr = (size_t)d->nrec_total;
@@ -1480,6 +1553,8 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(hipMemset(b->d_chi_ev, 0, sizeof(double) * std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_chi, 1));
   B_CHECK(dalloc(&b->d_iters, 3 * (size_t)std::max(ne, 1)));
+  // no rounds before the first run: the first reorder keeps the grouped batch order
+  B_CHECK(hipMemset(b->d_iters, 0, sizeof(int) * 3 * (size_t)std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_conv, 2 * (size_t)std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_skip, std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_fb, (size_t)std::max(ne, 1)));
@@ -1739,6 +1814,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   pd.slot_bytes = b->slot_bytes;
   pd.slot_plane = b->slot_plane;
   pd.order = b->d_order;
+  pd.order_adj = b->d_order_adj;
   pd.fwd_maxiter = kFwdMaxIter;
   pd.adj_maxiter = kAdjMaxIter;
   pd.polish = b->polish ? 1 : 0;
@@ -1759,6 +1835,15 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   auto mark = [&](int i, hipStream_t st) -> hipError_t {
     return capturing ? hipSuccess : hipEventRecord(mk[i], st);
   };
+  // the queues, longest pairs of the previous run first (its rounds are still in
+  // iters_f / iters_a until reset_kernel clears them)
+  if (b->order_by_rounds && ng && ne) {
+    hipLaunchKernelGGL(order_kernel, dim3((unsigned)ng), dim3(kOrderThreads), 0, b->stream, b->d_order,
+                       b->d_order_tmp, bd.iters_f, b->d_group_rng);
+    hipLaunchKernelGGL(order_kernel, dim3((unsigned)ng), dim3(kOrderThreads), 0, b->stream, b->d_order_adj,
+                       b->d_order_tmp, bd.iters_a, b->d_group_rng);
+    SAT_CHECK(hipGetLastError());
+  }
   hipLaunchKernelGGL(reset_kernel, dim3((unsigned)std::min<size_t>((n_counters + ne1 + 255) / 256, 4096)), dim3(256),
                      0, b->stream, b->d_counter, n_counters, b->d_skip, b->d_iters, bd.flags, b->d_base_flags, ne1);
   SAT_CHECK(hipGetLastError());
@@ -1834,6 +1919,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
           q.slot_plane = b->slot_plane64;
           q.want_skip = 1;
           q.ta_f32 = 1;
+          q.scaled_iso = 1;
           // the groups' re-solves run concurrently; each takes a share of the
           // re-solve slots in proportion to its events, as the main sweeps do
           auto res_slots = [&](int gi, PhaseDev& a, int ub) {
@@ -1852,7 +1938,9 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
             SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(init_bx, ub, 2), st, a));
             a.counter = resf_heads + gi;
             const dim3 grid = res_slots(gi, a, ub);
-            SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(0, cmw(gi), wmw(gi), grid, st, a, 0, 0, gi));
+            // isotropic grids: the scaled update (init_kernel wrote scaled gradient planes)
+            const int rmode = b->group_iso[gi] != kMetricGeneral ? kMetricIso : kMetricGeneral;
+            SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(rmode, cmw(gi), wmw(gi), grid, st, a, 0, 0, gi));
             return SAT_OK;
           });
           if (rc) return rc;

@@ -98,12 +98,17 @@ hipError_t launch_fwd_sweep_mw(int iso, int C, int W, dim3 grid, hipStream_t s, 
                                int count, int group) {
   if (C == kMaxC) return launch_fwd_sweep_wide<PREC>(iso, W, grid, s, pd, first, count, group);
   if constexpr (PREC == SAT_PREC_F64) {
-    if (iso) return hipErrorInvalidValue;
-    if (W == kMultiWave && C == 1) launch_fwd_mw1<PREC, kMultiWave, 1, kMetricGeneral>(grid, s, pd, first, count, group);
-    else if (W == kMultiWave && C == 2)
-      launch_fwd_mw1<PREC, kMultiWave, 2, kMetricGeneral>(grid, s, pd, first, count, group);
+    // the re-solves of FP32 batches: the general metric or the isotropic one in
+    // scaled coordinates (FwdF64RIso)
+    if (iso != kMetricGeneral && iso != kMetricIso) return hipErrorInvalidValue;
+    auto go = [&]<int WW, int CC>() {
+      if (iso == kMetricIso) launch_fwd_mw1<PREC, WW, CC, kMetricIso>(grid, s, pd, first, count, group);
+      else launch_fwd_mw1<PREC, WW, CC, kMetricGeneral>(grid, s, pd, first, count, group);
+    };
+    if (W == kMultiWave && C == 1) go.template operator()<kMultiWave, 1>();
+    else if (W == kMultiWave && C == 2) go.template operator()<kMultiWave, 2>();
     else if (W == kMultiWaveWide && C == 1)  // grids of 257..384 columns (resolve_waves)
-      launch_fwd_mw1<PREC, kMultiWaveWide, 1, kMetricGeneral>(grid, s, pd, first, count, group);
+      go.template operator()<kMultiWaveWide, 1>();
     else return hipErrorInvalidValue;
   } else {
     if (iso == kMetricIsoPolish) return hipErrorInvalidValue;  // polished sweeps: one wave per pair

@@ -87,9 +87,12 @@ struct PhaseDev {
   const int* list;      // non-null: events list[0, *list_count) (FP64 re-solve pass)
   const int* list_count;
   int ta_f32;           // masked Ta stored as float (FP64 re-solve of an FP32 batch)
+  int scaled_iso;       // FP64 re-solves of an FP32 batch: isotropic grids sweep in scaled
+                        // coordinates (gradient planes sqrt(a) T0x, sqrt(b) T0y; FwdF64RIso)
   char* slots;          // per resident wave: 3 swept planes
   size_t slot_bytes, slot_plane;
-  const int* order;
+  const int* order;      // the forward queue of each launch group (event ids)
+  const int* order_adj;  // the adjoint queue (the same events, ordered by adjoint rounds)
   int* counter;
   int fwd_maxiter, adj_maxiter;
   int polish;           // SAT_PREC_F32P batch: isotropic grids' T0 gradient planes are FP64, scaled
@@ -280,6 +283,10 @@ __global__ void init_kernel(PhaseDev pd) {
       // the polished isotropic FP32 sweeps' scaled gradient, FP64: sqrt(a) T0x, sqrt(b) T0y
       static_cast<double*>(pd.t0x[pass])[w] = T0x / (kR * cyy);
       static_cast<double*>(pd.t0y[pass])[w] = T0y / kR;
+    } else if ((PREC == SAT_PREC_F32 || pd.scaled_iso) && g.iso) {
+      // the isotropic FP32 sweeps' and FP64 re-solves' scaled gradient (FwdF32Iso, FwdF64RIso)
+      static_cast<V*>(pd.t0x[pass])[w] = (V)(T0x / (kR * cyy));
+      static_cast<V*>(pd.t0y[pass])[w] = (V)(T0y / kR);
     } else {
       static_cast<V*>(pd.t0x[pass])[w] = (V)T0x;
       static_cast<V*>(pd.t0y[pass])[w] = (V)T0y;
@@ -301,6 +308,7 @@ template <int PREC, int MODE, class Pol>
 __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const EventX& ev) {
   constexpr bool ISO = MODE == kMetricIso || MODE == kMetricIsoPolish;
   constexpr bool POL = PREC == SAT_PREC_F32 && MODE == kMetricIsoPolish;
+  constexpr bool RISO = std::is_same_v<Pol, FwdF64RIso>;  // FP64 re-solve, isotropic, scaled
   using PT = PhaseT<PREC>;
   using V = typename PT::V;
   typename Pol::Ctx cx;
@@ -316,7 +324,7 @@ __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const
     cx.t0y[o] = static_cast<const G*>(pd.t0y[o]) + ev.w_off;
     if constexpr (POL) cx.f[o] = g.wf64[o] + po;
     else cx.f[o] = static_cast<const V*>(g.wmed[3][o]) + po;
-    if constexpr (!(PREC == SAT_PREC_F32 && ISO)) {
+    if constexpr (!(PREC == SAT_PREC_F32 && ISO) && !RISO) {
       cx.a[o] = static_cast<const V*>(g.wmed[0][o]) + po;
       cx.b[o] = static_cast<const V*>(g.wmed[1][o]) + po;
       cx.c[o] = static_cast<const V*>(g.wmed[2][o]) + po;
@@ -326,7 +334,10 @@ __device__ inline auto phase_fwd_ctx(const PhaseDev& pd, const GridDev& g, const
       cx.dsy[o] = static_cast<const V*>(g.wmed[5][o]) + po;
     }
   }
-  if constexpr (PREC == SAT_PREC_F64) {
+  if constexpr (RISO) {
+    cx.rowha = g.rowha;
+    cx.kb = g.kb;
+  } else if constexpr (PREC == SAT_PREC_F64) {
     cx.dx = g.dx;
     cx.dy = g.dy;
     if constexpr (std::is_same_v<Pol, FwdF64R>) {
@@ -368,7 +379,8 @@ void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
   using U = typename PT::U;
   // multi-wave FP64 launches are the re-solves of FP32 batches (FwdF64R)
   using Pol = std::conditional_t<
-      (PREC == SAT_PREC_F64 && W > 1 && MODE != kMetricExact), FwdF64R,
+      (PREC == SAT_PREC_F64 && W > 1 && MODE != kMetricExact),
+      std::conditional_t<MODE == kMetricIso, FwdF64RIso, FwdF64R>,
       std::conditional_t<MODE == kMetricIsoPolish, typename PT::FwdIsoP,
                          std::conditional_t<MODE == kMetricIso, typename PT::FwdIso, typename PT::Fwd>>>;
   const int lane = threadIdx.x;
@@ -630,7 +642,7 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
   for (;;) {
     const int idx = pop_work<W>(pd.counter, &s_idx);
     if (idx >= n_work) break;
-    const int e = pd.list ? pd.list[idx] : pd.order[first + idx];
+    const int e = pd.list ? pd.list[idx] : pd.order_adj[first + idx];
     if (!pass_event(pd, e)) continue;
     const EventX ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];

@@ -116,6 +116,25 @@ __device__ __forceinline__ double sqrt_nr(double x) {
   return fma(d, h, g);
 }
 
+// min(old, smallest accepted candidate): a balanced tree of minima over the 16
+// candidates (rejected ones as +inf), then against the old value.  Accepted
+// candidates are never NaN (every acceptance test compares them), so this is
+// the value of the reference's scan (:281-283) in 5 dependent steps instead of
+// 32; the tree's minima are single v_min_f64 (no NaN reaches them), the last
+// step keeps std::min's rule (a NaN old value stays, as in the scan).
+template <int N>
+__device__ __forceinline__ double min_accepted(double old, const double (&cand)[N], const bool (&acc)[N]) {
+  static_assert((N & (N - 1)) == 0, "a power of two");
+  double m[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) m[i] = acc[i] ? cand[i] : __builtin_inf();
+#pragma unroll
+  for (int s = N / 2; s > 0; s >>= 1)
+#pragma unroll
+    for (int i = 0; i < s; i++) m[i] = __builtin_fmin(m[i], m[i + s]);
+  return dmin(old, m[0]);
+}
+
 // The candidates and acceptance tests of fwd_update_f64 (src/eikonal_solver.cpp:155-283)
 // with h = t0/dx and k = t0/dy formed as t0 * (1/dx), one reciprocal per
 // quadratic (1/(2A)), per one-point denominator and of det, and FMA contraction.
@@ -178,10 +197,7 @@ __device__ inline double fwd_update_f64r(double tau_self, double t0, double t0x,
       acc[8 + 2 * d + sgn] = ok & (tmp_tau * t0 >= tD * T0D) & (tmp_tau > tD * 0.5);
     }
   }
-  double tn = tau_self;
-#pragma unroll
-  for (int i = 0; i < 16; i++) tn = acc[i] ? dmin(tn, cand[i]) : tn;
-  return tn;
+  return min_accepted(tau_self, cand, acc);
 }
 
 struct FwdF64R : FwdF64 {
@@ -194,6 +210,118 @@ struct FwdF64R : FwdF64 {
                                   double uN, double aN, bool hN) {
     return fwd_update_f64r(h.u[k], h.aux[k], r.t0x[k], r.t0y[k], r.a[k], r.b[k], r.c[k], r.f[k], cx.idx, cx.idy,
                            hW, uW, aW, hE, uE, aE, hS, uS, aS, hN, uN, aN);
+  }
+};
+
+// The re-solve update of an isotropic metric (spha = sphb = 1, sphc = 0) in the
+// scaled coordinates of fwd_update_f32s: gradient planes sqrt(a) T0x and
+// sqrt(b) T0y, h = t0 sqrt(a)/dx, k = t0 sqrt(b)/dy.  Every quantity of
+// fwd_update_f64r scales by sqrt(a) or sqrt(b) (> 0), so each candidate and
+// each acceptance test is the same as there: eq_a, eq_b, eq_c keep their
+// values (without the cross terms, c = 0), the causality tests keep their
+// signs, and the one-point distance sqrt(f^2 b/det) is f / sqrt(a), i.e. f
+// after scaling (det = a b > 0).  About half the FP64 operations of the
+// general update, and no reciprocal or square root of det.
+__device__ inline double fwd_update_f64r_iso(double tau_self, double t0, double gx, double gy, double f, double ha,
+                                             double kb, bool hW, double tW, double T0W, bool hE, double tE,
+                                             double T0E, bool hS, double tS, double T0S, bool hN, double tN,
+                                             double T0N) {
+#pragma clang fp contract(fast)
+  const double h = t0 * ha, k = t0 * kb;
+  const double f2 = f * f;
+  double cand[16];
+  bool acc[16];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const double sx = (q < 2) ? -1.0 : 1.0;
+    const double sy = (q & 1) ? 1.0 : -1.0;
+    const bool ok0 = ((q < 2) ? hW : hE) & ((q & 1) ? hN : hS);
+    const double tX = (q < 2) ? tW : tE;
+    const double tY = (q & 1) ? tN : tS;
+    const double px = gx - sx * h;
+    const double qx = sx * h * tX;
+    const double py = gy - sy * k;
+    const double qy = sy * k * tY;
+    const double eq_a = px * px + py * py;
+    const double eq_b = 2.0 * (px * qx + py * qy);
+    const double eq_c = qx * qx + qy * qy - f2;
+    const double Delta = eq_b * eq_b - 4.0 * eq_a * eq_c;
+    const bool ok = ok0 & !(Delta < 0.0 || fabs(eq_a) < 1e-30);
+    const double sq = (Delta > 0.0) ? sqrt_nr(Delta) : 0.0;
+    const double r2a = rcp_nr(2.0 * eq_a);
+#pragma unroll
+    for (int sgn = 0; sgn < 2; sgn++) {
+      const double tmp_tau = (-eq_b + (sgn == 0 ? sq : -sq)) * r2a;
+      const double Tx = px * tmp_tau + qx;
+      const double Ty = py * tmp_tau + qy;
+      cand[2 * q + sgn] = tmp_tau;
+      acc[2 * q + sgn] = ok & (-sx * Tx >= 0.0) & (-sy * Ty >= 0.0);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const bool hd = (d == 0) ? hW : (d == 1) ? hE : (d == 2) ? hS : hN;
+    const double tD = (d == 0) ? tW : (d == 1) ? tE : (d == 2) ? tS : tN;
+    const double T0D = (d == 0) ? T0W : (d == 1) ? T0E : (d == 2) ? T0S : T0N;
+    const double g = (d < 2) ? gx : gy;
+    const double hh = (d < 2) ? h : k;
+    const double p = (d == 0 || d == 2) ? g + hh : g - hh;
+    const double qd = (d == 0 || d == 2) ? -hh * tD : hh * tD;
+    const double rp = rcp_nr(p);
+#pragma unroll
+    for (int sgn = 0; sgn < 2; sgn++) {
+      const double tmp_tau = ((sgn == 0 ? f : -f) - qd) * rp;
+      cand[8 + 2 * d + sgn] = tmp_tau;
+      acc[8 + 2 * d + sgn] = hd & (tmp_tau * t0 >= tD * T0D) & (tmp_tau > tD * 0.5);
+    }
+  }
+  return min_accepted(tau_self, cand, acc);
+}
+
+// FP64 re-solves of isotropic grids: scaled gradient planes, the slowness per
+// node and sqrt(a)/dx per row (GridDev::rowha).
+struct FwdF64RIso {
+  using V = double;
+  using A = double;
+  static constexpr bool kAux = true;
+  struct Ctx {
+    const GridDev* g;
+    const double *t0[2], *t0x[2], *t0y[2], *f[2];
+    const double* rowha;
+    double kb;
+    FixedBox fb;
+  };
+  template <int C> struct Head { double u[C], aux[C]; };
+  template <int C> struct Body { double t0x[C], t0y[C], f[C], st[C]; double ha; };
+  template <int C, int O>
+  __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
+#pragma unroll
+    for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.t0[O], base + k); }
+  }
+  template <int C, int O, bool NORM>
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int j, const double* tst) {
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      B.t0x[k] = gld(cx.t0x[O], base + k);
+      B.t0y[k] = gld(cx.t0y[O], base + k);
+      B.f[k] = gld(cx.f[O], base + k);
+      B.st[k] = NORM ? gld(tst, base + k) : 0.0;
+    }
+    B.ha = gld(cx.rowha, (unsigned)j);
+  }
+  __device__ static bool fixed(const Ctx& cx, int ix, int j) {
+    return j >= cx.fb.y0 && j <= cx.fb.y1 && ix >= cx.fb.x0 && ix <= cx.fb.x1;
+  }
+  template <int C>
+  __device__ static double update(const Ctx& cx, const Head<C>& h, const Body<C>& r, int k, int, double uW,
+                                  double aW, bool hW, double uE, double aE, bool hE, double uS, double aS, bool hS,
+                                  double uN, double aN, bool hN) {
+    return fwd_update_f64r_iso(h.u[k], h.aux[k], r.t0x[k], r.t0y[k], r.f[k], r.ha, cx.kb, hW, uW, aW, hE, uE, aE,
+                               hS, uS, aS, hN, uN, aN);
+  }
+  template <int C>
+  __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, double v) {
+    return fabs(v - r.st[k]) * h.aux[k];  // :295
   }
 };
 
@@ -462,8 +590,83 @@ struct FwdF32 {
   }
 };
 
-// Isotropic metric (spha = sphb = 1, sphc = 0 on every node): a, b per row from a
-// small table {a, b, 1/sqrt(a), 1/sqrt(b)}, only the slowness per node.
+// The FP32 update of an isotropic metric (spha = sphb = 1, sphc = 0) in the
+// scaled coordinates X = sqrt(a) Tx, Y = sqrt(b) Ty, where the node equation
+// a Tx^2 + b Ty^2 = f^2 reads X^2 + Y^2 = f^2.  The gradient planes hold
+// sqrt(a) T0x and sqrt(b) T0y, h = t0 sqrt(a)/dx and k = t0 sqrt(b)/dy (per
+// row), and every candidate of fwd_update_f32<true> keeps its value and its
+// acceptance test: px, py, Tx0, Ty0 and the roots' Tx, Ty all scale by sqrt(a)
+// or sqrt(b) (> 0, so no causality sign changes), the quadratic's
+// coefficients become Aq = px^2 + py^2 (= the reference's eq_a),
+// Bh = px X0 + py Y0 and Cq = X0^2 + Y0^2 - f^2, and the one-point distances
+// sqrt(f^2 b/det), sqrt(f^2 a/det) become f.  Aq needs no test against the
+// reference's 1e-30: it is a sum of squares, zero only where px = py = 0,
+// and there both roots give Tx = Ty = NaN, which no causality test accepts.
+__device__ __forceinline__ double fwd_update_f32s(double u_self, float t0, float t0x, float t0y, float f, float h,
+                                                  float kk, double uW, float T0W, double uE, float T0E, double uS,
+                                                  float T0S, double uN, float T0N) {
+#pragma clang fp contract(fast)
+  const float kInf = __builtin_huge_valf();
+  const double u_ref = fmin(fmin(uW, uE), fmin(uS, uN));
+  const float ts = (float)(1.0 + u_ref);
+  const float dW = (float)(uW - u_ref), dE = (float)(uE - u_ref);
+  const float dS = (float)(uS - u_ref), dN = (float)(uN - u_ref);
+  const float f2 = f * f;
+  const float pxm = t0x + h, pxp = t0x - h, pym = t0y + kk, pyp = t0y - kk;
+  const float Axm = pxm * pxm, Axp = pxp * pxp, Bym = pym * pym, Byp = pyp * pyp;
+  float best = kInf;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const bool xm = q < 2, ym = !(q & 1);
+    const float sxh = xm ? -h : h, syk = ym ? -kk : kk;
+    const float dX = xm ? dW : dE, dY = ym ? dS : dN;
+    const float px = xm ? pxm : pxp, py = ym ? pym : pyp;
+    const float dr = fminf(dX, dY);
+    const float tr = ts + dr;
+    const float X0 = t0x * tr + sxh * (dX - dr);
+    const float Y0 = t0y * tr + syk * (dY - dr);
+    const float Aq = (xm ? Axm : Axp) + (ym ? Bym : Byp);
+    const float Bh = px * X0 + py * Y0;
+    const float Cq = X0 * X0 + Y0 * Y0 - f2;
+    const float D = Bh * Bh - Aq * Cq;
+    const float sq = __builtin_amdgcn_sqrtf(D);  // NaN for D < 0: no candidate
+    const float sv = -(Bh + __builtin_copysignf(sq, Bh));
+    const float w0 = sv * __builtin_amdgcn_rcpf(Aq);
+    const float w1 = Cq * __builtin_amdgcn_rcpf(sv);
+    float cw[2];
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+      const float w = g ? w1 : w0;
+      const float X = X0 + px * w;
+      const float Y = Y0 + py * w;
+      const float m = fmaxf(xm ? -X : X, ym ? -Y : Y);
+      cw[g] = (m <= 0.0f) ? w : kInf;
+    }
+    best = fminf(best, dr + fminf(cw[0], cw[1]));
+  }
+  const float rt0 = __builtin_amdgcn_rcpf(t0);
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    const float dq = (d == 0) ? dW : (d == 1) ? dE : (d == 2) ? dS : dN;
+    const float T0n = (d == 0) ? T0W : (d == 1) ? T0E : (d == 2) ? T0S : T0N;
+    const float td = (d < 2) ? t0x : t0y;
+    const float p = (d == 0) ? pxm : (d == 1) ? pxp : (d == 2) ? pym : pyp;
+    const float rp = __builtin_amdgcn_rcpf(p);
+    const float tnd = ts + dq;
+    const float Td0 = td * tnd;
+    const float wp = (f - Td0) * rp;
+    const float wm = (-f - Td0) * rp;
+    const float thw = tnd * fmaxf(T0n * rt0 - 1.0f, -0.5f);
+    const float cp = (wp >= thw) ? wp : kInf;
+    const float cm = (wm >= thw) ? wm : kInf;
+    best = fminf(best, dq + fminf(cp, cm));
+  }
+  const double v = u_ref + (double)best;  // +inf when no candidate was accepted
+  return ((best < (float)(u_self - u_ref)) & (v < u_self)) ? v : u_self;
+}
+
+// Isotropic metric (spha = sphb = 1, sphc = 0 on every node): scaled gradient
+// planes, the slowness per node and {sqrt(a)/dx, sqrt(b)/dy} per row.
 struct FwdF32Iso {
   using V = double;
   using A = float;
@@ -471,12 +674,12 @@ struct FwdF32Iso {
   struct Ctx {
     const GridDev* g;
     const float *t0[2], *t0x[2], *t0y[2], *f[2];
-    const float4* rowtab;
+    const float2* rowtab;
     FixedBox fb;
     float idx, idy;
   };
   template <int C> struct Head { double u[C]; float aux[C]; };
-  template <int C> struct Body { double st[C]; float t0x[C], t0y[C], f[C]; float4 row; };
+  template <int C> struct Body { double st[C]; float t0x[C], t0y[C], f[C]; float2 row; };
   template <int C, int O>
   __device__ static void load_head(const Ctx& cx, Head<C>& H, int base, int, const double* tin) {
 #pragma unroll
@@ -491,7 +694,7 @@ struct FwdF32Iso {
       B.f[k] = gld(cx.f[O], base + k);
       B.st[k] = NORM ? gld(tst, base + k) : 0.0;
     }
-    B.row = gld(cx.rowtab, j);
+    B.row = gld(cx.rowtab, (unsigned)j);
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
     return j >= cx.fb.y0 && j <= cx.fb.y1 && ix >= cx.fb.x0 && ix <= cx.fb.x1;
@@ -501,11 +704,9 @@ struct FwdF32Iso {
                                   float aW, bool hW, double uE, float aE, bool hE, double uS, float aS, bool hS,
                                   double uN, float aN, bool hN) {
     const float t0 = hd.aux[k];
-    const float f = r.f[k];
     const double kNaN = __builtin_nan("");
-    return fwd_update_f32<true>(hd.u[k], t0, r.t0x[k], r.t0y[k], r.row.x, r.row.y, 0.0f, f * f, f * r.row.z,
-                                f * r.row.w, t0 * cx.idx, t0 * cx.idy, hW ? uW : kNaN, aW, hE ? uE : kNaN, aE,
-                                hS ? uS : kNaN, aS, hN ? uN : kNaN, aN);
+    return fwd_update_f32s(hd.u[k], t0, r.t0x[k], r.t0y[k], r.f[k], t0 * r.row.x, t0 * r.row.y, hW ? uW : kNaN,
+                           aW, hE ? uE : kNaN, aE, hS ? uS : kNaN, aS, hN ? uN : kNaN, aN);
   }
   template <int C>
   __device__ static double norm(const Head<C>& h, const Body<C>& r, int k, double v) {
