@@ -1,7 +1,9 @@
 """Batched runs are deterministic: the same batch run twice (and in a second
 batch object) gives the same bits for syn, the per-period kernels and chi, in
 FP64 and in FP32 (where the FP64 re-solves take pool slots in whatever order
-the pairs finish)."""
+the pairs finish).  A batch's second run pops its queues in a different order
+(longest pairs of the first run first, order_kernel) than its first run
+(batch order): the results must not depend on it."""
 import numpy as np
 import pytest
 
