@@ -408,6 +408,9 @@ struct Layout {
 
 // Launch groups: one per (columns per lane C, isotropic metric).
 constexpr int kMaxGroups = 2 * kMaxC;
+// Columns per lane of the FP32 adjoint of one-wave grids (GridHost::gadj):
+// 8 x 4 bytes, one 32-byte sector per lane and row.
+constexpr int kAdjC = 8;
 
 // Up to 512 columns: one wave, C = ceil(nx / 64) columns per lane.  Wider:
 // kMaxC columns per lane on W = wide_waves(nx) waves.
@@ -492,9 +495,21 @@ struct GridHost {
   bool p2 = false;
   GridDev gp2{};
   char* d_wmedp2 = nullptr;   // its wavefront-major media
+  // FP32 batches, one wave per pair, fewer than kAdjC columns per lane: the
+  // adjoint's geometry, kAdjC columns per lane (adjoint_geometry)
+  bool adj8 = false;
+  GridDev gadj{};
 
-  // wavefront-major elements of one event's work plane (either forward geometry)
-  size_t work_wplane() const { return p2 ? std::max(g.wplane, gp2.wplane) : g.wplane; }
+  // wavefront-major elements of one event's work plane (every geometry its
+  // forward and adjoint planes may take)
+  size_t work_wplane() const {
+    size_t w = g.wplane;
+    if (p2) w = std::max(w, gp2.wplane);
+    if (adj8) w = std::max(w, gadj.wplane);
+    return w;
+  }
+  // the geometry of the adjoint phase (face fluxes, delta, sweeps, Ta)
+  const GridDev& gadjoint() const { return adj8 ? gadj : g; }
   bool p2_on = false;          // the batch's groups run this grid's pairs two per wave
   // the forward geometry of the batched sweeps for the current grouping
   const GridDev& gfwd() const { return p2_on ? gp2 : g; }
@@ -632,6 +647,24 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
         for (int o = 0; o < 2; o++) q.wmed[i][o] = G.d_wmedp2 + (size_t)(2 * i + o) * pb2;
       G.p2 = true;
     }
+    if (prec == SAT_PREC_F32 && W == 1 && G.Wseg == 1 && g.C < kAdjC) {
+      // The FP32 adjoint with kAdjC columns per lane.  Its sweeps store 4-byte
+      // values, and the two sweeps of a round that change the planes'
+      // orientation (1 and 3) store each lane's C values into a different run;
+      // pieces of fewer than 32 bytes are written back as partial 32-byte
+      // sectors, each costing a whole sector (C = 4: twice the bytes written,
+      // measured with the TCC write counters).  Eight FP32 columns per lane
+      // make every piece a whole sector; nx <= 448 keeps one wave.
+      GridDev& a = G.gadj;
+      a = g;
+      a.C = kAdjC;
+      a.La = (nx + kAdjC - 1) / kAdjC;
+      a.P = a.La * a.C;
+      a.wplane = (size_t)(ny + a.La - 1) * a.La * a.C;
+      for (int i = 0; i < 6; i++) a.wmed[i][0] = a.wmed[i][1] = nullptr;
+      a.wf64[0] = a.wf64[1] = nullptr;
+      G.adj8 = true;
+    }
   }
   return SAT_OK;
 }
@@ -663,6 +696,7 @@ static int grid_media(GridHost& G, int mode, const double* p0, const double* p1,
     G.gmw.iso = G.g.iso;
     G.gseg.iso = G.g.iso;
     G.gp2.iso = G.g.iso;
+    G.gadj.iso = G.g.iso;
   }
   const size_t total = (size_t)G.nper * G.ny * G.g.P;
   const int threads = 256;
@@ -972,6 +1006,7 @@ struct sat_batch {
   GridDev* d_grids = nullptr;
   GridDev* d_grids_fwd = nullptr;  // the forward geometry of each grid (GridHost::gfwd)
   GridDev* d_grids_seg = nullptr;  // one-wave geometry of the per-pair kernels
+  GridDev* d_grids_adj = nullptr;  // the adjoint phase's geometry (GridHost::gadjoint)
   GridDev* d_grids64 = nullptr;    // FP32 batches: grids with FP64 media (per-pair re-solves)
   GridDev* d_grids64mw = nullptr;  // FP32 batches: FP64 media, re-solve geometry (batched re-solves)
   EventDev* d_events = nullptr;
@@ -1028,6 +1063,7 @@ struct sat_batch {
   std::vector<int> group_p2;       // > 0: the two-pairs-per-wave forward sweeps' columns per lane (one grid per group)
   std::vector<size_t> group_p2h;   // and their half-plane stride (pair2_hstride)
   std::vector<int> group_wide;     // 1: a grid wider than 512 columns (W waves of kMaxC columns, every kernel)
+  std::vector<int> group_adj8;     // 1: the FP32 adjoint sweeps kAdjC columns per lane on one wave (gadj)
   // per group: sweep slots [group_slot0, group_slot0 + group_nblk); launch order
   // of the groups (most work first)
   std::vector<int> group_slot0, group_nblk, launch_order;
@@ -1078,7 +1114,7 @@ struct sat_batch {
       if (p) (void)hipFree(p);
     d_aniso.clear();
     if (d_Kden) (void)hipFree(d_Kden);
-    void* ps[] = {d_grids, d_grids_fwd, d_grids_seg, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_order_adj, d_order_tmp, d_group_rng, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
+    void* ps[] = {d_grids, d_grids_fwd, d_grids_seg, d_grids_adj, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_order_adj, d_order_tmp, d_group_rng, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
                   d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_ov, d_evx64, d_work64, d_slots64, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
                   d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3};
     for (void* p : ps)
@@ -1140,6 +1176,7 @@ int build_groups(sat_batch* b) {
   b->group_p2.clear();
   b->group_p2h.clear();
   b->group_wide.clear();
+  b->group_adj8.clear();
   b->group_first.clear();
   b->group_count.clear();
   b->group_slot0.clear();
@@ -1168,6 +1205,7 @@ int build_groups(sat_batch* b) {
       const bool wide = std::get<4>(k) != 0;
       b->group_wC.push_back(wide ? kMaxC : (std::get<1>(k) + std::get<0>(k) - 1) / std::get<0>(k));
       b->group_wide.push_back(wide ? 1 : 0);
+      b->group_adj8.push_back(b->grids[b->he[order[i]].grid].adj8 ? 1 : 0);
       b->group_iso.push_back(std::get<2>(k));
       b->group_p2.push_back(p2 ? b->grids[std::get<3>(k)].gp2.C : 0);
       b->group_p2h.push_back(p2 ? pair2_hstride(b->grids[std::get<3>(k)].gp2) : 0);
@@ -1215,11 +1253,12 @@ int build_groups(sat_batch* b) {
     if (!rng.empty()) SAT_CHECK(hipMemcpy(b->d_group_rng, rng.data(), sizeof(int2) * rng.size(), hipMemcpyHostToDevice));
   }
   std::vector<GridDev> hg(b->grids.size()), hg64(b->grids.size()), hgmw(b->grids.size()), hgs(b->grids.size());
-  std::vector<GridDev> hgf(b->grids.size());
+  std::vector<GridDev> hgf(b->grids.size()), hga(b->grids.size());
   for (size_t gi = 0; gi < b->grids.size(); gi++) {
     b->grids[gi].p2_on = use_p2[gi] != 0;
     hg[gi] = b->grids[gi].g;
     hgf[gi] = b->grids[gi].gfwd();
+    hga[gi] = b->grids[gi].gadjoint();
     hg64[gi] = b->grids[gi].g64;
     hgmw[gi] = b->grids[gi].gmw;
     hgs[gi] = b->grids[gi].gseg;
@@ -1227,6 +1266,7 @@ int build_groups(sat_batch* b) {
   }
   SAT_CHECK(hipMemcpy(b->d_grids, hg.data(), sizeof(GridDev) * hg.size(), hipMemcpyHostToDevice));
   SAT_CHECK(hipMemcpy(b->d_grids_fwd, hgf.data(), sizeof(GridDev) * hgf.size(), hipMemcpyHostToDevice));
+  SAT_CHECK(hipMemcpy(b->d_grids_adj, hga.data(), sizeof(GridDev) * hga.size(), hipMemcpyHostToDevice));
   SAT_CHECK(hipMemcpy(b->d_grids_seg, hgs.data(), sizeof(GridDev) * hgs.size(), hipMemcpyHostToDevice));
   if (b->d_grids64) SAT_CHECK(hipMemcpy(b->d_grids64, hg64.data(), sizeof(GridDev) * hg64.size(), hipMemcpyHostToDevice));
   if (b->d_grids64mw)
@@ -1437,6 +1477,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(dalloc(&b->d_grids, d->ngrid));
   B_CHECK(dalloc(&b->d_grids_fwd, d->ngrid));
   B_CHECK(dalloc(&b->d_grids_seg, d->ngrid));
+  B_CHECK(dalloc(&b->d_grids_adj, d->ngrid));
   if (f32) B_CHECK(dalloc(&b->d_grids64, d->ngrid));
   if (f32) B_CHECK(dalloc(&b->d_grids64mw, d->ngrid));
 
@@ -1757,7 +1798,9 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   bd.f32_band = b->f32_band;
 
   PhaseDev pd{};
-  pd.grids = b->d_grids;
+  // the receivers and the adjoint phase (face fluxes, delta, sweeps, Ta) in the
+  // adjoint's geometry (GridHost::gadjoint)
+  pd.grids = b->d_grids_adj;
   pd.ev = b->d_evx;
   pd.ev_base = 0;
   pd.nevent = ne;
@@ -1831,6 +1874,9 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   auto cmw = [&](size_t gi) { return b->group_wide[gi] ? kMaxC : (b->group_C[gi] + wmw(gi) - 1) / wmw(gi); };
   // waves of the one-pair kernels (pair_kernel) of a group
   auto wseg = [&](size_t gi) { return b->group_wide[gi] ? b->group_W[gi] : 1; };
+  // columns per lane and waves of a group's batched adjoint sweeps
+  auto adj_cols = [&](size_t gi) { return b->group_adj8[gi] ? kAdjC : b->group_wC[gi]; };
+  auto adj_waves = [&](size_t gi) { return b->group_adj8[gi] ? 1 : b->group_W[gi]; };
 
   auto mark = [&](int i, hipStream_t st) -> hipError_t {
     return capturing ? hipSuccess : hipEventRecord(mk[i], st);
@@ -2068,7 +2114,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
           a.counter = adj_heads + gi;
           a.slots = b->d_slots + (size_t)b->group_slot0[gi] * b->slot_bytes;
           const dim3 grid(b->group_nblk[gi]);
-          const int C = b->group_wC[gi], W = b->group_W[gi];
+          const int C = adj_cols(gi), W = adj_waves(gi);
           SAT_CHECK(f32 ? launch_adj_sweep<SAT_PREC_F32>(C, W, grid, st, a, b->group_first[gi], b->group_count[gi])
                         : launch_adj_sweep<SAT_PREC_F64>(C, W, grid, st, a, b->group_first[gi], b->group_count[gi]));
           return SAT_OK;
@@ -2127,7 +2173,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
           PhaseDev a = pk;
           a.counter = adj_heads + gi;
           a.slots = b->d_slots + (size_t)b->group_slot0[gi] * b->slot_bytes;
-          const int C = b->group_wC[gi], W = b->group_W[gi];
+          const int C = adj_cols(gi), W = adj_waves(gi);
           SAT_CHECK(f32 ? launch_adj_sweep<SAT_PREC_F32>(C, W, dim3(b->group_nblk[gi]), st, a, b->group_first[gi],
                                                          b->group_count[gi])
                         : launch_adj_sweep<SAT_PREC_F64>(C, W, dim3(b->group_nblk[gi]), st, a, b->group_first[gi],

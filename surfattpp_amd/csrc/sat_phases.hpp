@@ -107,6 +107,11 @@ struct PhaseDev {
 constexpr int kRingDepth = SAT_RING_DEPTH;
 constexpr int kFwdDepth1 = 1;
 constexpr int kAdjDepth1 = 3;
+// the one-wave adjoint's rows in flight: the ring of kAdjDepth1 rows, or the
+// head/body pipeline for eight columns per lane (the ring's registers would
+// leave one wave per SIMD)
+template <int C>
+constexpr int adj_depth1() { return C >= 8 ? 1 : kAdjDepth1; }
 
 // The multi-wave FP64 sweeps carry the latency-critical re-solves of FP32
 // batches, which share the CUs with a throughput kernel on the other stream:
@@ -667,7 +672,7 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
       cx.dy = g.dy;
     }
     IterOut io;
-    const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? mw_adj_depth<PREC, C, W>() : kAdjDepth1>(uniform(cx), Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,
+    const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? mw_adj_depth<PREC, C, W>() : adj_depth1<C>()>(uniform(cx), Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,
                                                                            io, exchange_ptr<Pol, W>(xch), red);
     if (lane == 0 && !pd.kden_pass) {
       pd.iters_a[e] = io.iters;
