@@ -275,29 +275,36 @@ __global__ void init_kernel(PhaseDev pd) {
   const double tmp_c = dmax(-1.0, dmin(1.0, tmp));
   const double ac = (PREC == SAT_PREC_F64) ? cr::acos_cr(tmp_c) : acos(tmp_c);
   const double T0 = ac * kR * fun0;
-  const double sin_d = sqrt(dmax(0.0, 1.0 - tmp_c * tmp_c));
-  double T0x = 0.0, T0y = 0.0;
-  if (!(sin_d < 1e-14)) {
-    T0x = cyy * cy0 * sdl / sin_d * kR * fun0;
-    T0y = -(cyy * sy0 - syy * cy0 * cdl) / sin_d * kR * fun0;
-  }
-  if (pass < 2) {
-    const size_t w = ev.w_off + idx;
-    static_cast<V*>(pd.t0[pass])[w] = (V)T0;
-    if (PREC == SAT_PREC_F32 && pd.polish && g.iso && g.wf64[0]) {
-      // the polished isotropic FP32 sweeps' scaled gradient, FP64: sqrt(a) T0x, sqrt(b) T0y
-      static_cast<double*>(pd.t0x[pass])[w] = T0x / (kR * cyy);
-      static_cast<double*>(pd.t0y[pass])[w] = T0y / kR;
-    } else if ((PREC == SAT_PREC_F32 || pd.scaled_iso) && g.iso) {
-      // the isotropic FP32 sweeps' and FP64 re-solves' scaled gradient (FwdF32Iso, FwdF64RIso)
-      static_cast<V*>(pd.t0x[pass])[w] = (V)(T0x / (kR * cyy));
-      static_cast<V*>(pd.t0y[pass])[w] = (V)(T0y / kR);
-    } else {
-      static_cast<V*>(pd.t0x[pass])[w] = (V)T0x;
-      static_cast<V*>(pd.t0y[pass])[w] = (V)T0y;
-    }
-  } else {
+  if (pass == 2) {  // the FP64 column-major T0 of FP32 sweeps: no gradient
     pd.T0d[ev.pl_off + idx] = T0;
+    return;
+  }
+  const size_t w = ev.w_off + idx;
+  static_cast<V*>(pd.t0[pass])[w] = (V)T0;
+  const double sin_d = sqrt(dmax(0.0, 1.0 - tmp_c * tmp_c));
+  const bool flat = sin_d < 1e-14;
+  const double gx = cyy * cy0 * sdl, gy = -(cyy * sy0 - syy * cy0 * cdl);
+  const bool scaled = (PREC == SAT_PREC_F32 && pd.polish && g.iso && g.wf64[0]) ||
+                      ((PREC == SAT_PREC_F32 || pd.scaled_iso) && g.iso);
+  if (PREC == SAT_PREC_F64 && !scaled) {
+    // FP64 batches: the reference's expressions, in its operation order
+    static_cast<V*>(pd.t0x[pass])[w] = flat ? 0.0 : gx / sin_d * kR * fun0;
+    static_cast<V*>(pd.t0y[pass])[w] = flat ? 0.0 : gy / sin_d * kR * fun0;
+    return;
+  }
+  // FP32 sweeps and FP64 re-solves (the same quantities, one division per
+  // node): scaled by sqrt(a) = 1/(R cos y), sqrt(b) = 1/R on isotropic grids,
+  // T0x = cy cy0 sdl fun0 R / sin_d and T0y likewise otherwise
+  const double r = flat ? 0.0 : (scaled ? fun0 : kR * fun0) / sin_d;
+  const double tx = (scaled ? cy0 * sdl : gx) * r, ty = gy * r;
+  if (PREC == SAT_PREC_F32 && pd.polish && g.iso && g.wf64[0]) {
+    // the polished isotropic FP32 sweeps' scaled gradient, FP64 (FwdF32IsoP)
+    static_cast<double*>(pd.t0x[pass])[w] = tx;
+    static_cast<double*>(pd.t0y[pass])[w] = ty;
+  } else {
+    // FP32: the sweeps' gradient (FwdF32Iso scaled, FwdF32); FP64: the re-solves' (FwdF64RIso)
+    static_cast<V*>(pd.t0x[pass])[w] = (V)tx;
+    static_cast<V*>(pd.t0y[pass])[w] = (V)ty;
   }
 }
 
