@@ -312,16 +312,27 @@ def _time_pair(grids, runs=2):
         b64.run()
         ms64 = b64.last_ms()[0]
         r64 = b64.results()
-    def timed(prec):
+    def timed(prec, graph=False):
         with PP.EikonalBatch(grids, precision=prec) as bt:
             bt.run()
             t = []
             for _ in range(runs):
                 bt.run()
                 t.append(bt.last_ms()[0])
-            return float(np.mean(t)), bt.phase_ms(), bt.results(), bt.conv()[2]
+            out = [float(np.mean(t)), bt.phase_ms(), bt.results(), bt.conv()[2]]
+            if graph:
+                # the whole run (every launch group on its streams, the re-solves,
+                # the accumulation) as one captured graph, one launch per run
+                from surfattpp_amd import _lib
+                bt.run(mode=_lib.RUN_GRAPH)
+                tg = []
+                for _ in range(runs):
+                    bt.run(mode=_lib.RUN_GRAPH)
+                    tg.append(bt.last_ms()[0])
+                out.append(float(np.mean(tg)))
+            return out
 
-    ms32, ph, r32, nres = timed(PREC_F32)
+    ms32, ph, r32, nres, ms32g = timed(PREC_F32, graph=True)
     msp, php, rp, _ = timed(PREC_F32P)
     ne = len(r64["iters_fwd"])
 
@@ -330,6 +341,7 @@ def _time_pair(grids, runs=2):
                 "syn": float(np.abs(r["syn"] - r64["syn"]).max() / np.abs(r64["syn"]).max())}
     return {"pairs": ne, "value_f32": round(ne / (ms32 * 1e-3), 2), "ms_f32": round(ms32, 3),
             "value_f64": round(ne / (ms64 * 1e-3), 2), "ms_f64": round(ms64, 3),
+            "ms_f32_one_graph_launch": round(ms32g, 3),
             "err_vs_fp64": err(r32),
             "iters_fwd_equal": bool(np.array_equal(r32["iters_fwd"], r64["iters_fwd"])),
             "fp64_resolves": int(nres), "phases_ms_f32": {k: round(v, 3) for k, v in ph.items()},

@@ -202,6 +202,24 @@ int sat_batch_last_ms(sat_batch* b, double* total_ms, double* solve_ms);
 int sat_batch_phase_ms(sat_batch* b, double* ms, int n);
 /* Waves per pair of each grid's batched sweeps, as chosen at creation: waves[ngrid]. */
 int sat_batch_waves(sat_batch* b, int* waves);
+/* Scheduling options of the batched kernels (no counterpart in the reference;
+ * results do not depend on them beyond FP32 rounding, and FP64 batches ignore
+ * all three): the autotuner's knobs besides the waves per grid.
+ *   SAT_OPT_PAIR2      FP32 isotropic grids of at most 64 columns: two pairs per
+ *                      wave in the forward sweeps (1), one (0), the library's
+ *                      rule (-1, the default: two for one-column-per-lane grids)
+ *   SAT_OPT_ADJ_COLS   FP32 one-wave grids: the adjoint on 8 columns per lane
+ *                      (1, default) or on the forward's columns per lane (0)
+ *   SAT_OPT_ORDER      pop each launch group's queue in the order of the
+ *                      previous run's rounds, longest first (1, default), or
+ *                      in batch order (0)
+ * A change re-groups the events and re-captures a graph at the next run.
+ * SAT_EINVAL for an unknown option or value. */
+#define SAT_OPT_PAIR2 1
+#define SAT_OPT_ADJ_COLS 2
+#define SAT_OPT_ORDER 3
+int sat_batch_set_option(sat_batch* b, int option, int value);
+int sat_batch_get_option(sat_batch* b, int option, int* value);
 int sat_batch_destroy(sat_batch* b);
 
 /* ------------------------------------------------------------------------

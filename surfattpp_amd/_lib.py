@@ -25,6 +25,9 @@ FLAG_NOT_CONVERGED = 1
 FLAG_SRC_ON_BOUNDARY = 2
 FLAG_ADJ_MAXITER = 4
 FLAG_REC_OUTSIDE = 8
+OPT_PAIR2 = 1      # scheduling options of a batch (sat_batch_set_option)
+OPT_ADJ_COLS = 2
+OPT_ORDER = 3
 
 # Every symbol declared in include/surfatt_eik.h.
 EXPORTS = (
@@ -32,6 +35,7 @@ EXPORTS = (
     "sat_batch_create", "sat_batch_update_media", "sat_batch_update_obs", "sat_batch_run",
     "sat_batch_sync", "sat_batch_results", "sat_batch_last_ms", "sat_batch_destroy",
     "sat_batch_set_f32_band", "sat_batch_conv", "sat_batch_phase_ms", "sat_batch_waves",
+    "sat_batch_set_option", "sat_batch_get_option",
     "sat_combine_kernels", "sat_combine_kernels_ex", "sat_batch_combine_kernels", "sat_batch_aniso_kernels", "sat_batch_set_aniso_metric",
     "sat_batch_set_kden", "sat_batch_kden_results",
     "sat_topo_metric", "sat_surfdisp96", "sat_fwdsurf", "sat_device_info",
@@ -108,6 +112,8 @@ def load(path: str = LIB_PATH):
     L.sat_batch_conv.argtypes = [vp, vp, vp, vp]
     L.sat_batch_phase_ms.argtypes = [vp, dp, C.c_int]
     L.sat_batch_waves.argtypes = [vp, vp]
+    L.sat_batch_set_option.argtypes = [vp, C.c_int, C.c_int]
+    L.sat_batch_get_option.argtypes = [vp, C.c_int, vp]
     L.sat_combine_kernels.argtypes = [C.c_int] * 5 + [vp] * 9 + [dp]
     L.sat_combine_kernels_ex.argtypes = [C.POINTER(sat_combine_desc), dp]
     L.sat_batch_combine_kernels.argtypes = [vp, C.c_int, C.c_int, C.c_int] + [vp] * 8 + [dp]

@@ -1015,7 +1015,9 @@ struct sat_batch {
   int* d_order_adj = nullptr;      // and the adjoint queue, each longest pairs first (order_kernel)
   int* d_order_tmp = nullptr;
   int2* d_group_rng = nullptr;     // per launch group: first, count (positions in the queues)
-  bool order_by_rounds = true;     // SURFATT_ORDER=0: queues in batch order (grouped), never reordered
+  bool order_by_rounds = true;     // SAT_OPT_ORDER (SURFATT_ORDER=0 at creation: queues in batch order)
+  int opt_pair2 = -1;              // SAT_OPT_PAIR2: -1 the library's rule (SURFATT_PAIR2 overrides), 0, 1
+  bool opt_adj_cols = true;        // SAT_OPT_ADJ_COLS: the FP32 adjoint of one-wave grids on kAdjC columns
   int* d_counter = nullptr;        // re-solve counts | fwd heads | adj heads | re-solve heads (one per group),
                                    // then one head per event for per-solve launches
   double* d_rec = nullptr;         // rec_lon | rec_lat | tt | w | syn | adjsrc
@@ -1153,7 +1155,7 @@ int build_groups(sat_batch* b) {
   std::vector<char> use_p2(b->grids.size(), 0);
   {
     const char* env = std::getenv("SURFATT_PAIR2");
-    const int force = env ? std::atoi(env) : -1;
+    const int force = b->opt_pair2 >= 0 ? b->opt_pair2 : (env ? std::atoi(env) : -1);
     for (size_t gi = 0; gi < b->grids.size(); gi++) {
       const GridHost& G = b->grids[gi];
       if (!G.p2 || !G.g.iso) continue;
@@ -1205,7 +1207,7 @@ int build_groups(sat_batch* b) {
       const bool wide = std::get<4>(k) != 0;
       b->group_wC.push_back(wide ? kMaxC : (std::get<1>(k) + std::get<0>(k) - 1) / std::get<0>(k));
       b->group_wide.push_back(wide ? 1 : 0);
-      b->group_adj8.push_back(b->grids[b->he[order[i]].grid].adj8 ? 1 : 0);
+      b->group_adj8.push_back(b->opt_adj_cols && b->grids[b->he[order[i]].grid].adj8 ? 1 : 0);
       b->group_iso.push_back(std::get<2>(k));
       b->group_p2.push_back(p2 ? b->grids[std::get<3>(k)].gp2.C : 0);
       b->group_p2h.push_back(p2 ? pair2_hstride(b->grids[std::get<3>(k)].gp2) : 0);
@@ -1258,7 +1260,7 @@ int build_groups(sat_batch* b) {
     b->grids[gi].p2_on = use_p2[gi] != 0;
     hg[gi] = b->grids[gi].g;
     hgf[gi] = b->grids[gi].gfwd();
-    hga[gi] = b->grids[gi].gadjoint();
+    hga[gi] = b->opt_adj_cols ? b->grids[gi].gadjoint() : b->grids[gi].g;
     hg64[gi] = b->grids[gi].g64;
     hgmw[gi] = b->grids[gi].gmw;
     hgs[gi] = b->grids[gi].gseg;
@@ -2412,6 +2414,39 @@ int sat_batch_set_f32_band(sat_batch* b, double band) {
   if (band != b->f32_band) drop_graph(b);  // the band is a value captured in the graph
   b->f32_band = band;
   return SAT_OK;
+}
+
+int sat_batch_set_option(sat_batch* b, int option, int value) {
+  if (!b) return SAT_EINVAL;
+  switch (option) {
+    case SAT_OPT_PAIR2:
+      if (value < -1 || value > 1) return SAT_EINVAL;
+      if (value != b->opt_pair2) b->groups_dirty = true;
+      b->opt_pair2 = value;
+      return SAT_OK;
+    case SAT_OPT_ADJ_COLS:
+      if (value < 0 || value > 1) return SAT_EINVAL;
+      if ((value != 0) != b->opt_adj_cols) b->groups_dirty = true;
+      b->opt_adj_cols = value != 0;
+      return SAT_OK;
+    case SAT_OPT_ORDER:
+      if (value < 0 || value > 1) return SAT_EINVAL;
+      if ((value != 0) != b->order_by_rounds) b->groups_dirty = true;
+      b->order_by_rounds = value != 0;
+      return SAT_OK;
+    default:
+      return SAT_EINVAL;
+  }
+}
+
+int sat_batch_get_option(sat_batch* b, int option, int* value) {
+  if (!b || !value) return SAT_EINVAL;
+  switch (option) {
+    case SAT_OPT_PAIR2: *value = b->opt_pair2; return SAT_OK;
+    case SAT_OPT_ADJ_COLS: *value = b->opt_adj_cols ? 1 : 0; return SAT_OK;
+    case SAT_OPT_ORDER: *value = b->order_by_rounds ? 1 : 0; return SAT_OK;
+    default: return SAT_EINVAL;
+  }
 }
 
 int sat_batch_conv(sat_batch* b, double* linf_last, double* linf_prev, int* n_resolved) {

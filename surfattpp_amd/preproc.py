@@ -158,6 +158,19 @@ class EikonalBatch:
         _lib.check("sat_batch_waves", _lib.load().sat_batch_waves(self._h, buf))
         return list(buf)
 
+    OPTIONS = {"pair2": _lib.OPT_PAIR2, "adj_cols": _lib.OPT_ADJ_COLS, "order": _lib.OPT_ORDER}
+
+    def set_option(self, name: str, value: int):
+        """A scheduling option (sat_batch_set_option): pair2 (-1 the library's
+        rule, 0, 1), adj_cols (0, 1), order (0, 1).  Results do not depend on
+        them beyond FP32 rounding."""
+        _lib.check("sat_batch_set_option", _lib.load().sat_batch_set_option(self._h, self.OPTIONS[name], int(value)))
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int()
+        _lib.check("sat_batch_get_option", _lib.load().sat_batch_get_option(self._h, self.OPTIONS[name], C.byref(v)))
+        return v.value
+
     def set_f32_band(self, band: float):
         _lib.check("sat_batch_set_f32_band", _lib.load().sat_batch_set_f32_band(self._h, float(band)))
 

@@ -1,4 +1,4 @@
-"""Autotuner of the batched sweeps' work-group shape.
+"""Autotuner of the batched kernels' schedule.
 
 A grid's batched FP32 sweeps run one pair per work-group of W waves (W = 1, 2
 or 4; 64*W lanes, ceil(ceil(nx/64)/W) columns per lane).  More waves shorten a
@@ -8,11 +8,22 @@ rule (more waves for the largest grids while the batch stays within three
 waves per SIMD); `autotune` measures instead.  It times the whole batched run
 for the library's choice, then, grid by grid from the largest, for every
 other W with the remaining grids held, and keeps any change that shortens the
-run by more than 2 %.  Every W performs the same Gauss-Seidel order (results
-agree to FP32 rounding), so only time is compared.  The choice is stored in a JSON cache keyed by the device, the
-precision and the batch's shape (grid sizes, pairs per grid, isotropic or
-not), so a batch of the same shape -- every iteration of an inversion -- uses
-it without measuring again.
+run by more than 2 %.  Then, with the waves fixed, it tries the batch's
+scheduling options (sat_batch_set_option) the same way, one at a time:
+  pair2     two pairs per wave in the forward sweeps of grids of at most 64
+            columns (on / off against the library's rule; tried only when the
+            batch has such a grid),
+  adj_cols  the FP32 adjoint of one-wave grids on 8 columns per lane or on
+            the forward's columns per lane,
+  order     the queues popped longest pairs (of the previous run) first, or
+            in batch order.
+Every choice performs the same Gauss-Seidel order (results agree to FP32
+rounding), so only time is compared.  The FP32 re-solve band is not tuned: it
+is the margin of the iteration-count parity, not a speed knob; neither is the
+number of resident waves, which sizes the workspace at creation.  The choice
+is stored in a JSON cache keyed by the device, the precision and the batch's
+shape (grid sizes, pairs per grid, isotropic or not), so a batch of the same
+shape -- every iteration of an inversion -- uses it without measuring again.
 
 The cache file is $SURFATT_TUNE_CACHE, or ~/.cache/surfattpp_amd/tune.json.
 """
@@ -68,10 +79,14 @@ def _store(path: str, key: str, entry: dict) -> None:
     os.replace(tmp, path)
 
 
-def time_batch(grids, waves, precision: int = _lib.PREC_F32, calc_adj: bool = True, runs: int = 3):
-    """(best ms of `runs` timed runs after a warm-up, waves as created)."""
+def time_batch(grids, waves, precision: int = _lib.PREC_F32, calc_adj: bool = True, runs: int = 3,
+               options: dict | None = None):
+    """(best ms of `runs` timed runs after a warm-up, waves as created), with
+    the scheduling `options` set on the batch."""
     with PP.EikonalBatch(_grids(grids), precision=precision, calc_adj=calc_adj, waves=waves) as b:
-        b.run()
+        for k, v in (options or {}).items():
+            b.set_option(k, v)
+        b.run()  # warm-up; also the rounds the next runs order their queues by
         ms = []
         for _ in range(runs):
             b.run()
@@ -79,24 +94,37 @@ def time_batch(grids, waves, precision: int = _lib.PREC_F32, calc_adj: bool = Tr
         return min(ms), b.waves()
 
 
+def _option_trials(grids) -> list:
+    """(option, value) pairs worth timing for this batch, against the defaults."""
+    trials = []
+    if any(g["lon"].size <= 64 for g in grids):
+        trials += [("pair2", 0), ("pair2", 1)]
+    trials += [("adj_cols", 0), ("order", 0)]
+    return trials
+
+
 def autotune(grids, precision: int = _lib.PREC_F32, calc_adj: bool = True, runs: int = 3,
              gain: float = 0.02, path: str | None = None, retune: bool = False) -> dict:
-    """Waves per grid for this batch: from the cache, or measured and cached.
-    Returns dict(waves=[...], ms=..., default_waves=[...], default_ms=..., trials=[[waves, ms], ...])."""
+    """Waves per grid and scheduling options for this batch: from the cache, or
+    measured and cached.  Returns dict(waves=[...], options={...}, ms=...,
+    default_waves=[...], default_ms=..., trials=[[waves, options, ms], ...])."""
     grids = _grids(grids)
     path = path or cache_path()
     key = signature(grids, precision, calc_adj)
     if not retune:
         hit = _load_cache(path).get(key)
         if hit is not None:
+            hit.setdefault("options", {})
             return hit
-    if precision != _lib.PREC_F32:  # FP64 batches run one wave per pair
-        entry = dict(waves=[1] * len(grids), ms=None, default_waves=[1] * len(grids), default_ms=None, trials=[])
+    if precision != _lib.PREC_F32:  # FP64 batches run one wave per pair; the options are FP32 ones
+        entry = dict(waves=[1] * len(grids), options={}, ms=None, default_waves=[1] * len(grids),
+                     default_ms=None, trials=[])
         _store(path, key, entry)
         return entry
     ms0, cur = time_batch(grids, 0, precision, calc_adj, runs)
     best = ms0
-    trials = [[list(cur), ms0]]
+    opts: dict = {}
+    trials = [[list(cur), {}, ms0]]
     order = sorted(range(len(grids)), key=lambda i: -grids[i]["lon"].size * grids[i]["lat"].size)
     for gi in order:
         for w in WAVES:
@@ -105,15 +133,29 @@ def autotune(grids, precision: int = _lib.PREC_F32, calc_adj: bool = True, runs:
             trial = list(cur)
             trial[gi] = w
             ms, _ = time_batch(grids, trial, precision, calc_adj, runs)
-            trials.append([trial, ms])
+            trials.append([trial, {}, ms])
             if ms < best * (1.0 - gain):
                 best, cur = ms, trial
-    entry = dict(waves=list(cur), ms=best, default_waves=list(trials[0][0]), default_ms=ms0, trials=trials)
+    for name, value in _option_trials(grids):
+        if opts.get(name) == value:
+            continue
+        trial = dict(opts)
+        trial[name] = value
+        ms, _ = time_batch(grids, cur, precision, calc_adj, runs, trial)
+        trials.append([list(cur), dict(trial), ms])
+        if ms < best * (1.0 - gain):
+            best, opts = ms, trial
+    entry = dict(waves=list(cur), options=opts, ms=best, default_waves=list(trials[0][0]), default_ms=ms0,
+                 trials=trials)
     _store(path, key, entry)
     return entry
 
 
 def tuned_batch(grids, precision: int = _lib.PREC_F32, calc_adj: bool = True, **kw) -> PP.EikonalBatch:
-    """An EikonalBatch with the autotuned waves per grid (measured on first use)."""
-    waves = autotune(grids, precision, calc_adj)["waves"]
-    return PP.EikonalBatch(_grids(grids), precision=precision, calc_adj=calc_adj, waves=waves, **kw)
+    """An EikonalBatch with the autotuned waves per grid and scheduling options
+    (measured on first use)."""
+    entry = autotune(grids, precision, calc_adj)
+    b = PP.EikonalBatch(_grids(grids), precision=precision, calc_adj=calc_adj, waves=entry["waves"], **kw)
+    for k, v in entry.get("options", {}).items():
+        b.set_option(k, v)
+    return b

@@ -1,0 +1,83 @@
+"""The batch's scheduling options (sat_batch_set_option: two pairs per wave,
+the adjoint's columns per lane, the queue order) change where and in which
+order pairs are swept, never what is computed: every setting keeps the FP64
+iteration counts and stays within the FP32 bar of the FP64 batch (which
+equals the oracle), and settings that keep the same arithmetic give the same
+bits."""
+import numpy as np
+import pytest
+
+from surfattpp_amd import PREC_F32, PREC_F64, _lib
+from surfattpp_amd import preproc as PP
+from surfattpp_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, r):
+    return float(np.abs(a - r).max() / np.abs(r).max())
+
+
+@pytest.fixture(scope="module")
+def grids():
+    # 61 columns: one column per lane (two pairs per wave eligible); 201: four
+    g1 = synth.config_grf(seed=61, nx=61, ny=71, nsta=12, nperiod=3, spacing_km=2.0)
+    g2 = synth.config_grf(seed=62, nx=201, ny=121, nsta=10, nperiod=2)
+    return [g1, g2]
+
+
+@pytest.fixture(scope="module")
+def ref64(grids):
+    with PP.EikonalBatch(grids, precision=PREC_F64) as b:
+        b.run()
+        return b.results()
+
+
+def _run(grids, opts, runs=2):
+    with PP.EikonalBatch(grids, precision=PREC_F32) as b:
+        for k, v in opts.items():
+            b.set_option(k, v)
+            assert b.get_option(k) == v
+        out = []
+        for _ in range(runs):  # the second run pops its queues reordered (order = 1)
+            b.run()
+            out.append(b.results())
+        return out
+
+
+@pytest.mark.parametrize("opts", [
+    {}, {"pair2": 0}, {"pair2": 1}, {"adj_cols": 0}, {"order": 0},
+    {"pair2": 1, "adj_cols": 0, "order": 0},
+])
+def test_options_keep_results(grids, ref64, opts):
+    outs = _run(grids, opts)
+    for r in outs:
+        np.testing.assert_array_equal(r["iters_fwd"], ref64["iters_fwd"])
+        for k32, k64 in zip(r["adj_s"], ref64["adj_s"]):
+            for p in range(k64.shape[0]):
+                assert _rel(k32[p], k64[p]) <= 1e-5
+        assert _rel(r["syn"], ref64["syn"]) <= 1e-6
+    # the same options, run again after the queues were reordered: the same bits
+    assert np.array_equal(outs[0]["syn"], outs[1]["syn"])
+    for a, c in zip(outs[0]["adj_s"], outs[1]["adj_s"]):
+        assert np.array_equal(a, c)
+    print(f"{opts}: chi {outs[0]['chi']:.9f} (FP64 {ref64['chi']:.9f})", flush=True)
+
+
+def test_order_changes_no_bits(grids):
+    """Queue order alone (same geometry, same kernels): identical results."""
+    a = _run(grids, {"order": 0}, runs=1)[0]
+    b = _run(grids, {"order": 1}, runs=2)[1]
+    assert np.array_equal(a["syn"], b["syn"])
+    for x, y in zip(a["adj_s"], b["adj_s"]):
+        assert np.array_equal(x, y)
+    assert a["chi"] == b["chi"]
+
+
+def test_invalid_options(grids):
+    with PP.EikonalBatch(grids[:1], precision=PREC_F32) as b:
+        for name, v in (("pair2", 2), ("adj_cols", -1), ("order", 5)):
+            with pytest.raises(_lib.SatError):
+                b.set_option(name, v)
+        with pytest.raises(_lib.SatError):
+            _lib.check("sat_batch_set_option", _lib.load().sat_batch_set_option(b._h, 99, 0))

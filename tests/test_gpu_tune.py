@@ -1,5 +1,6 @@
-"""The work-group shape autotuner (surfattpp_amd.tune): measures waves per grid
-for a batch, caches the choice by batch shape, and reuses it without measuring."""
+"""The autotuner (surfattpp_amd.tune): measures waves per grid and the
+scheduling options for a batch, caches the choice by batch shape, and reuses it
+without measuring."""
 import json
 
 import numpy as np
@@ -20,14 +21,21 @@ def test_autotune_measures_caches_and_reuses(tmp_path, monkeypatch):
     print("tuned", e["waves"], e["ms"], "library", e["default_waves"], e["default_ms"], flush=True)
     assert all(w in (1, 2, 4) for w in e["waves"])
     assert e["ms"] <= e["default_ms"]
-    assert len(e["trials"]) >= 3
+    # waves of each grid, then the options (no grid of <= 64 columns: adj_cols, order)
+    assert len(e["trials"]) >= 5
+    tried = [t[1] for t in e["trials"] if t[1]]
+    assert {"adj_cols": 0} in tried and {"order": 0} in tried
+    assert set(e["options"]) <= {"pair2", "adj_cols", "order"}
     cache = json.load(open(path))
     assert list(cache.values())[0]["waves"] == e["waves"]
+    assert list(cache.values())[0]["options"] == e["options"]
     # the same shape again: served from the cache (no new trials are measured)
     monkeypatch.setattr(tune, "time_batch", lambda *a, **k: pytest.fail("measured again"))
     assert tune.autotune([g1, g2])["waves"] == e["waves"]
     with tune.tuned_batch([g1, g2]) as b:
         assert b.waves() == e["waves"]
+        for k, v in e["options"].items():
+            assert b.get_option(k) == v
         b.run()
         r = b.results()
     assert np.isfinite(r["chi"]) and r["chi"] > 0
