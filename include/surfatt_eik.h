@@ -204,7 +204,7 @@ int sat_batch_phase_ms(sat_batch* b, double* ms, int n);
 int sat_batch_waves(sat_batch* b, int* waves);
 /* Scheduling options of the batched kernels (no counterpart in the reference;
  * results do not depend on them beyond FP32 rounding, and FP64 batches ignore
- * all three): the autotuner's knobs besides the waves per grid.
+ * them): the autotuner's knobs besides the waves per grid.
  *   SAT_OPT_PAIR2      FP32 isotropic grids of at most 64 columns: two pairs per
  *                      wave in the forward sweeps (1), one (0), the library's
  *                      rule (-1, the default: two for one-column-per-lane grids)
@@ -213,11 +213,20 @@ int sat_batch_waves(sat_batch* b, int* waves);
  *   SAT_OPT_ORDER      pop each launch group's queue in the order of the
  *                      previous run's rounds, longest first (1, default), or
  *                      in batch order (0)
+ *   SAT_OPT_SPECULATE  FP32 batches with a re-solve band: the pairs re-solved
+ *                      in FP64 in one of the last 8 runs (and still near the
+ *                      tolerance) are solved in FP64 from the start of the
+ *                      run, beside the FP32 forward sweeps, which skip them
+ *                      (1, default), or only the band rule's pairs, after
+ *                      the FP32 forward sweeps (0).  A speculated pair's
+ *                      result is its FP64 solution; one-chunk batched and
+ *                      graph runs only.
  * A change re-groups the events and re-captures a graph at the next run.
  * SAT_EINVAL for an unknown option or value. */
 #define SAT_OPT_PAIR2 1
 #define SAT_OPT_ADJ_COLS 2
 #define SAT_OPT_ORDER 3
+#define SAT_OPT_SPECULATE 4
 int sat_batch_set_option(sat_batch* b, int option, int value);
 int sat_batch_get_option(sat_batch* b, int option, int* value);
 int sat_batch_destroy(sat_batch* b);

@@ -28,6 +28,7 @@ FLAG_REC_OUTSIDE = 8
 OPT_PAIR2 = 1      # scheduling options of a batch (sat_batch_set_option)
 OPT_ADJ_COLS = 2
 OPT_ORDER = 3
+OPT_SPECULATE = 4
 
 # Every symbol declared in include/surfatt_eik.h.
 EXPORTS = (

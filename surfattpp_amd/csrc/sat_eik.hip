@@ -368,6 +368,87 @@ __global__ __launch_bounds__(kOrderThreads) void order_kernel(int* order, int* t
   }
 }
 
+// Speculative FP64 re-solves.  An FP32 batch re-solves in FP64 the pairs whose
+// stop decision lies within the band of the tolerance, so that every forward
+// iteration count is the FP64 one; the decision is known only when the pair's
+// FP32 sweeps end, so those re-solves wait for the whole FP32 forward launch
+// and then run alone at the end of the run.  Consecutive runs of an inversion
+// differ by one model update, so a pair that needed a re-solve in a recent run
+// is likely to need one again: such pairs are solved in FP64 from the start of
+// the run, on the second stream beside the FP32 forward sweeps, and the FP32
+// pipeline passes over them (mark kSkipSpec).  Their result is their FP64
+// solution, which the band rule would have given them or which is closer to
+// the reference than the FP32 one; the band rule still re-solves the pairs it
+// finds among the others.
+
+// Runs a prediction lasts: refreshed while the pair is re-solved by the band
+// rule, or speculated and its FP64 stop decision is within the band itself.
+constexpr unsigned char kSpecAge = 8;
+
+// Before a run's reset: each pair's prediction age from the previous run's
+// re-solve marks (and, for speculated pairs, their FP64 convergence norms).
+__global__ void spec_age_kernel(unsigned char* age, const unsigned char* skip, const double* conv, double band,
+                                int ne) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const unsigned char s = skip[e];
+  unsigned char a = age[e];
+  if (s == 1 || s == 2) {
+    a = kSpecAge;
+  } else if (s == kSkipSpec) {
+    const double l = conv[2 * (size_t)e], lp = conv[2 * (size_t)e + 1];
+    const bool near = (l >= (1.0 - band) * kFwdTol && l < kFwdTol) || (lp >= 0.0 && lp < (1.0 + band) * kFwdTol);
+    a = near ? kSpecAge : (a > 0 ? (unsigned char)(a - 1) : (unsigned char)0);
+  } else {
+    a = 0;
+  }
+  age[e] = a;
+}
+
+// After the reset: the speculated pairs of every launch group, in queue order
+// (group g's list at list[rng[g].x], its length count[g]), each with a slot of
+// the FP64 pool, in group order until the pool is full; the pool counter then
+// starts the band rule's re-solves past them.  One block: the choice of pairs
+// does not depend on timing.
+__global__ __launch_bounds__(kOrderThreads) void spec_kernel(const unsigned char* age, unsigned char* skip,
+                                                             const int* order, const int2* rng, int ng, int* list,
+                                                             int* count, int* pool_counter, int pool_size,
+                                                             size_t pool_stride, EventX* ev64) {
+  constexpr int kWaves = kOrderThreads / kWave;
+  __shared__ int wcnt[kWaves];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  int total = 0;
+  for (int g = 0; g < ng; g++) {
+    const int first = rng[g].x, n = rng[g].y;
+    int cg = 0;
+    for (int base = 0; base < n; base += kOrderThreads) {
+      const int i = base + tid;
+      const int e = i < n ? order[first + i] : -1;
+      const bool p = e >= 0 && age[e] > 0;
+      const unsigned long long m = __ballot(p);
+      if (lane == 0) wcnt[wv] = __popcll(m);
+      __syncthreads();
+      int rank = __popcll(m & ((1ull << lane) - 1ull)), tot = 0;
+      for (int w = 0; w < kWaves; w++) {
+        if (w < wv) rank += wcnt[w];
+        tot += wcnt[w];
+      }
+      const int slot = total + cg + rank;
+      if (p && slot < pool_size) {
+        list[first + cg + rank] = e;
+        ev64[e].w_off = (size_t)slot * pool_stride;
+        skip[e] = kSkipSpec;
+      }
+      cg += tot;
+      __syncthreads();  // wcnt is rewritten by the next pass
+    }
+    const int kept = imax(0, imin(cg, pool_size - total));
+    if (tid == 0) count[g] = kept;
+    total += kept;
+  }
+  if (tid == 0) *pool_counter = total;
+}
+
 // One block of kChiThreads: each pass stages kChiThreads per-event terms in LDS
 // with one coalesced load, and thread 0 adds them in event order (the
 // reference's sequential sum; the loads of a pass are all in flight at once).
@@ -1018,6 +1099,10 @@ struct sat_batch {
   bool order_by_rounds = true;     // SAT_OPT_ORDER (SURFATT_ORDER=0 at creation: queues in batch order)
   int opt_pair2 = -1;              // SAT_OPT_PAIR2: -1 the library's rule (SURFATT_PAIR2 overrides), 0, 1
   bool opt_adj_cols = true;        // SAT_OPT_ADJ_COLS: the FP32 adjoint of one-wave grids on kAdjC columns
+  bool opt_spec = true;            // SAT_OPT_SPECULATE: FP64 from the start for recently re-solved pairs
+  unsigned char* d_age = nullptr;  // per event: runs its speculation lasts (spec_age_kernel)
+  int* d_spec = nullptr;           // speculated pairs per launch group, group g at group_first[g]
+  hipEvent_t spec_ev = nullptr;    // the column tables are ready: the speculated solves may start
   int* d_counter = nullptr;        // re-solve counts | fwd heads | adj heads | re-solve heads (one per group),
                                    // then one head per event for per-solve launches
   double* d_rec = nullptr;         // rec_lon | rec_lat | tt | w | syn | adjsrc
@@ -1118,7 +1203,7 @@ struct sat_batch {
     if (d_Kden) (void)hipFree(d_Kden);
     void* ps[] = {d_grids, d_grids_fwd, d_grids_seg, d_grids_adj, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_order_adj, d_order_tmp, d_group_rng, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
                   d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_ov, d_evx64, d_work64, d_slots64, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
-                  d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3};
+                  d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3, d_age, d_spec};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -1127,6 +1212,7 @@ struct sat_batch {
       if (e) (void)hipEventDestroy(e);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (fork2_ev) (void)hipEventDestroy(fork2_ev);
+    if (spec_ev) (void)hipEventDestroy(spec_ev);
     for (int i = 0; i < kSide; i++) {
       if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
       if (join2_ev[i]) (void)hipEventDestroy(join2_ev[i]);
@@ -1442,6 +1528,7 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   for (int i = 0; i < sat_batch::kMarks; i++) B_CHECK(hipEventCreate(&b->mk[i]));
   B_CHECK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
   B_CHECK(hipEventCreateWithFlags(&b->fork2_ev, hipEventDisableTiming));
+  B_CHECK(hipEventCreateWithFlags(&b->spec_ev, hipEventDisableTiming));
   for (int i = 0; i < sat_batch::kSide; i++) {
     B_CHECK(hipStreamCreate(&b->side[i]));
     B_CHECK(hipStreamCreate(&b->side2[i]));
@@ -1600,10 +1687,15 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(hipMemset(b->d_iters, 0, sizeof(int) * 3 * (size_t)std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_conv, 2 * (size_t)std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_skip, std::max(ne, 1)));
+  B_CHECK(hipMemset(b->d_skip, 0, std::max(ne, 1)));  // no re-solves before the first run
+  B_CHECK(dalloc(&b->d_age, std::max(ne, 1)));
+  B_CHECK(hipMemset(b->d_age, 0, std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_fb, (size_t)std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_ov, (size_t)std::max(ne, 1)));
-  // groups <= events; then one re-solve pool counter per chunk
-  B_CHECK(dalloc(&b->d_counter, 8 * (size_t)std::max(ne, 1) + 2 + b->chunk_first.size()));
+  B_CHECK(dalloc(&b->d_spec, (size_t)std::max(ne, 1)));
+  // groups <= events; then one re-solve pool counter per chunk, then the
+  // speculated pairs' counts and queue heads (3 per group)
+  B_CHECK(dalloc(&b->d_counter, 11 * (size_t)std::max(ne, 1) + 2 + b->chunk_first.size()));
 
   // per-event planes of the phase pipeline (sized for the largest chunk)
   B_CHECK(dalloc(&b->d_evs, 3 * (size_t)std::max(ne, 1)));
@@ -1773,7 +1865,14 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   int* ov_heads = b->d_counter + 6 * ng;
   int* solo_heads = b->d_counter + 7 * ng + 1;
   int* pool_counters = b->d_counter + 8 * ne1 + 2;  // one per chunk
-  const size_t n_counters = 8 * ne1 + 2 + b->chunk_first.size();
+  const size_t nch = b->chunk_first.size();
+  int* spec_count = b->d_counter + 8 * ne1 + 2 + nch;  // speculated pairs per group
+  int* spec_fheads = spec_count + ng;
+  int* spec_aheads = spec_count + 2 * ng;
+  const size_t n_counters = 8 * ne1 + 2 + nch + 3 * ng;
+  // FP64 solves from the start of the run for the pairs re-solved recently
+  // (spec_kernel): one-chunk batched or graph runs of FP32 batches with a band
+  const bool spec = f32 && b->opt_spec && b->f32_band > 0.0 && mode != SAT_RUN_PER_SOLVE && nch == 1 && ng && ne;
 
   BatchDev bd{};
   bd.grids = b->d_grids_seg;
@@ -1892,9 +1991,22 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
                        b->d_order_tmp, bd.iters_a, b->d_group_rng);
     SAT_CHECK(hipGetLastError());
   }
+  // the pairs' speculation ages from the previous run's re-solve marks, which
+  // reset_kernel clears
+  if (f32 && b->f32_band > 0.0 && ne) {
+    hipLaunchKernelGGL(spec_age_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, b->stream, b->d_age,
+                       b->d_skip, b->d_conv, b->f32_band, ne);
+    SAT_CHECK(hipGetLastError());
+  }
   hipLaunchKernelGGL(reset_kernel, dim3((unsigned)std::min<size_t>((n_counters + ne1 + 255) / 256, 4096)), dim3(256),
                      0, b->stream, b->d_counter, n_counters, b->d_skip, b->d_iters, bd.flags, b->d_base_flags, ne1);
   SAT_CHECK(hipGetLastError());
+  if (spec) {
+    hipLaunchKernelGGL(spec_kernel, dim3(1), dim3(kOrderThreads), 0, b->stream, b->d_age, b->d_skip, b->d_order,
+                       b->d_group_rng, (int)ng, b->d_spec, spec_count, pool_counters, b->pool_size, b->pool_stride,
+                       b->d_evx64);
+    SAT_CHECK(hipGetLastError());
+  }
 
   // The chunks run one after the other through the same workspace (one chunk
   // unless the batch exceeds the workspace cap).  The phase marks describe the
@@ -1940,82 +2052,94 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
       return fan_out_on(b->stream2, b->side2, b->fork2_ev, b->join2_ev, fn);
     };
 
+    // The FP64 phase kernels on stream2 over pool lists (group g's list at
+    // lists + group_first[g], its length counts[g]; queue heads fheads, aheads):
+    // init, forward sweeps, receivers, adjoint set-up, delta and adjoint sweeps
+    // of the events whose re-solve mark is `want`.
+    auto pool_pipeline = [&](int* lists, int* counts, int* fheads, int* aheads, int want) -> int {
+      hipStream_t s2 = b->stream2;
+      PhaseDev q = pc;
+      q.grids = b->d_grids64mw;
+      q.ev = b->d_evx64;
+      const size_t wb = b->pool_stride * (size_t)b->pool_size * sizeof(double);
+      for (int o = 0; o < 2; o++) {
+        q.t0[o] = b->d_work64 + (0 + o) * wb;
+        q.t0x[o] = b->d_work64 + (2 + o) * wb;
+        q.t0y[o] = b->d_work64 + (4 + o) * wb;
+        q.fxy[o] = b->d_work64 + 2 * o * wb;
+        q.dl[o] = b->d_work64 + (4 + o) * wb;
+      }
+      q.slots = b->d_slots64;
+      q.slot_bytes = b->slot_bytes64;
+      q.slot_plane = b->slot_plane64;
+      q.want_skip = want;
+      q.ta_f32 = 1;
+      q.scaled_iso = 1;
+      // the groups' re-solves run concurrently; each takes a share of the
+      // re-solve slots in proportion to its events, as the main sweeps do
+      auto res_slots = [&](int gi, PhaseDev& a, int ub) {
+        const int share = std::max(1, (int)(((long)b->nslots64p * b->group_count[gi] + cn - 1) / cn));
+        int base = 0;
+        for (size_t k = g_lo; k < (size_t)gi; k++)
+          base += std::max(1, (int)(((long)b->nslots64p * b->group_count[k] + cn - 1) / cn));
+        a.slots = b->d_slots64 + (size_t)base * b->slot_bytes64;
+        return dim3(std::min(ub, share));
+      };
+      int rc = fan_out2([&](int gi, hipStream_t st) -> int {
+        const int ub = std::min(b->group_count[gi], b->pool_size);
+        PhaseDev a = q;
+        a.list = lists + b->group_first[gi];
+        a.list_count = counts + gi;
+        SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(init_bx, ub, 2), st, a));
+        a.counter = fheads + gi;
+        const dim3 grid = res_slots(gi, a, ub);
+        // isotropic grids: the scaled update (init_kernel wrote scaled gradient planes)
+        const int rmode = b->group_iso[gi] != kMetricGeneral ? kMetricIso : kMetricGeneral;
+        SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(rmode, cmw(gi), wmw(gi), grid, st, a, 0, 0, gi));
+        return SAT_OK;
+      });
+      if (rc) return rc;
+      if (nrec_c) SAT_CHECK(launch_recv(dim3(rec_blocks), s2, q, nrec_c));
+      if (b->calc_adj) {
+        rc = fan_out2([&](int gi, hipStream_t st) -> int {
+          const int ub = std::min(b->group_count[gi], b->pool_size);
+          PhaseDev a = q;
+          a.list = lists + b->group_first[gi];
+          a.list_count = counts + gi;
+          SAT_CHECK(launch_adj_setup<SAT_PREC_F64>(dim3(node_bx, ub), st, a));
+          return SAT_OK;
+        });
+        if (rc) return rc;
+        SAT_CHECK(launch_delta<SAT_PREC_F64>(dim3((unsigned)cn), s2, q));  // one wave per event
+        rc = fan_out2([&](int gi, hipStream_t st) -> int {
+          const int ub = std::min(b->group_count[gi], b->pool_size);
+          PhaseDev a = q;
+          a.list = lists + b->group_first[gi];
+          a.list_count = counts + gi;
+          a.counter = aheads + gi;
+          const dim3 grid = res_slots(gi, a, ub);
+          SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(cmw(gi), wmw(gi), grid, st, a, 0, 0));
+          return SAT_OK;
+        });
+        if (rc) return rc;
+      }
+      return SAT_OK;
+    };
+
     // FP64 re-solves of the FP32 pairs whose stop decision was within the band,
-    // on stream2 once the forward sweeps (which list them) are done.  Batched
+    // on stream2 once the forward sweeps (which list them) are done (after the
+    // speculated pairs' solves queued there at the start of the run).  Batched
     // runs: the FP64 phase kernels over the pool lists, the per-pair kernel over
     // the overflow lists; per-solve runs: the per-pair kernel.
     auto launch_resolves = [&](bool phase) -> int {
       SAT_CHECK(hipEventRecord(mk[2], b->stream));
       SAT_CHECK(hipStreamWaitEvent(b->stream2, mk[2], 0));
-      SAT_CHECK(mark(5, b->stream2));
+      if (!spec) SAT_CHECK(mark(5, b->stream2));
       if (f32 && cn) {
         hipStream_t s2 = b->stream2;
         if (phase) {
-          PhaseDev q = pc;
-          q.grids = b->d_grids64mw;
-          q.ev = b->d_evx64;
-          const size_t wb = b->pool_stride * (size_t)b->pool_size * sizeof(double);
-          for (int o = 0; o < 2; o++) {
-            q.t0[o] = b->d_work64 + (0 + o) * wb;
-            q.t0x[o] = b->d_work64 + (2 + o) * wb;
-            q.t0y[o] = b->d_work64 + (4 + o) * wb;
-            q.fxy[o] = b->d_work64 + 2 * o * wb;
-            q.dl[o] = b->d_work64 + (4 + o) * wb;
-          }
-          q.slots = b->d_slots64;
-          q.slot_bytes = b->slot_bytes64;
-          q.slot_plane = b->slot_plane64;
-          q.want_skip = 1;
-          q.ta_f32 = 1;
-          q.scaled_iso = 1;
-          // the groups' re-solves run concurrently; each takes a share of the
-          // re-solve slots in proportion to its events, as the main sweeps do
-          auto res_slots = [&](int gi, PhaseDev& a, int ub) {
-            const int share = std::max(1, (int)(((long)b->nslots64p * b->group_count[gi] + cn - 1) / cn));
-            int base = 0;
-            for (size_t k = g_lo; k < (size_t)gi; k++)
-              base += std::max(1, (int)(((long)b->nslots64p * b->group_count[k] + cn - 1) / cn));
-            a.slots = b->d_slots64 + (size_t)base * b->slot_bytes64;
-            return dim3(std::min(ub, share));
-          };
-          int rc = fan_out2([&](int gi, hipStream_t st) -> int {
-            const int ub = std::min(b->group_count[gi], b->pool_size);
-            PhaseDev a = q;
-            a.list = b->d_fb + b->group_first[gi];
-            a.list_count = fb_count + gi;
-            SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(init_bx, ub, 2), st, a));
-            a.counter = resf_heads + gi;
-            const dim3 grid = res_slots(gi, a, ub);
-            // isotropic grids: the scaled update (init_kernel wrote scaled gradient planes)
-            const int rmode = b->group_iso[gi] != kMetricGeneral ? kMetricIso : kMetricGeneral;
-            SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(rmode, cmw(gi), wmw(gi), grid, st, a, 0, 0, gi));
-            return SAT_OK;
-          });
+          const int rc = pool_pipeline(b->d_fb, fb_count, resf_heads, resa_heads, 1);
           if (rc) return rc;
-          if (nrec_c) SAT_CHECK(launch_recv(dim3(rec_blocks), s2, q, nrec_c));
-          if (b->calc_adj) {
-            rc = fan_out2([&](int gi, hipStream_t st) -> int {
-              const int ub = std::min(b->group_count[gi], b->pool_size);
-              PhaseDev a = q;
-              a.list = b->d_fb + b->group_first[gi];
-              a.list_count = fb_count + gi;
-              SAT_CHECK(launch_adj_setup<SAT_PREC_F64>(dim3(node_bx, ub), st, a));
-              return SAT_OK;
-            });
-            if (rc) return rc;
-            SAT_CHECK(launch_delta<SAT_PREC_F64>(dim3((unsigned)cn), s2, q));  // one wave per event
-            rc = fan_out2([&](int gi, hipStream_t st) -> int {
-              const int ub = std::min(b->group_count[gi], b->pool_size);
-              PhaseDev a = q;
-              a.list = b->d_fb + b->group_first[gi];
-              a.list_count = fb_count + gi;
-              a.counter = resa_heads + gi;
-              const dim3 grid = res_slots(gi, a, ub);
-              SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(cmw(gi), wmw(gi), grid, st, a, 0, 0));
-              return SAT_OK;
-            });
-            if (rc) return rc;
-          }
         }
         for (size_t gi = g_lo; gi < g_hi; gi++) {
           BatchDev a = bd;
@@ -2069,6 +2193,15 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
       if (cn) {
         const dim3 col_grid((unsigned)((b->max_nx + 63) / 64), (unsigned)cn);
         SAT_CHECK(f32 ? launch_trig<SAT_PREC_F32>(col_grid, b->stream, pc) : launch_trig<SAT_PREC_F64>(col_grid, b->stream, pc));
+        if (spec) {
+          // the speculated pairs' FP64 solves start on stream2 once their column
+          // tables exist, beside the FP32 init and forward sweeps
+          SAT_CHECK(hipEventRecord(b->spec_ev, b->stream));
+          SAT_CHECK(hipStreamWaitEvent(b->stream2, b->spec_ev, 0));
+          SAT_CHECK(mark(5, b->stream2));
+          const int rc = pool_pipeline(b->d_spec, spec_count, spec_fheads, spec_aheads, kSkipSpec);
+          if (rc) return rc;
+        }
         PhaseDev pf = pc;  // the forward geometry (two pairs per wave where a grid runs them)
         pf.grids = b->d_grids_fwd;
         SAT_CHECK(f32 ? launch_init<SAT_PREC_F32>(dim3(init_bx, cn, 3), b->stream, pf)
@@ -2434,6 +2567,11 @@ int sat_batch_set_option(sat_batch* b, int option, int value) {
       if ((value != 0) != b->order_by_rounds) b->groups_dirty = true;
       b->order_by_rounds = value != 0;
       return SAT_OK;
+    case SAT_OPT_SPECULATE:
+      if (value < 0 || value > 1) return SAT_EINVAL;
+      if ((value != 0) != b->opt_spec) drop_graph(b);  // a different set of launches
+      b->opt_spec = value != 0;
+      return SAT_OK;
     default:
       return SAT_EINVAL;
   }
@@ -2445,6 +2583,7 @@ int sat_batch_get_option(sat_batch* b, int option, int* value) {
     case SAT_OPT_PAIR2: *value = b->opt_pair2; return SAT_OK;
     case SAT_OPT_ADJ_COLS: *value = b->opt_adj_cols ? 1 : 0; return SAT_OK;
     case SAT_OPT_ORDER: *value = b->order_by_rounds ? 1 : 0; return SAT_OK;
+    case SAT_OPT_SPECULATE: *value = b->opt_spec ? 1 : 0; return SAT_OK;
     default: return SAT_EINVAL;
   }
 }
@@ -2469,6 +2608,10 @@ int sat_batch_conv(sat_batch* b, double* linf_last, double* linf_prev, int* n_re
       SAT_CHECK(hipMemcpy(cnt.data(), b->d_counter, sizeof(int) * ng, hipMemcpyDeviceToHost));
       for (int c : cnt) *n_resolved += c;
       SAT_CHECK(hipMemcpy(cnt.data(), b->d_counter + 5 * ng, sizeof(int) * ng, hipMemcpyDeviceToHost));
+      for (int c : cnt) *n_resolved += c;
+      // the speculated pairs (solved in FP64 from the start of the run)
+      const size_t spec_off = 8 * (size_t)std::max(ne, 1) + 2 + b->chunk_first.size();
+      SAT_CHECK(hipMemcpy(cnt.data(), b->d_counter + spec_off, sizeof(int) * ng, hipMemcpyDeviceToHost));
       for (int c : cnt) *n_resolved += c;
     }
   }

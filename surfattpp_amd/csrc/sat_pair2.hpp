@@ -153,13 +153,15 @@ void fwd_sweep_kernel_pair2(PhaseDev pd, int first, int count, int group, int hs
   double linf = -1.0, linf_prev = -1.0;
   int rot = 0;        // B0 = Q[rot], B1 = Q[rot + 1], B2 = Q[rot + 2] (mod 3)
   for (;;) {
-    // a half without a pair takes the next one and writes its initial sigma into B0
-    if (!act && more) {
+    // a half without a pair takes the next one (passing over the pairs solved in
+    // FP64 on the second stream) and writes its initial sigma into B0
+    while (!act && more) {
       int idx = 0;
       if (lp == 0) idx = atomicAdd(pd.counter, 1);
       idx = __shfl(idx, half * kHalfWave);
       if (idx < n_work) {
         e = pd.order[first + idx];
+        if (pd.skip[e] == kSkipSpec) continue;
         e_last = e;
         act = true;
         it = 0;

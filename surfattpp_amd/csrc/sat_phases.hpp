@@ -70,7 +70,8 @@ struct PhaseDev {
   int* flags;
   double* conv;
   unsigned char* skip;  // FP32 batches: 0 FP32 result, 1 FP64 re-solve by these kernels,
-                        // 2 FP64 re-solve by the per-pair kernel (pool full)
+                        // 2 FP64 re-solve by the per-pair kernel (pool full), kSkipSpec
+                        // FP64 solve from the start of the run (speculated; no FP32 sweeps)
   int want_skip;        // the skip value of the events this pass processes (< 0: every event)
   int kden_pass;        // the density-kernel adjoint (src/preproc.cpp:128-136): receiver sources -w,
                         // the field not masked, syn, chi, iteration counts and flags left as they are
@@ -183,6 +184,9 @@ __device__ inline int pop_work(int* counter, int* s_idx) {
     return __builtin_amdgcn_readfirstlane(*s_idx);
   }
 }
+
+// The re-solve mark of a pair solved in FP64 from the start of its run.
+constexpr unsigned char kSkipSpec = 3;
 
 // Whether this pass processes event e (its re-solve mark is the pass's).
 __device__ __forceinline__ bool pass_event(const PhaseDev& pd, int e) {
@@ -411,6 +415,7 @@ void fwd_sweep_kernel(PhaseDev pd, int first, int count, int group) {
     const int idx = pop_work<W>(pd.counter, &s_idx);
     if (idx >= n_work) break;
     const int e = pd.list ? pd.list[idx] : pd.order[first + idx];
+    if (!pd.list && pd.skip[e] == kSkipSpec) continue;  // solved in FP64 on the second stream
     const EventX ev = pd.ev[e];
     const GridDev& g = pd.grids[ev.grid];
     const int nx = g.nx, n = (int)g.wplane;

@@ -158,12 +158,14 @@ class EikonalBatch:
         _lib.check("sat_batch_waves", _lib.load().sat_batch_waves(self._h, buf))
         return list(buf)
 
-    OPTIONS = {"pair2": _lib.OPT_PAIR2, "adj_cols": _lib.OPT_ADJ_COLS, "order": _lib.OPT_ORDER}
+    OPTIONS = {"pair2": _lib.OPT_PAIR2, "adj_cols": _lib.OPT_ADJ_COLS, "order": _lib.OPT_ORDER,
+               "speculate": _lib.OPT_SPECULATE}
 
     def set_option(self, name: str, value: int):
         """A scheduling option (sat_batch_set_option): pair2 (-1 the library's
-        rule, 0, 1), adj_cols (0, 1), order (0, 1).  Results do not depend on
-        them beyond FP32 rounding."""
+        rule, 0, 1), adj_cols (0, 1), order (0, 1), speculate (0, 1).  Results
+        do not depend on them beyond FP32 rounding (speculate: pairs re-solved
+        in FP64 in recent runs are solved in FP64 from the start of the run)."""
         _lib.check("sat_batch_set_option", _lib.load().sat_batch_set_option(self._h, self.OPTIONS[name], int(value)))
 
     def get_option(self, name: str) -> int:
