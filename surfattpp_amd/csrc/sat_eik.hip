@@ -384,6 +384,9 @@ __global__ __launch_bounds__(kOrderThreads) void order_kernel(int* order, int* t
 // Runs a prediction lasts: refreshed while the pair is re-solved by the band
 // rule, or speculated and its FP64 stop decision is within the band itself.
 constexpr unsigned char kSpecAge = 8;
+// Counters of the speculated solves: the three list lengths (spec_kernel), the
+// forward queue heads of the two update kinds, the adjoint queue head.
+constexpr int kSpecCounters = 8;
 
 // Before a run's reset: each pair's prediction age from the previous run's
 // re-solve marks (and, for speculated pairs, their FP64 convergence norms).
@@ -405,22 +408,24 @@ __global__ void spec_age_kernel(unsigned char* age, const unsigned char* skip, c
   age[e] = a;
 }
 
-// After the reset: the speculated pairs of every launch group, in queue order
-// (group g's list at list[rng[g].x], its length count[g]), each with a slot of
-// the FP64 pool, in group order until the pool is full; the pool counter then
-// starts the band rule's re-solves past them.  One block: the choice of pairs
-// does not depend on timing.
+// After the reset: the speculated pairs, in launch-group and queue order, each
+// with a slot of the FP64 pool until the pool is full: list[0, count[0]) holds
+// all of them, list[ne, ne + count[1]) those of the groups whose re-solves take
+// the isotropic update, list[2 ne, 2 ne + count[2]) the others; the pool
+// counter then starts the band rule's re-solves past them.  One block: the
+// choice of pairs does not depend on timing.
 __global__ __launch_bounds__(kOrderThreads) void spec_kernel(const unsigned char* age, unsigned char* skip,
-                                                             const int* order, const int2* rng, int ng, int* list,
-                                                             int* count, int* pool_counter, int pool_size,
-                                                             size_t pool_stride, EventX* ev64) {
+                                                             const int* order, const int2* rng, const int* group_iso,
+                                                             int ng, int ne, int* list, int* count,
+                                                             int* pool_counter, int pool_size, size_t pool_stride,
+                                                             EventX* ev64) {
   constexpr int kWaves = kOrderThreads / kWave;
   __shared__ int wcnt[kWaves];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
-  int total = 0;
+  int total = 0, n_iso = 0, n_gen = 0;
   for (int g = 0; g < ng; g++) {
     const int first = rng[g].x, n = rng[g].y;
-    int cg = 0;
+    const bool iso = group_iso[g] != 0;
     for (int base = 0; base < n; base += kOrderThreads) {
       const int i = base + tid;
       const int e = i < n ? order[first + i] : -1;
@@ -433,20 +438,26 @@ __global__ __launch_bounds__(kOrderThreads) void spec_kernel(const unsigned char
         if (w < wv) rank += wcnt[w];
         tot += wcnt[w];
       }
-      const int slot = total + cg + rank;
+      const int slot = total + rank;
       if (p && slot < pool_size) {
-        list[first + cg + rank] = e;
+        list[slot] = e;
+        list[(iso ? ne + n_iso : 2 * ne + n_gen) + rank] = e;
         ev64[e].w_off = (size_t)slot * pool_stride;
         skip[e] = kSkipSpec;
       }
-      cg += tot;
+      const int kept = imax(0, imin(tot, pool_size - total));
+      total += kept;
+      if (iso) n_iso += kept;
+      else n_gen += kept;
       __syncthreads();  // wcnt is rewritten by the next pass
     }
-    const int kept = imax(0, imin(cg, pool_size - total));
-    if (tid == 0) count[g] = kept;
-    total += kept;
   }
-  if (tid == 0) *pool_counter = total;
+  if (tid == 0) {
+    count[0] = total;
+    count[1] = n_iso;
+    count[2] = n_gen;
+    *pool_counter = total;
+  }
 }
 
 // One block of kChiThreads: each pass stages kChiThreads per-event terms in LDS
@@ -609,7 +620,7 @@ static size_t elem_size(int prec) { return prec == SAT_PREC_F64 ? sizeof(double)
 // wave: also the wavefront-major media of the batched sweeps, laid out for W
 // waves per pair (W * 64 lanes, ceil(ceil(nx / 64) / W) columns per lane).
 static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const double* lon, const double* lat,
-                      bool wave, int W = 1, bool polish = false) {
+                      bool wave, int W = 1, bool polish = false, int wr = 0) {
   if (nx < 3 || ny < 3 || nper < 1 || !lon || !lat) return SAT_EINVAL;
   if (!uniform_axis(lon, nx) || !uniform_axis(lat, ny)) return SAT_EINVAL;
   const Layout L = layout_for(nx);
@@ -689,11 +700,11 @@ static int grid_setup(GridHost& G, int prec, int nx, int ny, int nper, const dou
       g.wf64[1] = G.d_wf64 + pf;
     }
     if (prec == SAT_PREC_F32) {
-      // re-solve geometry: Wr = resolve_waves(ceil(nx / 64)) waves per grid,
-      // C = ceil(nx / (64 Wr))
+      // re-solve geometry: Wr = resolve_waves(ceil(nx / 64)) waves per grid, or
+      // the batch's common wr, C = ceil(nx / (64 Wr))
       GridDev& m = G.gmw;
       m = G.g64;
-      const WaveGeom wg = wave_geom(nx, ny, resolve_waves((nx + kWave - 1) / kWave));
+      const WaveGeom wg = wave_geom(nx, ny, wr > 0 ? wr : resolve_waves((nx + kWave - 1) / kWave));
       m.C = wg.C;
       m.La = wg.La;
       m.P = m.La * m.C;
@@ -1101,7 +1112,9 @@ struct sat_batch {
   bool opt_adj_cols = true;        // SAT_OPT_ADJ_COLS: the FP32 adjoint of one-wave grids on kAdjC columns
   bool opt_spec = true;            // SAT_OPT_SPECULATE: FP64 from the start for recently re-solved pairs
   unsigned char* d_age = nullptr;  // per event: runs its speculation lasts (spec_age_kernel)
-  int* d_spec = nullptr;           // speculated pairs per launch group, group g at group_first[g]
+  int* d_spec = nullptr;           // speculated pairs: all | isotropic re-solve update | general (spec_kernel)
+  int* d_group_iso = nullptr;      // per launch group: 1 if its re-solves take the isotropic update
+  int wr_all = 0;                  // FP32 batches: waves per pair of every grid's re-solves (0: per grid)
   hipEvent_t spec_ev = nullptr;    // the column tables are ready: the speculated solves may start
   int* d_counter = nullptr;        // re-solve counts | fwd heads | adj heads | re-solve heads (one per group),
                                    // then one head per event for per-solve launches
@@ -1154,6 +1167,7 @@ struct sat_batch {
   // per group: sweep slots [group_slot0, group_slot0 + group_nblk); launch order
   // of the groups (most work first)
   std::vector<int> group_slot0, group_nblk, launch_order;
+  std::vector<double> group_work;  // sweep steps x columns per lane over the group's pairs
   // per group: dynamic LDS of its one-wave per-pair launches (the adjoint's LDS
   // tier; 0 where no grid of the group fits), batch precision and FP64
   std::vector<size_t> group_lds, group_lds64;
@@ -1203,7 +1217,7 @@ struct sat_batch {
     if (d_Kden) (void)hipFree(d_Kden);
     void* ps[] = {d_grids, d_grids_fwd, d_grids_seg, d_grids_adj, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_order_adj, d_order_tmp, d_group_rng, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
                   d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_ov, d_evx64, d_work64, d_slots64, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
-                  d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3, d_age, d_spec};
+                  d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3, d_age, d_spec, d_group_iso};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -1272,6 +1286,7 @@ int build_groups(sat_batch* b) {
   b->group_lds.clear();
   b->group_lds64.clear();
   b->launch_order.clear();
+  b->group_work.clear();
   b->chunk_g0.assign(1, 0);
   const int nchunk = (int)b->chunk_first.size();
   for (int c = 0; c < nchunk; c++) {
@@ -1327,6 +1342,7 @@ int build_groups(sat_batch* b) {
       b->group_lds.push_back(lds);
       b->group_lds64.push_back(lds64);
     }
+    b->group_work.insert(b->group_work.end(), work.begin(), work.end());
     std::vector<int> lo(g_hi - g_lo);
     std::iota(lo.begin(), lo.end(), (int)g_lo);
     std::stable_sort(lo.begin(), lo.end(), [&](int a, int d) { return work[a - g_lo] > work[d - g_lo]; });
@@ -1339,6 +1355,9 @@ int build_groups(sat_batch* b) {
     std::vector<int2> rng(b->group_first.size());
     for (size_t gi = 0; gi < rng.size(); gi++) rng[gi] = make_int2(b->group_first[gi], b->group_count[gi]);
     if (!rng.empty()) SAT_CHECK(hipMemcpy(b->d_group_rng, rng.data(), sizeof(int2) * rng.size(), hipMemcpyHostToDevice));
+    std::vector<int> giso(b->group_iso.size());
+    for (size_t gi = 0; gi < giso.size(); gi++) giso[gi] = b->group_iso[gi] != kMetricGeneral ? 1 : 0;
+    if (!giso.empty()) SAT_CHECK(hipMemcpy(b->d_group_iso, giso.data(), sizeof(int) * giso.size(), hipMemcpyHostToDevice));
   }
   std::vector<GridDev> hg(b->grids.size()), hg64(b->grids.size()), hgmw(b->grids.size()), hgs(b->grids.size());
   std::vector<GridDev> hgf(b->grids.size()), hga(b->grids.size());
@@ -1545,12 +1564,26 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
     rc = choose_waves(*d, f32, b->polish, prop.multiProcessorCount, waves);
     if (rc) return fail(rc);
   }
+  // FP32 batches whose grids (up to 512 columns) would re-solve on different
+  // geometries re-solve all of them on kMultiWaveWide waves of one column per
+  // lane, so that one launch per phase covers every speculated pair
+  if (f32) {
+    int w0 = 0, c0 = 0;
+    for (int gi = 0; gi < d->ngrid; gi++) {
+      const int c1 = (d->grids[gi].nx + kWave - 1) / kWave;
+      if (c1 > kMaxC) continue;  // a wide grid keeps its own geometry
+      const int w = resolve_waves(c1), c = (c1 + w - 1) / w;
+      if (w0 && (w != w0 || c != c0)) b->wr_all = kMultiWaveWide;
+      w0 = w;
+      c0 = c;
+    }
+  }
   size_t max_seg = 0, max_plane = 0, max_wplane = 0;
   int max_nx = 0;
   for (int gi = 0; gi < d->ngrid; gi++) {
     const sat_grid_desc& gd = d->grids[gi];
     rc = grid_setup(b->grids[gi], b->precision, gd.nx, gd.ny, gd.nperiod, gd.lon_deg, gd.lat_deg, true, waves[gi],
-                    b->polish);
+                    b->polish, b->wr_all);
     if (rc) return fail(rc);
     rc = grid_media(b->grids[gi], 0, gd.svel, gd.m11, gd.m12, gd.m22, b->stream);
     if (rc) return fail(rc);
@@ -1692,10 +1725,11 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(hipMemset(b->d_age, 0, std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_fb, (size_t)std::max(ne, 1)));
   B_CHECK(dalloc(&b->d_ov, (size_t)std::max(ne, 1)));
-  B_CHECK(dalloc(&b->d_spec, (size_t)std::max(ne, 1)));
+  B_CHECK(dalloc(&b->d_spec, 3 * (size_t)std::max(ne, 1)));
+  B_CHECK(dalloc(&b->d_group_iso, (size_t)std::max(ne, 1)));  // groups <= events
   // groups <= events; then one re-solve pool counter per chunk, then the
-  // speculated pairs' counts and queue heads (3 per group)
-  B_CHECK(dalloc(&b->d_counter, 11 * (size_t)std::max(ne, 1) + 2 + b->chunk_first.size()));
+  // speculated pairs' counts and queue heads
+  B_CHECK(dalloc(&b->d_counter, 8 * (size_t)std::max(ne, 1) + 2 + b->chunk_first.size() + kSpecCounters));
 
   // per-event planes of the phase pipeline (sized for the largest chunk)
   B_CHECK(dalloc(&b->d_evs, 3 * (size_t)std::max(ne, 1)));
@@ -1866,13 +1900,18 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   int* solo_heads = b->d_counter + 7 * ng + 1;
   int* pool_counters = b->d_counter + 8 * ne1 + 2;  // one per chunk
   const size_t nch = b->chunk_first.size();
-  int* spec_count = b->d_counter + 8 * ne1 + 2 + nch;  // speculated pairs per group
-  int* spec_fheads = spec_count + ng;
-  int* spec_aheads = spec_count + 2 * ng;
-  const size_t n_counters = 8 * ne1 + 2 + nch + 3 * ng;
+  int* spec_count = b->d_counter + 8 * ne1 + 2 + nch;  // spec_kernel's list lengths, then queue heads
+  const size_t n_counters = 8 * ne1 + 2 + nch + kSpecCounters;
   // FP64 solves from the start of the run for the pairs re-solved recently
-  // (spec_kernel): one-chunk batched or graph runs of FP32 batches with a band
-  const bool spec = f32 && b->opt_spec && b->f32_band > 0.0 && mode != SAT_RUN_PER_SOLVE && nch == 1 && ng && ne;
+  // (spec_kernel): one-chunk batched or graph runs of FP32 batches with a band,
+  // no grid wider than 512 columns
+  bool any_wide = false, spec_mode[2] = {false, false};  // isotropic / general re-solve update
+  for (size_t gi = 0; gi < ng; gi++) {
+    any_wide |= b->group_wide[gi] != 0;
+    spec_mode[b->group_iso[gi] != kMetricGeneral ? 0 : 1] = true;
+  }
+  const bool spec = f32 && b->opt_spec && b->f32_band > 0.0 && mode != SAT_RUN_PER_SOLVE && nch == 1 && ng && ne &&
+                    !any_wide;
 
   BatchDev bd{};
   bd.grids = b->d_grids_seg;
@@ -1971,7 +2010,9 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   // waves per pair and columns per lane of the re-solve geometry:
   // Wr = resolve_waves(C), ceil(nx / (64 Wr)) = ceil(C / Wr)
   // (wide grids: the batch's own W waves of kMaxC columns)
-  auto wmw = [&](size_t gi) { return b->group_wide[gi] ? b->group_W[gi] : resolve_waves(b->group_C[gi]); };
+  auto wmw = [&](size_t gi) {
+    return b->group_wide[gi] ? b->group_W[gi] : (b->wr_all ? b->wr_all : resolve_waves(b->group_C[gi]));
+  };
   auto cmw = [&](size_t gi) { return b->group_wide[gi] ? kMaxC : (b->group_C[gi] + wmw(gi) - 1) / wmw(gi); };
   // waves of the one-pair kernels (pair_kernel) of a group
   auto wseg = [&](size_t gi) { return b->group_wide[gi] ? b->group_W[gi] : 1; };
@@ -2003,8 +2044,8 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   SAT_CHECK(hipGetLastError());
   if (spec) {
     hipLaunchKernelGGL(spec_kernel, dim3(1), dim3(kOrderThreads), 0, b->stream, b->d_age, b->d_skip, b->d_order,
-                       b->d_group_rng, (int)ng, b->d_spec, spec_count, pool_counters, b->pool_size, b->pool_stride,
-                       b->d_evx64);
+                       b->d_group_rng, b->d_group_iso, (int)ng, (int)ne1, b->d_spec, spec_count, pool_counters,
+                       b->pool_size, b->pool_stride, b->d_evx64);
     SAT_CHECK(hipGetLastError());
   }
 
@@ -2026,17 +2067,25 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
 
     // Launch fn(gi, stream) for every group of the chunk, concurrently: from the
     // base stream (main or stream2) over its side streams, joined back at the end.
-    auto fan_out_on = [&](hipStream_t base, hipStream_t* sides, hipEvent_t fork, hipEvent_t* joins,
+    // A stream's kernels run one after the other (so do those of streams that
+    // share a hardware queue): each group goes to the stream with the least work
+    // so far, in launch order (largest first).
+    auto fan_out_on = [&](hipStream_t base, hipStream_t* sides, hipEvent_t fork, hipEvent_t* joins, int max_streams,
                           auto&& fn) -> int {
       const size_t ngc = g_hi - g_lo;
-      const int nused = (int)std::min<size_t>(ngc, sat_batch::kSide + 1);
+      const int nused = (int)std::min<size_t>(ngc, (size_t)max_streams);
       if (nused > 1) {
         SAT_CHECK(hipEventRecord(fork, base));
         for (int k = 1; k < nused; k++) SAT_CHECK(hipStreamWaitEvent(sides[k - 1], fork, 0));
       }
+      double load[sat_batch::kSide + 1] = {};
       for (size_t k = 0; k < ngc; k++) {
-        const int sk = (int)(k % (size_t)nused);
-        const int rc = fn(b->launch_order[g_lo + k], sk == 0 ? base : sides[sk - 1]);
+        const int gi = b->launch_order[g_lo + k];
+        int sk = 0;
+        for (int j = 1; j < nused; j++)
+          if (load[j] < load[sk]) sk = j;
+        load[sk] += b->group_work[gi];
+        const int rc = fn(gi, sk == 0 ? base : sides[sk - 1]);
         if (rc) return rc;
       }
       if (nused > 1) {
@@ -2047,17 +2096,22 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
       }
       return SAT_OK;
     };
-    auto fan_out = [&](auto&& fn) -> int { return fan_out_on(b->stream, b->side, b->fork_ev, b->join_ev, fn); };
+    // the main launch groups keep one hardware queue free for the speculated
+    // solves on stream2 (streams are spread over the device's queues in creation
+    // order: stream, stream2, then the side streams)
+    const int main_streams = spec ? sat_batch::kSide : sat_batch::kSide + 1;
+    auto fan_out = [&](auto&& fn) -> int {
+      return fan_out_on(b->stream, b->side, b->fork_ev, b->join_ev, main_streams, fn);
+    };
     auto fan_out2 = [&](auto&& fn) -> int {
-      return fan_out_on(b->stream2, b->side2, b->fork2_ev, b->join2_ev, fn);
+      return fan_out_on(b->stream2, b->side2, b->fork2_ev, b->join2_ev, sat_batch::kSide + 1, fn);
     };
 
     // The FP64 phase kernels on stream2 over pool lists (group g's list at
     // lists + group_first[g], its length counts[g]; queue heads fheads, aheads):
     // init, forward sweeps, receivers, adjoint set-up, delta and adjoint sweeps
     // of the events whose re-solve mark is `want`.
-    auto pool_pipeline = [&](int* lists, int* counts, int* fheads, int* aheads, int want) -> int {
-      hipStream_t s2 = b->stream2;
+    auto pool_phase = [&](int want) {
       PhaseDev q = pc;
       q.grids = b->d_grids64mw;
       q.ev = b->d_evx64;
@@ -2075,6 +2129,11 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
       q.want_skip = want;
       q.ta_f32 = 1;
       q.scaled_iso = 1;
+      return q;
+    };
+    auto pool_pipeline = [&](int* lists, int* counts, int* fheads, int* aheads, int want) -> int {
+      hipStream_t s2 = b->stream2;
+      const PhaseDev q = pool_phase(want);
       // the groups' re-solves run concurrently; each takes a share of the
       // re-solve slots in proportion to its events, as the main sweeps do
       auto res_slots = [&](int gi, PhaseDev& a, int ub) {
@@ -2122,6 +2181,40 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
           return SAT_OK;
         });
         if (rc) return rc;
+      }
+      return SAT_OK;
+    };
+
+    // The speculated pairs' FP64 solves: one chain on stream2, each phase one
+    // launch over all of them (the batch's grids share one re-solve geometry):
+    // init, forward sweeps (one launch per update kind), receivers, adjoint
+    // set-up, delta, adjoint sweeps.
+    auto spec_pipeline = [&]() -> int {
+      hipStream_t s2 = b->stream2;
+      const PhaseDev q = pool_phase(kSkipSpec);
+      size_t g0 = g_lo;
+      while (b->group_wide[g0]) g0++;  // (no wide group when speculating)
+      const int Wr = wmw(g0), Cr = cmw(g0);
+      const int ub = b->pool_size;
+      const dim3 sgrid((unsigned)std::min(ub, b->nslots64p));
+      PhaseDev a = q;
+      a.list = b->d_spec;
+      a.list_count = spec_count;
+      SAT_CHECK(launch_init<SAT_PREC_F64>(dim3(init_bx, ub, 2), s2, a));
+      for (int m = 0; m < 2; m++) {
+        if (!spec_mode[m]) continue;
+        PhaseDev f = q;
+        f.list = b->d_spec + (size_t)(1 + m) * ne1;
+        f.list_count = spec_count + 1 + m;
+        f.counter = spec_count + 3 + m;
+        SAT_CHECK(launch_fwd_sweep<SAT_PREC_F64>(m == 0 ? kMetricIso : kMetricGeneral, Cr, Wr, sgrid, s2, f, 0, 0, 0));
+      }
+      if (nrec_c) SAT_CHECK(launch_recv(dim3(rec_blocks), s2, q, nrec_c));
+      if (b->calc_adj) {
+        SAT_CHECK(launch_adj_setup<SAT_PREC_F64>(dim3(node_bx, ub), s2, a));
+        SAT_CHECK(launch_delta<SAT_PREC_F64>(dim3((unsigned)cn), s2, q));  // one wave per event
+        a.counter = spec_count + 5;
+        SAT_CHECK(launch_adj_sweep<SAT_PREC_F64>(Cr, Wr, sgrid, s2, a, 0, 0));
       }
       return SAT_OK;
     };
@@ -2199,7 +2292,7 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
           SAT_CHECK(hipEventRecord(b->spec_ev, b->stream));
           SAT_CHECK(hipStreamWaitEvent(b->stream2, b->spec_ev, 0));
           SAT_CHECK(mark(5, b->stream2));
-          const int rc = pool_pipeline(b->d_spec, spec_count, spec_fheads, spec_aheads, kSkipSpec);
+          const int rc = spec_pipeline();
           if (rc) return rc;
         }
         PhaseDev pf = pc;  // the forward geometry (two pairs per wave where a grid runs them)
@@ -2611,8 +2704,9 @@ int sat_batch_conv(sat_batch* b, double* linf_last, double* linf_prev, int* n_re
       for (int c : cnt) *n_resolved += c;
       // the speculated pairs (solved in FP64 from the start of the run)
       const size_t spec_off = 8 * (size_t)std::max(ne, 1) + 2 + b->chunk_first.size();
-      SAT_CHECK(hipMemcpy(cnt.data(), b->d_counter + spec_off, sizeof(int) * ng, hipMemcpyDeviceToHost));
-      for (int c : cnt) *n_resolved += c;
+      int ns = 0;
+      SAT_CHECK(hipMemcpy(&ns, b->d_counter + spec_off, sizeof(int), hipMemcpyDeviceToHost));
+      *n_resolved += ns;
     }
   }
   return SAT_OK;
