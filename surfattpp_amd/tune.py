@@ -16,9 +16,13 @@ scheduling options (sat_batch_set_option) the same way, one at a time:
   adj_cols  the FP32 adjoint of one-wave grids on 8 columns per lane or on
             the forward's columns per lane,
   order     the queues popped longest pairs (of the previous run) first, or
-            in batch order.
+            in batch order,
+  speculate the pairs re-solved in FP64 in recent runs solved in FP64 from
+            the start of the run, beside the FP32 forward sweeps, or only
+            after them (the band rule alone).
 Every choice performs the same Gauss-Seidel order (results agree to FP32
-rounding), so only time is compared.  The FP32 re-solve band is not tuned: it
+rounding; a speculated pair has its FP64 result, which the band rule gives it
+when the inputs repeat, as they do here), so only time is compared.  The FP32 re-solve band is not tuned: it
 is the margin of the iteration-count parity, not a speed knob; neither is the
 number of resident waves, which sizes the workspace at creation.  The choice
 is stored in a JSON cache keyed by the device, the precision and the batch's
@@ -99,7 +103,7 @@ def _option_trials(grids) -> list:
     trials = []
     if any(g["lon"].size <= 64 for g in grids):
         trials += [("pair2", 0), ("pair2", 1)]
-    trials += [("adj_cols", 0), ("order", 0)]
+    trials += [("adj_cols", 0), ("order", 0), ("speculate", 0)]
     return trials
 
 

@@ -21,11 +21,11 @@ def test_autotune_measures_caches_and_reuses(tmp_path, monkeypatch):
     print("tuned", e["waves"], e["ms"], "library", e["default_waves"], e["default_ms"], flush=True)
     assert all(w in (1, 2, 4) for w in e["waves"])
     assert e["ms"] <= e["default_ms"]
-    # waves of each grid, then the options (no grid of <= 64 columns: adj_cols, order)
-    assert len(e["trials"]) >= 5
+    # waves of each grid, then the options (no grid of <= 64 columns: adj_cols, order, speculate)
+    assert len(e["trials"]) >= 6
     tried = [t[1] for t in e["trials"] if t[1]]
-    assert {"adj_cols": 0} in tried and {"order": 0} in tried
-    assert set(e["options"]) <= {"pair2", "adj_cols", "order"}
+    assert {"adj_cols": 0} in tried and {"order": 0} in tried and {"speculate": 0} in tried
+    assert set(e["options"]) <= {"pair2", "adj_cols", "order", "speculate"}
     cache = json.load(open(path))
     assert list(cache.values())[0]["waves"] == e["waves"]
     assert list(cache.values())[0]["options"] == e["options"]
