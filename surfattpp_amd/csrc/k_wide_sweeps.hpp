@@ -37,7 +37,8 @@ hipError_t launch_fwd_sweep_wide(int mode, int W, dim3 grid, hipStream_t s, cons
 template <int PREC>
 hipError_t launch_adj_sweep_wide(int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count) {
   auto go = [&]<int WW>() {
-    hipLaunchKernelGGL((adj_sweep_kernel<PREC, kMaxC, WW>), grid, dim3(kWave * WW), 0, s, pd, first, count);
+    hipLaunchKernelGGL((adj_sweep_kernel<PREC, kMaxC, WW>), grid, dim3(kWave * WW), adj_dyn_lds<PREC>(pd), s, pd,
+                       first, count);
   };
   if (W == 2) go.template operator()<2>();
   else if (W == 4) go.template operator()<4>();

@@ -1143,6 +1143,8 @@ struct sat_batch {
   char* d_work = nullptr;          // wavefront-major: t0, t0x, t0y x 2 orientations, then the
                                    // adjoint's face fluxes and delta x 2 in the same bytes
   double* d_T0d = nullptr;
+  unsigned long long* d_dmask = nullptr;  // FP32: per event, the delta row mask (PhaseDev::dmask)
+  int dmask_stride = 0;
   double* d_T = nullptr;
   char* d_ta = nullptr;            // masked Ta, sweep precision
   char* d_slots = nullptr;         // phase sweep slots
@@ -1217,7 +1219,7 @@ struct sat_batch {
     if (d_Kden) (void)hipFree(d_Kden);
     void* ps[] = {d_grids, d_grids_fwd, d_grids_seg, d_grids_adj, d_grids64, d_grids64mw, d_events, d_evx, d_order, d_order_adj, d_order_tmp, d_group_rng, d_counter, d_rec, d_rec_ev, d_rcell, d_rwt, d_chi_ev,
                   d_chi, d_iters, d_base_flags, d_conv, d_skip, d_fb, d_ov, d_evx64, d_work64, d_slots64, d_evs, d_coltab, d_work, d_T0d, d_T, d_ta,
-                  d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3, d_age, d_spec, d_group_iso};
+                  d_slots, d_ws, d_ws64, d_K, d_red_sz, d_red_i, (void*)d_red_v3, d_age, d_spec, d_group_iso, d_dmask};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -1739,6 +1741,21 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   B_CHECK(dalloc(&b->d_work, work_bytes));
   B_CHECK(hipMemset(b->d_work, 0, work_bytes));
   if (f32) B_CHECK(dalloc(&b->d_T0d, std::max<size_t>(pl_total, 1)));
+  if (f32) {
+    // the delta row masks: per event, ny rows of one bit per lane of either
+    // adjoint geometry; the adjoint sweeps hold one in LDS (beyond kMaskLds
+    // bytes the planes are read whole instead)
+    constexpr size_t kMaskLds = 48 << 10;
+    size_t stride = 1;
+    for (const GridHost& G : b->grids) {
+      const int la = std::max(G.g.La, G.adj8 ? G.gadj.La : 0);
+      stride = std::max(stride, (size_t)G.g.ny * (size_t)((la + kWave - 1) / kWave));
+    }
+    if (stride * sizeof(unsigned long long) <= kMaskLds) {
+      b->dmask_stride = (int)stride;
+      B_CHECK(dalloc(&b->d_dmask, stride * (size_t)std::max(ne, 1)));
+    }
+  }
   B_CHECK(dalloc(&b->d_T, std::max<size_t>(pl_total, 1)));
   if (b->calc_adj) {
     B_CHECK(dalloc(&b->d_ta, std::max<size_t>(pl_total, 1) * es));
@@ -1981,6 +1998,9 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
     for (const EventX& x : b->hx) max_nrec = std::max(max_nrec, x.nrec);
     pd.delta_cap = std::min(kDeltaShares, std::max(4 * max_nrec, kScan));
   }
+  pd.dmask = b->d_dmask;
+  pd.dmask_stride = b->dmask_stride;
+  pd.dmask_lds = f32 ? (int)(b->d_dmask ? b->dmask_stride * sizeof(unsigned long long) : sizeof(unsigned long long)) : 0;
   pd.ov_list = b->d_ov;
   pd.ov_count = ov_count;
   pd.pool_counter = pool_counters;

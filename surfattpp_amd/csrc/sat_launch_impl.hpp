@@ -143,7 +143,8 @@ template <int PREC>
 hipError_t launch_adj_sweep(int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count) {
   if (W != 1) return launch_adj_sweep_mw<PREC>(C, W, grid, s, pd, first, count);
   return with_C(C, [&]<int CC>() {
-    hipLaunchKernelGGL((adj_sweep_kernel<PREC, CC, 1>), grid, dim3(kWave), 0, s, pd, first, count);
+    hipLaunchKernelGGL((adj_sweep_kernel<PREC, CC, 1>), grid, dim3(kWave), adj_dyn_lds<PREC>(pd), s, pd, first,
+                       count);
   });
 }
 
@@ -151,7 +152,8 @@ template <int PREC>
 hipError_t launch_adj_sweep_mw(int C, int W, dim3 grid, hipStream_t s, const PhaseDev& pd, int first, int count) {
   if (C == kMaxC) return launch_adj_sweep_wide<PREC>(W, grid, s, pd, first, count);
   auto go = [&]<int WW, int CC>() {
-    hipLaunchKernelGGL((adj_sweep_kernel<PREC, CC, WW>), grid, dim3(kWave * WW), 0, s, pd, first, count);
+    hipLaunchKernelGGL((adj_sweep_kernel<PREC, CC, WW>), grid, dim3(kWave * WW), adj_dyn_lds<PREC>(pd), s, pd,
+                       first, count);
   };
   if constexpr (PREC == SAT_PREC_F64) {
     if (W == kMultiWave && C == 1) go.template operator()<kMultiWave, 1>();

@@ -76,6 +76,12 @@ struct PhaseDev {
   int kden_pass;        // the density-kernel adjoint (src/preproc.cpp:128-136): receiver sources -w,
                         // the field not masked, syn, chi, iteration counts and flags left as they are
   int delta_cap;        // shares per event the receiver spread holds in LDS (launch's dynamic LDS)
+  unsigned long long* dmask;  // FP32 batches: per event, dmask_stride words: per row of the adjoint
+                              // geometry, ceil(La / 64) words of one bit per lane, set where the
+                              // lane's columns hold a receiver share (delta_kernel); null: the adjoint
+                              // reads every delta (adj_setup_kernel zeroes the planes)
+  int dmask_stride;
+  int dmask_lds;        // bytes of dynamic LDS of the FP32 adjoint sweeps (one event's mask)
   int* fb_list;         // FP32 forward: pool re-solves, per launch group
   int* fb_count;
   int* ov_list;         // FP32 forward: re-solves beyond the pool, per launch group
@@ -505,11 +511,12 @@ __global__ void adj_setup_kernel(PhaseDev pd) {
   F v;
   if constexpr (PREC == SAT_PREC_F64) v = make_double2(fx, fy);
   else v = make_float2((float)(fx / g.dx), (float)(fy / g.dy));
+  const bool zero_dl = PREC == SAT_PREC_F64 || !pd.dmask;  // (FP32 with the row mask: delta_kernel)
 #pragma unroll
   for (int o = 0; o < 2; o++) {
     const size_t w = ev.w_off + wave_index(g, o, ix, iy);
     static_cast<F*>(pd.fxy[o])[w] = v;
-    static_cast<V*>(pd.dl[o])[w] = V(0);
+    if (zero_dl) static_cast<V*>(pd.dl[o])[w] = V(0);
   }
 }
 
@@ -564,6 +571,34 @@ __global__ __launch_bounds__(kWave) void delta_kernel(PhaseDev pd) {
     j = cc.y + (m & 1);
     return (m == 0) ? w * (1 - r1) * (1 - r2) : (m == 1) ? w * (1 - r1) * r2 : (m == 2) ? w * r1 * (1 - r2) : w * r1 * r2;
   };
+  if constexpr (PREC == SAT_PREC_F32) {
+    if (pd.dmask) {
+      // The row mask: the adjoint reads the delta planes only in the lane
+      // segments (C columns of one row) marked here, which are zeroed first in
+      // both orientations; elsewhere the planes hold the forward's values.
+      unsigned long long* msk = pd.dmask + (size_t)e * pd.dmask_stride;
+      const int mw = (g.La + kWave - 1) / kWave;
+      for (int i = lane; i < g.ny * mw; i += kWave) msk[i] = 0ull;
+      for (int c = lane; c < ns; c += kWave) {
+        int i, j;
+        (void)share(c, i, j);
+        const int l = i / g.C;
+        for (int k = 0; k < g.C; k++) {
+          const int ix = l * g.C + k;
+          dl0[wave_index(g, 0, ix, j)] = V(0);
+          dl1[wave_index(g, 1, ix, j)] = V(0);
+        }
+      }
+      __threadfence_block();  // the zeroes are written before the values and the marks
+      __syncthreads();
+      for (int c = lane; c < ns; c += kWave) {
+        int i, j;
+        (void)share(c, i, j);
+        const int l = i / g.C;
+        atomicOr(msk + (size_t)j * mw + (l >> 6), 1ull << (l & 63));
+      }
+    }
+  }
   if (ns > cap) {
     if (lane != 0) return;
     double* acc = pd.T0d + ev.pl_off;  // FP32: an FP64 accumulator (T0d is no longer needed once T is formed)
@@ -638,6 +673,12 @@ __global__ __launch_bounds__(kWave) void delta_kernel(PhaseDev pd) {
   }
 }
 
+// Dynamic LDS of an adjoint sweep launch: FP32, one event's delta row mask.
+template <int PREC>
+inline size_t adj_dyn_lds(const PhaseDev& pd) {
+  return PREC == SAT_PREC_F32 ? (size_t)pd.dmask_lds : 0;
+}
+
 // Persistent adjoint sweeps, one work-group of W waves per event; mask and store.
 template <int PREC, int C, int W>
 __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int first, int count) {
@@ -650,6 +691,7 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
   __shared__ ExchangeLds<Pol, W> xch;
   __shared__ double red[W];
   __shared__ int s_idx;
+  extern __shared__ unsigned long long s_mask[];  // FP32: the event's delta row mask (pd.dmask_lds bytes)
   latency_critical_priority<PREC, W>();
   char* sb = pd.slots + (size_t)blockIdx.x * pd.slot_bytes;
   V* Q0 = reinterpret_cast<V*>(sb);
@@ -669,6 +711,16 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
       const bool real = wave_decode0(g, q, ix, iy);
       Q0[q] = (real && ix >= 1 && ix < nx - 1 && iy >= 1 && iy < ny - 1) ? V(100) : V(0);
     }
+    // FP32: the row mask in LDS (without one: a single all-lanes word for every row)
+    const int mw = (PREC == SAT_PREC_F32 && pd.dmask) ? (g.La + kWave - 1) / kWave : 0;
+    if constexpr (PREC == SAT_PREC_F32) {
+      if (mw) {
+        const unsigned long long* msk = pd.dmask + (size_t)e * pd.dmask_stride;
+        for (int i = lane; i < ny * mw; i += nthr) s_mask[i] = msk[i];
+      } else if (lane == 0) {
+        s_mask[0] = ~0ull;
+      }
+    }
     __syncthreads();
     typename Pol::Ctx cx;
     cx.g = &g;
@@ -682,6 +734,10 @@ __global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int firs
     if constexpr (PREC == SAT_PREC_F64) {
       cx.dx = g.dx;
       cx.dy = g.dy;
+    } else {
+      cx.rmask = (typename Pol::MaskLds)s_mask;
+      cx.mw = mw;
+      cx.dl_bytes = (int)(g.wplane * sizeof(V));
     }
     IterOut io;
     const V* ta = iterate<Pol, WaveLayout, C, W, (W > 1) ? mw_adj_depth<PREC, C, W>() : adj_depth1<C>()>(uniform(cx), Q0, Q1, Q2, pd.adj_maxiter, kAdjTol,

@@ -974,15 +974,26 @@ struct AdjF64Lds : AdjF64 {
 // pre-divided by dx and dy: g = (a1/dx, b1/dy), so d = max(gE.x,0) - min(g.x,0)
 // + max(gN.y,0) - min(g.y,0) and Ta = (delta + max(g.x,0)*Ta_W - min(gE.x,0)*Ta_E
 // + max(g.y,0)*Ta_S - min(gN.y,0)*Ta_N) / d.
+// The receiver term delta is zero except in the cells of the event's receivers.
+// With a row mask (rmask: per row, mw words of one bit per lane, in LDS) a lane
+// reads its C delta values from the plane only in the rows where its columns
+// hold a receiver share (delta_kernel zeroes and marks those segments); its
+// other rows' raw buffer loads carry an offset past the plane, which the
+// hardware answers with zeros and no memory access: the sweeps stream 16
+// instead of 20 bytes per node.
 struct AdjF32Face {
   using V = float;
   using A = float2;
   static constexpr bool kAux = true;
+  using MaskLds = const __attribute__((address_space(3))) unsigned long long*;
   struct Ctx {
     const GridDev* g;
     int nx, ny;
     const float2* fxy[2];
     const float* dl[2];
+    MaskLds rmask;
+    int mw;
+    int dl_bytes;  // one delta plane of the event
   };
   template <int C> struct Head { float u[C]; float2 aux[C]; };
   template <int C> struct Body { float dl[C], st[C]; };
@@ -992,12 +1003,29 @@ struct AdjF32Face {
     for (int k = 0; k < C; k++) { H.u[k] = gld(tin, base + k); H.aux[k] = gld(cx.fxy[O], base + k); }
   }
   template <int C, int O, bool NORM>
-  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int, const float* tst) {
+  __device__ static void load_body(const Ctx& cx, Body<C>& B, int base, int j, const float* tst) {
+    const int lane = threadIdx.x;
+    // mw == 0: one word of all lanes for every row (no mask)
+    const unsigned long long m = cx.rmask[cx.mw ? j * cx.mw + imin(lane >> 6, cx.mw - 1) : 0];
+    const int off = ((m >> (lane & 63)) & 1ull) ? base * (int)sizeof(float) : kDropOffset;
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc(const_cast<float*>(cx.dl[O]), cx.dl_bytes);
+    if constexpr (C % 4 == 0) {
+      typedef float v4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-    for (int k = 0; k < C; k++) {
-      B.dl[k] = gld(cx.dl[O], base + k);
-      B.st[k] = NORM ? gld(tst, base + k) : 0.0f;
+      for (int k = 0; k < C; k += 4) {
+        const v4 v = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(rs, off + k * 4, 0, 0));
+        B.dl[k] = v.x;
+        B.dl[k + 1] = v.y;
+        B.dl[k + 2] = v.z;
+        B.dl[k + 3] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < C; k++)
+        B.dl[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, off + k * 4, 0, 0));
     }
+#pragma unroll
+    for (int k = 0; k < C; k++) B.st[k] = NORM ? gld(tst, base + k) : 0.0f;
   }
   __device__ static bool fixed(const Ctx& cx, int ix, int j) {
     return ix == 0 || ix >= cx.nx - 1 || j == 0 || j == cx.ny - 1;
