@@ -673,6 +673,13 @@ __global__ __launch_bounds__(kWave) void delta_kernel(PhaseDev pd) {
   }
 }
 
+// Occupancy of the one-wave FP32 adjoint sweeps on eight columns per lane:
+// three waves per SIMD, so that a batch of up to three pairs per SIMD is
+// resident at once (the memory-bound sweeps gain more from waves in flight than
+// from the registers beyond 168).
+template <int PREC, int C, int W>
+constexpr int adj_waves_per_eu() { return (PREC == SAT_PREC_F32 && W == 1 && C == 8) ? 3 : 1; }
+
 // Dynamic LDS of an adjoint sweep launch: FP32, one event's delta row mask.
 template <int PREC>
 inline size_t adj_dyn_lds(const PhaseDev& pd) {
@@ -681,7 +688,8 @@ inline size_t adj_dyn_lds(const PhaseDev& pd) {
 
 // Persistent adjoint sweeps, one work-group of W waves per event; mask and store.
 template <int PREC, int C, int W>
-__global__ __launch_bounds__(64 * W) void adj_sweep_kernel(PhaseDev pd, int first, int count) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(adj_waves_per_eu<PREC, C, W>())))
+void adj_sweep_kernel(PhaseDev pd, int first, int count) {
   using PT = PhaseT<PREC>;
   using V = typename PT::V;
   using F = typename PT::F;
