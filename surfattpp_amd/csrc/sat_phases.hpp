@@ -259,7 +259,8 @@ __host__ __device__ inline bool wave_decode(const GridDev& g, int o, int q, int&
 // blockIdx.z = pass: 0 and 1 write the orientation-0 / -1 wavefront-major
 // planes (T0, T0x, T0y; thread = plane element, padding skipped), 2 the FP64
 // column-major T0 of FP32 sweeps (thread = node): every store coalesced, the
-// node's arithmetic repeated per pass.
+// node's arithmetic repeated per pass (measured: faster than forming T0 once
+// and gathering it for the wavefront-major passes).
 template <int PREC>
 __global__ void init_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
@@ -506,8 +507,18 @@ __global__ void adj_setup_kernel(PhaseDev pd) {
   const double* b = g.b_plain + (size_t)ev.iper * g.plane;
   const double* c = g.c_plain + (size_t)ev.iper * g.plane;
   double fx = 0.0, fy = 0.0;
-  if (ix >= 1 && iy >= 1 && iy < ny - 1) fx = face_a1(g, T, a, c, ix, iy);
-  if (iy >= 1 && ix >= 1 && ix < nx - 1) fy = face_b1(g, T, b, c, ix, iy);
+  if (PREC == SAT_PREC_F32 && g.iso) {
+    // isotropic metric: a and b depend on the row only and c = 0, so the
+    // cross terms vanish; the node's row values stand for its neighbours'
+    // (the same (a + a) / 2 and (b + b') / 2), and the a, b, c planes are not read
+    const double ar = a[(size_t)iy * nx];
+    if (ix >= 1 && iy >= 1 && iy < ny - 1) fx = -(T[p] - T[p - 1]) / g.dx * (ar + ar) / 2.0;
+    if (iy >= 1 && ix >= 1 && ix < nx - 1)
+      fy = -(T[p] - T[p - nx]) / g.dy * (b[(size_t)iy * nx] + b[(size_t)(iy - 1) * nx]) / 2.0;
+  } else {
+    if (ix >= 1 && iy >= 1 && iy < ny - 1) fx = face_a1(g, T, a, c, ix, iy);
+    if (iy >= 1 && ix >= 1 && ix < nx - 1) fy = face_b1(g, T, b, c, ix, iy);
+  }
   F v;
   if constexpr (PREC == SAT_PREC_F64) v = make_double2(fx, fy);
   else v = make_float2((float)(fx / g.dx), (float)(fy / g.dy));
