@@ -2022,10 +2022,10 @@ int enqueue_run(sat_batch* b, int mode, bool capturing) {
   pd.adj_maxiter = kAdjMaxIter;
   pd.polish = b->polish ? 1 : 0;
   const unsigned node_bx = (unsigned)((b->max_plane + 255) / 256);
-  // init_kernel: one thread per wavefront-major element (either geometry) or node
+  // init_kernel: kInitPerThread wavefront-major elements (either geometry) or nodes per thread
   size_t init_n = b->max_plane;
   for (const GridHost& G : b->grids) init_n = std::max({init_n, G.g.wplane, G.gmw.wplane});
-  const unsigned init_bx = (unsigned)((init_n + 255) / 256);
+  const unsigned init_bx = (unsigned)((init_n + 256 * kInitPerThread - 1) / (256 * kInitPerThread));
   hipEvent_t* mk = b->mk;
   // waves per pair and columns per lane of the re-solve geometry:
   // Wr = resolve_waves(C), ceil(nx / (64 Wr)) = ceil(C / Wr)

@@ -260,7 +260,9 @@ __host__ __device__ inline bool wave_decode(const GridDev& g, int o, int q, int&
 // planes (T0, T0x, T0y; thread = plane element, padding skipped), 2 the FP64
 // column-major T0 of FP32 sweeps (thread = node): every store coalesced, the
 // node's arithmetic repeated per pass (measured: faster than forming T0 once
-// and gathering it for the wavefront-major passes).
+// and gathering it for the wavefront-major passes).  A thread takes
+// kInitPerThread elements (fewer work-groups to dispatch).
+constexpr int kInitPerThread = 4;
 template <int PREC>
 __global__ void init_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
@@ -269,53 +271,55 @@ __global__ void init_kernel(PhaseDev pd) {
   const EventX ev = pd.ev[e];
   const GridDev& g = pd.grids[ev.grid];
   const int pass = blockIdx.z;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  int ix, iy;
-  if (pass < 2) {
-    if (idx >= (int)g.wplane || !wave_decode(g, pass, idx, ix, iy)) return;
-  } else {
-    if (idx >= (int)g.plane) return;
-    iy = idx / g.nx;
-    ix = idx - iy * g.nx;
-  }
   const double fun0 = pd.evs[3 * (size_t)e], sy0 = pd.evs[3 * (size_t)e + 1], cy0 = pd.evs[3 * (size_t)e + 2];
-  const double2 rw = g.trig_yy[iy];
-  const double2 cl = pd.coltab[ev.col_off + ix];
-  const double syy = rw.x, cyy = rw.y, sdl = cl.x, cdl = cl.y;
-  const double tmp = syy * sy0 + cyy * cy0 * cdl;
-  const double tmp_c = dmax(-1.0, dmin(1.0, tmp));
-  const double ac = (PREC == SAT_PREC_F64) ? cr::acos_cr(tmp_c) : acos(tmp_c);
-  const double T0 = ac * kR * fun0;
-  if (pass == 2) {  // the FP64 column-major T0 of FP32 sweeps: no gradient
-    pd.T0d[ev.pl_off + idx] = T0;
-    return;
-  }
-  const size_t w = ev.w_off + idx;
-  static_cast<V*>(pd.t0[pass])[w] = (V)T0;
-  const double sin_d = sqrt(dmax(0.0, 1.0 - tmp_c * tmp_c));
-  const bool flat = sin_d < 1e-14;
-  const double gx = cyy * cy0 * sdl, gy = -(cyy * sy0 - syy * cy0 * cdl);
   const bool scaled = (PREC == SAT_PREC_F32 && pd.polish && g.iso && g.wf64[0]) ||
                       ((PREC == SAT_PREC_F32 || pd.scaled_iso) && g.iso);
-  if (PREC == SAT_PREC_F64 && !scaled) {
-    // FP64 batches: the reference's expressions, in its operation order
-    static_cast<V*>(pd.t0x[pass])[w] = flat ? 0.0 : gx / sin_d * kR * fun0;
-    static_cast<V*>(pd.t0y[pass])[w] = flat ? 0.0 : gy / sin_d * kR * fun0;
-    return;
-  }
-  // FP32 sweeps and FP64 re-solves (the same quantities, one division per
-  // node): scaled by sqrt(a) = 1/(R cos y), sqrt(b) = 1/R on isotropic grids,
-  // T0x = cy cy0 sdl fun0 R / sin_d and T0y likewise otherwise
-  const double r = flat ? 0.0 : (scaled ? fun0 : kR * fun0) / sin_d;
-  const double tx = (scaled ? cy0 * sdl : gx) * r, ty = gy * r;
-  if (PREC == SAT_PREC_F32 && pd.polish && g.iso && g.wf64[0]) {
-    // the polished isotropic FP32 sweeps' scaled gradient, FP64 (FwdF32IsoP)
-    static_cast<double*>(pd.t0x[pass])[w] = tx;
-    static_cast<double*>(pd.t0y[pass])[w] = ty;
-  } else {
-    // FP32: the sweeps' gradient (FwdF32Iso scaled, FwdF32); FP64: the re-solves' (FwdF64RIso)
-    static_cast<V*>(pd.t0x[pass])[w] = (V)tx;
-    static_cast<V*>(pd.t0y[pass])[w] = (V)ty;
+  const int n = pass < 2 ? (int)g.wplane : (int)g.plane;
+  // kInitPerThread elements per thread, the block's threads side by side
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+    int ix, iy;
+    if (pass < 2) {
+      if (!wave_decode(g, pass, idx, ix, iy)) continue;
+    } else {
+      iy = idx / g.nx;
+      ix = idx - iy * g.nx;
+    }
+    const double2 rw = g.trig_yy[iy];
+    const double2 cl = pd.coltab[ev.col_off + ix];
+    const double syy = rw.x, cyy = rw.y, sdl = cl.x, cdl = cl.y;
+    const double tmp = syy * sy0 + cyy * cy0 * cdl;
+    const double tmp_c = dmax(-1.0, dmin(1.0, tmp));
+    const double ac = (PREC == SAT_PREC_F64) ? cr::acos_cr(tmp_c) : acos(tmp_c);
+    const double T0 = ac * kR * fun0;
+    if (pass == 2) {  // the FP64 column-major T0 of FP32 sweeps: no gradient
+      pd.T0d[ev.pl_off + idx] = T0;
+      continue;
+    }
+    const size_t w = ev.w_off + idx;
+    static_cast<V*>(pd.t0[pass])[w] = (V)T0;
+    const double sin_d = sqrt(dmax(0.0, 1.0 - tmp_c * tmp_c));
+    const bool flat = sin_d < 1e-14;
+    const double gx = cyy * cy0 * sdl, gy = -(cyy * sy0 - syy * cy0 * cdl);
+    if (PREC == SAT_PREC_F64 && !scaled) {
+      // FP64 batches: the reference's expressions, in its operation order
+      static_cast<V*>(pd.t0x[pass])[w] = flat ? 0.0 : gx / sin_d * kR * fun0;
+      static_cast<V*>(pd.t0y[pass])[w] = flat ? 0.0 : gy / sin_d * kR * fun0;
+      continue;
+    }
+    // FP32 sweeps and FP64 re-solves (the same quantities, one division per
+    // node): scaled by sqrt(a) = 1/(R cos y), sqrt(b) = 1/R on isotropic grids,
+    // T0x = cy cy0 sdl fun0 R / sin_d and T0y likewise otherwise
+    const double r = flat ? 0.0 : (scaled ? fun0 : kR * fun0) / sin_d;
+    const double tx = (scaled ? cy0 * sdl : gx) * r, ty = gy * r;
+    if (PREC == SAT_PREC_F32 && pd.polish && g.iso && g.wf64[0]) {
+      // the polished isotropic FP32 sweeps' scaled gradient, FP64 (FwdF32IsoP)
+      static_cast<double*>(pd.t0x[pass])[w] = tx;
+      static_cast<double*>(pd.t0y[pass])[w] = ty;
+    } else {
+      // FP32: the sweeps' gradient (FwdF32Iso scaled, FwdF32); FP64: the re-solves' (FwdF64RIso)
+      static_cast<V*>(pd.t0x[pass])[w] = (V)tx;
+      static_cast<V*>(pd.t0y[pass])[w] = (V)ty;
+    }
   }
 }
 
