@@ -1,6 +1,7 @@
 """The batch's scheduling options (sat_batch_set_option: two pairs per wave,
-the adjoint's columns per lane, the queue order) change where and in which
-order pairs are swept, never what is computed: every setting keeps the FP64
+the adjoint's columns per lane, the queue order, the speculated FP64 solves)
+change where and in which order pairs are swept, never what is computed: every
+setting keeps the FP64
 iteration counts and stays within the FP32 bar of the FP64 batch (which
 equals the oracle), and settings that keep the same arithmetic give the same
 bits."""
@@ -47,7 +48,7 @@ def _run(grids, opts, runs=2):
 
 @pytest.mark.parametrize("opts", [
     {}, {"pair2": 0}, {"pair2": 1}, {"adj_cols": 0}, {"order": 0},
-    {"pair2": 1, "adj_cols": 0, "order": 0},
+    {"pair2": 1, "adj_cols": 0, "order": 0}, {"speculate": 0},
 ])
 def test_options_keep_results(grids, ref64, opts):
     outs = _run(grids, opts)
@@ -76,7 +77,7 @@ def test_order_changes_no_bits(grids):
 
 def test_invalid_options(grids):
     with PP.EikonalBatch(grids[:1], precision=PREC_F32) as b:
-        for name, v in (("pair2", 2), ("adj_cols", -1), ("order", 5)):
+        for name, v in (("pair2", 2), ("adj_cols", -1), ("order", 5), ("speculate", 2)):
             with pytest.raises(_lib.SatError):
                 b.set_option(name, v)
         with pytest.raises(_lib.SatError):
