@@ -1744,8 +1744,10 @@ int sat_batch_create(const sat_batch_desc* d, sat_batch** out) {
   if (f32) {
     // the delta row masks: per event, ny rows of one bit per lane of either
     // adjoint geometry; the adjoint sweeps hold one in LDS (beyond kMaskLds
-    // bytes the planes are read whole instead)
-    constexpr size_t kMaskLds = 48 << 10;
+    // bytes the planes are read whole instead; SURFATT_MASK_LDS lowers the
+    // limit, for tests of that path)
+    size_t kMaskLds = 48 << 10;
+    if (const char* v = std::getenv("SURFATT_MASK_LDS")) kMaskLds = std::min<size_t>(kMaskLds, std::strtoul(v, nullptr, 10));
     size_t stride = 1;
     for (const GridHost& G : b->grids) {
       const int la = std::max(G.g.La, G.adj8 ? G.gadj.La : 0);
