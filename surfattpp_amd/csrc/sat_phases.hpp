@@ -262,7 +262,7 @@ __host__ __device__ inline bool wave_decode(const GridDev& g, int o, int q, int&
 // node's arithmetic repeated per pass (measured: faster than forming T0 once
 // and gathering it for the wavefront-major passes).  A thread takes
 // kInitPerThread elements (fewer work-groups to dispatch).
-constexpr int kInitPerThread = 4;
+constexpr int kInitPerThread = 8;
 template <int PREC>
 __global__ void init_kernel(PhaseDev pd) {
   using V = typename PhaseT<PREC>::V;
